@@ -1,0 +1,249 @@
+/*
+ * latec.h -- C ABI of the MI355X-native LATEC codec hot path (liblatec.so).
+ *
+ * Plain pointers and sizes only.  "d_" pointers are device (HBM) pointers,
+ * all others are host pointers.  `stream` is a hipStream_t (NULL = default
+ * stream).  Every function returns 0 (or a non-negative size) on success and
+ * a negative LATEC_E_* code on failure; latec_last_error() then holds a
+ * message (thread-local).  Callers own every buffer they pass in; the library
+ * never frees caller memory.  Device work is asynchronous on `stream`.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   plan / stage geometry ..... latec/levels.py:100-147, 221-282, 297-386, 410-574
+ *   interp encode/decode ...... latec/predictor.py:75-317 fused with
+ *                               latec/quantizer.py:28-120 and the stage loop
+ *                               latec/codec.py:384-510
+ *   quantize / dequantize ..... latec/quantizer.py:28-120
+ *   tables / fit .............. latec/rans.py:162-180, 247-267, 270-278,
+ *                               305-329, 377-430, 676-722
+ *   rANS coder ................ constriction 0.4 stack AnsCoder +
+ *                               Categorical(perfect=False) as bound at
+ *                               latec/rans.py:466-535, 629-657
+ *   stage record / frame ...... latec/bitstream.py:155-164, 178-344
+ *                               (zstandard bound at bitstream.py:156-163,
+ *                               210-212, 312)
+ */
+#ifndef LATEC_H
+#define LATEC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LATEC_OK 0
+#define LATEC_E_ARG -1
+#define LATEC_E_HIP -2
+#define LATEC_E_CAPACITY -3
+#define LATEC_E_FORMAT -4
+#define LATEC_E_COUNT -5
+#define LATEC_MAX_DIM 8
+
+/* Last error message of the calling thread ("" if none). */
+const char *latec_last_error(void);
+/* ABI version (bumped on any signature change). */
+int latec_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Schedule plan: the dyadic stage schedule of one region shape.       */
+/* latec/levels.py:100-147 (iter_stage_plan), 221-282 (stage_counts),  */
+/* 410-574 (parity-class layouts with affine stage rank).              */
+/* ------------------------------------------------------------------ */
+typedef struct latec_plan latec_plan;
+
+/* Validation and messages follow levels.py:54-76 (_check_plan). */
+int latec_plan_create(int ndim, const int64_t *shape, int channels, int levels,
+                      int anchor_stride, int anchor_block, latec_plan **out);
+void latec_plan_destroy(latec_plan *p);
+int latec_plan_n_stages(const latec_plan *p);
+/* Points per channel of `stage`, its lattice stride, weight (0 anchors, 1
+ * single axis, >=2 fused) and number of parity classes. */
+int latec_plan_stage_info(const latec_plan *p, int stage, int64_t *count,
+                          int *stride, int *weight, int *n_classes);
+/* One parity class: out[0]=parity bits, [1]=grid step, [2]=rank offset,
+ * [3]=size, then starts[ndim], sizes[ndim], rank strides[ndim]. */
+int latec_plan_class_info(const latec_plan *p, int stage, int cls,
+                          int64_t *out);
+/* Offset of `stage` in the concatenated per-channel stage arrays. */
+int64_t latec_plan_stage_offset(const latec_plan *p, int stage);
+
+/* ------------------------------------------------------------------ */
+/* Fused interpolation + quantization stage kernels                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int32_t cubic;      /* 1 cubic, 0 linear (predictor.py:187-217) */
+  int32_t center;     /* 0 average odd axes, 1 first, 2 last */
+  int32_t end_mode;   /* END_QUAD=1 | END_EXTRAP=2 (predictor.py:46-48) */
+  int32_t round_mode; /* 0 none, 1 rint (integer source), 2 (span, offset) */
+  double span, offset;
+  uint32_t radius;    /* quantizer radius R, codes in [0, 2R) */
+  int32_t code_bytes; /* 2 (R <= 32768) or 4 */
+} latec_interp_opts;
+
+/* Encode stages [s0, s1) of the plan in place (codec.py:384-459):
+ *  d_x, d_recon ... (C, N) float32; d_recon must hold the reconstruction of
+ *                   stages < s0 and zeros elsewhere.
+ *  d_codes ........ C * N codes (u16/u32); stage i occupies
+ *                   [C*off_i, C*off_i + C*n_i) channel-major, stage order.
+ *  d_hist ......... n_stages * 2R u32 code histograms, zeroed by the caller.
+ *  d_out_rank/val . outlier (index in the stage array, value) pairs in
+ *                   arbitrary order; stage i's list starts at C*off_i.
+ *  d_out_count .... n_stages u32 counters, zeroed by the caller.
+ *  ebs ............ host array of per-stage absolute error bounds.
+ * Stage 0 (anchors) is quantized against a zero prediction. */
+int latec_interp_encode(latec_plan *p, int s0, int s1, const float *d_x,
+                        float *d_recon, void *d_codes, uint32_t *d_hist,
+                        uint32_t *d_out_rank, float *d_out_val,
+                        uint32_t *d_out_count, const double *ebs,
+                        const latec_interp_opts *o, void *stream);
+
+/* Decode stages [s0, s1) in place (codec.py:462-510).  d_out_rank must be
+ * sorted ascending within each stage; d_out_count (device) per stage. */
+int latec_interp_decode(latec_plan *p, int s0, int s1, const void *d_codes,
+                        const uint32_t *d_out_rank, const float *d_out_val,
+                        const uint32_t *d_out_count, float *d_recon,
+                        const double *ebs, const latec_interp_opts *o,
+                        void *stream);
+
+/* Capture the whole encode (or decode) of the plan as a hipGraph on first
+ * use and replay it afterwards; same arguments as the calls above with
+ * s0 = 0, s1 = n_stages.  The graph is cached per plan and argument set. */
+int latec_interp_encode_graph(latec_plan *p, const float *d_x, float *d_recon,
+                              void *d_codes, uint32_t *d_hist,
+                              uint32_t *d_out_rank, float *d_out_val,
+                              uint32_t *d_out_count, const double *ebs,
+                              const latec_interp_opts *o, void *stream);
+
+/* Prediction only (InterpPredictor.predict, predictor.py:255-317): the
+ * stage's (C, n) float32 predictions in stage order from d_recon. */
+int latec_interp_predict(latec_plan *p, int stage, const float *d_recon,
+                         float *d_pred, const latec_interp_opts *o,
+                         void *stream);
+
+/* Add the histogram of n codes (u16/u32) into d_hist (2R bins). */
+int latec_code_hist(const void *d_codes, int64_t n, int code_bytes,
+                    uint32_t radius, uint32_t *d_hist, void *stream);
+
+/* Generic quantizer on device buffers (quantizer.py:28-87): codes and the
+ * outlier (index, value) pairs as above; *d_out_count is one counter. */
+int latec_quantize(const float *d_x, const float *d_pred, int64_t n, double eb,
+                   const latec_interp_opts *o, void *d_codes,
+                   uint32_t *d_out_rank, float *d_out_val,
+                   uint32_t *d_out_count, uint32_t *d_hist, void *stream);
+/* dequantize (quantizer.py:90-120) with sorted outlier positions. */
+int latec_dequantize(const float *d_pred, const void *d_codes, int64_t n,
+                     const uint32_t *d_out_rank, const float *d_out_val,
+                     const uint32_t *d_out_count, double eb,
+                     const latec_interp_opts *o, float *d_recon, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Laplace-mixture tables and the per-stage fit, built on the GPU       */
+/* ------------------------------------------------------------------ */
+/* _level_weights (rans.py:247-267): d_out (64, 2^(kexp+1)) float64. */
+int latec_level_weights(int kexp, double *d_out, void *stream);
+/* _shaped_weights + _with_outlier_weight (rans.py:377-384, 172-180). */
+int latec_model_weights(int kexp, int shape, int owx, double *d_out,
+                        void *stream);
+/* _neg_log2_weights (rans.py:387-392): (64, A) float32. */
+int latec_neg_log2_weights(int kexp, int shape, int owx, int precision,
+                           float *d_out, void *stream);
+/* _build_cdfs_cached rows (rans.py:305-329, 676-722): rows [l0, l1) of the
+ * (64, A+1) uint32 CDF table into d_out ((l1-l0) * (A+1)). */
+int latec_build_cdfs(int kexp, int shape, int precision, int owx, int l0,
+                     int l1, uint32_t *d_out, void *stream);
+
+typedef struct {
+  int32_t kexp, level, shape, owx;
+  float bits;        /* model bits of the fitted level (rans.py:430) */
+  uint32_t n_codes;  /* C * n */
+  uint32_t n_out;    /* outliers (histogram bin 0) */
+  uint32_t pad;
+} latec_fit;
+
+/* choose_kexp + choose_const_level (rans.py:70-80, 395-430) for stages
+ * [s0, s1) from the code histograms d_hist (n_stages * 2R); d_codes is read
+ * only for stages whose largest bin reaches 2^24, to reproduce the
+ * reference's per-2^22-tile float32 histogram accumulation.  Result: host
+ * array fits[s1 - s0] (the call synchronizes `stream`). */
+int latec_fit_const(latec_plan *p, int s0, int s1, const uint32_t *d_hist,
+                    const void *d_codes, uint32_t radius, int code_bytes,
+                    int precision, latec_fit *fits, void *stream);
+/* The same fit for one code array (no plan): n codes. */
+int latec_fit_codes(const uint32_t *d_hist, const void *d_codes, int64_t n,
+                    uint32_t radius, int code_bytes, int precision,
+                    latec_fit *fit, void *stream);
+
+/* scale_to_level (rans.py:270-278) on device: levels u8. */
+int latec_scale_to_level(const float *d_scale, int64_t n, double eb,
+                         uint8_t *d_levels, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Host entropy coding and records                                      */
+/* ------------------------------------------------------------------ */
+/* Stack rANS with a 24-bit categorical model quantized from the float32
+ * pmf of `cdf` (A+1 entries): the design of constriction's AnsCoder<u32,u64>
+ * + Categorical(perfect=False).  Symbols are pushed last-to-first so that
+ * decoding yields them first-to-last (rans.py:490-498).  Returns words
+ * written or LATEC_E_CAPACITY. */
+int64_t latec_rans_encode(const uint32_t *syms, int64_t n,
+                          const uint32_t *cdf, int64_t A, uint32_t *words,
+                          int64_t cap);
+/* Per-point models: symbols grouped by level, levels pushed 63..0
+ * (rans.py:546-578); cdfs is (64, A+1). */
+int64_t latec_rans_encode_levels(const uint32_t *syms, const uint8_t *levels,
+                                 int64_t n, const uint32_t *cdfs, int64_t A,
+                                 uint32_t *words, int64_t cap);
+/* Returns 0, or LATEC_E_FORMAT for trailing / malformed data. */
+int latec_rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf,
+                      int64_t A, uint32_t *syms, int64_t n);
+int latec_rans_decode_levels(const uint32_t *words, int64_t nw,
+                             const uint8_t *levels, const uint32_t *cdfs,
+                             int64_t A, uint32_t *syms, int64_t n);
+
+/* pack_stage, const-level path (bitstream.py:178-243): codes are u16/u32
+ * (code_bytes), `fit` from latec_fit_*, `cdf_row` the (A+1) CDF of
+ * (fit->kexp, fit->shape, fit->owx, fit->level).  Returns bytes written. */
+int64_t latec_pack_stage(const void *codes, int code_bytes, int64_t n,
+                         const float *outl, int64_t n_out, uint32_t radius,
+                         int zstd_level, const latec_fit *fit,
+                         const uint32_t *cdf_row, uint8_t *out, int64_t cap);
+/* Parse a stage record header (bitstream.py:288-300): n_codes and framing
+ * fields; returns 12 or LATEC_E_FORMAT. */
+int latec_stage_header(const uint8_t *buf, int64_t len, uint32_t *n_codes,
+                       int64_t *hlen, int *kexp, int *shape, int *level,
+                       int *owx, int *literal);
+/* unpack_stage, const-level path (bitstream.py:288-344).  cdf_row is the row
+ * named by the header (ignored for literal records).  Writes codes (u16/u32)
+ * and the sorted positions + values of the outliers (code 0).  Returns bytes
+ * consumed. */
+int64_t latec_unpack_stage(const uint8_t *buf, int64_t len, uint32_t radius,
+                           const uint32_t *cdf_row, int64_t n_expect,
+                           void *codes, int code_bytes, uint32_t *out_rank,
+                           float *out_val, int64_t out_cap, int64_t *n_out);
+
+/* Multi-threaded pack of many stages into one payload buffer: stage i's
+ * codes at codes + code_bytes * code_off[i], outliers at outl + out_off[i]
+ * (already in stage order), cdf rows at cdf_rows + row_off[i].  Writes the
+ * concatenated records; rec_len[i] receives each record's size. */
+int64_t latec_pack_stages(int n_stages, const void *codes, int code_bytes,
+                          const int64_t *code_off, const int64_t *n_codes,
+                          const float *outl, const int64_t *out_off,
+                          const int64_t *n_out, uint32_t radius,
+                          int zstd_level, const latec_fit *fits,
+                          const uint32_t *cdf_rows, const int64_t *row_off,
+                          uint8_t *out, int64_t cap, int64_t *rec_len,
+                          int threads);
+
+/* zstd one-shot frames (the reference's zstandard.ZstdCompressor(level)
+ * .compress / ZstdDecompressor().decompress). */
+int64_t latec_zstd_bound(int64_t n);
+int64_t latec_zstd_compress(const uint8_t *src, int64_t n, int level,
+                            uint8_t *dst, int64_t cap);
+int64_t latec_zstd_content_size(const uint8_t *src, int64_t n);
+int64_t latec_zstd_decompress(const uint8_t *src, int64_t n, uint8_t *dst,
+                              int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LATEC_H */

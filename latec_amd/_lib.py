@@ -1,0 +1,140 @@
+"""ctypes binding of liblatec.so (include/latec.h).
+
+The library is built in-tree by ``latec_amd.csrc.build`` (``__graft_entry__``
+``build()``).  There is no fallback: if the shared library is missing or a
+call fails, this module raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "liblatec.so")
+
+LATEC_OK = 0
+LATEC_E_ARG = -1
+LATEC_E_HIP = -2
+LATEC_E_CAPACITY = -3
+LATEC_E_FORMAT = -4
+LATEC_E_COUNT = -5
+
+
+class InterpOpts(C.Structure):
+    _fields_ = [
+        ("cubic", C.c_int32),
+        ("center", C.c_int32),
+        ("end_mode", C.c_int32),
+        ("round_mode", C.c_int32),
+        ("span", C.c_double),
+        ("offset", C.c_double),
+        ("radius", C.c_uint32),
+        ("code_bytes", C.c_int32),
+    ]
+
+
+class Fit(C.Structure):
+    _fields_ = [
+        ("kexp", C.c_int32),
+        ("level", C.c_int32),
+        ("shape", C.c_int32),
+        ("owx", C.c_int32),
+        ("bits", C.c_float),
+        ("n_codes", C.c_uint32),
+        ("n_out", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
+p = C.c_void_p
+i64, i32, u32, f64 = C.c_int64, C.c_int, C.c_uint32, C.c_double
+
+# name -> (restype, argtypes); every symbol declared in include/latec.h
+SIGNATURES = {
+    "latec_last_error": (C.c_char_p, []),
+    "latec_abi_version": (i32, []),
+    "latec_plan_create": (i32, [i32, p, i32, i32, i32, i32, C.POINTER(p)]),
+    "latec_plan_destroy": (None, [p]),
+    "latec_plan_n_stages": (i32, [p]),
+    "latec_plan_stage_info": (i32, [p, i32, p, p, p, p]),
+    "latec_plan_class_info": (i32, [p, i32, i32, p]),
+    "latec_plan_stage_offset": (i64, [p, i32]),
+    "latec_interp_encode": (i32, [p, i32, i32, p, p, p, p, p, p, p, p, p, p]),
+    "latec_interp_decode": (i32, [p, i32, i32, p, p, p, p, p, p, p, p]),
+    "latec_interp_encode_graph": (i32, [p, p, p, p, p, p, p, p, p, p, p]),
+    "latec_interp_predict": (i32, [p, i32, p, p, p, p]),
+    "latec_code_hist": (i32, [p, i64, i32, u32, p, p]),
+    "latec_quantize": (i32, [p, p, i64, f64, p, p, p, p, p, p, p]),
+    "latec_dequantize": (i32, [p, p, i64, p, p, p, f64, p, p, p]),
+    "latec_level_weights": (i32, [i32, p, p]),
+    "latec_model_weights": (i32, [i32, i32, i32, p, p]),
+    "latec_neg_log2_weights": (i32, [i32, i32, i32, i32, p, p]),
+    "latec_build_cdfs": (i32, [i32, i32, i32, i32, i32, i32, p, p]),
+    "latec_fit_const": (i32, [p, i32, i32, p, p, u32, i32, i32, p, p]),
+    "latec_fit_codes": (i32, [p, p, i64, u32, i32, i32, p, p]),
+    "latec_scale_to_level": (i32, [p, i64, f64, p, p]),
+    "latec_rans_encode": (i64, [p, i64, p, i64, p, i64]),
+    "latec_rans_encode_levels": (i64, [p, p, i64, p, i64, p, i64]),
+    "latec_rans_decode": (i32, [p, i64, p, i64, p, i64]),
+    "latec_rans_decode_levels": (i32, [p, i64, p, p, i64, p, i64]),
+    "latec_pack_stage": (i64, [p, i32, i64, p, i64, u32, i32, p, p, p, i64]),
+    "latec_stage_header": (i32, [p, i64, p, p, p, p, p, p, p]),
+    "latec_unpack_stage": (i64, [p, i64, u32, p, i64, p, i32, p, p, i64, p]),
+    "latec_pack_stages": (i64, [i32, p, i32, p, p, p, p, p, u32, i32, p, p, p, p, i64, p, i32]),
+    "latec_zstd_bound": (i64, [i64]),
+    "latec_zstd_compress": (i64, [p, i64, i32, p, i64]),
+    "latec_zstd_content_size": (i64, [p, i64]),
+    "latec_zstd_decompress": (i64, [p, i64, p, i64]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m latec_amd.csrc.build` "
+                "(the HIP path has no CPU fallback)"
+            )
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().latec_last_error().decode("utf-8", "replace")
+
+
+def check(rc: int, what: str = "") -> int:
+    """Raise ValueError (argument/format errors) or RuntimeError (HIP errors)."""
+    if rc < 0:
+        msg = last_error() or what
+        if rc == LATEC_E_HIP:
+            raise RuntimeError(msg)
+        raise ValueError(msg)
+    return rc
+
+
+def ptr(a) -> C.c_void_p:
+    """Data pointer of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return C.c_void_p(0)
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def stream_ptr(stream=None) -> C.c_void_p:
+    """hipStream_t of a torch stream (None -> torch's current stream)."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)

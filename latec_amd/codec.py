@@ -1,0 +1,607 @@
+"""Closed-loop plain codec (the API of latec/codec.py).
+
+``compress``/``decompress`` keep the reference's signatures, tune sweep,
+header and stats keys (codec.py:45-322).  With the package's
+``InterpPredictor`` the whole dyadic schedule runs as fused GPU stage
+kernels (predict + quantize + reconstruct in place, one launch per stage),
+the per-stage level fit and tables run on the GPU, and the host packs the
+stage records (multi-threaded stack rANS / literal zstd) into the same
+LATEC001 v4 stream.  Any other predictor object goes through the generic
+loop: prediction in Python, quantization and fit on the GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import _device as D
+from . import rans as R
+from ._lib import Fit, check, lib, ptr
+from .bitstream import (DTYPE_CODES, DTYPE_IDS, END_MODE_SHIFT, FLAG_CUBIC, FLAG_INTERP,
+                        FLAG_RANS, Header, pack_record, pack_stage, parse_record,
+                        read_stream, unpack_record, unpack_stage, write_stream)
+from .levels import mask_tiles, stage_ebs, stage_masks
+from .predictor import (END_MODE_MAX, InterpPredictor, default_interp_center,
+                        default_interp_end_mode, get_plan)
+from .quantizer import dequantize, opts_for, quantize
+
+_THREADS = int(os.environ.get("LATEC_HOST_THREADS", min(16, os.cpu_count() or 1)))
+
+
+def _optr(o):
+    return ctypes.cast(ctypes.pointer(o), ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# GPU fast path
+# ---------------------------------------------------------------------------
+
+class GpuEncodeResult:
+    """Device-side result of the fused stage kernels for one encode."""
+
+    def __init__(self, plan, d_recon, d_codes, d_hist, d_rank, d_val, d_cnt, opts):
+        self.plan, self.d_recon, self.d_codes, self.d_hist = plan, d_recon, d_codes, d_hist
+        self.d_rank, self.d_val, self.d_cnt, self.opts = d_rank, d_val, d_cnt, opts
+
+
+class Workspace:
+    """Reusable device buffers for one plan + radius (sized to the field)."""
+
+    def __init__(self, plan, radius):
+        C, N, ns = plan.channels, plan.npoints, plan.n_stages
+        cb = 2 if radius <= 32768 else 4
+        self.recon = D.empty(C * N, np.float32)
+        self.codes = D.empty(C * N, np.uint16 if cb == 2 else np.uint32)
+        self.hist = D.empty(ns * 2 * radius, np.uint32)
+        self.rank = D.empty(C * N, np.uint32)
+        self.val = D.empty(C * N, np.float32)
+        self.cnt = D.empty(ns, np.uint32)
+
+
+_WS: dict = {}
+
+
+def workspace(plan, radius) -> Workspace:
+    key = (id(plan), radius)
+    ws = _WS.get(key)
+    if ws is None:
+        _WS.clear()  # one field at a time: release the previous one
+        ws = Workspace(plan, radius)
+        _WS[key] = ws
+    return ws
+
+
+def encode_device(d_x, plan, ebs, o, graph: bool = False, ws: Workspace = None):
+    """Run the fused encode over all stages of ``plan`` on device input
+    ``d_x`` ((C*N,) float32).  Returns a GpuEncodeResult (asynchronous)."""
+    ws = ws or workspace(plan, o.radius)
+    ws.recon.zero_()
+    ws.hist.zero_()
+    ws.cnt.zero_()
+    ebs_arr = np.ascontiguousarray(ebs, np.float64)
+    fn = lib().latec_interp_encode_graph if graph else None
+    if graph:
+        check(fn(plan.handle, ptr(d_x), ptr(ws.recon), ptr(ws.codes), ptr(ws.hist),
+                 ptr(ws.rank), ptr(ws.val), ptr(ws.cnt), ptr(ebs_arr), _optr(o), ptr(None)))
+    else:
+        check(lib().latec_interp_encode(plan.handle, 0, plan.n_stages, ptr(d_x), ptr(ws.recon),
+                                        ptr(ws.codes), ptr(ws.hist), ptr(ws.rank), ptr(ws.val),
+                                        ptr(ws.cnt), ptr(ebs_arr), _optr(o), ptr(None)))
+    return GpuEncodeResult(plan, ws.recon, ws.codes, ws.hist, ws.rank, ws.val, ws.cnt, o)
+
+
+def fit_device(res: GpuEncodeResult, precision: int = 24):
+    """Per-stage (kexp, level, shape, owx, bits) fitted on the GPU."""
+    plan, o = res.plan, res.opts
+    fits = (Fit * plan.n_stages)()
+    check(lib().latec_fit_const(plan.handle, 0, plan.n_stages, ptr(res.d_hist),
+                                ptr(res.d_codes), o.radius, o.code_bytes, precision,
+                                ctypes.cast(fits, ctypes.c_void_p), ptr(None)))
+    return fits
+
+
+def pack_device(res: GpuEncodeResult, fits, zstd_level: int, threads: int = _THREADS):
+    """Copy codes/outliers to the host and pack every stage record."""
+    plan, o = res.plan, res.opts
+    C, ns = plan.channels, plan.n_stages
+    cb = o.code_bytes
+    codes = D.to_host(res.d_codes, np.uint16 if cb == 2 else np.uint32)
+    cnt = D.to_host(res.d_cnt, np.uint32)
+    code_off = np.array([C * plan.offsets[s] for s in range(ns)], np.int64)
+    n_codes = np.array([C * plan.counts[s] for s in range(ns)], np.int64)
+    n_out = cnt.astype(np.int64)
+    outl = np.zeros(max(int(n_out.sum()), 1), np.float32)
+    out_off = np.zeros(ns, np.int64)
+    at = 0
+    for s in range(ns):
+        out_off[s] = at
+        k = int(n_out[s])
+        if k:
+            lo = int(code_off[s])
+            rk = D.to_host(res.d_rank[lo:lo + k], np.uint32)
+            vl = D.to_host(res.d_val[lo:lo + k])
+            outl[at:at + k] = vl[np.argsort(rk, kind="stable")]
+            at += k
+    rows, row_off, pos = [], np.zeros(ns, np.int64), 0
+    for s in range(ns):
+        f = fits[s]
+        row = (R.cdf_row(f.kexp, f.shape, f.owx, f.level) if n_codes[s]
+               else np.zeros((2 << 1) + 1, np.uint32))
+        rows.append(row)
+        row_off[s] = pos
+        pos += len(row)
+    cdf_rows = np.ascontiguousarray(np.concatenate(rows), np.uint32)
+    cap = int(64 * ns + 4 * (n_codes.sum() + 2 * ns) + 4 * n_out.sum() + (1 << 16))
+    payload = np.empty(cap, np.uint8)
+    rec_len = np.zeros(ns, np.int64)
+    size = check(lib().latec_pack_stages(ns, ptr(codes), cb, ptr(code_off), ptr(n_codes),
+                                         ptr(outl), ptr(out_off), ptr(n_out), o.radius,
+                                         int(zstd_level), ctypes.cast(fits, ctypes.c_void_p),
+                                         ptr(cdf_rows), ptr(row_off), ptr(payload), cap,
+                                         ptr(rec_len), int(threads)))
+    return payload[:size].tobytes(), rec_len, n_out
+
+
+def decode_device(payload: bytes, plan, ebs, o, threads: int = _THREADS):
+    """Parse and entropy-decode every stage record (host threads), then run
+    the fused decode kernels; returns the device reconstruction."""
+    C, ns = plan.channels, plan.n_stages
+    offs, off = [], 0
+    for s in range(ns):
+        offs.append(off)
+        off += parse_record(payload, off)[7]
+    if off != len(payload):
+        raise ValueError("trailing bytes in LATEC payload")
+    cb = o.code_bytes
+    codes = np.empty(max(C * plan.npoints, 1), np.uint16 if cb == 2 else np.uint32)
+    rank = np.empty(max(C * plan.npoints, 1), np.uint32)
+    val = np.empty(max(C * plan.npoints, 1), np.float32)
+    cnt = np.zeros(ns, np.uint32)
+
+    def one(s):
+        n = C * plan.counts[s]
+        c, r, v, _ = unpack_record(payload, offs[s], n, o.radius)
+        lo = C * plan.offsets[s]
+        codes[lo:lo + n] = c
+        rank[lo:lo + len(r)] = r
+        val[lo:lo + len(v)] = v
+        cnt[s] = len(r)
+
+    order = sorted(range(ns), key=lambda s: -plan.counts[s])
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(one, order))
+    ws = workspace(plan, o.radius)
+    ws.codes.copy_(D.to_device(codes[:C * plan.npoints]))
+    ws.rank.copy_(D.to_device(rank[:C * plan.npoints]))
+    ws.val.copy_(D.to_device(val[:C * plan.npoints]))
+    ws.cnt.copy_(D.to_device(cnt))
+    ws.recon.zero_()
+    ebs_arr = np.ascontiguousarray(ebs, np.float64)
+    check(lib().latec_interp_decode(plan.handle, 0, ns, ptr(ws.codes), ptr(ws.rank),
+                                    ptr(ws.val), ptr(ws.cnt), ptr(ws.recon), ptr(ebs_arr),
+                                    _optr(o), ptr(None)))
+    return ws.recon
+
+
+def _interp_opts(predictor, ndim, radius, round_output):
+    center, end = predictor.resolved(ndim)
+    return opts_for(radius, round_output, cubic=predictor.order == "cubic", center=center,
+                    end_mode=end)
+
+
+def _compress_region_gpu(field, ebs, predictor, radius, round_output, stats, zstd_level):
+    C = field.shape[0]
+    region = field.shape[1:]
+    plan = get_plan(region, C, predictor.levels, predictor.anchor_stride,
+                    predictor.anchor_block)
+    o = _interp_opts(predictor, len(region), radius, round_output)
+    t0 = time.time()
+    d_x = field if hasattr(field, "data_ptr") else D.to_device(field.reshape(-1))
+    res = encode_device(d_x.reshape(-1), plan, ebs, o)
+    fits = fit_device(res)
+    stats["quantize_s"] += time.time() - t0
+    t0 = time.time()
+    payload, rec_len, n_out = pack_device(res, fits, zstd_level)
+    stats["entropy_s"] += time.time() - t0
+    recon = D.to_host(res.d_recon).reshape(C, *region)
+    x = field if isinstance(field, np.ndarray) else D.to_host(field).reshape(C, *region)
+    for s in range(plan.n_stages):
+        stats["stage_codes"][s] += C * plan.counts[s]
+        stats["stage_outliers"][s] += int(n_out[s])
+        stats["stage_payload_bytes"][s] += int(rec_len[s])
+        stats["stage_model_bits"][s] += float(fits[s].bits)
+    stats["outliers"] += int(n_out.sum())
+    _region_error_stats(stats, plan, x, recon)
+    return payload, recon
+
+
+def _region_error_stats(stats, plan, x, recon):
+    """Per-stage reconstruction error moments (codec.py:353-381), from the
+    final reconstruction gathered by stage mask on the host."""
+    err = np.abs(x.astype(np.float32) - recon)
+    C = x.shape[0]
+    for s, mask in enumerate(stage_masks(plan.shape, plan.levels, plan.stride, plan.block)):
+        if not plan.counts[s]:
+            continue
+        e = err.reshape(C, -1)[:, mask.reshape(-1)]
+        stats["stage_recon_sae"][s] += float(e.sum(dtype=np.float64))
+        stats["stage_recon_sse"][s] += float(np.square(e, dtype=np.float64).sum())
+        stats["stage_recon_max"][s] = max(stats["stage_recon_max"][s], float(e.max(initial=0)))
+
+
+# ---------------------------------------------------------------------------
+# generic predictor loop (codec.py:384-510)
+# ---------------------------------------------------------------------------
+
+def _gather_stage(field, pos, n):
+    out = np.empty((field.shape[0], n), field.dtype)
+    flat_field = field.reshape(field.shape[0], -1)
+    at = 0
+    for flat in mask_tiles(pos, field.shape[1:]):
+        out[:, at:at + len(flat)] = flat_field[:, flat]
+        at += len(flat)
+    return out
+
+
+def _scatter_stage(recon, pos, values):
+    flat_recon = recon.reshape(recon.shape[0], -1)
+    at = 0
+    for flat in mask_tiles(pos, recon.shape[1:]):
+        flat_recon[:, flat] = values[:, at:at + len(flat)]
+        at += len(flat)
+
+
+def _stage_error_stats(stats, s, values, pred, recon):
+    pe = values - pred
+    re = np.abs(values - recon)
+    stats["stage_pred_sae"][s] += float(np.abs(pe).sum(dtype=np.float64))
+    stats["stage_pred_sse"][s] += float(np.square(pe, dtype=np.float64).sum())
+    stats["stage_recon_sae"][s] += float(re.sum(dtype=np.float64))
+    stats["stage_recon_sse"][s] += float(np.square(re, dtype=np.float64).sum())
+    stats["stage_recon_max"][s] = max(stats["stage_recon_max"][s], float(re.max(initial=0.0)))
+
+
+def _compress_region(field, masks, ebs, predictor, radius, round_output, stats, zstd_level=9):
+    c = field.shape[0]
+    recon = np.zeros_like(field)
+    known = np.zeros(field.shape[1:], bool)
+    parts = []
+    use_rans = getattr(predictor, "provides_scale", False)
+    for s, pos in enumerate(masks):
+        n = int(np.count_nonzero(pos))
+        if n == 0:
+            if use_rans:
+                parts.append(pack_stage(np.zeros(0, np.uint32), np.zeros(0, np.float32),
+                                        rans_levels=np.zeros(0, np.uint8),
+                                        rans_tables=R.build_laplace_tables(ebs[s], radius)))
+            else:
+                parts.append(pack_stage(np.zeros(0, np.uint32), np.zeros(0, np.float32),
+                                        radius=radius, zstd_level=zstd_level))
+            continue
+        eb = ebs[s]
+        scale = None
+        if s == 0:
+            pred = np.zeros((c, n), np.float32)
+            if use_rans:
+                scale = np.full((c, n), eb, np.float32)
+        else:
+            t0 = time.time()
+            if use_rans:
+                pred, scale = predictor.predict(field_recon_view(recon), known, pos, eb=eb)
+            else:
+                pred = predictor.predict(recon, known, pos)
+            stats["predict_s"] += time.time() - t0
+        t0 = time.time()
+        values = _gather_stage(field, pos, n)
+        codes, outliers = quantize(values, pred, eb, radius, round_output=round_output)
+        stage_recon = dequantize(pred, codes, outliers, eb, radius).reshape(c, n)
+        _scatter_stage(recon, pos, stage_recon)
+        _stage_error_stats(stats, s, values, np.asarray(pred, np.float32).reshape(c, n),
+                           stage_recon)
+        stats["quantize_s"] += time.time() - t0
+        known |= pos
+        stats["outliers"] += len(outliers)
+        stats["stage_outliers"][s] += len(outliers)
+        stats["stage_codes"][s] += n * c
+        t0 = time.time()
+        if use_rans:
+            tables = R.build_laplace_tables(eb, radius)
+            levels64 = R.scale_to_level(scale, eb).reshape(-1)
+            kexp = R.choose_kexp(codes, radius)
+            shape = R.choose_shape(codes, levels64, tables, kexp)
+            stats["stage_model_bits"][s] += R.model_bits(codes, levels64, tables, kexp=kexp,
+                                                         shape=shape)
+            part = pack_stage(codes, outliers, rans_levels=levels64, rans_tables=tables)
+        else:
+            part = pack_stage(codes, outliers, radius=radius, zstd_level=zstd_level)
+        stats["stage_payload_bytes"][s] += len(part)
+        parts.append(part)
+        stats["entropy_s"] += time.time() - t0
+    return b"".join(parts), recon
+
+
+def field_recon_view(recon):
+    return recon
+
+
+def _decompress_region(payload, masks, ebs, header, predictor):
+    c = header.channels
+    region = tuple(header.spatial)
+    recon = np.zeros((c, *region), np.float32)
+    known = np.zeros(region, bool)
+    off = 0
+    use_rans = bool(header.flags & FLAG_RANS)
+    for s, pos in enumerate(masks):
+        n = int(np.count_nonzero(pos))
+        if n == 0:
+            if use_rans:
+                _, _, off = unpack_stage(payload, off, rans_levels=np.zeros(0, np.uint8),
+                                         rans_tables=R.build_laplace_tables(ebs[s],
+                                                                            header.radius))
+            else:
+                _, _, off = unpack_stage(payload, off, radius=header.radius)
+            continue
+        if s == 0:
+            pred = np.zeros((c, n), np.float32)
+            scale = np.full((c, n), ebs[s], np.float32)
+        else:
+            if use_rans:
+                pred, scale = predictor.predict(recon, known, pos, eb=ebs[s])
+            elif getattr(predictor, "provides_scale", False):
+                pred, _ = predictor.predict(recon, known, pos, eb=ebs[s])
+            else:
+                pred = predictor.predict(recon, known, pos)
+        if use_rans:
+            tables = R.build_laplace_tables(ebs[s], header.radius)
+            levels64 = R.scale_to_level(scale, ebs[s]).reshape(-1)
+            codes, outliers, off = unpack_stage(payload, off, rans_levels=levels64,
+                                                rans_tables=tables)
+        else:
+            codes, outliers, off = unpack_stage(payload, off, radius=header.radius)
+        _scatter_stage(recon, pos, dequantize(pred, codes, outliers, ebs[s],
+                                              header.radius).reshape(c, n))
+        known |= pos
+    return recon
+
+
+def _finalize(canvas, header):
+    """codec.py:513-524."""
+    out = np.moveaxis(canvas, 0, -1)
+    dtype = DTYPE_CODES[header.src_dtype]
+    if np.issubdtype(dtype, np.integer):
+        info = np.iinfo(dtype)
+        out = np.clip(np.rint(out), info.min, info.max)
+    out = out.astype(dtype, copy=False)
+    if header.channels == 1:
+        out = out[..., 0]
+    return out
+
+
+def _fast_path(predictor) -> bool:
+    return isinstance(predictor, InterpPredictor) and not getattr(predictor, "provides_scale",
+                                                                 False)
+
+
+def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, radius=1 << 15,
+             zstd_level=9, eb_ratio=None, tune="fast", tune_size_slack=1.05, verbose=False):
+    """codec.py:45-285: returns (stream, stats)."""
+    is_image = img.ndim <= 2 or (img.ndim == 3 and img.shape[-1] in (1, 3))
+    if is_image:
+        if img.ndim == 2:
+            img = img[..., None]
+        spatial = img.shape[:-1]
+        c = img.shape[-1]
+        fimg = np.asarray(img, np.float32)
+    else:
+        spatial = img.shape
+        c = 1
+        fimg = np.asarray(img, np.float32)[..., None]
+    src_dtype = DTYPE_IDS[np.dtype(img.dtype)]
+    vmin = float(fimg.min())
+    vmax = float(fimg.max())
+    if vmax <= vmin:
+        vmax = vmin + 1.0
+    for name, val in (("levels", levels), ("anchor_stride", anchor_stride),
+                      ("anchor_block", anchor_block)):
+        got = getattr(predictor, name, val)
+        if got != val:
+            raise ValueError(f"predictor {name}={got} != compress {name}={val}")
+    field = np.ascontiguousarray(np.moveaxis(fimg, -1, 0))
+    region_shape = field.shape[1:]
+    make_masks = getattr(predictor, "stage_masks", stage_masks)
+    masks = make_masks(region_shape, levels, anchor_stride, anchor_block)
+    flags = getattr(predictor, "stream_flag", 0)
+    use_rans = getattr(predictor, "provides_scale", False)
+    flags |= FLAG_RANS if use_rans else 0
+    round_output = np.issubdtype(np.dtype(img.dtype), np.integer)
+    fast = _fast_path(predictor)
+    d_field = D.to_device(field.reshape(-1)) if fast else None
+
+    def new_stats():
+        k = len(masks)
+        return {"predict_s": 0.0, "quantize_s": 0.0, "entropy_s": 0.0, "outliers": 0,
+                "stage_codes": [0] * k, "stage_outliers": [0] * k,
+                "stage_payload_bytes": [0] * k, "stage_model_bits": [0.0] * k,
+                "stage_pred_sae": [0.0] * k, "stage_pred_sse": [0.0] * k,
+                "stage_recon_sae": [0.0] * k, "stage_recon_sse": [0.0] * k,
+                "stage_recon_max": [0.0] * k}
+
+    def encode(ebs):
+        st = new_stats()
+        if fast:
+            payload, recon = _compress_region_gpu_dev(field, d_field, ebs, predictor, radius,
+                                                      round_output, st, zstd_level)
+        else:
+            payload, recon = _compress_region(field, masks, ebs, predictor, radius,
+                                              round_output, st, zstd_level)
+        return payload, recon, len(payload), st
+
+    def ebs_for(ratio):
+        return stage_ebs(region_shape, levels, anchor_stride, anchor_block, eb, ratio)
+
+    def make_header(center, ratio, end_mode):
+        return Header(channels=c, src_dtype=src_dtype, spatial=tuple(spatial), eb=float(eb),
+                      levels=levels, anchor_stride=anchor_stride, anchor_block=anchor_block,
+                      radius=radius, max_radius=getattr(predictor, "max_radius", 0),
+                      agg_level=(-1 if getattr(predictor, "agg_level", None) is None
+                                 else int(predictor.agg_level)),
+                      vmin=vmin, vmax=vmax,
+                      ckpt_hash=getattr(predictor, "checkpoint_hash", b"\0" * 16),
+                      flags=flags | (int(end_mode) << END_MODE_SHIFT), interp_center=center,
+                      eb_ratio=ratio)
+
+    if tune not in ("fast", "size", "rd"):
+        raise ValueError("tune must be 'fast', 'size', or 'rd'")
+    tunable = getattr(predictor, "tunable", False)
+    has_center = hasattr(predictor, "center")
+    has_end_mode = hasattr(predictor, "end_mode")
+    if has_center and getattr(predictor, "center") is None:
+        predictor.center = default_interp_center(len(region_shape))
+    if has_end_mode and getattr(predictor, "end_mode") is None:
+        predictor.end_mode = default_interp_end_mode(len(region_shape))
+    if eb_ratio is not None:
+        ratio_cands = [eb_ratio]
+    elif tunable and tune != "fast":
+        ratio_cands = [1.0, 0.9, 0.8, 0.7]
+    else:
+        ratio_cands = [getattr(predictor, "fast_eb_ratio", 1.0)]
+    sweep = tunable and tune != "fast" and eb_ratio is None
+    base_center = getattr(predictor, "center", 0)
+    center_cands = [base_center]
+    if sweep and has_center:
+        center_cands += [x for x in (0, 1, 2) if x != base_center]
+    base_end = getattr(predictor, "end_mode", 0)
+    candidates = []
+
+    def run(ratio, center, end_mode):
+        if has_center:
+            predictor.center = center
+        if has_end_mode:
+            predictor.end_mode = end_mode
+        payload, rc, raw_bytes, st = encode(ebs_for(ratio))
+        header = make_header(center, ratio, end_mode)
+        t0 = time.time()
+        stream = write_stream(header, payload, zstd_level)
+        st["entropy_s"] += time.time() - t0
+        st["raw_payload_bytes"] = raw_bytes
+        st["compressed_bytes"] = len(stream)
+        st["recon_sse"] = float(sum(st["stage_recon_sse"]))
+        cand = (len(stream), st["recon_sse"], ratio, center, end_mode, stream, rc, st)
+        candidates.append(cand)
+        return cand
+
+    seen = set()
+
+    def run_once(ratio, center, end_mode):
+        if (ratio, center, end_mode) not in seen:
+            seen.add((ratio, center, end_mode))
+            run(ratio, center, end_mode)
+
+    run_once(ratio_cands[0], base_center, base_end)
+    chosen_end = base_end
+    if sweep and has_end_mode:
+        for end_mode in range(END_MODE_MAX + 1):
+            run_once(ratio_cands[0], base_center, end_mode)
+        chosen_end = min(candidates, key=lambda x: (x[0], x[1]))[4]
+    for ratio in ratio_cands:
+        for center in center_cands:
+            run_once(ratio, center, chosen_end)
+
+    if tune == "rd" and eb_ratio is None and tunable and len(candidates) > 1:
+        min_size = min(x[0] for x in candidates)
+        feasible = [x for x in candidates if x[0] <= min_size * tune_size_slack]
+        best = min(feasible, key=lambda x: (x[1], x[0], x[2], x[3], x[4]))
+    else:
+        best = min(candidates, key=lambda x: (x[0], x[1], x[2], x[3], x[4]))
+    _, _, chosen_ratio, chosen_center, chosen_end, stream, recon_canvas, stats = best
+    stats["tune_candidates"] = len(candidates)
+    stats["search_predict_s"] = sum(x[7]["predict_s"] for x in candidates)
+    stats["search_quantize_s"] = sum(x[7]["quantize_s"] for x in candidates)
+    stats["search_entropy_s"] = sum(x[7]["entropy_s"] for x in candidates)
+    if has_center:
+        predictor.center = chosen_center
+    if has_end_mode:
+        predictor.end_mode = chosen_end
+    stats["eb_ratio"] = chosen_ratio
+    stats["interp_center"] = chosen_center
+    stats["interp_end_mode"] = chosen_end
+    stats["tune"] = tune
+    stats["tune_size_slack"] = (tune_size_slack if tune == "rd" and eb_ratio is None and tunable
+                                else 1.0)
+    if verbose:
+        print(f"tuned: eb_ratio={chosen_ratio} center={chosen_center} end_mode={chosen_end} "
+              f"compressed={len(stream)} bytes raw={stats['raw_payload_bytes']} bytes "
+              f"sse={stats['recon_sse']:.6g}")
+    stats["recon"] = _finalize(recon_canvas, make_header(chosen_center, chosen_ratio,
+                                                         chosen_end))
+    stats["original_bytes"] = img.nbytes
+    stats["ratio"] = img.nbytes / len(stream)
+    return stream, stats
+
+
+def _compress_region_gpu_dev(field, d_field, ebs, predictor, radius, round_output, stats,
+                             zstd_level):
+    return _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_output,
+                                     stats, zstd_level)
+
+
+def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_output, stats,
+                              zstd_level):
+    C = field.shape[0]
+    region = field.shape[1:]
+    plan = get_plan(region, C, predictor.levels, predictor.anchor_stride,
+                    predictor.anchor_block)
+    o = _interp_opts(predictor, len(region), radius, round_output)
+    t0 = time.time()
+    res = encode_device(d_field, plan, ebs, o)
+    fits = fit_device(res)
+    stats["quantize_s"] += time.time() - t0
+    t0 = time.time()
+    payload, rec_len, n_out = pack_device(res, fits, zstd_level)
+    stats["entropy_s"] += time.time() - t0
+    recon = D.to_host(res.d_recon).reshape(C, *region)
+    for s in range(plan.n_stages):
+        stats["stage_codes"][s] += C * plan.counts[s]
+        stats["stage_outliers"][s] += int(n_out[s])
+        stats["stage_payload_bytes"][s] += int(rec_len[s])
+        stats["stage_model_bits"][s] += float(fits[s].bits)
+    stats["outliers"] += int(n_out.sum())
+    _region_error_stats(stats, plan, field, recon)
+    return payload, recon
+
+
+def decompress(stream: bytes, predictor_factory=None) -> np.ndarray:
+    """codec.py:288-322."""
+    header, payload = read_stream(stream)
+    if predictor_factory is None:
+        if header.flags & FLAG_INTERP:
+            def predictor_factory(hdr):
+                return InterpPredictor("cubic" if hdr.flags & FLAG_CUBIC else "linear",
+                                       hdr.levels, hdr.anchor_stride, hdr.anchor_block)
+        else:
+            raise ValueError("stream needs a predictor_factory")
+    predictor = predictor_factory(header)
+    if hasattr(predictor, "center"):
+        predictor.center = header.interp_center
+    if hasattr(predictor, "end_mode"):
+        predictor.end_mode = (header.flags >> END_MODE_SHIFT) & END_MODE_MAX
+    spatial = tuple(header.spatial)
+    ebs = stage_ebs(spatial, header.levels, header.anchor_stride, header.anchor_block,
+                    header.eb, header.eb_ratio)
+    if _fast_path(predictor) and not header.flags & FLAG_RANS:
+        plan = get_plan(spatial, header.channels, header.levels, header.anchor_stride,
+                        header.anchor_block)
+        round_output = np.issubdtype(DTYPE_CODES[header.src_dtype], np.integer)
+        o = _interp_opts(predictor, len(spatial), header.radius, round_output)
+        d_recon = decode_device(payload, plan, ebs, o)
+        canvas = D.to_host(d_recon).reshape(header.channels, *spatial)
+        return _finalize(canvas, header)
+    make_masks = getattr(predictor, "stage_masks", stage_masks)
+    masks = make_masks(spatial, header.levels, header.anchor_stride, header.anchor_block)
+    canvas = _decompress_region(payload, masks, ebs, header, predictor)
+    return _finalize(canvas, header)

@@ -1,0 +1,502 @@
+// latec_entropy.cpp -- host entropy stage: stack rANS, stage records, zstd.
+//
+// Stack rANS: the design of constriction 0.4's AnsCoder<u32, u64> driven
+// by Categorical(probabilities_f32, perfect=False) at 24-bit precision, as
+// bound by latec/rans.py:466-535 and 629-657 (model = the float32 pmf of one
+// CDF row; symbols pushed last to first so they decode first to last):
+//   model ..... left[k] = trunc(cum_f32(k) * f32((2^24 - A) / sum)) + k
+//   encode .... if state >> 40 >= prob: emit low word, state >>= 32;
+//               state = (state / prob) << 24 | (left + state % prob)
+//   flush ..... the final state as its low word then (if nonzero) high word
+//   decode .... the inverse, reading words from the end of the array.
+// constriction is absent from this environment, so the word stream is a
+// restatement of that published design, not a byte-checked one.
+//
+// Stage record and literal back end: latec/bitstream.py:178-344.
+#include <stdint.h>
+#include <string.h>
+#include <zstd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "latec_common.h"
+
+namespace latec {
+
+static const int PREC = 24;
+
+struct SymInfo {
+  uint32_t left, prob;
+  uint64_t rcp;  // floor(2^64 / prob), 0 for prob == 1
+};
+
+struct Model {
+  int64_t A = 0;
+  std::vector<uint32_t> left;  // A + 1 entries
+  std::vector<SymInfo> sym;
+  std::vector<uint32_t> lut;   // quantile >> 8 -> first candidate symbol
+};
+
+static void build_model(const uint32_t *cdf, int64_t A, Model &m, bool for_decode) {
+  m.A = A;
+  m.left.resize(A + 1);
+  float sum = 0.0f;
+  for (int64_t k = 0; k < A; k++) sum += (float)(cdf[k + 1] - cdf[k]);
+  const float scale = (float)((1 << PREC) - A) / sum;
+  float cum = 0.0f;
+  for (int64_t k = 0; k < A; k++) {
+    m.left[k] = (uint32_t)(cum * scale) + (uint32_t)k;
+    cum += (float)(cdf[k + 1] - cdf[k]);
+  }
+  m.left[A] = 1u << PREC;
+  m.sym.resize(A);
+  for (int64_t k = 0; k < A; k++) {
+    SymInfo &s = m.sym[k];
+    s.left = m.left[k];
+    s.prob = m.left[k + 1] - m.left[k];
+    s.rcp = s.prob > 1 ? (uint64_t)((((unsigned __int128)1) << 64) / s.prob) : 0;
+  }
+  if (for_decode) {
+    m.lut.assign((size_t)1 << (PREC - 8), 0);
+    int64_t k = 0;
+    for (size_t b = 0; b < m.lut.size(); b++) {
+      const uint32_t q = (uint32_t)(b << 8);
+      while (k + 1 < A && m.left[k + 1] <= q) k++;
+      m.lut[b] = (uint32_t)k;
+    }
+  }
+}
+
+static inline void enc_put(uint64_t &state, const SymInfo &s, uint32_t *words,
+                           int64_t &nw) {
+  if ((state >> (64 - PREC)) >= s.prob) {
+    words[nw++] = (uint32_t)state;
+    state >>= 32;
+  }
+  uint64_t q, r;
+  if (s.prob == 1) {
+    q = state;
+    r = 0;
+  } else {
+    q = (uint64_t)(((unsigned __int128)state * s.rcp) >> 64);
+    r = state - q * s.prob;
+    if (r >= s.prob) {
+      q++;
+      r -= s.prob;
+    }
+  }
+  state = (q << PREC) | (s.left + r);
+}
+
+static inline int64_t enc_flush(uint64_t state, uint32_t *words, int64_t nw) {
+  if (state) {
+    words[nw++] = (uint32_t)state;
+    if (state >> 32) words[nw++] = (uint32_t)(state >> 32);
+  }
+  return nw;
+}
+
+// Worst case: one word per symbol plus the two-word flush.
+static inline int64_t enc_cap(int64_t n) { return n + 2; }
+
+static inline uint32_t dec_get(uint64_t &state, const Model &m, const uint32_t *words,
+                               int64_t &top) {
+  const uint32_t qt = (uint32_t)(state & ((1u << PREC) - 1));
+  uint32_t k = m.lut[qt >> 8];
+  while (m.left[k + 1] <= qt) k++;
+  const SymInfo &s = m.sym[k];
+  state = (state >> PREC) * s.prob + (qt - s.left);
+  if (state < ((uint64_t)1 << 32) && top > 0) state = (state << 32) | words[--top];
+  return k;
+}
+
+static inline bool dec_init(uint64_t &state, const uint32_t *words, int64_t nw,
+                            int64_t &top) {
+  top = nw;
+  state = 0;
+  if (top > 0) {
+    const uint32_t w0 = words[--top];
+    if (w0 == 0) return false;
+    state = w0;
+    if (top > 0) state = (state << 32) | words[--top];
+  }
+  return true;
+}
+
+int64_t rans_encode(const uint32_t *syms, int64_t n, const uint32_t *cdf, int64_t A,
+                    uint32_t *words, int64_t cap) {
+  Model m;
+  build_model(cdf, A, m, false);
+  std::vector<uint32_t> tmp;
+  uint32_t *out = words;
+  if (cap < enc_cap(n)) {
+    tmp.resize(enc_cap(n));
+    out = tmp.data();
+  }
+  uint64_t state = 0;
+  int64_t nw = 0;
+  for (int64_t i = n - 1; i >= 0; i--) {
+    if (syms[i] >= (uint64_t)A) {
+      set_error("code outside the narrowed rANS alphabet");
+      return LATEC_E_ARG;
+    }
+    enc_put(state, m.sym[syms[i]], out, nw);
+  }
+  nw = enc_flush(state, out, nw);
+  if (out != words) {
+    if (nw > cap) {
+      set_error("rANS output capacity");
+      return LATEC_E_CAPACITY;
+    }
+    memcpy(words, out, sizeof(uint32_t) * nw);
+  }
+  return nw;
+}
+
+int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t n,
+                           const uint32_t *cdfs, int64_t A, uint32_t *words, int64_t cap) {
+  // groups by level, levels pushed 63..0, each group in reverse (encode_reverse)
+  std::vector<std::vector<int64_t>> idx(64);
+  for (int64_t i = 0; i < n; i++) {
+    if (levels[i] >= 64) {
+      set_error("scale level exceeds table count");
+      return LATEC_E_ARG;
+    }
+    idx[levels[i]].push_back(i);
+  }
+  std::vector<uint32_t> tmp(enc_cap(n));
+  uint64_t state = 0;
+  int64_t nw = 0;
+  for (int l = 63; l >= 0; l--) {
+    if (idx[l].empty()) continue;
+    Model m;
+    build_model(cdfs + (int64_t)l * (A + 1), A, m, false);
+    for (int64_t j = (int64_t)idx[l].size() - 1; j >= 0; j--) {
+      const uint32_t s = syms[idx[l][j]];
+      if (s >= (uint64_t)A) {
+        set_error("code outside the narrowed rANS alphabet");
+        return LATEC_E_ARG;
+      }
+      enc_put(state, m.sym[s], tmp.data(), nw);
+    }
+  }
+  nw = enc_flush(state, tmp.data(), nw);
+  if (nw > cap) {
+    set_error("rANS output capacity");
+    return LATEC_E_CAPACITY;
+  }
+  memcpy(words, tmp.data(), sizeof(uint32_t) * nw);
+  return nw;
+}
+
+int rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t A,
+                uint32_t *syms, int64_t n) {
+  Model m;
+  build_model(cdf, A, m, true);
+  uint64_t state;
+  int64_t top;
+  if (!dec_init(state, words, nw, top)) {
+    set_error("malformed rANS payload");
+    return LATEC_E_FORMAT;
+  }
+  for (int64_t i = 0; i < n; i++) syms[i] = dec_get(state, m, words, top);
+  if (top != 0 || state != 0) {
+    set_error("trailing data in rANS payload");
+    return LATEC_E_FORMAT;
+  }
+  return LATEC_OK;
+}
+
+int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
+                       const uint32_t *cdfs, int64_t A, uint32_t *syms, int64_t n) {
+  uint64_t state;
+  int64_t top;
+  if (!dec_init(state, words, nw, top)) {
+    set_error("malformed rANS payload");
+    return LATEC_E_FORMAT;
+  }
+  std::vector<std::vector<int64_t>> idx(64);
+  for (int64_t i = 0; i < n; i++) {
+    if (levels[i] >= 64) {
+      set_error("scale level exceeds table count");
+      return LATEC_E_ARG;
+    }
+    idx[levels[i]].push_back(i);
+  }
+  for (int l = 0; l < 64; l++) {
+    if (idx[l].empty()) continue;
+    Model m;
+    build_model(cdfs + (int64_t)l * (A + 1), A, m, true);
+    for (int64_t i : idx[l]) syms[i] = dec_get(state, m, words, top);
+  }
+  if (top != 0 || state != 0) {
+    set_error("trailing data in rANS payload");
+    return LATEC_E_FORMAT;
+  }
+  return LATEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// records
+// ---------------------------------------------------------------------------
+
+static inline uint32_t code_at(const void *codes, int cb, int64_t i) {
+  return cb == 2 ? ((const uint16_t *)codes)[i] : ((const uint32_t *)codes)[i];
+}
+
+static const int64_t LITERAL_MIN = 1 << 12;  // bitstream.py:172
+static const double SCOUT_SLACK = 0.85;      // bitstream.py:175
+
+int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int64_t n_out,
+                   uint32_t R, int zstd_level, const latec_fit *fit, const uint32_t *cdf_row,
+                   uint8_t *out, int64_t cap) {
+  const int kexp = fit->kexp;
+  if (kexp < 1 || kexp > 15) {
+    set_error("rANS coder window does not fit the stage frame");
+    return LATEC_E_ARG;
+  }
+  const int64_t A = (int64_t)2 << kexp;
+  const int64_t head = 12, tail = 4 + 4 * n_out;
+  if (cap < head + tail) {
+    set_error("record capacity");
+    return LATEC_E_CAPACITY;
+  }
+  uint8_t *blob = out + head;
+  const int64_t blob_cap = cap - head - tail;
+  int64_t hlen = -1;
+  int literal = 0;
+  std::vector<uint32_t> syms((size_t)n);
+  for (int64_t i = 0; i < n; i++) {
+    const uint32_t s = narrow_sym(code_at(codes, cb, i), R, kexp);
+    if (s >= (uint64_t)A) {
+      set_error("code outside the narrowed rANS alphabet");
+      return LATEC_E_ARG;
+    }
+    syms[i] = s;
+  }
+  if (n >= LITERAL_MIN) {
+    const bool wide = kexp > 7;
+    std::vector<uint8_t> lit((size_t)n * (wide ? 2 : 1));
+    for (int64_t i = 0; i < n; i++) {
+      if (wide)
+        ((uint16_t *)lit.data())[i] = (uint16_t)syms[i];
+      else
+        lit[i] = (uint8_t)syms[i];
+    }
+    std::vector<uint8_t> tmp(ZSTD_compressBound(lit.size()));
+    const size_t scout = ZSTD_compress(tmp.data(), tmp.size(), lit.data(), lit.size(), 1);
+    if (!ZSTD_isError(scout) && (double)scout * SCOUT_SLACK < (double)fit->bits / 8.0) {
+      const size_t full =
+          ZSTD_compress(tmp.data(), tmp.size(), lit.data(), lit.size(), zstd_level);
+      if (!ZSTD_isError(full) && (double)full * 8.0 < (double)fit->bits) {
+        if ((int64_t)full > blob_cap) {
+          set_error("record capacity");
+          return LATEC_E_CAPACITY;
+        }
+        memcpy(blob, tmp.data(), full);
+        hlen = (int64_t)full;
+        literal = 1;
+      }
+    }
+  }
+  if (hlen < 0 && n == 0) hlen = 0;  // empty stage: empty word stream
+  if (hlen < 0) {
+    if (fit->level < 0 || fit->level >= 64) {
+      set_error("scale level exceeds table count");
+      return LATEC_E_ARG;
+    }
+    std::vector<uint32_t> words((size_t)enc_cap(n));
+    const int64_t nw = rans_encode(syms.data(), n, cdf_row, A, words.data(), enc_cap(n));
+    if (nw < 0) return nw;
+    if (nw * 4 > blob_cap) {
+      set_error("record capacity");
+      return LATEC_E_CAPACITY;
+    }
+    memcpy(blob, words.data(), (size_t)nw * 4);
+    hlen = nw * 4;
+  }
+  const uint64_t framed = (uint64_t)hlen | ((uint64_t)kexp << 45) |
+                          ((uint64_t)fit->shape << 49) | ((uint64_t)fit->level << 54) |
+                          ((uint64_t)fit->owx << 60) | ((uint64_t)literal << 63);
+  const uint32_t nc = (uint32_t)n;
+  memcpy(out, &nc, 4);
+  memcpy(out + 4, &framed, 8);
+  uint8_t *p = blob + hlen;
+  const uint32_t no = (uint32_t)n_out;
+  memcpy(p, &no, 4);
+  if (n_out) memcpy(p + 4, outl, 4 * (size_t)n_out);
+  return head + hlen + tail;
+}
+
+int stage_header(const uint8_t *buf, int64_t len, uint32_t *n_codes, int64_t *hlen,
+                 int *kexp, int *shape, int *level, int *owx, int *literal) {
+  if (len < 12) {
+    set_error("truncated stage record");
+    return LATEC_E_FORMAT;
+  }
+  uint64_t framed;
+  memcpy(n_codes, buf, 4);
+  memcpy(&framed, buf + 4, 8);
+  *hlen = (int64_t)(framed & (((uint64_t)1 << 45) - 1));
+  *kexp = (int)((framed >> 45) & 15);
+  *shape = (int)((framed >> 49) & 31);
+  *level = (int)((framed >> 54) & 63);
+  *owx = (int)((framed >> 60) & 7);
+  *literal = (int)(framed >> 63);
+  return 12;
+}
+
+int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t *cdf_row,
+                     int64_t n_expect, void *codes, int cb, uint32_t *out_rank,
+                     float *out_val, int64_t out_cap, int64_t *n_out) {
+  uint32_t n;
+  int64_t hlen;
+  int kexp, shape, level, owx, literal;
+  int rc = stage_header(buf, len, &n, &hlen, &kexp, &shape, &level, &owx, &literal);
+  if (rc < 0) return rc;
+  if (!kexp) {
+    set_error("rANS stage record has no coder window");
+    return LATEC_E_FORMAT;
+  }
+  if ((int64_t)n != n_expect) {
+    set_error("stage code count mismatch");
+    return LATEC_E_COUNT;
+  }
+  if (12 + hlen + 4 > len) {
+    set_error("truncated stage record");
+    return LATEC_E_FORMAT;
+  }
+  const uint8_t *blob = buf + 12;
+  const int64_t A = (int64_t)2 << kexp;
+  std::vector<uint32_t> syms((size_t)n);
+  if (literal) {
+    const bool wide = kexp > 7;
+    const size_t sb = (size_t)n * (wide ? 2 : 1);
+    std::vector<uint8_t> lit(sb ? sb : 1);
+    const size_t got = ZSTD_decompress(lit.data(), sb, blob, (size_t)hlen);
+    if (ZSTD_isError(got) || got < sb) {
+      set_error("bad literal stage frame");
+      return LATEC_E_FORMAT;
+    }
+    for (uint32_t i = 0; i < n; i++)
+      syms[i] = wide ? ((const uint16_t *)lit.data())[i] : lit[i];
+  } else if (n == 0) {
+    if (hlen) {
+      set_error("non-empty rANS blob for an empty stage");
+      return LATEC_E_FORMAT;
+    }
+  } else {
+    if (hlen % 4) {
+      set_error("rANS payload is not aligned to 32-bit words");
+      return LATEC_E_FORMAT;
+    }
+    std::vector<uint32_t> words((size_t)hlen / 4);
+    memcpy(words.data(), blob, (size_t)hlen);
+    rc = rans_decode(words.data(), hlen / 4, cdf_row, A, syms.data(), n);
+    if (rc) return rc;
+  }
+  uint32_t no;
+  memcpy(&no, blob + hlen, 4);
+  if (12 + hlen + 4 + 4 * (int64_t)no > len || (int64_t)no > out_cap) {
+    set_error("truncated stage record");
+    return LATEC_E_FORMAT;
+  }
+  int64_t k = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t c = widen_sym(syms[i], R, kexp);
+    if (cb == 2)
+      ((uint16_t *)codes)[i] = (uint16_t)c;
+    else
+      ((uint32_t *)codes)[i] = c;
+    if (c == 0) {
+      if (k < (int64_t)no) out_rank[k] = i;
+      k++;
+    }
+  }
+  if (k != (int64_t)no) {
+    set_error("outlier count mismatch: " + std::to_string(k) + " zeros vs " +
+              std::to_string(no) + " values");
+    return LATEC_E_COUNT;
+  }
+  memcpy(out_val, blob + hlen + 4, 4 * (size_t)no);
+  *n_out = no;
+  return 12 + hlen + 4 + 4 * (int64_t)no;
+}
+
+int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
+                    const int64_t *n_codes, const float *outl, const int64_t *out_off,
+                    const int64_t *n_out, uint32_t R, int zstd_level, const latec_fit *fits,
+                    const uint32_t *cdf_rows, const int64_t *row_off, uint8_t *out,
+                    int64_t cap, int64_t *rec_len, int threads) {
+  std::vector<std::vector<uint8_t>> recs(ns);
+  std::vector<int64_t> rcs(ns, 0);
+  std::vector<int> order(ns);
+  for (int i = 0; i < ns; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return n_codes[a] > n_codes[b]; });
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    while (true) {
+      const int j = next.fetch_add(1);
+      if (j >= ns) break;
+      const int s = order[j];
+      const int64_t rc_cap = 16 + 4 * n_out[s] + 4 * (n_codes[s] + 2) + 64;
+      recs[s].resize((size_t)rc_cap);
+      const void *cs = (const uint8_t *)codes + (size_t)cb * code_off[s];
+      rcs[s] = pack_stage(cs, cb, n_codes[s], outl + out_off[s], n_out[s], R, zstd_level,
+                          &fits[s], cdf_rows + row_off[s], recs[s].data(), rc_cap);
+    }
+  };
+  if (threads < 1) threads = 1;
+  threads = std::min(threads, ns);
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; t++) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  int64_t off = 0;
+  for (int s = 0; s < ns; s++) {
+    if (rcs[s] < 0) {
+      set_error("stage " + std::to_string(s) + ": pack failed");
+      return rcs[s];
+    }
+    if (off + rcs[s] > cap) {
+      set_error("payload capacity");
+      return LATEC_E_CAPACITY;
+    }
+    memcpy(out + off, recs[s].data(), (size_t)rcs[s]);
+    rec_len[s] = rcs[s];
+    off += rcs[s];
+  }
+  return off;
+}
+
+int64_t zstd_compress(const uint8_t *src, int64_t n, int level, uint8_t *dst, int64_t cap) {
+  const size_t r = ZSTD_compress(dst, (size_t)cap, src, (size_t)n, level);
+  if (ZSTD_isError(r)) {
+    set_error(std::string("zstd: ") + ZSTD_getErrorName(r));
+    return LATEC_E_CAPACITY;
+  }
+  return (int64_t)r;
+}
+
+int64_t zstd_content_size(const uint8_t *src, int64_t n) {
+  const unsigned long long r = ZSTD_getFrameContentSize(src, (size_t)n);
+  if (r == ZSTD_CONTENTSIZE_ERROR || r == ZSTD_CONTENTSIZE_UNKNOWN) {
+    set_error("not a zstd frame with a content size");
+    return LATEC_E_FORMAT;
+  }
+  return (int64_t)r;
+}
+
+int64_t zstd_decompress(const uint8_t *src, int64_t n, uint8_t *dst, int64_t cap) {
+  const size_t r = ZSTD_decompress(dst, (size_t)cap, src, (size_t)n);
+  if (ZSTD_isError(r)) {
+    set_error(std::string("zstd: ") + ZSTD_getErrorName(r));
+    return LATEC_E_FORMAT;
+  }
+  return (int64_t)r;
+}
+
+}  // namespace latec

@@ -1,0 +1,702 @@
+// latec_interp.hip -- fused interpolation + quantization stage kernels (gfx950).
+//
+// One launch per dyadic sub-stage.  The grid is (x: work, y: parity class);
+// each parity class is a box of lattice points with an affine rank into the
+// stage array (levels.py:389-512), so a point's grid address and its slot in
+// the stage array are both a few integer ops away from its class index.
+//
+// Per point and channel (FP64, no contraction -- built with
+// -ffp-contract=off, every expression below is one IEEE op per operator):
+//   prediction ....... predictor.py:97-156 (_interp_axis_at) along each odd
+//                      axis, combined by `center` as predictor.py:286-315;
+//                      cast to float32 (predictor.py:313)
+//   quantization ..... quantizer.py:57-83: q = rint((x - p) / (2 eb)),
+//                      recon = f32(p + 2eb * q), float32 |x - chk| <= f32(eb)
+//   write-back ....... recon in place (codec.py:340-347), code into the stage
+//                      array, outliers appended as (rank, value), histogram.
+//
+// Masked even-axis terms of the reference's centre-0 sum (v * 0.0) are not
+// evaluated: with anchor_block == 1 they read only not-yet-revealed points,
+// which hold 0, so they add a signed zero, and a signed-zero prediction
+// yields the same code and reconstruction (q = 0 => recon = f32(p + 0.0)).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "latec_plan.h"
+
+namespace latec {
+
+struct DeviceBufs {
+  ClassDesc *classes = nullptr;     // all stages' classes, concatenated
+  uint32_t *excluded = nullptr;     // all stages' excluded ranks
+  std::vector<int64_t> class_off;   // per stage: index of first class
+  std::vector<int64_t> excl_off;    // per stage: index of first excluded rank
+  int device = -1;
+  // cached graphs
+  struct GraphKey {
+    const void *a[8];
+    bool operator<(const GraphKey &o) const { return memcmp(a, o.a, sizeof a) < 0; }
+  };
+  std::map<GraphKey, hipGraphExec_t> graphs;
+};
+
+static int hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    set_hip_error(what, (int)e);
+    return LATEC_E_HIP;
+  }
+  return LATEC_OK;
+}
+
+int plan_upload(latec_plan *p) {
+  if (p->dev) return LATEC_OK;
+  auto *d = new DeviceBufs();
+  std::vector<ClassDesc> cls;
+  std::vector<uint32_t> ex;
+  for (auto &st : p->stages) {
+    d->class_off.push_back((int64_t)cls.size());
+    d->excl_off.push_back((int64_t)ex.size());
+    cls.insert(cls.end(), st.classes.begin(), st.classes.end());
+    ex.insert(ex.end(), st.excluded.begin(), st.excluded.end());
+  }
+  int rc = hip_check(hipGetDevice(&d->device), "hipGetDevice");
+  if (!rc && !cls.empty())
+    rc = hip_check(hipMalloc(&d->classes, cls.size() * sizeof(ClassDesc)), "hipMalloc");
+  if (!rc && !cls.empty())
+    rc = hip_check(hipMemcpy(d->classes, cls.data(), cls.size() * sizeof(ClassDesc),
+                             hipMemcpyHostToDevice), "hipMemcpy");
+  if (!rc && !ex.empty())
+    rc = hip_check(hipMalloc(&d->excluded, ex.size() * sizeof(uint32_t)), "hipMalloc");
+  if (!rc && !ex.empty())
+    rc = hip_check(hipMemcpy(d->excluded, ex.data(), ex.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice), "hipMemcpy");
+  if (rc) {
+    if (d->classes) (void)hipFree(d->classes);
+    if (d->excluded) (void)hipFree(d->excluded);
+    delete d;
+    return rc;
+  }
+  p->dev = d;
+  return LATEC_OK;
+}
+
+void plan_release(latec_plan *p) {
+  if (!p->dev) return;
+  for (auto &kv : p->dev->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (p->dev->classes) (void)hipFree(p->dev->classes);
+  if (p->dev->excluded) (void)hipFree(p->dev->excluded);
+  delete p->dev;
+  p->dev = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+struct GridArgs {
+  int32_t ndim, channels;
+  int32_t s, stage_stride, block;
+  int32_t cubic, center, end_mode, round_mode;
+  uint32_t R;
+  int64_t N;        // points per channel
+  int64_t n;        // stage points per channel
+  int64_t code_base;  // C * stage offset
+  int64_t shape[LATEC_MAX_DIM];
+  int64_t gstr[LATEC_MAX_DIM];
+  double eb, span, offset;
+  int64_t n_excl;
+};
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t magic, uint32_t shift) {
+  return (uint32_t)(((uint64_t)__umulhi(n, magic) + n) >> shift);
+}
+
+// Decompose a class-local index into lattice coords; returns false for an
+// anchor point excluded from a dyadic stage.  Outputs flat grid index, the
+// stage rank, and the grid coordinates.
+__device__ __forceinline__ bool locate(const ClassDesc &c, const GridArgs &g,
+                                       uint32_t idx, const uint32_t *excl,
+                                       int64_t *flat, int64_t *rank,
+                                       int64_t *coord) {
+  int64_t f = 0, r = c.offset;
+  bool anchor = true;
+#pragma unroll
+  for (int j = LATEC_MAX_DIM - 1; j >= 0; j--) {
+    if (j >= g.ndim) continue;
+    uint32_t q = fdiv(idx, c.magic[j], c.shift[j]);
+    uint32_t m = idx - q * c.sizes[j];
+    idx = q;
+    int64_t x;
+    if (c.flags & 1)
+      x = (int64_t)(m / (uint32_t)g.block) * g.stage_stride + (m % (uint32_t)g.block);
+    else
+      x = c.starts[j] + (int64_t)c.step * m;
+    coord[j] = x;
+    f += x * g.gstr[j];
+    r += (int64_t)m * c.rstr[j];
+    anchor = anchor && ((x % g.stage_stride) < g.block);
+  }
+  *flat = f;
+  if (g.n_excl && anchor) return false;
+  if (g.n_excl) {  // rebase the rank past the excluded anchors before it
+    int64_t lo = 0, hi = g.n_excl;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)excl[mid] < r) lo = mid + 1; else hi = mid;
+    }
+    r -= lo;
+  }
+  *rank = r;
+  return true;
+}
+
+// 1-D interpolation along one odd axis (c >= s always): predictor.py:97-156.
+__device__ __forceinline__ double interp_axis(const float *__restrict__ W,
+                                              int64_t flat, int64_t c,
+                                              int64_t n, int64_t gs, int64_t s,
+                                              int cubic, int end_mode) {
+  const bool vp1 = c + s < n, vm3 = c >= 3 * s, vp3 = c + 3 * s < n;
+  const double B = (double)W[flat - s * gs];
+  if (!vp1) {
+    if ((end_mode & 2) && vm3) {
+      const double A = (double)W[flat - 3 * s * gs];
+      return 1.5 * B - 0.5 * A;
+    }
+    return B;
+  }
+  const double C = (double)W[flat + s * gs];
+  if (cubic) {
+    if (vm3 && vp3) {
+      const double A = (double)W[flat - 3 * s * gs];
+      const double D = (double)W[flat + 3 * s * gs];
+      return (((-A + 9.0 * B) + 9.0 * C) - D) * 0.0625;
+    }
+    if (end_mode & 1) {
+      if (vp3) {  // !vm3
+        const double D = (double)W[flat + 3 * s * gs];
+        return ((3.0 * B + 6.0 * C) - D) * 0.125;
+      }
+      if (vm3) {  // !vp3
+        const double A = (double)W[flat - 3 * s * gs];
+        return ((-A + 6.0 * B) + 3.0 * C) * 0.125;
+      }
+    }
+  }
+  return 0.5 * (B + C);
+}
+
+__device__ __forceinline__ float predict_point(const float *__restrict__ W,
+                                               const GridArgs &g, int parity,
+                                               int64_t flat,
+                                               const int64_t *coord) {
+  double total = 0.0, v_first = 0.0, v_last = 0.0;
+  int count = 0;
+#pragma unroll
+  for (int a = 0; a < LATEC_MAX_DIM; a++) {
+    if (a >= g.ndim || !((parity >> a) & 1)) continue;
+    double v = interp_axis(W, flat, coord[a], g.shape[a], g.gstr[a], g.s,
+                           g.cubic, g.end_mode);
+    total = count ? total + v : v;
+    if (!count) v_first = v;
+    v_last = v;
+    count++;
+  }
+  double res;
+  if (g.center == 0 || count == 1)
+    res = count == 1 ? total : total / (double)count;
+  else
+    res = g.center == 1 ? v_first : v_last;
+  return (float)res;
+}
+
+// Quantize one value: quantizer.py:57-83 + _recon_from_codes :114-120.
+// Returns the code (0 = outlier); *rec receives the reconstruction.
+__device__ __forceinline__ uint32_t quantize_one(float x, float p,
+                                                 const GridArgs &g,
+                                                 float *rec) {
+  const double w = 2.0 * g.eb;
+  const double qd = rint(((double)x - (double)p) / w);
+  const bool inr = fabs(qd) < (double)g.R;
+  uint32_t code = 0;
+  float r = x;
+  if (inr) {
+    const int64_t q = (int64_t)qd;
+    code = (uint32_t)(q + (int64_t)g.R);
+    const float cand = (float)((double)p + w * (double)q);
+    float chk = cand;
+    if (g.round_mode == 1)
+      chk = rintf(cand);
+    else if (g.round_mode == 2)
+      chk = (float)((rint((double)cand * g.span + g.offset) - g.offset) / g.span);
+    if (fabsf(x - chk) <= (float)g.eb)
+      r = cand;
+    else
+      code = 0;
+  }
+  *rec = r;
+  return code;
+}
+
+#define HIST_WIN 1024
+
+template <typename CodeT>
+__device__ __forceinline__ void emit(CodeT *__restrict__ codes, int64_t slot,
+                                     uint32_t code, float x, uint32_t rank,
+                                     uint32_t *__restrict__ out_rank,
+                                     float *__restrict__ out_val,
+                                     uint32_t *__restrict__ out_count,
+                                     uint32_t *lhist, uint32_t *__restrict__ ghist,
+                                     uint32_t win_lo, bool active) {
+  if (active) {
+    codes[slot] = (CodeT)code;
+    uint32_t b = code - win_lo;
+    if (code != 0 && b < HIST_WIN)
+      atomicAdd(&lhist[b], 1u);
+    else
+      atomicAdd(&ghist[code], 1u);
+  }
+  // wave-aggregated outlier append
+  const bool out = active && code == 0;
+  const uint64_t m = __ballot(out);
+  if (m) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(out_count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (out) {
+      const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      out_rank[k] = rank;
+      out_val[k] = x;
+    }
+  }
+}
+
+template <typename CodeT>
+__global__ void __launch_bounds__(256) encode_stage_kernel(
+    GridArgs g, const ClassDesc *__restrict__ classes,
+    const uint32_t *__restrict__ excl, const float *__restrict__ x,
+    float *__restrict__ recon, CodeT *__restrict__ codes,
+    uint32_t *__restrict__ ghist, uint32_t *__restrict__ out_rank,
+    float *__restrict__ out_val, uint32_t *__restrict__ out_count,
+    int anchors) {
+  __shared__ uint32_t lhist[HIST_WIN];
+  const ClassDesc c = classes[blockIdx.y];
+  const uint32_t win_lo = g.R > HIST_WIN / 2 ? g.R - HIST_WIN / 2 : 0;
+  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x) lhist[i] = 0;
+  __syncthreads();
+  const uint32_t size = (uint32_t)c.size;
+  const uint32_t nthreads = gridDim.x * blockDim.x;
+  // iterate so that every lane of a wave reaches the ballot together
+  for (uint32_t base = blockIdx.x * blockDim.x; base < size; base += nthreads) {
+    const uint32_t idx = base + threadIdx.x;
+    int64_t flat = 0, rank = 0, coord[LATEC_MAX_DIM];
+    bool active = idx < size;
+    if (active) active = locate(c, g, idx, excl, &flat, &rank, coord);
+    for (int ch = 0; ch < g.channels; ch++) {
+      uint32_t code = 0;
+      float xv = 0.f;
+      int64_t slot = 0;
+      if (active) {
+        const float *W = recon + ch * g.N;
+        const float p = anchors ? 0.0f : predict_point(W, g, c.parity, flat, coord);
+        xv = x[ch * g.N + flat];
+        float rec;
+        code = quantize_one(xv, p, g, &rec);
+        recon[ch * g.N + flat] = rec;
+        slot = g.code_base + ch * g.n + rank;
+      }
+      emit<CodeT>(codes, slot, code, xv, (uint32_t)(ch * g.n + rank), out_rank,
+                  out_val, out_count, lhist, ghist, win_lo, active);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
+    if (lhist[i]) atomicAdd(&ghist[win_lo + i], lhist[i]);
+}
+
+// Decode: predict, then recon = f32(p + 2eb q) or the outlier value found
+// by binary search of the stage's sorted outlier ranks.
+template <typename CodeT>
+__global__ void __launch_bounds__(256) decode_stage_kernel(
+    GridArgs g, const ClassDesc *__restrict__ classes,
+    const uint32_t *__restrict__ excl, const CodeT *__restrict__ codes,
+    const uint32_t *__restrict__ out_rank, const float *__restrict__ out_val,
+    const uint32_t *__restrict__ out_count, float *__restrict__ recon,
+    int anchors) {
+  const ClassDesc c = classes[blockIdx.y];
+  const uint32_t size = (uint32_t)c.size;
+  const uint32_t nthreads = gridDim.x * blockDim.x;
+  const double w = 2.0 * g.eb;
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size;
+       idx += nthreads) {
+    int64_t flat, rank, coord[LATEC_MAX_DIM];
+    if (!locate(c, g, idx, excl, &flat, &rank, coord)) continue;
+    for (int ch = 0; ch < g.channels; ch++) {
+      const float *W = recon + ch * g.N;
+      const float p = anchors ? 0.0f : predict_point(W, g, c.parity, flat, coord);
+      const int64_t srank = ch * g.n + rank;
+      const uint32_t code = (uint32_t)codes[g.code_base + srank];
+      float r;
+      if (code) {
+        r = (float)((double)p + w * (double)((int64_t)code - (int64_t)g.R));
+      } else {
+        int64_t lo = 0, hi = *out_count;
+        while (lo < hi) {
+          int64_t mid = (lo + hi) >> 1;
+          if ((int64_t)out_rank[mid] < srank) lo = mid + 1; else hi = mid;
+        }
+        r = out_val[lo];
+      }
+      recon[ch * g.N + flat] = r;
+    }
+  }
+}
+
+// Generic quantizer over compact arrays (foreign predictors, GNN stages).
+template <typename CodeT>
+__global__ void __launch_bounds__(256) quantize_kernel(
+    GridArgs g, const float *__restrict__ x, const float *__restrict__ pred,
+    int64_t n, CodeT *__restrict__ codes, uint32_t *__restrict__ out_rank,
+    float *__restrict__ out_val, uint32_t *__restrict__ out_count,
+    uint32_t *__restrict__ ghist) {
+  __shared__ uint32_t lhist[HIST_WIN];
+  const uint32_t win_lo = g.R > HIST_WIN / 2 ? g.R - HIST_WIN / 2 : 0;
+  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x) lhist[i] = 0;
+  __syncthreads();
+  const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += nthreads) {
+    const int64_t i = base + threadIdx.x;
+    const bool active = i < n;
+    uint32_t code = 0;
+    float xv = 0.f, rec;
+    if (active) {
+      xv = x[i];
+      code = quantize_one(xv, pred[i], g, &rec);
+    }
+    emit<CodeT>(codes, i, code, xv, (uint32_t)i, out_rank, out_val, out_count,
+                lhist, ghist, win_lo, active);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
+    if (lhist[i]) atomicAdd(&ghist[win_lo + i], lhist[i]);
+}
+
+template <typename CodeT>
+__global__ void __launch_bounds__(256) dequantize_kernel(
+    GridArgs g, const float *__restrict__ pred, const CodeT *__restrict__ codes,
+    int64_t n, const uint32_t *__restrict__ out_rank,
+    const float *__restrict__ out_val, const uint32_t *__restrict__ out_count,
+    float *__restrict__ recon) {
+  const double w = 2.0 * g.eb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t code = (uint32_t)codes[i];
+    float r;
+    if (code) {
+      r = (float)((double)pred[i] + w * (double)((int64_t)code - (int64_t)g.R));
+    } else {
+      int64_t lo = 0, hi = *out_count;
+      while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)out_rank[mid] < i) lo = mid + 1; else hi = mid;
+      }
+      r = out_val[lo];
+    }
+    recon[i] = r;
+  }
+}
+
+// Prediction only, written in stage order (C, n): InterpPredictor.predict.
+__global__ void __launch_bounds__(256) predict_stage_kernel(
+    GridArgs g, const ClassDesc *__restrict__ classes,
+    const uint32_t *__restrict__ excl, const float *__restrict__ recon,
+    float *__restrict__ pred) {
+  const ClassDesc c = classes[blockIdx.y];
+  const uint32_t size = (uint32_t)c.size;
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size;
+       idx += gridDim.x * blockDim.x) {
+    int64_t flat, rank, coord[LATEC_MAX_DIM];
+    if (!locate(c, g, idx, excl, &flat, &rank, coord)) continue;
+    for (int ch = 0; ch < g.channels; ch++)
+      pred[ch * g.n + rank] = predict_point(recon + ch * g.N, g, c.parity, flat, coord);
+  }
+}
+
+template <typename CodeT>
+__global__ void __launch_bounds__(256) code_hist_kernel(const CodeT *__restrict__ codes,
+                                                        int64_t n, uint32_t R,
+                                                        uint32_t *__restrict__ ghist) {
+  __shared__ uint32_t lhist[HIST_WIN];
+  const uint32_t win_lo = R > HIST_WIN / 2 ? R - HIST_WIN / 2 : 0;
+  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x) lhist[i] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t code = (uint32_t)codes[i];
+    const uint32_t b = code - win_lo;
+    if (code != 0 && b < HIST_WIN)
+      atomicAdd(&lhist[b], 1u);
+    else if (code < 2 * R)
+      atomicAdd(&ghist[code], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
+    if (lhist[i]) atomicAdd(&ghist[win_lo + i], lhist[i]);
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+
+static int grid_blocks(int64_t work) {
+  int64_t b = (work + 255) / 256;
+  const int64_t cap = 256 * 16;  // enough waves to fill 256 CUs several times
+  if (b > cap) b = cap;
+  return (int)(b < 1 ? 1 : b);
+}
+
+static int check_opts(const latec_interp_opts *o) {
+  if (!o || (o->code_bytes != 2 && o->code_bytes != 4)) {
+    set_error("code_bytes must be 2 or 4");
+    return LATEC_E_ARG;
+  }
+  if (o->radius < 1 || (o->code_bytes == 2 && o->radius > 32768)) {
+    set_error("radius does not fit the code width");
+    return LATEC_E_ARG;
+  }
+  if (o->round_mode == 2 && !(o->span > 0)) {
+    set_error("round_output span must be > 0");
+    return LATEC_E_ARG;
+  }
+  return LATEC_OK;
+}
+
+static GridArgs make_args(const latec_plan *p, int stage, double eb,
+                          const latec_interp_opts *o) {
+  GridArgs g;
+  memset(&g, 0, sizeof g);
+  const StageDesc &st = p->stages[stage];
+  g.ndim = p->ndim;
+  g.channels = p->channels;
+  g.s = st.stride;
+  g.stage_stride = p->stride;
+  g.block = p->block;
+  g.cubic = o->cubic;
+  g.center = o->center;
+  g.end_mode = o->end_mode;
+  g.round_mode = o->round_mode;
+  g.R = o->radius;
+  g.N = p->npoints;
+  g.n = st.count;
+  g.code_base = (int64_t)p->channels * st.offset;
+  for (int j = 0; j < p->ndim; j++) {
+    g.shape[j] = p->shape[j];
+    g.gstr[j] = p->gstr[j];
+  }
+  g.eb = eb;
+  g.span = o->span;
+  g.offset = o->offset;
+  g.n_excl = (int64_t)st.excluded.size();
+  return g;
+}
+
+static int64_t max_class(const StageDesc &st) {
+  int64_t m = 0;
+  for (auto &c : st.classes) m = c.size > m ? c.size : m;
+  return m;
+}
+
+int interp_encode(latec_plan *p, int s0, int s1, const float *x, float *recon,
+                  void *codes, uint32_t *hist, uint32_t *out_rank,
+                  float *out_val, uint32_t *out_count, const double *ebs,
+                  const latec_interp_opts *o, hipStream_t stream) {
+  int rc = check_opts(o);
+  if (rc) return rc;
+  if (s0 < 0 || s1 > (int)p->stages.size() || s0 > s1) {
+    set_error("stage range out of bounds");
+    return LATEC_E_ARG;
+  }
+  if ((rc = plan_upload(p))) return rc;
+  const int64_t twoR = 2 * (int64_t)o->radius;
+  for (int s = s0; s < s1; s++) {
+    const StageDesc &st = p->stages[s];
+    if (!st.count) continue;
+    if (!(ebs[s] > 0)) {
+      set_error("error bound must be > 0");
+      return LATEC_E_ARG;
+    }
+    GridArgs g = make_args(p, s, ebs[s], o);
+    dim3 grid(grid_blocks(max_class(st)), (unsigned)st.classes.size());
+    const ClassDesc *cls = p->dev->classes + p->dev->class_off[s];
+    const uint32_t *ex = p->dev->excluded ? p->dev->excluded + p->dev->excl_off[s] : nullptr;
+    const int64_t lo = (int64_t)p->channels * st.offset;
+    if (o->code_bytes == 2)
+      hipLaunchKernelGGL(encode_stage_kernel<uint16_t>, grid, dim3(256), 0, stream, g, cls,
+                         ex, x, recon, (uint16_t *)codes, hist + s * twoR, out_rank + lo,
+                         out_val + lo, out_count + s, s == 0);
+    else
+      hipLaunchKernelGGL(encode_stage_kernel<uint32_t>, grid, dim3(256), 0, stream, g, cls,
+                         ex, x, recon, (uint32_t *)codes, hist + s * twoR, out_rank + lo,
+                         out_val + lo, out_count + s, s == 0);
+    if ((rc = hip_check(hipGetLastError(), "encode_stage_kernel"))) return rc;
+  }
+  return LATEC_OK;
+}
+
+int interp_decode(latec_plan *p, int s0, int s1, const void *codes,
+                  const uint32_t *out_rank, const float *out_val,
+                  const uint32_t *out_count, float *recon, const double *ebs,
+                  const latec_interp_opts *o, hipStream_t stream) {
+  int rc = check_opts(o);
+  if (rc) return rc;
+  if (s0 < 0 || s1 > (int)p->stages.size() || s0 > s1) {
+    set_error("stage range out of bounds");
+    return LATEC_E_ARG;
+  }
+  if ((rc = plan_upload(p))) return rc;
+  for (int s = s0; s < s1; s++) {
+    const StageDesc &st = p->stages[s];
+    if (!st.count) continue;
+    GridArgs g = make_args(p, s, ebs[s], o);
+    dim3 grid(grid_blocks(max_class(st)), (unsigned)st.classes.size());
+    const ClassDesc *cls = p->dev->classes + p->dev->class_off[s];
+    const uint32_t *ex = p->dev->excluded ? p->dev->excluded + p->dev->excl_off[s] : nullptr;
+    const int64_t lo = (int64_t)p->channels * st.offset;
+    if (o->code_bytes == 2)
+      hipLaunchKernelGGL(decode_stage_kernel<uint16_t>, grid, dim3(256), 0, stream, g, cls,
+                         ex, (const uint16_t *)codes, out_rank + lo, out_val + lo,
+                         out_count + s, recon, s == 0);
+    else
+      hipLaunchKernelGGL(decode_stage_kernel<uint32_t>, grid, dim3(256), 0, stream, g, cls,
+                         ex, (const uint32_t *)codes, out_rank + lo, out_val + lo,
+                         out_count + s, recon, s == 0);
+    if ((rc = hip_check(hipGetLastError(), "decode_stage_kernel"))) return rc;
+  }
+  return LATEC_OK;
+}
+
+int interp_encode_graph(latec_plan *p, const float *x, float *recon, void *codes,
+                        uint32_t *hist, uint32_t *out_rank, float *out_val,
+                        uint32_t *out_count, const double *ebs,
+                        const latec_interp_opts *o, hipStream_t stream) {
+  int rc = plan_upload(p);
+  if (rc) return rc;
+  // The graph bakes in pointers, ebs and options: key on all of them.
+  DeviceBufs::GraphKey key;
+  memset(&key, 0, sizeof key);
+  key.a[0] = x; key.a[1] = recon; key.a[2] = codes; key.a[3] = hist;
+  key.a[4] = out_rank; key.a[5] = out_val; key.a[6] = out_count;
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void *b, size_t n) {
+    for (size_t i = 0; i < n; i++) h = (h ^ ((const uint8_t *)b)[i]) * 1099511628211ull;
+  };
+  mix(ebs, sizeof(double) * p->stages.size());
+  mix(o, sizeof *o);
+  key.a[7] = (const void *)(uintptr_t)h;
+  auto it = p->dev->graphs.find(key);
+  if (it == p->dev->graphs.end()) {
+    hipStream_t cap;
+    if ((rc = hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate")))
+      return rc;
+    hipGraph_t graph;
+    rc = hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "capture");
+    if (!rc) {
+      int erc = interp_encode(p, 0, (int)p->stages.size(), x, recon, codes, hist, out_rank,
+                              out_val, out_count, ebs, o, cap);
+      int crc = hip_check(hipStreamEndCapture(cap, &graph), "end capture");
+      rc = erc ? erc : crc;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (!rc) rc = hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+    if (!rc) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cap);
+    if (rc) return rc;
+    it = p->dev->graphs.emplace(key, exec).first;
+  }
+  return hip_check(hipGraphLaunch(it->second, stream), "hipGraphLaunch");
+}
+
+int interp_predict(latec_plan *p, int stage, const float *recon, float *pred,
+                   const latec_interp_opts *o, hipStream_t stream) {
+  int rc = check_opts(o);
+  if (rc) return rc;
+  if (stage < 1 || stage >= (int)p->stages.size()) {
+    set_error("known mask does not match the interp schedule");
+    return LATEC_E_ARG;
+  }
+  if ((rc = plan_upload(p))) return rc;
+  const StageDesc &st = p->stages[stage];
+  if (!st.count) return LATEC_OK;
+  GridArgs g = make_args(p, stage, 1.0, o);
+  dim3 grid(grid_blocks(max_class(st)), (unsigned)st.classes.size());
+  const uint32_t *ex = p->dev->excluded ? p->dev->excluded + p->dev->excl_off[stage] : nullptr;
+  hipLaunchKernelGGL(predict_stage_kernel, grid, dim3(256), 0, stream, g,
+                     p->dev->classes + p->dev->class_off[stage], ex, recon, pred);
+  return hip_check(hipGetLastError(), "predict_stage_kernel");
+}
+
+int code_hist(const void *codes, int64_t n, int cb, uint32_t R, uint32_t *hist,
+              hipStream_t stream) {
+  if (!n) return LATEC_OK;
+  if (cb == 2)
+    hipLaunchKernelGGL(code_hist_kernel<uint16_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
+                       (const uint16_t *)codes, n, R, hist);
+  else
+    hipLaunchKernelGGL(code_hist_kernel<uint32_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
+                       (const uint32_t *)codes, n, R, hist);
+  return hip_check(hipGetLastError(), "code_hist_kernel");
+}
+
+int quantize(const float *x, const float *pred, int64_t n, double eb,
+             const latec_interp_opts *o, void *codes, uint32_t *out_rank,
+             float *out_val, uint32_t *out_count, uint32_t *hist,
+             hipStream_t stream) {
+  int rc = check_opts(o);
+  if (rc) return rc;
+  if (!(eb > 0)) {
+    set_error("error bound must be > 0, got " + std::to_string(eb));
+    return LATEC_E_ARG;
+  }
+  if (n > 0xFFFFFFFFll) {
+    set_error("too many values for 32-bit outlier ranks");
+    return LATEC_E_ARG;
+  }
+  if (!n) return LATEC_OK;
+  GridArgs g;
+  memset(&g, 0, sizeof g);
+  g.R = o->radius; g.eb = eb; g.round_mode = o->round_mode; g.span = o->span;
+  g.offset = o->offset;
+  if (o->code_bytes == 2)
+    hipLaunchKernelGGL(quantize_kernel<uint16_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
+                       g, x, pred, n, (uint16_t *)codes, out_rank, out_val, out_count, hist);
+  else
+    hipLaunchKernelGGL(quantize_kernel<uint32_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
+                       g, x, pred, n, (uint32_t *)codes, out_rank, out_val, out_count, hist);
+  return hip_check(hipGetLastError(), "quantize_kernel");
+}
+
+int dequantize(const float *pred, const void *codes, int64_t n,
+               const uint32_t *out_rank, const float *out_val,
+               const uint32_t *out_count, double eb, const latec_interp_opts *o,
+               float *recon, hipStream_t stream) {
+  int rc = check_opts(o);
+  if (rc) return rc;
+  if (!n) return LATEC_OK;
+  GridArgs g;
+  memset(&g, 0, sizeof g);
+  g.R = o->radius; g.eb = eb;
+  if (o->code_bytes == 2)
+    hipLaunchKernelGGL(dequantize_kernel<uint16_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
+                       g, pred, (const uint16_t *)codes, n, out_rank, out_val, out_count, recon);
+  else
+    hipLaunchKernelGGL(dequantize_kernel<uint32_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
+                       g, pred, (const uint32_t *)codes, n, out_rank, out_val, out_count, recon);
+  return hip_check(hipGetLastError(), "dequantize_kernel");
+}
+
+}  // namespace latec

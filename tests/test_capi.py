@@ -1,0 +1,92 @@
+"""The C-ABI library loads and exports every symbol include/latec.h
+declares; its host-side coder matches the oracle word for word."""
+
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+from latec_amd import _lib
+from oracle import oracle as O
+
+HDR = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "latec.h")
+
+
+def declared():
+    txt = open(HDR).read()
+    return sorted(set(re.findall(r"\b(latec_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    L = _lib.lib()
+    names = declared()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, n
+
+
+def test_rans_words_match_oracle():
+    rng = np.random.RandomState(3)
+    for kexp, level in ((1, 0), (4, 20), (7, 33), (10, 50)):
+        cdf = O.build_cdfs(kexp, 3, 24, 2)[level]
+        pmf = np.diff(cdf.astype(np.int64))
+        syms = rng.choice(len(pmf), 30000, p=pmf / pmf.sum()).astype(np.uint32)
+        ref = O.rans_encode_syms(syms, cdf)
+        words = np.empty(len(syms) + 2, np.uint32)
+        nw = _lib.check(_lib.lib().latec_rans_encode(_lib.ptr(syms), len(syms), _lib.ptr(cdf),
+                                                     len(cdf) - 1, _lib.ptr(words), len(words)))
+        assert np.array_equal(words[:nw], ref)
+        out = np.empty(len(syms), np.uint32)
+        _lib.check(_lib.lib().latec_rans_decode(_lib.ptr(ref), len(ref), _lib.ptr(cdf),
+                                                len(cdf) - 1, _lib.ptr(out), len(out)))
+        assert np.array_equal(out, syms)
+
+
+def test_rans_trailing_data_rejected():
+    cdf = O.build_cdfs(2, 0)[10]
+    w = O.rans_encode_syms(np.array([1, 2, 3], np.uint32), cdf)
+    bad = np.concatenate([[7], w]).astype(np.uint32)
+    out = np.empty(3, np.uint32)
+    rc = _lib.lib().latec_rans_decode(_lib.ptr(bad), len(bad), _lib.ptr(cdf), len(cdf) - 1,
+                                      _lib.ptr(out), 3)
+    assert rc == _lib.LATEC_E_FORMAT
+    assert "trailing" in _lib.last_error()
+
+
+def test_zstd_frame_matches_oracle():
+    data = np.random.RandomState(0).randint(0, 7, 100000).astype(np.uint8).tobytes()
+    cap = _lib.lib().latec_zstd_bound(len(data))
+    dst = np.empty(cap, np.uint8)
+    src = np.frombuffer(data, np.uint8)
+    n = _lib.lib().latec_zstd_compress(_lib.ptr(src), len(src), 9, _lib.ptr(dst), cap)
+    assert dst[:n].tobytes() == O._zstd(data, 9)
+
+
+def test_unpack_stage_of_oracle_records():
+    """Records written by the oracle unpack with the native parser."""
+    R = 1 << 15
+    rng = np.random.RandomState(1)
+    for n, sc in ((300, 1.5), (6000, 0.3), (9000, 40.0)):
+        codes = (np.rint(rng.laplace(0, sc, n)) + R).astype(np.uint32)
+        codes[::53] = 0
+        outl = rng.randn(int((codes == 0).sum())).astype(np.float32)
+        rec = np.frombuffer(O.pack_stage(codes, outl), np.uint8)
+        _, framed = struct.unpack_from("<IQ", rec.tobytes(), 0)
+        kexp, shape, level, owx = ((framed >> 45) & 15, (framed >> 49) & 31,
+                                   (framed >> 54) & 63, (framed >> 60) & 7)
+        row = O.build_cdfs(int(kexp), int(shape), 24, int(owx))[int(level)]
+        c = np.empty(n, np.uint16)
+        rk = np.empty(n, np.uint32)
+        vl = np.empty(n, np.float32)
+        import ctypes
+        no = ctypes.c_int64()
+        used = _lib.check(_lib.lib().latec_unpack_stage(_lib.ptr(rec), len(rec), R, _lib.ptr(row),
+                                                        n, _lib.ptr(c), 2, _lib.ptr(rk),
+                                                        _lib.ptr(vl), n, ctypes.byref(no)))
+        assert used == len(rec)
+        assert np.array_equal(c.astype(np.uint32), codes)
+        assert np.array_equal(rk[:no.value], np.flatnonzero(codes == 0))
+        assert np.array_equal(vl[:no.value], outl)

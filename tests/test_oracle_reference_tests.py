@@ -1,0 +1,264 @@
+"""The reference's own known-answer tests and closed-form oracles, restated
+against the C oracle (tests/test_interp_predictor.py, test_quantizer.py,
+test_levels.py, test_rans.py, test_bitstream.py of the reference)."""
+
+import itertools
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+END_MODES = [0, 1, 2, 3]
+
+
+def _interp_axis_full(V, axis, s, order, end_mode=0):
+    """Whole-grid 1-D interpolation: the closed form of the reference's
+    tests/test_interp_predictor.py:21-57, evaluated with NumPy."""
+    n = V.shape[axis]
+
+    def gather(off):
+        j = np.arange(n) + off
+        return np.take(V, np.clip(j, 0, n - 1), axis=axis), (j >= 0) & (j < n)
+
+    def bc(m):
+        sh = [1] * V.ndim
+        sh[axis] = m.shape[0]
+        return m.reshape(sh)
+
+    Lm1, vm1 = gather(-s)
+    Lp1, vp1 = gather(+s)
+    lin = 0.5 * (Lm1 + Lp1)
+    pred = lin
+    Lm3, vm3 = gather(-3 * s)
+    Lp3, vp3 = gather(+3 * s)
+    if order == "cubic":
+        cub = (-Lm3 + 9 * Lm1 + 9 * Lp1 - Lp3) / 16.0
+        one_far = lin
+        if end_mode & 1:
+            one_far = np.where(bc(vp3 & ~vm3), (3 * Lm1 + 6 * Lp1 - Lp3) / 8.0,
+                               np.where(bc(vm3 & ~vp3), (-Lm3 + 6 * Lm1 + 3 * Lp1) / 8.0, lin))
+        pred = np.where(bc(vm3 & vp3), cub, one_far)
+    left, right = Lm1, Lp1
+    if end_mode & 2:
+        left = np.where(bc(vm3), 1.5 * Lm1 - 0.5 * Lm3, Lm1)
+        right = np.where(bc(vp3), 1.5 * Lp1 - 0.5 * Lp3, Lp1)
+    return np.where(bc(vm1 & vp1), pred, np.where(bc(vm1 & ~vp1), left, right))
+
+
+@pytest.mark.parametrize("order", ["linear", "cubic"])
+@pytest.mark.parametrize("center", [0, 1, 2])
+@pytest.mark.parametrize("end_mode", END_MODES)
+def test_oracle_predict_matches_full_grid_2d(order, center, end_mode):
+    """reference tests/test_interp_predictor.py:60-86 on the oracle."""
+    shape, levels, stride = (70, 90), 3, 8
+    recon = np.random.RandomState(0).rand(2, *shape).astype(np.float32) * 50
+    for mask, s, w, axis in O.stage_plan(shape, levels, stride, 1):
+        if w == 0 or not mask.any():
+            continue
+        axes = (axis,) if w == 1 else tuple(range(len(shape)))
+        got = O.predict(recon, mask, s, w, axis, order, center, end_mode)
+        W = recon.astype(np.float64)
+        if center == 0 or len(axes) == 1:
+            ref = sum(_interp_axis_full(W, a + 1, s, order, end_mode) for a in axes) / len(axes)
+        else:
+            a = axes[0] if center == 1 else axes[-1]
+            ref = _interp_axis_full(W, a + 1, s, order, end_mode)
+        assert np.array_equal(got, ref.astype(np.float32)[:, mask])
+
+
+@pytest.mark.parametrize("order", ["linear", "cubic"])
+@pytest.mark.parametrize("center", [0, 1, 2])
+def test_oracle_fused_stage_3d(order, center):
+    """reference tests/test_interp_predictor.py:89-132 on the oracle."""
+    shape, levels, stride = (16, 16, 16), 2, 4
+    recon = np.random.RandomState(0).rand(2, *shape).astype(np.float32) * 50
+    saw = False
+    for mask, s, w, axis in O.stage_plan(shape, levels, stride, 1):
+        if w == 0 or not mask.any():
+            continue
+        axes = (axis,) if w == 1 else (0, 1, 2)
+        got = O.predict(recon, mask, s, w, axis, order, center, 0)
+        W = recon.astype(np.float64)
+
+        def odd_full(a):
+            sel = (np.arange(shape[a]) // s) % 2 == 1
+            sh = [1, 1, 1, 1]
+            sh[a + 1] = shape[a]
+            return sel.reshape(sh)
+
+        if center == 0 or len(axes) == 1:
+            total = count = None
+            for a in axes:
+                v = _interp_axis_full(W, a + 1, s, order, 0)
+                m = odd_full(a).astype(np.float64)
+                total = v * m if total is None else total + v * m
+                count = m if count is None else count + m
+            with np.errstate(divide="ignore", invalid="ignore"):
+                ref = np.broadcast_to(total / count, W.shape)
+        else:
+            ref = None
+            for a in (reversed(axes) if center == 1 else axes):
+                v = _interp_axis_full(W, a + 1, s, order, 0)
+                ref = v if ref is None else np.where(odd_full(a), v, ref)
+            ref = np.broadcast_to(ref, W.shape)
+        saw |= len(axes) > 1
+        assert np.array_equal(got, ref.astype(np.float32)[:, mask])
+    assert saw
+
+
+def test_interp_line_end_kat():
+    """reference tests/test_gnn_codec_chunked.py:552-565: W = k^2 on a line,
+    stride 1 linear midpoints -> [2, 10, 22] (exact averages)."""
+    W = np.array([0, 1, 4, 9, 16, 25], np.float32).reshape(1, 6)
+    mask = np.zeros(6, bool)
+    mask[[1, 3, 5]] = True
+    got = O.predict(W, mask, 1, 1, 0, "linear", 0, 0)[0]
+    assert np.array_equal(got, np.array([2, 10, 16], np.float32))  # last: edge copy of b=16
+    got_x = O.predict(W, mask, 1, 1, 0, "linear", 0, 2)[0]
+    assert got_x[-1] == np.float32(1.5 * 16 - 0.5 * 4)
+
+
+# ---- quantizer (reference tests/test_quantizer.py) -------------------------
+
+@pytest.mark.parametrize("scale,eb", [(1.0, 0.01), (255.0, 2.0), (1e6, 1e-3), (1e-3, 1e-7)])
+def test_quantizer_bound(scale, eb):
+    rng = np.random.RandomState(42)
+    x = (rng.randn(10000) * scale).astype(np.float32)
+    pred = x + (rng.randn(10000) * eb * 10).astype(np.float32)
+    codes, outl = O.quantize(x, pred, eb)
+    rec = O.dequantize(pred, codes, outl, eb)
+    assert np.abs(x - rec).max() <= eb
+
+
+def test_quantizer_closed_form():
+    """codes equal the three-line closed form of quantizer.py:57-83."""
+    rng = np.random.RandomState(3)
+    x = (rng.randn(50000) * 3).astype(np.float32)
+    pred = (x + rng.randn(50000).astype(np.float32) * 0.2).astype(np.float32)
+    eb = 0.013
+    codes, outl = O.quantize(x, pred, eb)
+    w = 2.0 * eb
+    q = np.rint((x.astype(np.float64) - pred.astype(np.float64)) / w).astype(np.int64)
+    inr = np.abs(q) < (1 << 15)
+    blk = np.where(inr, q + (1 << 15), 0).astype(np.uint32)
+    rec = (pred.astype(np.float64) + w * (blk.astype(np.int64) - (1 << 15))).astype(np.float32)
+    ok = inr & (np.abs(x - rec) <= np.float32(eb))
+    blk[~ok] = 0
+    assert np.array_equal(codes, blk)
+    assert np.array_equal(outl, x[blk == 0])
+
+
+def test_quantizer_ties_absorption_radius_rounding():
+    eb = 0.1
+    x = (np.arange(-4, 5) * 2 + 1).astype(np.float32) * eb
+    c, o = O.quantize(x, np.zeros(9, np.float32), eb)
+    assert np.abs(x - O.dequantize(np.zeros(9, np.float32), c, o, eb)).max() <= eb
+    x = np.full(100, 1e7, np.float32) + np.linspace(0, 1, 100, dtype=np.float32)
+    p = np.full(100, 1e7, np.float32)
+    c, o = O.quantize(x, p, 1e-3)
+    assert np.abs(x - O.dequantize(p, c, o, 1e-3)).max() <= 1e-3
+    c, o = O.quantize(np.array([0.0, 1e6], np.float32), np.zeros(2, np.float32), 0.5)
+    assert c[1] == 0 and len(o) == 1
+    x = np.arange(0, 256, dtype=np.float32)
+    c, o = O.quantize(x, x + 1.5, 1.5, round_output=True)
+    assert np.abs(x - np.rint(O.dequantize(x + 1.5, c, o, 1.5))).max() <= 1.5
+    with pytest.raises(ValueError):
+        O.quantize(np.zeros(1, np.float32), np.zeros(1, np.float32), 0.0)
+
+
+def test_quantizer_nonfinite_are_outliers():
+    x = np.array([np.nan, np.inf, -np.inf, 1.0], np.float32)
+    c, o = O.quantize(x, np.zeros(4, np.float32), 0.1)
+    assert list(c[:3]) == [0, 0, 0] and len(o) == 3
+
+
+# ---- schedule (reference tests/test_levels.py) ------------------------------
+
+@pytest.mark.parametrize("shape,levels,stride,block", [
+    ((33,), 3, 8, 1), ((20, 17), 3, 8, 1), ((16, 16), 4, 16, 4), ((9, 10, 11), 2, 4, 1),
+    ((12, 9, 7), 3, 8, 2), ((6, 5, 4, 5), 2, 4, 1), ((8, 8), 5, 16, 1), ((5, 6, 3, 4, 3), 1, 2, 1)])
+def test_schedule_partition(shape, levels, stride, block):
+    cov = np.zeros(shape, int)
+    n = 0
+    for mask, s, w, a in O.stage_plan(shape, levels, stride, block):
+        cov += mask
+        n += 1
+    assert n == O.lib().orc_n_stages(len(shape), levels)
+    assert (cov == 1).all()
+
+
+def test_stage_ebs_closed_form():
+    ebs = O.stage_ebs(3, 5, 32, 1e-3, 0.9)
+    assert ebs[0] == 1e-3 * 0.9 ** 5
+    assert ebs[-1] == 1e-3
+
+
+# ---- rANS coder + records ---------------------------------------------------
+
+def test_rans_roundtrip_and_size_tracks_model():
+    """reference tests/test_rans.py:31-54: round trip, length ~ model bits."""
+    rng = np.random.RandomState(1)
+    cdf = O.build_cdfs(7, 0)[30]
+    pmf = np.diff(cdf.astype(np.int64))
+    syms = rng.choice(len(pmf), 20000, p=pmf / pmf.sum()).astype(np.uint32)
+    words = O.rans_encode_syms(syms, cdf)
+    assert np.array_equal(O.rans_decode_syms(words, len(syms), cdf), syms)
+    bits = -np.log2(pmf[syms] / (1 << 24)).sum()
+    assert abs(len(words) * 32 - bits) / bits < 0.01
+
+
+def test_rans_empty_and_trailing():
+    cdf = O.build_cdfs(2, 0)[10]
+    assert len(O.rans_encode_syms(np.zeros(0, np.uint32), cdf)) == 0
+    w = O.rans_encode_syms(np.array([1, 2, 3], np.uint32), cdf)
+    with pytest.raises(ValueError):
+        O.rans_decode_syms(np.concatenate([[5], w]).astype(np.uint32), 3, cdf)
+
+
+def test_record_framing_and_roundtrip():
+    """bitstream.py:249-285 bit layout; record round trips incl. literal."""
+    assert struct.calcsize("<8sHHIIBBBBIHhddd16sd") == 80
+    rng = np.random.RandomState(0)
+    R = 1 << 15
+    for n, scale in ((0, 1), (100, 2.0), (5000, 0.4), (20000, 30.0)):
+        q = np.rint(rng.laplace(0, scale, n)).astype(np.int64)
+        codes = (q + R).astype(np.uint32)
+        if n:
+            codes[::97] = 0
+        outl = rng.randn(int((codes == 0).sum())).astype(np.float32)
+        rec = O.pack_stage(codes, outl)
+        n_codes, framed = struct.unpack_from("<IQ", rec, 0)
+        assert n_codes == n and (framed >> 45) & 15 >= 1
+        c2, o2, off = O.unpack_stage(rec, 0, n)
+        assert off == len(rec) and np.array_equal(c2, codes) and np.array_equal(o2, outl)
+    with pytest.raises(ValueError, match="count mismatch"):
+        O.unpack_stage(O.pack_stage(np.full(10, R, np.uint32), []), 0, 11)
+
+
+def test_literal_backend_chosen_for_runs():
+    """reference tests/test_bitstream.py:109-133: run-structured codes go to
+    the literal (zstd) back end, iid codes to rANS."""
+    R = 1 << 15
+    runs = np.repeat(np.arange(50) % 7 + R - 3, 400).astype(np.uint32)
+    rec = O.pack_stage(runs, [])
+    assert struct.unpack_from("<IQ", rec, 0)[1] >> 63 == 1
+    rng = np.random.RandomState(2)
+    iid = (np.rint(rng.laplace(0, 3.0, 20000)) + R).astype(np.uint32)
+    rec = O.pack_stage(iid, [])
+    assert struct.unpack_from("<IQ", rec, 0)[1] >> 63 == 0
+
+
+@pytest.mark.parametrize("shape,block,order,center,end", [
+    ((64, 48), 4, "cubic", 0, 1), ((40, 33, 17), 1, "cubic", 0, 3), ((40, 33, 17), 1, "linear", 2, 0),
+    ((12, 11, 10, 9), 1, "cubic", 1, 1)])
+def test_oracle_stream_roundtrip(shape, block, order, center, end):
+    rng = np.random.RandomState(5)
+    x = np.cumsum(np.cumsum(rng.randn(*shape), 0), -1).astype(np.float32)
+    eb = 1e-2 * float(x.max() - x.min())
+    levels, stride = (4, 16) if len(shape) == 2 else (3, 8)
+    stream, recon = O.compress(x, eb, order, levels, stride, block, center, end)
+    dec = O.decompress(stream)
+    assert np.array_equal(dec, recon)
+    assert np.abs(dec[0] - x).max() <= eb
