@@ -21,6 +21,7 @@
 // yields the same code and reconstruction (q = 0 => recon = f32(p + 0.0)).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -107,7 +108,9 @@ struct GridArgs {
   int64_t code_base;  // C * stage offset
   int64_t shape[LATEC_MAX_DIM];
   int64_t gstr[LATEC_MAX_DIM];
-  double eb, span, offset;
+  double eb, w, inv_w, span, offset;
+  float feb;
+  int32_t pad;
   int64_t n_excl;
 };
 
@@ -115,73 +118,36 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t magic, uint32_t sh
   return (uint32_t)(((uint64_t)__umulhi(n, magic) + n) >> shift);
 }
 
-// Decompose a class-local index into lattice coords; returns false for an
-// anchor point excluded from a dyadic stage.  Outputs flat grid index, the
-// stage rank, and the grid coordinates.
-__device__ __forceinline__ bool locate(const ClassDesc &c, const GridArgs &g,
-                                       uint32_t idx, const uint32_t *excl,
-                                       int64_t *flat, int64_t *rank,
-                                       int64_t *coord) {
-  int64_t f = 0, r = c.offset;
-  bool anchor = true;
-#pragma unroll
-  for (int j = LATEC_MAX_DIM - 1; j >= 0; j--) {
-    if (j >= g.ndim) continue;
-    uint32_t q = fdiv(idx, c.magic[j], c.shift[j]);
-    uint32_t m = idx - q * c.sizes[j];
-    idx = q;
-    int64_t x;
-    if (c.flags & 1)
-      x = (int64_t)(m / (uint32_t)g.block) * g.stage_stride + (m % (uint32_t)g.block);
-    else
-      x = c.starts[j] + (int64_t)c.step * m;
-    coord[j] = x;
-    f += x * g.gstr[j];
-    r += (int64_t)m * c.rstr[j];
-    anchor = anchor && ((x % g.stage_stride) < g.block);
-  }
-  *flat = f;
-  if (g.n_excl && anchor) return false;
-  if (g.n_excl) {  // rebase the rank past the excluded anchors before it
-    int64_t lo = 0, hi = g.n_excl;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if ((int64_t)excl[mid] < r) lo = mid + 1; else hi = mid;
-    }
-    r -= lo;
-  }
-  *rank = r;
-  return true;
-}
-
-// 1-D interpolation along one odd axis (c >= s always): predictor.py:97-156.
-__device__ __forceinline__ double interp_axis(const float *__restrict__ W,
-                                              int64_t flat, int64_t c,
-                                              int64_t n, int64_t gs, int64_t s,
+// 1-D interpolation along one odd axis at coordinate c (c >= s always):
+// predictor.py:97-156.  `flat` and `gs` in the index type I.
+template <typename I>
+__device__ __forceinline__ double interp_axis(const float *__restrict__ W, I flat,
+                                              int64_t c, int64_t n, I gs, int64_t s,
                                               int cubic, int end_mode) {
   const bool vp1 = c + s < n, vm3 = c >= 3 * s, vp3 = c + 3 * s < n;
-  const double B = (double)W[flat - s * gs];
+  const I o1 = (I)s * gs, o3 = (I)(3 * s) * gs;
+  const double B = (double)W[flat - o1];
   if (!vp1) {
     if ((end_mode & 2) && vm3) {
-      const double A = (double)W[flat - 3 * s * gs];
+      const double A = (double)W[flat - o3];
       return 1.5 * B - 0.5 * A;
     }
     return B;
   }
-  const double C = (double)W[flat + s * gs];
+  const double C = (double)W[flat + o1];
   if (cubic) {
     if (vm3 && vp3) {
-      const double A = (double)W[flat - 3 * s * gs];
-      const double D = (double)W[flat + 3 * s * gs];
+      const double A = (double)W[flat - o3];
+      const double D = (double)W[flat + o3];
       return (((-A + 9.0 * B) + 9.0 * C) - D) * 0.0625;
     }
     if (end_mode & 1) {
-      if (vp3) {  // !vm3
-        const double D = (double)W[flat + 3 * s * gs];
+      if (vp3) {  // !vm3: interp_quad_1
+        const double D = (double)W[flat + o3];
         return ((3.0 * B + 6.0 * C) - D) * 0.125;
       }
-      if (vm3) {  // !vp3
-        const double A = (double)W[flat - 3 * s * gs];
+      if (vm3) {  // !vp3: interp_quad_2
+        const double A = (double)W[flat - o3];
         return ((-A + 6.0 * B) + 3.0 * C) * 0.125;
       }
     }
@@ -189,17 +155,21 @@ __device__ __forceinline__ double interp_axis(const float *__restrict__ W,
   return 0.5 * (B + C);
 }
 
+// Centre combine over the class's odd axes (predictor.py:286-315), cast to
+// float32.  ND > 0: compile-time rank; ND == 0: runtime g.ndim.
+template <int ND, typename I>
 __device__ __forceinline__ float predict_point(const float *__restrict__ W,
                                                const GridArgs &g, int parity,
-                                               int64_t flat,
-                                               const int64_t *coord) {
+                                               I flat, const int64_t *coord) {
   double total = 0.0, v_first = 0.0, v_last = 0.0;
   int count = 0;
+  constexpr int NMAX = ND > 0 ? ND : LATEC_MAX_DIM;
 #pragma unroll
-  for (int a = 0; a < LATEC_MAX_DIM; a++) {
-    if (a >= g.ndim || !((parity >> a) & 1)) continue;
-    double v = interp_axis(W, flat, coord[a], g.shape[a], g.gstr[a], g.s,
-                           g.cubic, g.end_mode);
+  for (int a = 0; a < NMAX; a++) {
+    if (ND == 0 && a >= g.ndim) continue;
+    if (!((parity >> a) & 1)) continue;
+    const double v = interp_axis<I>(W, flat, coord[a], g.shape[a], (I)g.gstr[a], g.s,
+                                    g.cubic, g.end_mode);
     total = count ? total + v : v;
     if (!count) v_first = v;
     v_last = v;
@@ -214,25 +184,29 @@ __device__ __forceinline__ float predict_point(const float *__restrict__ W,
 }
 
 // Quantize one value: quantizer.py:57-83 + _recon_from_codes :114-120.
-// Returns the code (0 = outlier); *rec receives the reconstruction.
-__device__ __forceinline__ uint32_t quantize_one(float x, float p,
-                                                 const GridArgs &g,
+// q = rint(fl((x - p) / w)) is obtained from the product with 1/w and
+// re-derived by the IEEE division whenever the product lies within 1e-12
+// (relative) of a rounding boundary, so the code is always the division's.
+__device__ __forceinline__ uint32_t quantize_one(float x, float p, const GridArgs &g,
                                                  float *rec) {
-  const double w = 2.0 * g.eb;
-  const double qd = rint(((double)x - (double)p) / w);
+  const double d = (double)x - (double)p;
+  const double t = d * g.inv_w;
+  double qd = rint(t);
+  const double dist = 0.5 - fabs(t - qd);
+  if (!(dist > 1e-12 * fmax(fabs(t), 1.0))) qd = rint(d / g.w);
   const bool inr = fabs(qd) < (double)g.R;
   uint32_t code = 0;
   float r = x;
   if (inr) {
-    const int64_t q = (int64_t)qd;
-    code = (uint32_t)(q + (int64_t)g.R);
-    const float cand = (float)((double)p + w * (double)q);
+    const int q = (int)qd;
+    code = (uint32_t)(q + (int)g.R);
+    const float cand = (float)((double)p + g.w * (double)q);
     float chk = cand;
     if (g.round_mode == 1)
       chk = rintf(cand);
     else if (g.round_mode == 2)
       chk = (float)((rint((double)cand * g.span + g.offset) - g.offset) / g.span);
-    if (fabsf(x - chk) <= (float)g.eb)
+    if (fabsf(x - chk) <= g.feb)
       r = cand;
     else
       code = 0;
@@ -253,7 +227,7 @@ __device__ __forceinline__ void emit(CodeT *__restrict__ codes, int64_t slot,
                                      uint32_t win_lo, bool active) {
   if (active) {
     codes[slot] = (CodeT)code;
-    uint32_t b = code - win_lo;
+    const uint32_t b = code - win_lo;
     if (code != 0 && b < HIST_WIN)
       atomicAdd(&lhist[b], 1u);
     else
@@ -276,85 +250,148 @@ __device__ __forceinline__ void emit(CodeT *__restrict__ codes, int64_t slot,
   }
 }
 
-template <typename CodeT>
+// Row walker shared by the encode / decode / predict kernels.  Threads of a
+// 256-thread block are split into 256/tr rows of tr lanes; a row is one run
+// of the class along the last axis, whose stage ranks are consecutive.  The
+// prefix (all but the last axis) is decomposed once per row.
+//   ND    compile-time rank (0 = runtime)
+//   I     grid index type (uint32_t when C*N < 2^32, else int64_t)
+//   BLK1  anchor_block == 1 (no anchor exclusion, no anchor-block coords)
+template <int ND, typename I, bool BLK1, typename F>
+__device__ __forceinline__ void walk_rows(const ClassDesc &c, const GridArgs &g,
+                                          const uint32_t *__restrict__ excl, F &&fn) {
+  constexpr int NMAX = ND > 0 ? ND : LATEC_MAX_DIM;
+  const int nd = ND > 0 ? ND : g.ndim;
+  const int tr = c.tr, rb = 256 / tr;
+  const int lrow = threadIdx.x / tr, lcol = threadIdx.x % tr;
+  const uint32_t L = c.row_len, nrows = c.nrows;
+  const int64_t start_last = c.starts[nd - 1];
+  // XCD-aware order: the hardware deals blocks round-robin to the 8 XCDs, so
+  // block b runs on XCD b % 8; renumbering makes each XCD walk one contiguous
+  // band of rows, whose shared tap lines then hit in that XCD's L2.
+  const uint32_t G = gridDim.x;
+  const uint32_t lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  for (uint32_t base = lb * rb; base < nrows; base += G * rb) {
+    const uint32_t row = base + lrow;
+    const bool rv = row < nrows;
+    int64_t coord[NMAX];
+    I fbase = 0;
+    int64_t rbase = c.offset;
+    bool anc_pref = true;
+    uint32_t idx = rv ? row : 0;
+#pragma unroll
+    for (int j = NMAX - 2; j >= 0; j--) {
+      if (ND == 0 && j > nd - 2) continue;
+      const uint32_t q = fdiv(idx, c.magic[j], c.shift[j]);
+      const uint32_t m = idx - q * c.sizes[j];
+      idx = q;
+      int64_t x;
+      if (!BLK1 && (c.flags & 1))
+        x = (int64_t)(m / (uint32_t)g.block) * g.stage_stride + (m % (uint32_t)g.block);
+      else
+        x = c.starts[j] + (int64_t)c.step * m;
+      coord[j] = x;
+      fbase += (I)x * (I)g.gstr[j];
+      rbase += (int64_t)m * c.rstr[j];
+      if (!BLK1) anc_pref = anc_pref && ((x % g.stage_stride) < g.block);
+    }
+    for (uint32_t col0 = 0; col0 < L; col0 += tr) {
+      const uint32_t col = col0 + lcol;
+      bool active = rv && col < L;
+      int64_t xl;
+      if (!BLK1 && (c.flags & 1))
+        xl = (int64_t)(col / (uint32_t)g.block) * g.stage_stride + (col % (uint32_t)g.block);
+      else
+        xl = start_last + (int64_t)c.step * col;
+      coord[nd - 1] = xl;
+      const I flat = fbase + (I)xl;
+      int64_t rank = rbase + col;
+      if (!BLK1 && g.n_excl) {
+        if (anc_pref && (xl % g.stage_stride) < g.block) active = false;
+        int64_t lo = 0, hi = g.n_excl;  // rebase past the excluded anchors
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if ((int64_t)excl[mid] < rank) lo = mid + 1; else hi = mid;
+        }
+        rank -= lo;
+      }
+      fn(active, flat, rank, (const int64_t *)coord);
+    }
+  }
+}
+
+template <int ND, typename I, bool BLK1, typename CodeT>
 __global__ void __launch_bounds__(256) encode_stage_kernel(
     GridArgs g, const ClassDesc *__restrict__ classes,
     const uint32_t *__restrict__ excl, const float *__restrict__ x,
     float *__restrict__ recon, CodeT *__restrict__ codes,
     uint32_t *__restrict__ ghist, uint32_t *__restrict__ out_rank,
-    float *__restrict__ out_val, uint32_t *__restrict__ out_count,
-    int anchors) {
+    float *__restrict__ out_val, uint32_t *__restrict__ out_count, int anchors) {
   __shared__ uint32_t lhist[HIST_WIN];
-  const ClassDesc c = classes[blockIdx.y];
+  const ClassDesc &c = classes[blockIdx.y];
   const uint32_t win_lo = g.R > HIST_WIN / 2 ? g.R - HIST_WIN / 2 : 0;
   for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x) lhist[i] = 0;
   __syncthreads();
-  const uint32_t size = (uint32_t)c.size;
-  const uint32_t nthreads = gridDim.x * blockDim.x;
-  // iterate so that every lane of a wave reaches the ballot together
-  for (uint32_t base = blockIdx.x * blockDim.x; base < size; base += nthreads) {
-    const uint32_t idx = base + threadIdx.x;
-    int64_t flat = 0, rank = 0, coord[LATEC_MAX_DIM];
-    bool active = idx < size;
-    if (active) active = locate(c, g, idx, excl, &flat, &rank, coord);
+  const int parity = c.parity;
+  walk_rows<ND, I, BLK1>(c, g, excl, [&](bool active, I flat, int64_t rank,
+                                          const int64_t *coord) {
     for (int ch = 0; ch < g.channels; ch++) {
       uint32_t code = 0;
       float xv = 0.f;
-      int64_t slot = 0;
+      const I cofs = (I)ch * (I)g.N;
       if (active) {
-        const float *W = recon + ch * g.N;
-        const float p = anchors ? 0.0f : predict_point(W, g, c.parity, flat, coord);
-        xv = x[ch * g.N + flat];
+        const float p = anchors ? 0.0f
+                                : predict_point<ND, I>(recon + cofs, g, parity, flat, coord);
+        xv = x[cofs + flat];
         float rec;
         code = quantize_one(xv, p, g, &rec);
-        recon[ch * g.N + flat] = rec;
-        slot = g.code_base + ch * g.n + rank;
+        recon[cofs + flat] = rec;
       }
-      emit<CodeT>(codes, slot, code, xv, (uint32_t)(ch * g.n + rank), out_rank,
-                  out_val, out_count, lhist, ghist, win_lo, active);
+      const int64_t srank = (int64_t)ch * g.n + rank;
+      emit<CodeT>(codes, g.code_base + srank, code, xv, (uint32_t)srank, out_rank, out_val,
+                  out_count, lhist, ghist, win_lo, active);
     }
-  }
+  });
   __syncthreads();
   for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
     if (lhist[i]) atomicAdd(&ghist[win_lo + i], lhist[i]);
 }
 
-// Decode: predict, then recon = f32(p + 2eb q) or the outlier value found
-// by binary search of the stage's sorted outlier ranks.
-template <typename CodeT>
+// Decode: predict, then recon = f32(p + 2eb q) or the outlier value found by
+// binary search of the stage's sorted outlier ranks.
+template <int ND, typename I, bool BLK1, typename CodeT>
 __global__ void __launch_bounds__(256) decode_stage_kernel(
     GridArgs g, const ClassDesc *__restrict__ classes,
     const uint32_t *__restrict__ excl, const CodeT *__restrict__ codes,
     const uint32_t *__restrict__ out_rank, const float *__restrict__ out_val,
-    const uint32_t *__restrict__ out_count, float *__restrict__ recon,
-    int anchors) {
-  const ClassDesc c = classes[blockIdx.y];
-  const uint32_t size = (uint32_t)c.size;
-  const uint32_t nthreads = gridDim.x * blockDim.x;
-  const double w = 2.0 * g.eb;
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size;
-       idx += nthreads) {
-    int64_t flat, rank, coord[LATEC_MAX_DIM];
-    if (!locate(c, g, idx, excl, &flat, &rank, coord)) continue;
+    const uint32_t *__restrict__ out_count, float *__restrict__ recon, int anchors) {
+  const ClassDesc &c = classes[blockIdx.y];
+  const int parity = c.parity;
+  walk_rows<ND, I, BLK1>(c, g, excl, [&](bool active, I flat, int64_t rank,
+                                          const int64_t *coord) {
+    if (!active) return;
     for (int ch = 0; ch < g.channels; ch++) {
-      const float *W = recon + ch * g.N;
-      const float p = anchors ? 0.0f : predict_point(W, g, c.parity, flat, coord);
-      const int64_t srank = ch * g.n + rank;
+      const I cofs = (I)ch * (I)g.N;
+      const float p = anchors ? 0.0f
+                              : predict_point<ND, I>(recon + cofs, g, parity, flat, coord);
+      const int64_t srank = (int64_t)ch * g.n + rank;
       const uint32_t code = (uint32_t)codes[g.code_base + srank];
       float r;
       if (code) {
-        r = (float)((double)p + w * (double)((int64_t)code - (int64_t)g.R));
+        r = (float)((double)p + g.w * (double)((int)code - (int)g.R));
       } else {
-        int64_t lo = 0, hi = *out_count;
+        const int64_t cnt = *out_count;
+        int64_t lo = 0, hi = cnt;
         while (lo < hi) {
-          int64_t mid = (lo + hi) >> 1;
+          const int64_t mid = (lo + hi) >> 1;
           if ((int64_t)out_rank[mid] < srank) lo = mid + 1; else hi = mid;
         }
-        r = out_val[lo];
+        r = (lo < cnt && (int64_t)out_rank[lo] == srank) ? out_val[lo]
+                                                         : __int_as_float(0x7fc00000);
       }
-      recon[ch * g.N + flat] = r;
+      recon[cofs + flat] = r;
     }
-  }
+  });
 }
 
 // Generic quantizer over compact arrays (foreign predictors, GNN stages).
@@ -378,8 +415,8 @@ __global__ void __launch_bounds__(256) quantize_kernel(
       xv = x[i];
       code = quantize_one(xv, pred[i], g, &rec);
     }
-    emit<CodeT>(codes, i, code, xv, (uint32_t)i, out_rank, out_val, out_count,
-                lhist, ghist, win_lo, active);
+    emit<CodeT>(codes, i, code, xv, (uint32_t)i, out_rank, out_val, out_count, lhist, ghist,
+                win_lo, active);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
@@ -392,39 +429,40 @@ __global__ void __launch_bounds__(256) dequantize_kernel(
     int64_t n, const uint32_t *__restrict__ out_rank,
     const float *__restrict__ out_val, const uint32_t *__restrict__ out_count,
     float *__restrict__ recon) {
-  const double w = 2.0 * g.eb;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t code = (uint32_t)codes[i];
     float r;
     if (code) {
-      r = (float)((double)pred[i] + w * (double)((int64_t)code - (int64_t)g.R));
+      r = (float)((double)pred[i] + g.w * (double)((int)code - (int)g.R));
     } else {
-      int64_t lo = 0, hi = *out_count;
+      const int64_t cnt = *out_count;
+      int64_t lo = 0, hi = cnt;
       while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
         if ((int64_t)out_rank[mid] < i) lo = mid + 1; else hi = mid;
       }
-      r = out_val[lo];
+      r = (lo < cnt && (int64_t)out_rank[lo] == i) ? out_val[lo] : __int_as_float(0x7fc00000);
     }
     recon[i] = r;
   }
 }
 
 // Prediction only, written in stage order (C, n): InterpPredictor.predict.
+template <int ND, typename I, bool BLK1>
 __global__ void __launch_bounds__(256) predict_stage_kernel(
     GridArgs g, const ClassDesc *__restrict__ classes,
     const uint32_t *__restrict__ excl, const float *__restrict__ recon,
     float *__restrict__ pred) {
-  const ClassDesc c = classes[blockIdx.y];
-  const uint32_t size = (uint32_t)c.size;
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < size;
-       idx += gridDim.x * blockDim.x) {
-    int64_t flat, rank, coord[LATEC_MAX_DIM];
-    if (!locate(c, g, idx, excl, &flat, &rank, coord)) continue;
+  const ClassDesc &c = classes[blockIdx.y];
+  const int parity = c.parity;
+  walk_rows<ND, I, BLK1>(c, g, excl, [&](bool active, I flat, int64_t rank,
+                                          const int64_t *coord) {
+    if (!active) return;
     for (int ch = 0; ch < g.channels; ch++)
-      pred[ch * g.n + rank] = predict_point(recon + ch * g.N, g, c.parity, flat, coord);
-  }
+      pred[(int64_t)ch * g.n + rank] =
+          predict_point<ND, I>(recon + (I)ch * (I)g.N, g, parity, flat, coord);
+  });
 }
 
 template <typename CodeT>
@@ -460,12 +498,23 @@ static int grid_blocks(int64_t work) {
   return (int)(b < 1 ? 1 : b);
 }
 
+// Blocks along x for a stage: enough rows of every class, capped.
+static int row_blocks(const StageDesc &st) {
+  int64_t b = 1;
+  for (auto &c : st.classes) {
+    const int64_t rb = 256 / c.tr;
+    b = std::max<int64_t>(b, ((int64_t)c.nrows + rb - 1) / rb);
+  }
+  b = std::min<int64_t>(b, 8192);
+  return (int)((b + 7) / 8 * 8);  // a multiple of the 8 XCDs
+}
+
 static int check_opts(const latec_interp_opts *o) {
   if (!o || (o->code_bytes != 2 && o->code_bytes != 4)) {
     set_error("code_bytes must be 2 or 4");
     return LATEC_E_ARG;
   }
-  if (o->radius < 1 || (o->code_bytes == 2 && o->radius > 32768)) {
+  if (o->radius < 1 || (o->code_bytes == 2 && o->radius > 32768) || o->radius > (1u << 30)) {
     set_error("radius does not fit the code width");
     return LATEC_E_ARG;
   }
@@ -499,17 +548,30 @@ static GridArgs make_args(const latec_plan *p, int stage, double eb,
     g.gstr[j] = p->gstr[j];
   }
   g.eb = eb;
+  g.w = 2.0 * eb;
+  g.inv_w = 1.0 / g.w;
+  g.feb = (float)eb;
   g.span = o->span;
   g.offset = o->offset;
   g.n_excl = (int64_t)st.excluded.size();
   return g;
 }
 
-static int64_t max_class(const StageDesc &st) {
-  int64_t m = 0;
-  for (auto &c : st.classes) m = c.size > m ? c.size : m;
-  return m;
+// Kernel selection: a compile-time rank with 32-bit grid indices for fields
+// below 2^32 values and ranks 1..4, the generic kernel otherwise.
+enum class KSel { Fast, Generic };
+static KSel pick(const latec_plan *p) {
+  const bool small = (int64_t)p->channels * p->npoints < 0xFFFFFFFFll;
+  return (small && p->ndim <= 4) ? KSel::Fast : KSel::Generic;
 }
+
+#define LATEC_DISPATCH_ND(KERNEL, BLK1, CODET, ...)                                      \
+  switch (p->ndim) {                                                                     \
+    case 1: hipLaunchKernelGGL((KERNEL<1, uint32_t, BLK1, CODET>), __VA_ARGS__); break;  \
+    case 2: hipLaunchKernelGGL((KERNEL<2, uint32_t, BLK1, CODET>), __VA_ARGS__); break;  \
+    case 3: hipLaunchKernelGGL((KERNEL<3, uint32_t, BLK1, CODET>), __VA_ARGS__); break;  \
+    default: hipLaunchKernelGGL((KERNEL<4, uint32_t, BLK1, CODET>), __VA_ARGS__); break; \
+  }
 
 int interp_encode(latec_plan *p, int s0, int s1, const float *x, float *recon,
                   void *codes, uint32_t *hist, uint32_t *out_rank,
@@ -523,6 +585,7 @@ int interp_encode(latec_plan *p, int s0, int s1, const float *x, float *recon,
   }
   if ((rc = plan_upload(p))) return rc;
   const int64_t twoR = 2 * (int64_t)o->radius;
+  const KSel sel = pick(p);
   for (int s = s0; s < s1; s++) {
     const StageDesc &st = p->stages[s];
     if (!st.count) continue;
@@ -531,18 +594,30 @@ int interp_encode(latec_plan *p, int s0, int s1, const float *x, float *recon,
       return LATEC_E_ARG;
     }
     GridArgs g = make_args(p, s, ebs[s], o);
-    dim3 grid(grid_blocks(max_class(st)), (unsigned)st.classes.size());
+    dim3 grid(row_blocks(st), (unsigned)st.classes.size());
     const ClassDesc *cls = p->dev->classes + p->dev->class_off[s];
     const uint32_t *ex = p->dev->excluded ? p->dev->excluded + p->dev->excl_off[s] : nullptr;
     const int64_t lo = (int64_t)p->channels * st.offset;
-    if (o->code_bytes == 2)
-      hipLaunchKernelGGL(encode_stage_kernel<uint16_t>, grid, dim3(256), 0, stream, g, cls,
-                         ex, x, recon, (uint16_t *)codes, hist + s * twoR, out_rank + lo,
+    uint32_t *hs = hist + s * twoR;
+    if (sel == KSel::Fast && o->code_bytes == 2) {
+      if (p->block == 1) {
+        LATEC_DISPATCH_ND(encode_stage_kernel, true, uint16_t, grid, dim3(256), 0, stream, g,
+                          cls, ex, x, recon, (uint16_t *)codes, hs, out_rank + lo,
+                          out_val + lo, out_count + s, s == 0);
+      } else {
+        LATEC_DISPATCH_ND(encode_stage_kernel, false, uint16_t, grid, dim3(256), 0, stream, g,
+                          cls, ex, x, recon, (uint16_t *)codes, hs, out_rank + lo,
+                          out_val + lo, out_count + s, s == 0);
+      }
+    } else if (o->code_bytes == 2) {
+      hipLaunchKernelGGL((encode_stage_kernel<0, int64_t, false, uint16_t>), grid, dim3(256), 0,
+                         stream, g, cls, ex, x, recon, (uint16_t *)codes, hs, out_rank + lo,
                          out_val + lo, out_count + s, s == 0);
-    else
-      hipLaunchKernelGGL(encode_stage_kernel<uint32_t>, grid, dim3(256), 0, stream, g, cls,
-                         ex, x, recon, (uint32_t *)codes, hist + s * twoR, out_rank + lo,
+    } else {
+      hipLaunchKernelGGL((encode_stage_kernel<0, int64_t, false, uint32_t>), grid, dim3(256), 0,
+                         stream, g, cls, ex, x, recon, (uint32_t *)codes, hs, out_rank + lo,
                          out_val + lo, out_count + s, s == 0);
+    }
     if ((rc = hip_check(hipGetLastError(), "encode_stage_kernel"))) return rc;
   }
   return LATEC_OK;
@@ -559,22 +634,34 @@ int interp_decode(latec_plan *p, int s0, int s1, const void *codes,
     return LATEC_E_ARG;
   }
   if ((rc = plan_upload(p))) return rc;
+  const KSel sel = pick(p);
   for (int s = s0; s < s1; s++) {
     const StageDesc &st = p->stages[s];
     if (!st.count) continue;
     GridArgs g = make_args(p, s, ebs[s], o);
-    dim3 grid(grid_blocks(max_class(st)), (unsigned)st.classes.size());
+    dim3 grid(row_blocks(st), (unsigned)st.classes.size());
     const ClassDesc *cls = p->dev->classes + p->dev->class_off[s];
     const uint32_t *ex = p->dev->excluded ? p->dev->excluded + p->dev->excl_off[s] : nullptr;
     const int64_t lo = (int64_t)p->channels * st.offset;
-    if (o->code_bytes == 2)
-      hipLaunchKernelGGL(decode_stage_kernel<uint16_t>, grid, dim3(256), 0, stream, g, cls,
-                         ex, (const uint16_t *)codes, out_rank + lo, out_val + lo,
-                         out_count + s, recon, s == 0);
-    else
-      hipLaunchKernelGGL(decode_stage_kernel<uint32_t>, grid, dim3(256), 0, stream, g, cls,
-                         ex, (const uint32_t *)codes, out_rank + lo, out_val + lo,
-                         out_count + s, recon, s == 0);
+    if (sel == KSel::Fast && o->code_bytes == 2) {
+      if (p->block == 1) {
+        LATEC_DISPATCH_ND(decode_stage_kernel, true, uint16_t, grid, dim3(256), 0, stream, g,
+                          cls, ex, (const uint16_t *)codes, out_rank + lo, out_val + lo,
+                          out_count + s, recon, s == 0);
+      } else {
+        LATEC_DISPATCH_ND(decode_stage_kernel, false, uint16_t, grid, dim3(256), 0, stream, g,
+                          cls, ex, (const uint16_t *)codes, out_rank + lo, out_val + lo,
+                          out_count + s, recon, s == 0);
+      }
+    } else if (o->code_bytes == 2) {
+      hipLaunchKernelGGL((decode_stage_kernel<0, int64_t, false, uint16_t>), grid, dim3(256), 0,
+                         stream, g, cls, ex, (const uint16_t *)codes, out_rank + lo,
+                         out_val + lo, out_count + s, recon, s == 0);
+    } else {
+      hipLaunchKernelGGL((decode_stage_kernel<0, int64_t, false, uint32_t>), grid, dim3(256), 0,
+                         stream, g, cls, ex, (const uint32_t *)codes, out_rank + lo,
+                         out_val + lo, out_count + s, recon, s == 0);
+    }
     if ((rc = hip_check(hipGetLastError(), "decode_stage_kernel"))) return rc;
   }
   return LATEC_OK;
@@ -633,9 +720,9 @@ int interp_predict(latec_plan *p, int stage, const float *recon, float *pred,
   const StageDesc &st = p->stages[stage];
   if (!st.count) return LATEC_OK;
   GridArgs g = make_args(p, stage, 1.0, o);
-  dim3 grid(grid_blocks(max_class(st)), (unsigned)st.classes.size());
+  dim3 grid(row_blocks(st), (unsigned)st.classes.size());
   const uint32_t *ex = p->dev->excluded ? p->dev->excluded + p->dev->excl_off[stage] : nullptr;
-  hipLaunchKernelGGL(predict_stage_kernel, grid, dim3(256), 0, stream, g,
+  hipLaunchKernelGGL((predict_stage_kernel<0, int64_t, false>), grid, dim3(256), 0, stream, g,
                      p->dev->classes + p->dev->class_off[stage], ex, recon, pred);
   return hip_check(hipGetLastError(), "predict_stage_kernel");
 }
@@ -670,7 +757,7 @@ int quantize(const float *x, const float *pred, int64_t n, double eb,
   GridArgs g;
   memset(&g, 0, sizeof g);
   g.R = o->radius; g.eb = eb; g.round_mode = o->round_mode; g.span = o->span;
-  g.offset = o->offset;
+  g.offset = o->offset; g.w = 2.0 * eb; g.inv_w = 1.0 / g.w; g.feb = (float)eb;
   if (o->code_bytes == 2)
     hipLaunchKernelGGL(quantize_kernel<uint16_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
                        g, x, pred, n, (uint16_t *)codes, out_rank, out_val, out_count, hist);
@@ -689,7 +776,7 @@ int dequantize(const float *pred, const void *codes, int64_t n,
   if (!n) return LATEC_OK;
   GridArgs g;
   memset(&g, 0, sizeof g);
-  g.R = o->radius; g.eb = eb;
+  g.R = o->radius; g.eb = eb; g.w = 2.0 * eb;
   if (o->code_bytes == 2)
     hipLaunchKernelGGL(dequantize_kernel<uint16_t>, dim3(grid_blocks(n)), dim3(256), 0, stream,
                        g, pred, (const uint16_t *)codes, n, out_rank, out_val, out_count, recon);

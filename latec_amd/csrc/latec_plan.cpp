@@ -44,6 +44,10 @@ static ClassDesc make_class(int ndim, int parity, const int64_t *starts,
     c.size *= sizes[j];
     fastdiv_magic(sizes[j] ? (uint32_t)sizes[j] : 1u, &c.magic[j], &c.shift[j]);
   }
+  c.row_len = (uint32_t)sizes[ndim - 1];
+  c.nrows = c.row_len ? (uint32_t)(c.size / c.row_len) : 0;
+  c.tr = 1;
+  while (c.tr < 256 && (uint32_t)c.tr < c.row_len) c.tr <<= 1;
   return c;
 }
 

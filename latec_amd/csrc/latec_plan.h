@@ -25,6 +25,10 @@ struct ClassDesc {
   uint32_t sizes[LATEC_MAX_DIM];
   uint32_t magic[LATEC_MAX_DIM];  // fast division by sizes[j]
   uint32_t shift[LATEC_MAX_DIM];
+  uint32_t row_len;  // sizes[ndim-1]: points per row along the last axis
+  uint32_t nrows;    // size / row_len
+  int32_t tr;        // threads per row in a 256-thread block (power of two)
+  int32_t pad;
 };
 
 struct StageDesc {

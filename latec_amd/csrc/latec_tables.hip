@@ -22,6 +22,7 @@
 // compares its float32 sgemv results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <list>
@@ -655,11 +656,14 @@ int fit_hists(int nst, const uint32_t *d_hist, const void *d_codes,
     const int kexp = f.kexp;
     const int64_t A = (int64_t)2 << kexp, span = ((int64_t)1 << kexp) - 1;
     // float32 histogram over the reduced alphabet (rans.py:413-420)
-    std::vector<uint32_t> full(A);
-    std::vector<uint32_t> win((size_t)(2 * span + 1));
-    rc = hip_ok(hipMemcpyAsync(win.data(), d_hist + s * twoR + (R - span),
-                               sizeof(uint32_t) * (2 * span + 1), hipMemcpyDeviceToHost, st),
-                "copy");
+    // window of codes [R - span, R + span], clipped to the histogram [0, 2R)
+    std::vector<uint32_t> win((size_t)(2 * span + 1), 0u);
+    const int64_t c_lo = std::max<int64_t>((int64_t)R - span, 1);
+    const int64_t c_hi = std::min<int64_t>((int64_t)R + span, twoR - 1);
+    if (c_hi >= c_lo)
+      rc = hip_ok(hipMemcpyAsync(win.data() + (c_lo - ((int64_t)R - span)), d_hist + s * twoR + c_lo,
+                                 sizeof(uint32_t) * (c_hi - c_lo + 1), hipMemcpyDeviceToHost, st),
+                  "copy");
     if (rc) return rc;
     if ((rc = hip_ok(hipStreamSynchronize(st), "sync"))) return rc;
     std::vector<float> h32(A);

@@ -83,10 +83,12 @@ def dequantize(pred, codes, outlier_vals, eb, radius=DEFAULT_RADIUS):
     o = opts_for(radius)
     hc = codes.astype(np.uint16) if o.code_bytes == 2 else codes
     out = D.empty(len(pred), np.float32)
+    # keep every device buffer referenced until the kernel has finished
+    d_pred, d_codes = D.to_device(pred), D.to_device(hc)
+    d_rank = D.to_device(zeros if len(zeros) else np.zeros(1, np.uint32))
+    d_val = D.to_device(outlier_vals if len(outlier_vals) else np.zeros(1, np.float32))
     cnt = D.to_device(np.array([len(zeros)], np.uint32))
-    check(lib().latec_dequantize(ptr(D.to_device(pred)), ptr(D.to_device(hc)), len(pred),
-                                 ptr(D.to_device(zeros if len(zeros) else np.zeros(1, np.uint32))),
-                                 ptr(D.to_device(outlier_vals if len(outlier_vals)
-                                                 else np.zeros(1, np.float32))),
+    check(lib().latec_dequantize(ptr(d_pred), ptr(d_codes), len(pred), ptr(d_rank), ptr(d_val),
                                  ptr(cnt), float(eb), _opts_ptr(o), ptr(out), ptr(None)))
+    D.torch().cuda.synchronize()
     return D.to_host(out)
