@@ -85,7 +85,9 @@ typedef struct {
  *                   stages < s0 and zeros elsewhere.
  *  d_codes ........ C * N codes (u16/u32); stage i occupies
  *                   [C*off_i, C*off_i + C*n_i) channel-major, stage order.
- *  d_hist ......... n_stages * 2R u32 code histograms, zeroed by the caller.
+ *  d_hist ......... n_stages * latec_hist_words(R) u32, zeroed by the caller:
+ *                   per stage the 2R code bins then striped copies of the
+ *                   central window (folded by latec_fit_const).
  *  d_out_rank/val . outlier (index in the stage array, value) pairs in
  *                   arbitrary order; stage i's list starts at C*off_i.
  *  d_out_count .... n_stages u32 counters, zeroed by the caller.
@@ -120,7 +122,10 @@ int latec_interp_predict(latec_plan *p, int stage, const float *d_recon,
                          float *d_pred, const latec_interp_opts *o,
                          void *stream);
 
-/* Add the histogram of n codes (u16/u32) into d_hist (2R bins). */
+/* u32 words of one stage histogram (2R bins + striped window copies). */
+int64_t latec_hist_words(uint32_t radius);
+
+/* Add the histogram of n codes (u16/u32) into d_hist (latec_hist_words). */
 int latec_code_hist(const void *d_codes, int64_t n, int code_bytes,
                     uint32_t radius, uint32_t *d_hist, void *stream);
 
@@ -161,7 +166,8 @@ typedef struct {
 } latec_fit;
 
 /* choose_kexp + choose_const_level (rans.py:70-80, 395-430) for stages
- * [s0, s1) from the code histograms d_hist (n_stages * 2R); d_codes is read
+ * [s0, s1) from the code histograms d_hist (n_stages * latec_hist_words);
+ * the striped window copies are folded (and cleared) first; d_codes is read
  * only for stages whose largest bin reaches 2^24, to reproduce the
  * reference's per-2^22-tile float32 histogram accumulation.  Result: host
  * array fits[s1 - s0] (the call synchronizes `stream`). */

@@ -65,6 +65,7 @@ SIGNATURES = {
     "latec_interp_encode_graph": (i32, [p, p, p, p, p, p, p, p, p, p, p]),
     "latec_interp_predict": (i32, [p, i32, p, p, p, p]),
     "latec_code_hist": (i32, [p, i64, i32, u32, p, p]),
+    "latec_hist_words": (i64, [u32]),
     "latec_quantize": (i32, [p, p, i64, f64, p, p, p, p, p, p, p]),
     "latec_dequantize": (i32, [p, p, i64, p, p, p, f64, p, p, p]),
     "latec_level_weights": (i32, [i32, p, p]),

@@ -57,7 +57,7 @@ class Workspace:
         cb = 2 if radius <= 32768 else 4
         self.recon = D.empty(C * N, np.float32)
         self.codes = D.empty(C * N, np.uint16 if cb == 2 else np.uint32)
-        self.hist = D.empty(ns * 2 * radius, np.uint32)
+        self.hist = D.empty(ns * lib().latec_hist_words(radius), np.uint32)
         self.rank = D.empty(C * N, np.uint32)
         self.val = D.empty(C * N, np.float32)
         self.cnt = D.empty(ns, np.uint32)

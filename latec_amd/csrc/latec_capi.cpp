@@ -43,6 +43,7 @@ int build_cdfs(int, int, int, int, int, int, uint32_t *, hipStream_t);
 int fit_hists(int, const uint32_t *, const void *, const int64_t *, const int64_t *, uint32_t,
               int, int, latec_fit *, hipStream_t);
 int scale_to_level(const float *, int64_t, double, uint8_t *, hipStream_t);
+int64_t hist_words(uint32_t R);
 // latec_entropy.cpp
 int64_t rans_encode(const uint32_t *, int64_t, const uint32_t *, int64_t, uint32_t *, int64_t);
 int64_t rans_encode_levels(const uint32_t *, const uint8_t *, int64_t, const uint32_t *,
@@ -219,7 +220,7 @@ int latec_fit_const(latec_plan *p, int s0, int s1, const uint32_t *d_hist, const
     off.push_back((int64_t)p->channels * p->stages[s].offset);
     n.push_back((int64_t)p->channels * p->stages[s].count);
   }
-  return fit_hists(s1 - s0, d_hist + (int64_t)s0 * 2 * radius, d_codes, off.data(), n.data(),
+  return fit_hists(s1 - s0, d_hist + (int64_t)s0 * hist_words(radius), d_codes, off.data(), n.data(),
                    radius, code_bytes, precision, fits, S(stream));
 }
 
@@ -229,6 +230,8 @@ int latec_fit_codes(const uint32_t *d_hist, const void *d_codes, int64_t n, uint
   return fit_hists(1, d_hist, d_codes, &off, &n, radius, code_bytes, precision, fit,
                    S(stream));
 }
+
+int64_t latec_hist_words(uint32_t radius) { return hist_words(radius); }
 
 int latec_scale_to_level(const float *d_scale, int64_t n, double eb, uint8_t *d_levels,
                          void *stream) {

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -43,6 +44,13 @@ struct DeviceBufs {
     bool operator<(const GraphKey &o) const { return memcmp(a, o.a, sizeof a) < 0; }
   };
   std::map<GraphKey, hipGraphExec_t> graphs;
+  // lattice pyramid (anchor_block == 1): level k runs on the compact grid
+  // x[::s_k] at stride 1; sub[k] is its one-level plan, X/R its buffers
+  // (index K = the caller's full-resolution arrays).
+  int K = 0;
+  std::vector<latec_plan *> sub;
+  std::vector<float *> X, R;
+  std::vector<int64_t> npts;
 };
 
 static int hip_check(hipError_t e, const char *what) {
@@ -85,8 +93,16 @@ int plan_upload(latec_plan *p) {
   return LATEC_OK;
 }
 
+void plan_release(latec_plan *p);
+
 void plan_release(latec_plan *p) {
   if (!p->dev) return;
+  for (auto *q : p->dev->sub)
+    if (q) plan_release(q), delete q;
+  for (size_t k = 0; k < p->dev->X.size(); k++) {
+    if (p->dev->X[k]) (void)hipFree(p->dev->X[k]);
+    if (p->dev->R[k]) (void)hipFree(p->dev->R[k]);
+  }
   for (auto &kv : p->dev->graphs) (void)hipGraphExecDestroy(kv.second);
   if (p->dev->classes) (void)hipFree(p->dev->classes);
   if (p->dev->excluded) (void)hipFree(p->dev->excluded);
@@ -112,6 +128,7 @@ struct GridArgs {
   float feb;
   int32_t pad;
   int64_t n_excl;
+  int64_t slab_lo, slab_hi;  // axis-0 grid coordinate range processed
 };
 
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t magic, uint32_t shift) {
@@ -216,6 +233,18 @@ __device__ __forceinline__ uint32_t quantize_one(float x, float p, const GridArg
 }
 
 #define HIST_WIN 1024
+#define HIST_COPIES 256
+// Per-stage histogram layout: [2R main bins][HIST_COPIES x HIST_WIN copies of
+// the central window].  A block flushes its LDS window into copy
+// (block id % HIST_COPIES), so thousands of blocks do not serialize their
+// atomics on the same few bins; the fit folds the copies into the main bins.
+__device__ __forceinline__ void flush_hist(const uint32_t *lhist, uint32_t *__restrict__ ghist,
+                                           uint32_t R) {
+  const uint32_t bid = blockIdx.y * gridDim.x + blockIdx.x;
+  uint32_t *gwin = ghist + 2 * (size_t)R + (size_t)(bid % HIST_COPIES) * HIST_WIN;
+  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
+    if (lhist[i]) atomicAdd(&gwin[i], lhist[i]);
+}
 
 template <typename CodeT>
 __device__ __forceinline__ void emit(CodeT *__restrict__ codes, int64_t slot,
@@ -264,16 +293,28 @@ __device__ __forceinline__ void walk_rows(const ClassDesc &c, const GridArgs &g,
   const int nd = ND > 0 ? ND : g.ndim;
   const int tr = c.tr, rb = 256 / tr;
   const int lrow = threadIdx.x / tr, lcol = threadIdx.x % tr;
-  const uint32_t L = c.row_len, nrows = c.nrows;
+  const uint32_t L = c.row_len;
   const int64_t start_last = c.starts[nd - 1];
+  // rows of this class whose axis-0 coordinate lies in [slab_lo, slab_hi)
+  uint32_t r0 = 0, nrows = c.nrows;
+  if (nd > 1 && (g.slab_lo > 0 || g.slab_hi < g.shape[0])) {
+    const int64_t st = c.starts[0], stp = c.step, n0 = c.sizes[0];
+    int64_t m_lo = g.slab_lo <= st ? 0 : (g.slab_lo - st + stp - 1) / stp;
+    int64_t m_hi = g.slab_hi <= st ? 0 : (g.slab_hi - st + stp - 1) / stp;
+    m_lo = m_lo < n0 ? m_lo : n0;
+    m_hi = m_hi < n0 ? m_hi : n0;
+    const uint32_t rpp = c.nrows / (uint32_t)n0;
+    r0 = (uint32_t)m_lo * rpp;
+    nrows = m_hi > m_lo ? (uint32_t)(m_hi - m_lo) * rpp : 0;
+  }
   // XCD-aware order: the hardware deals blocks round-robin to the 8 XCDs, so
   // block b runs on XCD b % 8; renumbering makes each XCD walk one contiguous
   // band of rows, whose shared tap lines then hit in that XCD's L2.
   const uint32_t G = gridDim.x;
   const uint32_t lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
   for (uint32_t base = lb * rb; base < nrows; base += G * rb) {
-    const uint32_t row = base + lrow;
-    const bool rv = row < nrows;
+    const uint32_t row = r0 + base + lrow;
+    const bool rv = base + lrow < nrows;
     int64_t coord[NMAX];
     I fbase = 0;
     int64_t rbase = c.offset;
@@ -353,8 +394,7 @@ __global__ void __launch_bounds__(256) encode_stage_kernel(
     }
   });
   __syncthreads();
-  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
-    if (lhist[i]) atomicAdd(&ghist[win_lo + i], lhist[i]);
+  flush_hist(lhist, ghist, g.R);
 }
 
 // Decode: predict, then recon = f32(p + 2eb q) or the outlier value found by
@@ -419,8 +459,7 @@ __global__ void __launch_bounds__(256) quantize_kernel(
                 win_lo, active);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
-    if (lhist[i]) atomicAdd(&ghist[win_lo + i], lhist[i]);
+  flush_hist(lhist, ghist, g.R);
 }
 
 template <typename CodeT>
@@ -483,13 +522,58 @@ __global__ void __launch_bounds__(256) code_hist_kernel(const CodeT *__restrict_
       atomicAdd(&ghist[code], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
-    if (lhist[i]) atomicAdd(&ghist[win_lo + i], lhist[i]);
+  flush_hist(lhist, ghist, R);
+}
+
+// Lattice gather / scatter between a grid (shape `fine`) and its every-other
+// point lattice (shape ceil(fine / 2)): gather coarse[m] = fine[2m],
+// scatter fine[2m] = coarse[m].  One thread per coarse point, C channels.
+struct LatticeArgs {
+  int32_t ndim, channels;
+  int64_t fine[LATEC_MAX_DIM], coarse[LATEC_MAX_DIM];
+  int64_t nfine, ncoarse;
+};
+
+template <bool GATHER>
+__global__ void __launch_bounds__(256) lattice_kernel(LatticeArgs a, float *__restrict__ fine,
+                                                      float *__restrict__ coarse) {
+  // rows of the coarse grid along the last axis; tr lanes per row
+  const int nd = a.ndim;
+  const uint32_t L = (uint32_t)a.coarse[nd - 1];
+  int tr = 1;
+  while (tr < 256 && (uint32_t)tr < L) tr <<= 1;
+  const int rb = 256 / tr, lrow = threadIdx.x / tr, lcol = threadIdx.x % tr;
+  const uint32_t nrows = (uint32_t)(a.ncoarse / L);
+  for (uint32_t base = blockIdx.x * rb; base < nrows; base += gridDim.x * rb) {
+    const uint32_t row = base + lrow;
+    if (row >= nrows) continue;
+    uint32_t rem = row;
+    uint32_t fbase = 0, fs = (uint32_t)a.fine[nd - 1];
+    for (int j = nd - 2; j >= 0; j--) {
+      const uint32_t cj = (uint32_t)a.coarse[j];
+      const uint32_t m = rem % cj;
+      rem /= cj;
+      fbase += 2 * m * fs;
+      fs *= (uint32_t)a.fine[j];
+    }
+    const uint32_t cbase = row * L;
+    for (uint32_t col = lcol; col < L; col += tr)
+      for (int ch = 0; ch < a.channels; ch++) {
+        const uint32_t fi = (uint32_t)(ch * a.nfine) + fbase + 2 * col;
+        const uint32_t ci = (uint32_t)(ch * a.ncoarse) + cbase + col;
+        if (GATHER)
+          coarse[ci] = fine[fi];
+        else
+          fine[fi] = coarse[ci];
+      }
+  }
 }
 
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
+
+int64_t hist_words(uint32_t R) { return 2 * (int64_t)R + (int64_t)HIST_COPIES * HIST_WIN; }
 
 static int grid_blocks(int64_t work) {
   int64_t b = (work + 255) / 256;
@@ -498,12 +582,26 @@ static int grid_blocks(int64_t work) {
   return (int)(b < 1 ? 1 : b);
 }
 
-// Blocks along x for a stage: enough rows of every class, capped.
-static int row_blocks(const StageDesc &st) {
+// Rows of class c whose axis-0 coordinate lies in [lo, hi) (as walk_rows).
+static int64_t slab_rows(const ClassDesc &c, int ndim, int64_t n0, int64_t lo, int64_t hi) {
+  if (ndim < 2 || (lo <= 0 && hi >= n0)) return c.nrows;
+  const int64_t st = c.starts[0], stp = c.step, s0 = c.sizes[0];
+  int64_t m_lo = lo <= st ? 0 : (lo - st + stp - 1) / stp;
+  int64_t m_hi = hi <= st ? 0 : (hi - st + stp - 1) / stp;
+  m_lo = std::min(m_lo, s0);
+  m_hi = std::min(m_hi, s0);
+  return m_hi > m_lo ? (m_hi - m_lo) * (c.nrows / s0) : 0;
+}
+
+// Blocks along x for a stage (or an axis-0 slab of it): enough rows of every
+// class, capped, rounded to a multiple of the 8 XCDs.
+static int row_blocks(const StageDesc &st, int ndim = 1, int64_t n0 = 0, int64_t lo = 0,
+                      int64_t hi = 0) {
   int64_t b = 1;
   for (auto &c : st.classes) {
     const int64_t rb = 256 / c.tr;
-    b = std::max<int64_t>(b, ((int64_t)c.nrows + rb - 1) / rb);
+    const int64_t rows = hi > lo ? slab_rows(c, ndim, n0, lo, hi) : (int64_t)c.nrows;
+    b = std::max<int64_t>(b, (rows + rb - 1) / rb);
   }
   b = std::min<int64_t>(b, 8192);
   return (int)((b + 7) / 8 * 8);  // a multiple of the 8 XCDs
@@ -554,6 +652,8 @@ static GridArgs make_args(const latec_plan *p, int stage, double eb,
   g.span = o->span;
   g.offset = o->offset;
   g.n_excl = (int64_t)st.excluded.size();
+  g.slab_lo = 0;
+  g.slab_hi = p->shape[0];
   return g;
 }
 
@@ -573,6 +673,12 @@ static KSel pick(const latec_plan *p) {
     default: hipLaunchKernelGGL((KERNEL<4, uint32_t, BLK1, CODET>), __VA_ARGS__); break; \
   }
 
+static bool pyramid_ok(const latec_plan *p);
+static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon, void *codes,
+                       uint32_t *hist, uint32_t *out_rank, float *out_val,
+                       uint32_t *out_count, const double *ebs, const latec_interp_opts *o,
+                       hipStream_t stream);
+
 int interp_encode(latec_plan *p, int s0, int s1, const float *x, float *recon,
                   void *codes, uint32_t *hist, uint32_t *out_rank,
                   float *out_val, uint32_t *out_count, const double *ebs,
@@ -584,7 +690,16 @@ int interp_encode(latec_plan *p, int s0, int s1, const float *x, float *recon,
     return LATEC_E_ARG;
   }
   if ((rc = plan_upload(p))) return rc;
-  const int64_t twoR = 2 * (int64_t)o->radius;
+  if (s0 == 0 && s1 == (int)p->stages.size() && pyramid_ok(p) && !getenv("LATEC_NO_PYRAMID")) {
+    for (int s = 0; s < s1; s++)
+      if (p->stages[s].count && !(ebs[s] > 0)) {
+        set_error("error bound must be > 0");
+        return LATEC_E_ARG;
+      }
+    return pyramid_run(p, true, x, recon, codes, hist, out_rank, out_val, out_count, ebs, o,
+                       stream);
+  }
+  const int64_t twoR = hist_words(o->radius);
   const KSel sel = pick(p);
   for (int s = s0; s < s1; s++) {
     const StageDesc &st = p->stages[s];
@@ -634,6 +749,9 @@ int interp_decode(latec_plan *p, int s0, int s1, const void *codes,
     return LATEC_E_ARG;
   }
   if ((rc = plan_upload(p))) return rc;
+  if (s0 == 0 && s1 == (int)p->stages.size() && pyramid_ok(p) && !getenv("LATEC_NO_PYRAMID"))
+    return pyramid_run(p, false, nullptr, recon, (void *)codes, nullptr, (uint32_t *)out_rank,
+                       (float *)out_val, (uint32_t *)out_count, ebs, o, stream);
   const KSel sel = pick(p);
   for (int s = s0; s < s1; s++) {
     const StageDesc &st = p->stages[s];
@@ -667,12 +785,183 @@ int interp_decode(latec_plan *p, int s0, int s1, const void *codes,
   return LATEC_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// lattice pyramid + slab wavefront
+// ---------------------------------------------------------------------------
+
+static bool pyramid_ok(const latec_plan *p) {
+  return p->block == 1 && p->ndim <= 4 && (int64_t)p->channels * p->npoints < 0xFFFFFFFFll;
+}
+
+static int pyramid_setup(latec_plan *p) {
+  int rc = plan_upload(p);
+  if (rc) return rc;
+  DeviceBufs *d = p->dev;
+  if (d->K) return LATEC_OK;
+  int K = 0;
+  while ((1 << (K + 1)) <= p->stride) K++;
+  K = std::min(K, p->levels);
+  d->sub.assign(K + 1, nullptr);
+  d->X.assign(K + 1, nullptr);
+  d->R.assign(K + 1, nullptr);
+  d->npts.assign(K + 1, 0);
+  for (int k = 1; k <= K; k++) {
+    const int64_t sk = (int64_t)p->stride >> k;
+    auto *q = new latec_plan();
+    q->ndim = p->ndim;
+    q->channels = p->channels;
+    q->levels = 1;
+    q->stride = 2;
+    q->block = 1;
+    for (int j = 0; j < p->ndim; j++) q->shape[j] = (p->shape[j] + sk - 1) / sk;
+    if ((rc = plan_build(q)) || (rc = plan_upload(q))) {
+      delete q;
+      return rc;
+    }
+    d->sub[k] = q;
+    d->npts[k] = q->npoints;
+    if (k < K) {
+      const size_t bytes = sizeof(float) * (size_t)p->channels * q->npoints;
+      if ((rc = hip_check(hipMalloc(&d->X[k], bytes), "hipMalloc")) ||
+          (rc = hip_check(hipMalloc(&d->R[k], bytes), "hipMalloc")))
+        return rc;
+    }
+  }
+  d->K = K;
+  return LATEC_OK;
+}
+
+static int lattice(const latec_plan *fine_p, const latec_plan *coarse_p, float *fine,
+                   float *coarse, bool gather, hipStream_t stream) {
+  LatticeArgs a;
+  memset(&a, 0, sizeof a);
+  a.ndim = fine_p->ndim;
+  a.channels = fine_p->channels;
+  for (int j = 0; j < a.ndim; j++) {
+    a.fine[j] = fine_p->shape[j];
+    a.coarse[j] = coarse_p->shape[j];
+  }
+  a.nfine = fine_p->npoints;
+  a.ncoarse = coarse_p->npoints;
+  const int64_t L = std::max<int64_t>(coarse_p->shape[a.ndim - 1], 1);
+  int tr = 1;
+  while (tr < 256 && tr < L) tr <<= 1;
+  const int64_t rows = a.ncoarse / L, rb = 256 / tr;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((rows + rb - 1) / rb, 8192));
+  if (gather)
+    hipLaunchKernelGGL(lattice_kernel<true>, dim3(blocks), dim3(256), 0, stream, a, fine, coarse);
+  else
+    hipLaunchKernelGGL(lattice_kernel<false>, dim3(blocks), dim3(256), 0, stream, a, fine, coarse);
+  return hip_check(hipGetLastError(), "lattice_kernel");
+}
+
+// One stage of a sub-plan addressed into the global plan's arrays.
+static int launch_stage(const latec_plan *gp, const latec_plan *q, int js, int gs, bool encode,
+                        const float *x, float *recon, void *codes, uint32_t *hist,
+                        uint32_t *out_rank, float *out_val, uint32_t *out_count,
+                        const double *ebs, const latec_interp_opts *o, int64_t slab_lo,
+                        int64_t slab_hi, hipStream_t stream) {
+  const StageDesc &st = q->stages[js];
+  if (!st.count) return LATEC_OK;
+  GridArgs g = make_args(q, js, ebs[gs], o);
+  const StageDesc &gst = gp->stages[gs];
+  g.code_base = (int64_t)gp->channels * gst.offset;
+  g.slab_lo = slab_lo;
+  g.slab_hi = slab_hi;
+  dim3 grid(row_blocks(st, q->ndim, q->shape[0], slab_lo, slab_hi),
+            (unsigned)st.classes.size());
+  const ClassDesc *cls = q->dev->classes + q->dev->class_off[js];
+  const int64_t lo = (int64_t)gp->channels * gst.offset;
+  const int64_t twoR = hist_words(o->radius);
+  const latec_plan *p = q;  // for LATEC_DISPATCH_ND
+  if (encode) {
+    if (o->code_bytes == 2) {
+      LATEC_DISPATCH_ND(encode_stage_kernel, true, uint16_t, grid, dim3(256), 0, stream, g, cls,
+                        nullptr, x, recon, (uint16_t *)codes, hist + gs * twoR, out_rank + lo,
+                        out_val + lo, out_count + gs, gs == 0);
+    } else {
+      hipLaunchKernelGGL((encode_stage_kernel<0, int64_t, false, uint32_t>), grid, dim3(256), 0,
+                         stream, g, cls, nullptr, x, recon, (uint32_t *)codes, hist + gs * twoR,
+                         out_rank + lo, out_val + lo, out_count + gs, gs == 0);
+    }
+  } else {
+    if (o->code_bytes == 2) {
+      LATEC_DISPATCH_ND(decode_stage_kernel, true, uint16_t, grid, dim3(256), 0, stream, g, cls,
+                        nullptr, (const uint16_t *)codes, out_rank + lo, out_val + lo,
+                        out_count + gs, recon, gs == 0);
+    } else {
+      hipLaunchKernelGGL((decode_stage_kernel<0, int64_t, false, uint32_t>), grid, dim3(256), 0,
+                         stream, g, cls, nullptr, (const uint32_t *)codes, out_rank + lo,
+                         out_val + lo, out_count + gs, recon, gs == 0);
+    }
+  }
+  return hip_check(hipGetLastError(), encode ? "encode_stage_kernel" : "decode_stage_kernel");
+}
+
+// Slab wavefront (opt-in, LATEC_SLAB_MB=<MiB per slab>): a level whose grid
+// exceeds four slabs runs its sub-stages slab by slab along axis 0 (weight-w
+// stages trail by w-1 slabs) so that each slab stays cache resident.
+
+static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon, void *codes,
+                       uint32_t *hist, uint32_t *out_rank, float *out_val,
+                       uint32_t *out_count, const double *ebs, const latec_interp_opts *o,
+                       hipStream_t stream) {
+  int rc = pyramid_setup(p);
+  if (rc) return rc;
+  DeviceBufs *d = p->dev;
+  const int K = d->K, nd = p->ndim, per = 2 * nd - 1;
+  auto X = [&](int k) { return k == K ? x : (const float *)d->X[k]; };
+  auto R = [&](int k) { return k == K ? recon : d->R[k]; };
+  if (encode) {  // lattice pyramid of the input, fine to coarse
+    for (int k = K - 1; k >= 1; k--) {
+      const latec_plan *fine = k + 1 == K ? p : d->sub[k + 1];
+      if ((rc = lattice(fine, d->sub[k], (float *)X(k + 1), d->X[k], true, stream))) return rc;
+    }
+  }
+  // anchors: stage 0 of the first level's plan (its even lattice)
+  if ((rc = launch_stage(p, d->sub[1], 0, 0, encode, X(1), R(1), codes, hist, out_rank, out_val,
+                         out_count, ebs, o, 0, d->sub[1]->shape[0], stream)))
+    return rc;
+  for (int k = 1; k <= K; k++) {
+    const latec_plan *q = d->sub[k];
+    if (k > 1 && (rc = lattice(q, d->sub[k - 1], R(k), d->R[k - 1], false, stream))) return rc;
+    const int g0 = 1 + (k - 1) * per;
+    const int64_t n0 = q->shape[0];
+    const int64_t plane = (int64_t)p->channels * (q->npoints / std::max<int64_t>(n0, 1)) * 8;
+    int64_t T = n0;
+    static const int64_t slab_bytes = getenv("LATEC_SLAB_MB") ? atoll(getenv("LATEC_SLAB_MB")) << 20 : 0;
+    if (slab_bytes && nd > 1 && (int64_t)p->channels * q->npoints * 8 > 4 * slab_bytes)
+      T = std::max<int64_t>(8, (slab_bytes / std::max<int64_t>(plane, 1)) & ~1ll);
+    const int64_t nslab = (n0 + T - 1) / T;
+    for (int64_t t = 0; t < nslab + nd - 1; t++) {
+      for (int w = 1; w <= nd; w++) {
+        const int64_t sl = t - (w - 1);
+        if (sl < 0 || sl >= nslab) continue;
+        const int64_t lo = sl * T, hi = std::min(n0, lo + T);
+        if (w == 1) {
+          for (int a = 0; a < nd; a++)
+            if ((rc = launch_stage(p, q, 1 + a, g0 + a, encode, X(k), R(k), codes, hist, out_rank,
+                                   out_val, out_count, ebs, o, lo, hi, stream)))
+              return rc;
+        } else if ((rc = launch_stage(p, q, 1 + nd + (w - 2), g0 + nd + (w - 2), encode, X(k),
+                                      R(k), codes, hist, out_rank, out_val, out_count, ebs, o,
+                                      lo, hi, stream))) {
+          return rc;
+        }
+      }
+    }
+  }
+  return LATEC_OK;
+}
+
 int interp_encode_graph(latec_plan *p, const float *x, float *recon, void *codes,
                         uint32_t *hist, uint32_t *out_rank, float *out_val,
                         uint32_t *out_count, const double *ebs,
                         const latec_interp_opts *o, hipStream_t stream) {
   int rc = plan_upload(p);
   if (rc) return rc;
+  if (pyramid_ok(p) && (rc = pyramid_setup(p))) return rc;  // no allocation inside capture
   // The graph bakes in pointers, ebs and options: key on all of them.
   DeviceBufs::GraphKey key;
   memset(&key, 0, sizeof key);

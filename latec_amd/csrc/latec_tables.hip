@@ -525,10 +525,29 @@ struct StageScan {  // per stage, from the full-alphabet histogram
   uint32_t pad;
 };
 
+int64_t hist_words(uint32_t R);
+
+// Fold the striped window copies into the main bins (one block per stage).
+__global__ void __launch_bounds__(1024) fold_hist_kernel(uint32_t *hist, int64_t stride,
+                                                         uint32_t R, int copies, int win) {
+  uint32_t *h = hist + blockIdx.x * stride;
+  const uint32_t win_lo = R > (uint32_t)win / 2 ? R - win / 2 : 0;
+  for (int i = threadIdx.x; i < win; i += blockDim.x) {
+    uint32_t acc = 0;
+    for (int c = 0; c < copies; c++) {
+      uint32_t *cp = h + 2 * (int64_t)R + (int64_t)c * win + i;
+      acc += *cp;
+      *cp = 0;
+    }
+    if (acc && win_lo + i < 2 * R) h[win_lo + i] += acc;
+  }
+}
+
 // one block per stage: kexp and the largest bin
-__global__ void __launch_bounds__(256) scan_hist_kernel(const uint32_t *hist, int64_t twoR,
+__global__ void __launch_bounds__(256) scan_hist_kernel(const uint32_t *hist, int64_t stride,
                                                         uint32_t R, StageScan *out) {
-  const uint32_t *h = hist + blockIdx.x * twoR;
+  const uint32_t *h = hist + blockIdx.x * stride;
+  const int64_t twoR = 2 * (int64_t)R;
   uint32_t peak = 0, mx = 0;
   bool any = false;
   for (int64_t c = 1 + threadIdx.x; c < twoR; c += blockDim.x) {
@@ -631,11 +650,14 @@ int fit_hists(int nst, const uint32_t *d_hist, const void *d_codes,
               int code_bytes, int precision, latec_fit *fits, hipStream_t st) {
   if (nst <= 0) return LATEC_OK;
   const int64_t twoR = 2 * (int64_t)R;
+  const int64_t HS = hist_words(R);
   int rc;
   StageScan *d_scan;
+  hipLaunchKernelGGL(fold_hist_kernel, dim3(nst), dim3(1024), 0, st, (uint32_t *)d_hist, HS, R,
+                     256, 1024);
   if ((rc = hip_ok(hipMallocAsync((void **)&d_scan, sizeof(StageScan) * nst, st), "hipMallocAsync")))
     return rc;
-  hipLaunchKernelGGL(scan_hist_kernel, dim3(nst), dim3(256), 0, st, d_hist, twoR, R, d_scan);
+  hipLaunchKernelGGL(scan_hist_kernel, dim3(nst), dim3(256), 0, st, d_hist, HS, R, d_scan);
   std::vector<StageScan> scan(nst);
   rc = hip_ok(hipMemcpyAsync(scan.data(), d_scan, sizeof(StageScan) * nst, hipMemcpyDeviceToHost, st),
               "copy");
@@ -661,7 +683,7 @@ int fit_hists(int nst, const uint32_t *d_hist, const void *d_codes,
     const int64_t c_lo = std::max<int64_t>((int64_t)R - span, 1);
     const int64_t c_hi = std::min<int64_t>((int64_t)R + span, twoR - 1);
     if (c_hi >= c_lo)
-      rc = hip_ok(hipMemcpyAsync(win.data() + (c_lo - ((int64_t)R - span)), d_hist + s * twoR + c_lo,
+      rc = hip_ok(hipMemcpyAsync(win.data() + (c_lo - ((int64_t)R - span)), d_hist + s * HS + c_lo,
                                  sizeof(uint32_t) * (c_hi - c_lo + 1), hipMemcpyDeviceToHost, st),
                   "copy");
     if (rc) return rc;

@@ -41,7 +41,7 @@ def quantize_device(dx, dpred, n: int, eb: float, o: InterpOpts, d_hist=None):
     rank = D.empty(max(n, 1), np.uint32)
     val = D.empty(max(n, 1), np.float32)
     cnt = D.zeros(1, np.uint32)
-    hist = d_hist if d_hist is not None else D.zeros(2 * o.radius, np.uint32)
+    hist = d_hist if d_hist is not None else D.zeros(lib().latec_hist_words(o.radius), np.uint32)
     check(lib().latec_quantize(ptr(dx), ptr(dpred), int(n), float(eb), _opts_ptr(o),
                                ptr(codes), ptr(rank), ptr(val), ptr(cnt), ptr(hist), ptr(None)))
     k = int(cnt.item())

@@ -201,7 +201,7 @@ def fit_codes_device(d_codes, n: int, radius: int, code_bytes: int, d_hist=None,
                      precision: int = DEFAULT_PRECISION) -> Fit:
     """choose_kexp + choose_const_level on device codes (rans.py:70-80, 395-430)."""
     if d_hist is None:
-        d_hist = D.zeros(2 * radius, np.uint32)
+        d_hist = D.zeros(lib().latec_hist_words(radius), np.uint32)
         check(lib().latec_code_hist(ptr(d_codes), int(n), code_bytes, int(radius), ptr(d_hist),
                                     ptr(None)))
     f = Fit()
