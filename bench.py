@@ -25,6 +25,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "encode_c2_512cubed_hbm_traffic.md")
+
+
+def profiled_traffic(shape):
+    """HBM bytes of one encode (all stage launches) from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary of the 512^3 workload, or None
+    for another shape or when the summary is absent."""
+    if tuple(shape) != (512, 512, 512) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_encode:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def parse():
@@ -203,8 +216,11 @@ def main():
         "max_abs_err": max_err,
         "kernel_ms": {"encode": t_kenc * 1e3, "decode": t_kdec * 1e3},
         "roofline": {"bound": "hbm", "achieved": ach_enc / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": ach_enc / HBM_PEAK, "traffic": None,
-                     "kernel": "encode_stage_kernel (26 launches, 10 B/value + 4 B/outlier)",
+                     "unit": "GB/s", "frac": ach_enc / HBM_PEAK,
+                     "traffic": profiled_traffic(shape), "algorithmic_bytes": enc_bytes,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+                     "kernel": "one encode: encode_stage_kernel x26 + lattice_kernel x8 "
+                               "(10 B/value + 4 B/outlier)",
                      "decode": {"achieved": ach_dec / 1e9, "frac": ach_dec / HBM_PEAK}},
         "cpu_baseline": cpu,
     }
