@@ -249,6 +249,111 @@ int64_t latec_zstd_content_size(const uint8_t *src, int64_t n);
 int64_t latec_zstd_decompress(const uint8_t *src, int64_t n, uint8_t *dst,
                               int64_t cap);
 
+/* ------------------------------------------------------------------ */
+/* GNN predictor (latec/gnn_predictor.py, latec/gnn_codec.py).         */
+/* ------------------------------------------------------------------ */
+
+typedef struct latec_gnn latec_gnn;
+
+#define LATEC_GNN_FP32 0 /* FP32 MFMA GEMMs, FP32 pools and head */
+#define LATEC_GNN_BF16 1 /* BF16 MFMA GEMMs (FP32 accumulate), FP32 head */
+
+/* Floats in the packed parameter block of a (d, agg_level) model: the
+ * reference checkpoint's state_dict tensors flattened in module order
+ * (build_model, gnn_predictor.py:593-933; init, dir_state, dir, bidir,
+ * line_pool, axis_pool, head, mix; weight before bias; AttnPool's wk, wv,
+ * null_k, null_v). */
+int64_t latec_gnn_param_count(int d, int agg_level);
+
+/* Load a model for rank-`ndim` fields (_load_inference_model,
+ * gnn_predictor.py:945-981, after the checkpoint is read on the host).
+ * d must be 32 or 64.  The handle owns its device weights and a workspace;
+ * calls on one handle must not overlap. */
+int latec_gnn_create(int d, int agg_level, int ndim, int numerics,
+                     const float *params, int64_t n_params, latec_gnn **out);
+void latec_gnn_destroy(latec_gnn *g);
+/* Half-directions (lines) of the model at its rank (half_directions,
+ * gnn_predictor.py:117-136). */
+int latec_gnn_n_lines(const latec_gnn *g);
+
+/* Where a chunk's embedding frame sits (ChunkedGNNPredictor frames,
+ * gnn_predictor.py:1010-1356): the frame is the chunk grown by `pad` cells on
+ * every side; halo cells are usable when inside the tensor and in a coded
+ * chunk (d_coded, raster chunk order) or anchors. */
+typedef struct latec_gnn_frame {
+  int32_t pad;       /* 0 (one chunk, no halo) or the anchor stride */
+  int32_t has_halo;
+  int64_t origin[LATEC_MAX_DIM];  /* chunk origin in the tensor */
+  int64_t tshape[LATEC_MAX_DIM];  /* tensor shape */
+  int64_t edges[LATEC_MAX_DIM];   /* chunk edges */
+  int64_t grid[LATEC_MAX_DIM];    /* chunks per axis */
+  const uint8_t *d_coded;         /* per chunk: already coded (halo only) */
+} latec_gnn_frame;
+
+/* Geometry of one stage of a chunk's local schedule (`p`: block 1, stride
+ * 2^levels, shape = chunk shape).  Replaces _StageGeom / _ChunkGeoms /
+ * _CompactGeom (gnn_predictor.py:270-443, 1010-1310): per stage point in
+ * stage order, d_qidx = frame flat index, d_gidx = tensor flat index and
+ * d_steps[m * L + l] = sp | sn << 16, the step to the nearest usable known
+ * sample along +/- line l within `stride` steps (0 = none). */
+int latec_gnn_geometry(latec_gnn *g, latec_plan *p, int stage,
+                       const latec_gnn_frame *fr, int64_t *d_qidx,
+                       int64_t *d_gidx, int32_t *d_steps, void *stream);
+
+/* Message pass + line pool of M query points (GNN.embed,
+ * gnn_predictor.py:818-906): d_E is (frame rows, ndim, d) in the model's
+ * storage type (bf16 or f32), frame_shape the host frame dims; writes the
+ * per-axis contexts d_ctx (M, ndim, d) in the storage type. */
+int latec_gnn_embed(latec_gnn *g, const void *d_E, const int64_t *frame_shape,
+                    const int32_t *d_steps, const int64_t *d_qidx, int64_t M,
+                    void *d_ctx, void *stream);
+
+/* Axis pool + PredHead (head_of, gnn_predictor.py:749-789, 922-931) and the
+ * predictor's output map (predict_wave_stage, gnn_predictor.py:1737-1774):
+ * pred = clamp(mu * (vmax - vmin) + vmin), scale = 2^log_b * (vmax - vmin);
+ * eb is the normalized bound.  d_mu / d_logb may be NULL. */
+int latec_gnn_head(latec_gnn *g, const void *d_ctx, int64_t M, double eb,
+                   float vmin, float vmax, float *d_pred, float *d_scale,
+                   float *d_mu, float *d_logb, void *stream);
+
+/* finalize (gnn_predictor.py:908-920, stage_forward :984-1000):
+ * E[qidx[m], k] = mix(ctx[m, k], init(clamp(recon[gidx[m]]))) . */
+int latec_gnn_finalize(latec_gnn *g, void *d_E, const void *d_ctx,
+                       const int64_t *d_qidx, const int64_t *d_gidx,
+                       const float *d_recon, int64_t M, float vmin, float vmax,
+                       void *stream);
+
+/* Halo rows: E[rows[m], k] = init(clamp(recon[gidx[m]])) for every axis k
+ * (value_halo_embed, gnn_predictor.py:1357-1366). */
+int latec_gnn_halo(latec_gnn *g, void *d_E, const int64_t *d_rows,
+                   const int64_t *d_gidx, const float *d_recon, int64_t M,
+                   float vmin, float vmax, void *stream);
+
+/* Gate rate proxy (_laplace_bits_t + the bucket sums of _gate_select_t,
+ * gnn_codec.py:777-797, 920-971): costs[c * 13 + b] = summed bits of
+ * candidate c in scale bucket b (c = 0: the model; c = 1 + 4 * kind + s:
+ * interpolation candidate `kind` with scale shift 2s).  thr63 / grid64 are
+ * the gate's scale snapping tables (gnn_codec.py:880-917).  Sums are in a
+ * fixed order. */
+int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
+                         const float *d_scale, const float *const *d_ip,
+                         int n_interp, int64_t n, double eb, uint32_t radius,
+                         const float *thr63, const double *grid64,
+                         double *costs, void *stream);
+
+/* _gate_apply_t (gnn_codec.py:974-995): where the scale is below (dir 0) or
+ * at/above (dir 1) `threshold`, pred = ip and scale *= mult. */
+int latec_gnn_gate_apply(float *d_pred, float *d_scale, const float *d_ip,
+                         int64_t n, double threshold, float mult, int dir,
+                         void *stream);
+
+/* dst[i] = src[idx[i]] / dst[idx[i]] = src[i] on device (the stage views of
+ * _stage_values / _write_stage, gnn_codec.py:685-701). */
+int latec_gather_f32(const float *d_src, const int64_t *d_idx, int64_t n,
+                     float *d_dst, void *stream);
+int latec_scatter_f32(const float *d_src, const int64_t *d_idx, int64_t n,
+                      float *d_dst, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,8 +47,20 @@ class Fit(C.Structure):
     ]
 
 
+class GnnFrame(C.Structure):
+    _fields_ = [
+        ("pad", C.c_int32),
+        ("has_halo", C.c_int32),
+        ("origin", C.c_int64 * 8),
+        ("tshape", C.c_int64 * 8),
+        ("edges", C.c_int64 * 8),
+        ("grid", C.c_int64 * 8),
+        ("d_coded", C.c_void_p),
+    ]
+
+
 p = C.c_void_p
-i64, i32, u32, f64 = C.c_int64, C.c_int, C.c_uint32, C.c_double
+i64, i32, u32, f64, f32 = C.c_int64, C.c_int, C.c_uint32, C.c_double, C.c_float
 
 # name -> (restype, argtypes); every symbol declared in include/latec.h
 SIGNATURES = {
@@ -87,6 +99,19 @@ SIGNATURES = {
     "latec_zstd_compress": (i64, [p, i64, i32, p, i64]),
     "latec_zstd_content_size": (i64, [p, i64]),
     "latec_zstd_decompress": (i64, [p, i64, p, i64]),
+    "latec_gnn_param_count": (i64, [i32, i32]),
+    "latec_gnn_create": (i32, [i32, i32, i32, i32, p, i64, C.POINTER(p)]),
+    "latec_gnn_destroy": (None, [p]),
+    "latec_gnn_n_lines": (i32, [p]),
+    "latec_gnn_geometry": (i32, [p, p, i32, p, p, p, p, p]),
+    "latec_gnn_embed": (i32, [p, p, p, p, p, i64, p, p]),
+    "latec_gnn_head": (i32, [p, p, i64, f64, f32, f32, p, p, p, p, p]),
+    "latec_gnn_finalize": (i32, [p, p, p, p, p, p, i64, f32, f32, p]),
+    "latec_gnn_halo": (i32, [p, p, p, p, p, i64, f32, f32, p]),
+    "latec_gnn_gate_costs": (i32, [p, p, p, p, p, i32, i64, f64, u32, p, p, p, p]),
+    "latec_gnn_gate_apply": (i32, [p, p, p, i64, f64, f32, i32, p]),
+    "latec_gather_f32": (i32, [p, p, i64, p, p]),
+    "latec_scatter_f32": (i32, [p, p, i64, p, p]),
 }
 
 _lib = None
