@@ -354,6 +354,53 @@ int latec_gather_f32(const float *d_src, const int64_t *d_idx, int64_t n,
 int latec_scatter_f32(const float *d_src, const int64_t *d_idx, int64_t n,
                       float *d_dst, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* GNN trainer ops (scripts/train_gnn.py:195-230, 373-459): FP32       */
+/* forward/backward kernels composed by torch.autograd.Functions.      */
+/* ------------------------------------------------------------------ */
+
+/* out[r, :] = src[idx[r], :] (rows of width w) / dsrc[idx[r], :] += dout[r, :]
+ * (atomic) / E[idx[r], :] = vals[r, :] (vals NULL: zeros). */
+int latec_train_gather(const float *d_src, int64_t w, const int64_t *d_idx,
+                       int64_t n, float *d_out, void *stream);
+int latec_train_scatter_add(const float *d_dout, int64_t w,
+                            const int64_t *d_idx, int64_t n, float *d_dsrc,
+                            void *stream);
+int latec_train_write_rows(float *d_E, int64_t w, const int64_t *d_idx,
+                           int64_t n, const float *d_vals, void *stream);
+/* C (M x N, row stride ldc) [+]= A * B (+ bias[N]) with generic strides
+ * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]; split-K partials
+ * in d_work (work_floats floats), summed in a fixed order. */
+int latec_train_gemm(int64_t M, int64_t N, int64_t K, const float *d_A,
+                     int64_t sam, int64_t sak, const float *d_B, int64_t sbk,
+                     int64_t sbn, float *d_C, int64_t ldc, const float *d_bias,
+                     int accumulate, float *d_work, int64_t work_floats,
+                     void *stream);
+/* out[n] [+]= sum_m X[m, n] in a fixed order. */
+int latec_train_colsum(int64_t M, int64_t N, const float *d_X, float *d_out,
+                       int accumulate, float *d_work, int64_t work_floats,
+                       void *stream);
+int latec_train_gelu(const float *d_x, float *d_y, int64_t n, void *stream);
+int latec_train_gelu_bwd(const float *d_x, const float *d_dy, float *d_dx,
+                         int64_t n, void *stream);
+/* AttnPool over CSR row segments + a null token (gnn_predictor.py:720-747):
+ * forward saves the softmax weights (w_rows, w_null) for the backward. */
+int latec_train_attn(int64_t n_items, const int64_t *d_off, const float *d_s,
+                     const float *d_v, int d, float null_s,
+                     const float *d_null_v, float *d_out, float *d_w_rows,
+                     float *d_w_null, void *stream);
+int latec_train_attn_bwd(int64_t n_items, const int64_t *d_off,
+                         const float *d_v, int d, const float *d_null_v,
+                         const float *d_w_rows, const float *d_w_null,
+                         const float *d_dout, float *d_ds, float *d_dv,
+                         float *d_dnull_s, float *d_dnull_v, void *stream);
+/* Head loss: mu = sigmoid(o0), log_b = log2(eb) + clamp(o1, -4, 12),
+ * bits = discretized-Laplace code length (+ mse_w (mu - t)^2); writes bits,
+ * d bits / d o (M x 2) and mu (may be NULL). */
+int latec_train_nll(int64_t M, const float *d_o, const float *d_t,
+                    const float *d_eb, float mse_w, float *d_bits, float *d_dO,
+                    float *d_mu, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

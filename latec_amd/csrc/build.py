@@ -23,7 +23,8 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ZSTD_INC = os.environ.get("LATEC_ZSTD_INC", "/opt/conda/include")
 ARCH = os.environ.get("LATEC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["latec_interp.hip", "latec_tables.hip", "latec_gnn.hip", "latec_capi.cpp"]
+HIP_SOURCES = ["latec_interp.hip", "latec_tables.hip", "latec_gnn.hip", "latec_train.hip",
+               "latec_capi.cpp"]
 CXX_SOURCES = ["latec_plan.cpp", "latec_entropy.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ZSTD_INC}",

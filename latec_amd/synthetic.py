@@ -194,3 +194,28 @@ def mixed_field(shape, kind, seed, device="cpu"):
     else:
         x = _standardize(grf(shape, 4.0, g, device) + jumps(shape, 3, g, device))
     return minmax(x).to(_torch().float32)
+
+
+def training_field(shape, corr_range, gen, device, turbulence_frac=0.5, n_jumps=3):
+    """One training sample (the trainer's generator family,
+    latec/synthetic_data.py:176-332 as a model): per-axis correlation lengths
+    uniform in ``corr_range``, a ``turbulence_frac`` share of k^-5/3
+    turbulence, up to ``n_jumps`` hyperplane discontinuities; [0, 1] float32."""
+    t = _torch()
+    lo, hi = corr_range
+    corr = (lo + (hi - lo) * t.rand(len(shape), dtype=t.float64, generator=gen,
+                                    device=gen.device)).cpu().numpy()
+    x = (1.0 - turbulence_frac) * grf(shape, corr, gen, device)
+    if turbulence_frac > 0:
+        x = x + turbulence_frac * turbulence(shape, gen, device)
+    k = int(t.randint(0, n_jumps + 1, (1,), generator=gen, device=gen.device).item()) if n_jumps else 0
+    if k:
+        x = _standardize(x) + jumps(shape, k, gen, device)
+    return minmax(x).to(t.float32)
+
+
+def training_batch(batch, shape, corr_range, gen, device, turbulence_frac=0.5, n_jumps=3):
+    """(batch, prod(shape)) float32 on ``device``."""
+    t = _torch()
+    return t.stack([training_field(shape, corr_range, gen, device, turbulence_frac,
+                                   n_jumps).reshape(-1) for _ in range(batch)])
