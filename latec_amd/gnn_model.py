@@ -83,7 +83,7 @@ def load_checkpoint(path):
         t = sd[name].detach().to("cpu").float().numpy()
         if tuple(t.shape) != shape:
             raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shape}")
-        params[name] = np.ascontiguousarray(t)
+        params[name] = np.array(t, dtype=np.float32, copy=True).reshape(shape)
     return d, agg, params
 
 
@@ -91,7 +91,7 @@ def save_checkpoint(path, d: int, agg: int, params: dict, extra: dict | None = N
     """Write a v7 checkpoint the reference's loader also accepts."""
     import torch
 
-    sd = {name: torch.from_numpy(np.ascontiguousarray(params[name], np.float32)).reshape(shape)
+    sd = {name: torch.from_numpy(np.array(params[name], np.float32).reshape(shape))
           for name, shape in param_shapes(d, agg)}
     ckpt = {"version": CKPT_VERSION, "d": int(d), "agg_level": int(agg), "state_dict": sd}
     if extra:

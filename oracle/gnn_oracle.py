@@ -281,3 +281,61 @@ def laplace_nll_bits(mu, log_b, target, eb):
     logp_cross = torch.log(cross.clamp_min(1e-30))
     logp = torch.where(abs_r >= eb, logp_tail, logp_cross)
     return -logp / math.log(2.0)
+
+
+# ---- gate rate proxy ---------------------------------------------------------
+
+SCALE_LO_DIV, SCALE_HI_MULT = 16.0, 4096.0
+GATE_T = np.arange(2, 14, dtype=np.float64)
+GATE_SHIFTS = (0, 2, 4, 6)
+
+
+def gate_scale_tables(eb):
+    """The gate's scale snapping (gnn_codec.py:880-917): level =
+    rint((log(clip(b)) - log lo) / step) on the 64-level grid [eb/16, 4096 eb],
+    as 63 float32 thresholds, and the grid exp(log lo + step * level)."""
+    lo_f, hi_f = eb / SCALE_LO_DIV, eb * SCALE_HI_MULT
+    log_lo = np.log(lo_f)
+    step = (np.log(hi_f) - log_lo) / 63.0
+    thr = np.empty(63, np.float32)
+    hi_bits = int(np.asarray(np.float32(hi_f)).view(np.uint32))
+    for t in range(1, 64):  # first float32 whose level reaches t (bisection)
+        lo, hi = 0, hi_bits
+        while lo < hi:
+            mid = (lo + hi) // 2
+            v = float(np.asarray(mid, np.uint32).view(np.float32))
+            lev = np.rint((np.log(np.clip(v, lo_f, hi_f)) - log_lo) / step)
+            if lev >= t:
+                hi = mid
+            else:
+                lo = mid + 1
+        thr[t - 1] = np.asarray(lo, np.uint32).view(np.float32)
+    return thr, np.exp(log_lo + step * np.arange(64, dtype=np.float64))
+
+
+def laplace_bits(absr, b, eb, radius):
+    """_laplace_bits_t (gnn_codec.py:777-797), float64 numpy."""
+    thr, grid = gate_scale_tables(eb)
+    bs = grid[np.searchsorted(thr, np.asarray(b, np.float32), side="right")]
+    k = np.rint(np.asarray(absr, np.float64) / (2 * eb))
+    with np.errstate(divide="ignore", over="ignore"):
+        p = np.where(k == 0, -np.expm1(-eb / bs),
+                     0.5 * np.exp(-((2 * k - 1) * eb / bs)) * -np.expm1(-2 * eb / bs))
+        bits = np.minimum(-np.log2(p), 24.0)
+    return np.where(k >= radius, 56.0, bits)
+
+
+def gate_costs(x, pred, scale, ips, eb, radius=1 << 15):
+    """Per-bucket bit sums W (1 + 4 * len(ips), 13): row 0 the model, then each
+    candidate at each scale shift; buckets by scale against eb * 2^T."""
+    x = np.asarray(x, np.float64)
+    scale = np.asarray(scale, np.float32)
+    bucket = np.searchsorted(eb * np.exp2(GATE_T), scale.astype(np.float64), side="left")
+    rows = [laplace_bits(np.abs(x - pred), scale, eb, radius)]
+    for ip in ips:
+        for sh in GATE_SHIFTS:
+            rows.append(laplace_bits(np.abs(x - ip), scale * np.float32(2.0 ** -sh), eb, radius))
+    W = np.zeros((len(rows), 13))
+    for r, bits in enumerate(rows):
+        np.add.at(W[r], bucket, bits)
+    return W
