@@ -51,7 +51,101 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--skip-decode", action="store_true")
+    ap.add_argument("--no-gnn", action="store_true")
+    ap.add_argument("--gnn-ckpt", default=os.path.join(ROOT, "latec_amd", "checkpoints",
+                                                       "gnn_d64_agg2.pt"))
+    ap.add_argument("--gnn-shape", default="64,64,64,64")
+    ap.add_argument("--gnn-levels", type=int, default=3)
+    ap.add_argument("--gnn-steps", type=int, default=2)
     return ap.parse_args()
+
+
+def gnn_leg(args):
+    """GNN codec on the 4-D anisotropic field (per-axis correlation U[1,8],
+    50% turbulence, 3 hyperplane jumps), levels 3, eb = 1e-3 * range: BF16 and
+    FP32 kernels against plain cubic interpolation on the same GPU, the BF16 /
+    FP32 prediction RMS against an FP64 run of the same network (oracle, on a
+    10^4 crop), and the model GEMM throughput."""
+    import numpy as np
+    import torch
+
+    from latec_amd import InterpPredictor, compress, decompress
+    from latec_amd.gnn_codec import GNNCodec, roundtrip_slack
+    from latec_amd.synthetic import config3_field
+
+    shape = tuple(int(v) for v in args.gnn_shape.split(","))
+    lv = args.gnn_levels
+    x = config3_field(3, shape, device="cuda").cpu().numpy()
+    rel = 1e-3
+    S = float(x.max() - x.min())
+    out = {"workload": f"GNN codec, {'x'.join(map(str, shape))} f32 4-D anisotropic GRF + "
+                       "turbulence + 3 jumps, levels 3, eb=1e-3*range, one chunk",
+           "checkpoint": os.path.relpath(args.gnn_ckpt, ROOT)}
+    res = {}
+    for numerics in ("bf16", "fp32"):
+        codec = GNNCodec(args.gnn_ckpt, numerics=numerics)
+        stream = codec.compress(x, rel, levels=lv)  # warm-up (geometry, tables)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.gnn_steps):
+            stream = codec.compress(x, rel, levels=lv)
+        torch.cuda.synchronize()
+        tc = (time.perf_counter() - t0) / args.gnn_steps
+        stats = dict(codec.last_stats)
+        t0 = time.perf_counter()
+        y = codec.uncompress(stream).numpy()
+        td = time.perf_counter() - t0
+        err = float(np.abs(y - x).max())
+        assert err <= rel * S + roundtrip_slack(x.min(), x.max())
+        codec.count_flops = True
+        codec.compress(x, rel, levels=lv)
+        flops = codec.last_stats.get("model_flops", 0)
+        res[numerics] = {"bits_per_value": 8 * len(stream) / x.size,
+                         "compress_GBps": x.nbytes / tc / 1e9, "decompress_GBps": x.nbytes / td / 1e9,
+                         "compress_s": tc, "decompress_s": td, "max_abs_err": err,
+                         "model_s": stats["model_s"], "model_gemm_flops": flops,
+                         "model_TFLOPs": flops / stats["model_s"] / 1e12,
+                         "gates_fired": stats.get("gates_fired"), "outliers": stats["outliers"]}
+        print(f"[bench] gnn {numerics}: {res[numerics]['bits_per_value']:.4f} bits/value, "
+              f"compress {tc:.2f} s, decompress {td:.2f} s, model {stats['model_s']:.3f} s",
+              file=sys.stderr, flush=True)
+    stride = 1 << lv
+    t0 = time.perf_counter()
+    s_int, _ = compress(x, rel * S, InterpPredictor("cubic", lv, stride, 1), levels=lv,
+                        anchor_stride=stride, anchor_block=1)
+    t_int = time.perf_counter() - t0
+    assert float(np.abs(decompress(s_int) - x).max()) <= rel * S
+    out.update({"bf16": res["bf16"], "fp32": res["fp32"],
+                "interp_bits_per_value": 8 * len(s_int) / x.size, "interp_compress_s": t_int,
+                "bf16_gain_bits_per_value": 8 * len(s_int) / x.size - res["bf16"]["bits_per_value"]})
+    # prediction RMS against the FP64 oracle on a crop (teacher-forced schedule)
+    from latec_amd.gnn_predictor import ChunkedGNNPredictor
+    from oracle import gnn_oracle as G
+
+    crop = tuple(min(10, n) for n in shape)
+    xc = x[tuple(slice(0, c) for c in crop)]
+    xc = ((xc - xc.min()) / (xc.max() - xc.min())).astype(np.float32)
+    from latec_amd import gnn_model as GM
+
+    d, agg, P = GM.load_checkpoint(args.gnn_ckpt)
+    Pd = {k: torch.from_numpy(np.asarray(v, np.float64)) for k, v in P.items()}
+    ref = G.run_schedule(Pd, torch.from_numpy(xc.reshape(1, -1)).double(), crop, lv, stride,
+                         rel, stride, agg)
+    ref_mu = np.concatenate([m.numpy()[0] for m, _ in ref])
+    for numerics in ("bf16", "fp32"):
+        pr = ChunkedGNNPredictor(args.gnn_ckpt, 0.0, 1.0, levels=lv, anchor_stride=stride,
+                                 numerics=numerics)
+        pr.begin(crop, tuple(-(-n // stride) * stride for n in crop))
+        rc = torch.from_numpy(xc).cuda()[None]
+        w = pr.start_wave([0], rc)
+        mus = []
+        for st in w["chain"][1:]:
+            _, _, mu, _ = pr.predict_wave_stage(st, rc, rel, want_head=True)
+            mus.append(mu.cpu().numpy().astype(np.float64))
+        pr.finish_wave(rc)
+        out[f"rms_{numerics}_vs_fp64"] = float(np.sqrt(np.mean((np.concatenate(mus) - ref_mu) ** 2)))
+    out["rms_crop"] = list(crop)
+    return out
 
 
 def main():
@@ -224,6 +318,8 @@ def main():
                      "decode": {"achieved": ach_dec / 1e9, "frac": ach_dec / HBM_PEAK}},
         "cpu_baseline": cpu,
     }
+    if not args.no_gnn and os.path.exists(args.gnn_ckpt):
+        line["gnn"] = gnn_leg(args)
     print(json.dumps(line))
 
 

@@ -434,6 +434,240 @@ __global__ void __launch_bounds__(256) decode_stage_kernel(
   });
 }
 
+// ---------------------------------------------------------------------------
+// Level-fused 3-D tile kernel (one dyadic level of the lattice pyramid, all
+// five sub-stages in one launch).
+//
+// On a level's compact grid (stride 1) the coarse points have even
+// coordinates on every axis; the new points form 7 parity classes: three
+// weight-1 classes (sub-stages 0..2, interpolated along their odd axis from
+// coarse points), three weight-2 classes (sub-stage 3, from weight-1 points)
+// and the weight-3 class (sub-stage 4, from weight-2 points).  A workgroup
+// owns a tile B of the grid; its LDS holds the recon window B +- 3.  Every
+// tap of an owned point lies within 3 cells of B along one axis and each
+// step of the dependency chain displaces along a different axis, so a class
+// is computed over B extended by 2 cells (its own even coordinates) along its
+// even axes: weight-1 classes over two extended axes, weight-2 over one,
+// weight-3 over B.  Points computed outside B (halo) are recomputed
+// identically by their owner; only owned points are written to HBM.  The
+// arithmetic is the per-stage kernels' (interp_axis / predict_point /
+// quantize_one), so codes, outliers and recon are bit-identical.
+// ---------------------------------------------------------------------------
+
+struct L3Stage {
+  double w, inv_w;
+  float feb;
+  uint32_t R;
+  int64_t code_base;  // codes + code_base + rank
+  int64_t out_lo;     // outlier arrays offset of the stage
+  int32_t gs, pad;    // global stage index (hist, out_count)
+};
+struct L3Class {
+  int64_t offset;
+  int64_t start[3], rstr[3];
+  int32_t step, pad;
+};
+struct Level3Args {
+  int64_t n[3];
+  int32_t cubic, center, end_mode, round_mode;
+  double span, offset;
+  int32_t T[3], tiles_total;
+  int64_t tiles[3];
+  int64_t hist_words;
+  L3Stage st[5];
+  L3Class cl[8];  // by parity 1..7
+};
+
+#define L3_HBINS 256
+#define L3_THREADS 512
+
+__device__ __forceinline__ int l3_stage(int parity) {
+  return parity == 1 ? 0 : parity == 2 ? 1 : parity == 4 ? 2 : parity == 7 ? 4 : 3;
+}
+
+template <bool ENC>
+__global__ void __launch_bounds__(L3_THREADS) level3_kernel(
+    Level3Args A, const float *__restrict__ x, float *__restrict__ recon,
+    uint16_t *__restrict__ codes, uint32_t *__restrict__ hist, uint32_t *__restrict__ out_rank,
+    float *__restrict__ out_val, uint32_t *__restrict__ out_count) {
+  extern __shared__ __align__(16) unsigned char l3_smem[];
+  uint32_t *lhist = (uint32_t *)l3_smem;  // 5 x L3_HBINS
+  float *win = (float *)(l3_smem + 5 * L3_HBINS * sizeof(uint32_t));
+  const int tid = threadIdx.x;
+  // XCD-aware tile order: consecutive tiles (along the last axis) share an XCD
+  const uint32_t G = gridDim.x;
+  const uint32_t tb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int W1 = A.T[1] + 6, W2 = A.T[2] + 6;
+  const int lstr0 = W1 * W2, lstr1 = W2;
+  // predictor arguments over the LDS window (global shape for validity)
+  GridArgs gp;
+  gp.ndim = 3;
+  gp.s = 1;
+  gp.cubic = A.cubic;
+  gp.center = A.center;
+  gp.end_mode = A.end_mode;
+  for (int j = 0; j < 3; j++) gp.shape[j] = A.n[j];
+  gp.gstr[0] = lstr0;
+  gp.gstr[1] = lstr1;
+  gp.gstr[2] = 1;
+  for (uint32_t t = tb; t < (uint32_t)A.tiles_total; t += G) {
+    const int64_t t2 = t % A.tiles[2], t01 = t / A.tiles[2];
+    const int64_t t1 = t01 % A.tiles[1], t0 = t01 / A.tiles[1];
+    int64_t blo[3], bhi[3], wlo[3];
+    const int64_t tt[3] = {t0, t1, t2};
+    for (int j = 0; j < 3; j++) {
+      blo[j] = tt[j] * A.T[j];
+      bhi[j] = blo[j] + A.T[j] < A.n[j] ? blo[j] + A.T[j] : A.n[j];
+      wlo[j] = blo[j] - 3;
+    }
+    for (int i = tid; i < 5 * L3_HBINS; i += L3_THREADS) lhist[i] = 0;
+    // coarse cells of the window: even coordinates in [blo-2, bhi+2] within the grid
+    {
+      int64_t c0lo[3], cnt[3];
+      for (int j = 0; j < 3; j++) {
+        int64_t lo = blo[j] - 2 < 0 ? 0 : blo[j] - 2;
+        int64_t hi = bhi[j] + 2 < A.n[j] - 1 ? bhi[j] + 2 : A.n[j] - 1;
+        lo += lo & 1;
+        c0lo[j] = lo;
+        cnt[j] = hi >= lo ? (hi - lo) / 2 + 1 : 0;
+      }
+      const int total = (int)(cnt[0] * cnt[1] * cnt[2]);
+      const int n1c = (int)cnt[1], n2c = (int)cnt[2];
+      for (int i = tid; i < total; i += L3_THREADS) {
+        const int i2 = i % n2c, i01 = i / n2c;
+        const int i1 = i01 % n1c, i0 = i01 / n1c;
+        const int64_t c0 = c0lo[0] + 2 * i0, c1 = c0lo[1] + 2 * i1, c2 = c0lo[2] + 2 * i2;
+        win[(c0 - wlo[0]) * lstr0 + (c1 - wlo[1]) * lstr1 + (c2 - wlo[2])] =
+            recon[(uint32_t)((c0 * A.n[1] + c1) * A.n[2] + c2)];
+      }
+    }
+    __syncthreads();
+    for (int phase = 0; phase < 3; phase++) {
+      for (int P = 1; P < 8; P++) {
+        const int w = __popc(P);
+        if (w != phase + 1) continue;
+        const int si = l3_stage(P);
+        const L3Stage &S = A.st[si];
+        const L3Class &K = A.cl[P];
+        int64_t first[3], cnt[3];
+        for (int j = 0; j < 3; j++) {
+          const bool odd = (P >> j) & 1;
+          int64_t lo = odd ? blo[j] : blo[j] - 2;
+          int64_t hi = odd ? bhi[j] - 1 : bhi[j] + 2;  // inclusive
+          if (lo < 0) lo = 0;
+          if (hi > A.n[j] - 1) hi = A.n[j] - 1;
+          if ((lo & 1) != (odd ? 1 : 0)) lo++;
+          first[j] = lo;
+          cnt[j] = hi >= lo ? (hi - lo) / 2 + 1 : 0;
+        }
+        const int total = (int)(cnt[0] * cnt[1] * cnt[2]);
+        const int n1c = (int)cnt[1], n2c = (int)cnt[2];
+        GridArgs gq;
+        gq.w = S.w;
+        gq.inv_w = S.inv_w;
+        gq.R = S.R;
+        gq.round_mode = A.round_mode;
+        gq.span = A.span;
+        gq.offset = A.offset;
+        gq.feb = S.feb;
+        uint32_t *lh = lhist + si * L3_HBINS;
+        const uint32_t win_lo = S.R > L3_HBINS / 2 ? S.R - L3_HBINS / 2 : 0;
+        uint32_t *ghist = hist + (int64_t)S.gs * A.hist_words;
+        for (int base = 0; base < total; base += L3_THREADS) {
+          const int i = base + tid;
+          const bool active = i < total;
+          int64_t coord[3] = {0, 0, 0};
+          bool owned = false;
+          int l = 0;
+          int64_t f = 0, rank = 0;
+          uint32_t code = 0;
+          float xv = 0.f, rec = 0.f;
+          if (active) {
+            const int i2 = i % n2c, i01 = i / n2c;
+            const int i1 = i01 % n1c, i0 = i01 / n1c;
+            coord[0] = first[0] + 2 * i0;
+            coord[1] = first[1] + 2 * i1;
+            coord[2] = first[2] + 2 * i2;
+            owned = true;
+            for (int j = 0; j < 3; j++) owned = owned && coord[j] >= blo[j] && coord[j] < bhi[j];
+            l = (int)((coord[0] - wlo[0]) * lstr0 + (coord[1] - wlo[1]) * lstr1 + (coord[2] - wlo[2]));
+            f = (coord[0] * A.n[1] + coord[1]) * A.n[2] + coord[2];
+            rank = K.offset;
+            for (int j = 0; j < 3; j++) rank += ((coord[j] - K.start[j]) >> 1) * K.rstr[j];
+            const float p = predict_point<3, int>(win, gp, P, l, coord);
+            if (ENC) {
+              xv = x[f];
+              code = quantize_one(xv, p, gq, &rec);
+            } else {
+              code = (uint32_t)codes[S.code_base + rank];
+              if (code) {
+                rec = (float)((double)p + S.w * (double)((int)code - (int)S.R));
+              } else {
+                const int64_t nout = out_count[S.gs];
+                const uint32_t *orr = out_rank + S.out_lo;
+                int64_t lo = 0, hi = nout;
+                while (lo < hi) {
+                  const int64_t mid = (lo + hi) >> 1;
+                  if ((int64_t)orr[mid] < rank) lo = mid + 1; else hi = mid;
+                }
+                rec = (lo < nout && (int64_t)orr[lo] == rank) ? out_val[S.out_lo + lo]
+                                                               : __int_as_float(0x7fc00000);
+              }
+            }
+            win[l] = rec;
+            if (owned) recon[f] = rec;
+          }
+          if (ENC) {
+            const bool emit_ok = active && owned;
+            if (emit_ok) {
+              codes[S.code_base + rank] = (uint16_t)code;
+              const uint32_t b = code - win_lo;
+              if (code != 0 && b < L3_HBINS)
+                atomicAdd(&lh[b], 1u);
+              else
+                atomicAdd(&ghist[code], 1u);
+            }
+            const bool out = emit_ok && code == 0;
+            const uint64_t m = __ballot(out);
+            if (m) {
+              const int lane = __lane_id();
+              const int leader = __ffsll((unsigned long long)m) - 1;
+              uint32_t obase = 0;
+              if (lane == leader) obase = atomicAdd(out_count + S.gs, (uint32_t)__popcll(m));
+              obase = __shfl(obase, leader);
+              if (out) {
+                const uint32_t k = obase + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+                out_rank[S.out_lo + k] = (uint32_t)rank;
+                out_val[S.out_lo + k] = xv;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (ENC) {  // flush the stage windows into the striped copies
+      for (int si = 0; si < 5; si++) {
+        const L3Stage &S = A.st[si];
+        const uint32_t win_lo = S.R > L3_HBINS / 2 ? S.R - L3_HBINS / 2 : 0;
+        const uint32_t glo = S.R > HIST_WIN / 2 ? S.R - HIST_WIN / 2 : 0;
+        uint32_t *gwin = hist + (int64_t)S.gs * A.hist_words + 2 * (int64_t)S.R +
+                         (int64_t)(t % HIST_COPIES) * HIST_WIN;
+        for (int i = tid; i < L3_HBINS; i += L3_THREADS) {
+          const uint32_t v = lhist[si * L3_HBINS + i];
+          const uint32_t code = win_lo + i;
+          if (!v) continue;
+          if (code >= glo && code - glo < HIST_WIN)
+            atomicAdd(&gwin[code - glo], v);
+          else
+            atomicAdd(&hist[(int64_t)S.gs * A.hist_words + code], v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Generic quantizer over compact arrays (foreign predictors, GNN stages).
 template <typename CodeT>
 __global__ void __launch_bounds__(256) quantize_kernel(
@@ -794,6 +1028,8 @@ static bool pyramid_ok(const latec_plan *p) {
   return p->block == 1 && p->ndim <= 4 && (int64_t)p->channels * p->npoints < 0xFFFFFFFFll;
 }
 
+static size_t level3_smem();
+
 static int pyramid_setup(latec_plan *p) {
   int rc = plan_upload(p);
   if (rc) return rc;
@@ -829,6 +1065,15 @@ static int pyramid_setup(latec_plan *p) {
     }
   }
   d->K = K;
+  if (p->ndim == 3) {  // the level kernel's window exceeds the default LDS limit
+    if ((rc = hip_check(hipFuncSetAttribute((const void *)level3_kernel<true>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)level3_smem()), "hipFuncSetAttribute")) ||
+        (rc = hip_check(hipFuncSetAttribute((const void *)level3_kernel<false>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)level3_smem()), "hipFuncSetAttribute")))
+      return rc;
+  }
   return LATEC_OK;
 }
 
@@ -899,6 +1144,77 @@ static int launch_stage(const latec_plan *gp, const latec_plan *q, int js, int g
   return hip_check(hipGetLastError(), encode ? "encode_stage_kernel" : "decode_stage_kernel");
 }
 
+// Level-fused launch for 3-D, one channel, u16 codes: all five sub-stages of
+// level k in one kernel (level3_kernel).  Returns 1 if not applicable.
+static const int L3_T[3] = {16, 16, 32};
+static size_t level3_smem() {
+  return 5 * L3_HBINS * sizeof(uint32_t) +
+         sizeof(float) * (size_t)(L3_T[0] + 6) * (L3_T[1] + 6) * (L3_T[2] + 6);
+}
+static bool level3_ok(const latec_plan *gp, const latec_plan *q, const latec_interp_opts *o) {
+  static const bool on = getenv("LATEC_LEVEL3") != nullptr;  // opt-in until it beats the stage kernels
+  if (!on) return false;
+  return gp->ndim == 3 && gp->channels == 1 && o->code_bytes == 2 && q->stages.size() == 6;
+}
+static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool encode,
+                         const float *x, float *recon, void *codes, uint32_t *hist,
+                         uint32_t *out_rank, float *out_val, uint32_t *out_count,
+                         const double *ebs, const latec_interp_opts *o, hipStream_t stream) {
+  Level3Args A;
+  memset(&A, 0, sizeof A);
+  for (int j = 0; j < 3; j++) {
+    A.n[j] = q->shape[j];
+    A.T[j] = L3_T[j];
+    A.tiles[j] = (q->shape[j] + L3_T[j] - 1) / L3_T[j];
+  }
+  const int64_t tiles = A.tiles[0] * A.tiles[1] * A.tiles[2];
+  if (tiles > 0x7fffffffll) return 1;
+  A.tiles_total = (int32_t)tiles;
+  A.cubic = o->cubic;
+  A.center = o->center;
+  A.end_mode = o->end_mode;
+  A.round_mode = o->round_mode;
+  A.span = o->span;
+  A.offset = o->offset;
+  A.hist_words = hist_words(o->radius);
+  for (int js = 1; js <= 5; js++) {
+    const int gs = g0 + js - 1;
+    const GridArgs g = make_args(q, js, ebs[gs], o);
+    L3Stage &S = A.st[js - 1];
+    S.w = g.w;
+    S.inv_w = g.inv_w;
+    S.feb = g.feb;
+    S.R = g.R;
+    S.code_base = (int64_t)gp->channels * gp->stages[gs].offset;
+    S.out_lo = S.code_base;
+    S.gs = gs;
+    for (const ClassDesc &c : q->stages[js].classes) {
+      if (c.parity < 1 || c.parity > 7 || c.ndim != 3 || c.step != 2) return 1;
+      L3Class &K = A.cl[c.parity];
+      K.offset = c.offset;
+      K.step = c.step;
+      for (int j = 0; j < 3; j++) {
+        K.start[j] = c.starts[j];
+        K.rstr[j] = c.rstr[j];
+      }
+    }
+  }
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  int64_t grid = std::min<int64_t>(tiles, (int64_t)ncu * 2);
+  grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
+  if (encode)
+    hipLaunchKernelGGL(level3_kernel<true>, dim3((unsigned)grid), dim3(L3_THREADS), level3_smem(),
+                       stream, A, x, recon, (uint16_t *)codes, hist, out_rank, out_val,
+                       out_count);
+  else
+    hipLaunchKernelGGL(level3_kernel<false>, dim3((unsigned)grid), dim3(L3_THREADS),
+                       level3_smem(), stream, A, nullptr, recon, (uint16_t *)codes, hist,
+                       out_rank, out_val, out_count);
+  return hip_check(hipGetLastError(), "level3_kernel");
+}
+
 // Slab wavefront (opt-in, LATEC_SLAB_MB=<MiB per slab>): a level whose grid
 // exceeds four slabs runs its sub-stages slab by slab along axis 0 (weight-w
 // stages trail by w-1 slabs) so that each slab stays cache resident.
@@ -934,6 +1250,12 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
     if (slab_bytes && nd > 1 && (int64_t)p->channels * q->npoints * 8 > 4 * slab_bytes)
       T = std::max<int64_t>(8, (slab_bytes / std::max<int64_t>(plane, 1)) & ~1ll);
     const int64_t nslab = (n0 + T - 1) / T;
+    if (nslab == 1 && level3_ok(p, q, o)) {
+      rc = launch_level3(p, q, g0, encode, X(k), R(k), codes, hist, out_rank, out_val, out_count,
+                         ebs, o, stream);
+      if (rc < 0) return rc;
+      if (rc == 0) continue;
+    }
     for (int64_t t = 0; t < nslab + nd - 1; t++) {
       for (int w = 1; w <= nd; w++) {
         const int64_t sl = t - (w - 1);

@@ -276,6 +276,7 @@ class GNNCompressorCodec:
         self.checkpoint_hash = load_model(self.checkpoint_path)[3]
         self._predictors: dict = {}
         self.last_stats: dict = {}
+        self.count_flops = False  # count the model's GEMM flops into last_stats (syncs)
 
     @staticmethod
     def _chunk_edges(shape, anchor_stride: int, cs):
@@ -433,7 +434,8 @@ class GNNCompressorCodec:
         empty = np.zeros(0, np.uint32), np.zeros(0, np.float32), np.zeros(0, np.uint8)
         pool = ThreadPoolExecutor(max_workers=8)
         futures = []
-        stats = {"outliers": 0, "stage_codes": 0}
+        stats = {"outliers": 0, "stage_codes": 0, "gate_words": 0}
+        events = []  # (start, end) HIP events around the model's stage passes
         for ci in range(n_chunks):
             sls = pred_obj.chunk_slices(ci)
             cshape = tuple(sl.stop - sl.start for sl in sls)
@@ -449,7 +451,13 @@ class GNNCompressorCodec:
                     continue
                 keep = sched.keep[s]
                 dkeep = D.to_device(keep) if keep is not None else None
+                e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+                e0.record()
                 p_full, sc_full = pred_obj.predict_wave_stage(s, recon, ebs[s])
+                e1.record()
+                events.append((e0, e1))
+                if self.count_flops:
+                    stats["model_flops"] = stats.get("model_flops", 0) + pred_obj.stage_flops(s)
                 gidx = pred_obj.stage_gidx(s)
                 if dkeep is not None:
                     p, sc, gidx = p_full[0][dkeep], sc_full[0][dkeep], gidx[dkeep]
@@ -513,6 +521,10 @@ class GNNCompressorCodec:
             pred_obj.finish_wave(recon)
         parts.extend(f.result() for f in futures)
         pool.shutdown()
+        T.cuda.synchronize()
+        stats["model_s"] = sum(a.elapsed_time(b) for a, b in events) * 1e-3
+        stats["gate_words"] = len(gates or []) + len(fine_gates or [])
+        stats["gates_fired"] = sum(1 for g in (gates or []) + (fine_gates or []) if g)
         self.last_stats = stats
         return b"".join(parts), gates, fine_gates
 

@@ -266,6 +266,27 @@ class ChunkedGNNPredictor:
                                         ptr(None)))
         return ctx
 
+    def stage_flops(self, s: int) -> int:
+        """Multiply-add flops of the model's GEMMs for stage s of the current
+        chunk: per live (line, point) pair and axis the message MLP (two-sided
+        2*(2d*h) + 2*h*d, one-sided 2*d*h + 2*h*d) and the attention key
+        (2*d*d); per point and axis of the previous stage the mix MLP
+        (2*2d*h + 2*h*d)."""
+        T = D.torch()
+        m, w = self.model, self._wave
+        g = w["geoms"][s]
+        if not g.M:
+            return 0
+        st = g.steps[: g.M * m.L].to(T.int64)
+        sp, sn = st & 0xFFFF, st >> 16
+        nb = int(((sp > 0) & (sn > 0)).sum())
+        nd = int(((sp > 0) ^ (sn > 0)).sum())
+        d, h, K = m.d, 2 * m.d, m.ndim
+        j = w["chain"].index(s)
+        prev = w["geoms"][w["chain"][j - 1]].M if j > 0 else 0
+        return (K * (nb * (2 * 2 * d * h + 2 * h * d) + nd * (2 * d * h + 2 * h * d)
+                     + (nb + nd) * 2 * d * d) + prev * K * (2 * 2 * d * h + 2 * h * d))
+
     def stage_gidx(self, s: int):
         """Tensor flat indices of the current chunk's stage-s points (device)."""
         return self._wave["geoms"][s].gidx[: self._wave["geoms"][s].M]
