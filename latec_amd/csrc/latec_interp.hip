@@ -485,6 +485,29 @@ __device__ __forceinline__ int l3_stage(int parity) {
   return parity == 1 ? 0 : parity == 2 ? 1 : parity == 4 ? 2 : parity == 7 ? 4 : 3;
 }
 
+// The seven classes in dependency order (weight 1, 2, 3).
+__constant__ int l3_order[7] = {1, 2, 4, 3, 5, 6, 7};
+
+// Region of class P computed for the tile [blo, bhi): odd axes over the tile,
+// even axes over the tile +- 2, clipped to the grid; first coordinate and count.
+__device__ __forceinline__ void l3_region(int P, const int64_t *blo, const int64_t *bhi,
+                                          const int64_t *n, int64_t *first, int *cnt) {
+  for (int j = 0; j < 3; j++) {
+    const bool odd = (P >> j) & 1;
+    int64_t lo = odd ? blo[j] : blo[j] - 2;
+    int64_t hi = odd ? bhi[j] - 1 : bhi[j] + 2;  // inclusive
+    if (lo < 0) lo = 0;
+    if (hi > n[j] - 1) hi = n[j] - 1;
+    if ((lo & 1) != (odd ? 1 : 0)) lo++;
+    first[j] = lo;
+    cnt[j] = hi >= lo ? (int)((hi - lo) / 2 + 1) : 0;
+  }
+}
+
+// Per tile: (1) one bulk phase loads the coarse window and, for every point
+// the tile computes, its input (x to encode, the code to decode) into LDS,
+// with all loads of a thread independent; (2) three compute phases (weight
+// 1, 2, 3) read only LDS and write owned results to HBM.
 template <bool ENC>
 __global__ void __launch_bounds__(L3_THREADS) level3_kernel(
     Level3Args A, const float *__restrict__ x, float *__restrict__ recon,
@@ -492,14 +515,13 @@ __global__ void __launch_bounds__(L3_THREADS) level3_kernel(
     float *__restrict__ out_val, uint32_t *__restrict__ out_count) {
   extern __shared__ __align__(16) unsigned char l3_smem[];
   uint32_t *lhist = (uint32_t *)l3_smem;  // 5 x L3_HBINS
+  const int W0 = A.T[0] + 6, W1 = A.T[1] + 6, W2 = A.T[2] + 6;
   float *win = (float *)(l3_smem + 5 * L3_HBINS * sizeof(uint32_t));
+  float *xs = win + W0 * W1 * W2;  // per computed point: x (encode) or code (decode)
   const int tid = threadIdx.x;
-  // XCD-aware tile order: consecutive tiles (along the last axis) share an XCD
   const uint32_t G = gridDim.x;
   const uint32_t tb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
-  const int W1 = A.T[1] + 6, W2 = A.T[2] + 6;
   const int lstr0 = W1 * W2, lstr1 = W2;
-  // predictor arguments over the LDS window (global shape for validity)
   GridArgs gp;
   gp.ndim = 3;
   gp.s = 1;
@@ -521,47 +543,68 @@ __global__ void __launch_bounds__(L3_THREADS) level3_kernel(
       wlo[j] = blo[j] - 3;
     }
     for (int i = tid; i < 5 * L3_HBINS; i += L3_THREADS) lhist[i] = 0;
-    // coarse cells of the window: even coordinates in [blo-2, bhi+2] within the grid
+    // --- bulk load: coarse cells of the window (even coordinates, tile +- 2)
     {
-      int64_t c0lo[3], cnt[3];
+      int64_t c0lo[3];
+      int cnt[3];
       for (int j = 0; j < 3; j++) {
         int64_t lo = blo[j] - 2 < 0 ? 0 : blo[j] - 2;
         int64_t hi = bhi[j] + 2 < A.n[j] - 1 ? bhi[j] + 2 : A.n[j] - 1;
         lo += lo & 1;
         c0lo[j] = lo;
-        cnt[j] = hi >= lo ? (hi - lo) / 2 + 1 : 0;
+        cnt[j] = hi >= lo ? (int)((hi - lo) / 2 + 1) : 0;
       }
-      const int total = (int)(cnt[0] * cnt[1] * cnt[2]);
-      const int n1c = (int)cnt[1], n2c = (int)cnt[2];
+      const int total = cnt[0] * cnt[1] * cnt[2];
       for (int i = tid; i < total; i += L3_THREADS) {
-        const int i2 = i % n2c, i01 = i / n2c;
-        const int i1 = i01 % n1c, i0 = i01 / n1c;
+        const int i2 = i % cnt[2], i01 = i / cnt[2];
+        const int i1 = i01 % cnt[1], i0 = i01 / cnt[1];
         const int64_t c0 = c0lo[0] + 2 * i0, c1 = c0lo[1] + 2 * i1, c2 = c0lo[2] + 2 * i2;
         win[(c0 - wlo[0]) * lstr0 + (c1 - wlo[1]) * lstr1 + (c2 - wlo[2])] =
             recon[(uint32_t)((c0 * A.n[1] + c1) * A.n[2] + c2)];
       }
     }
+    // --- bulk load: the input of every point the tile computes
+    {
+      int off = 0;
+      for (int oi = 0; oi < 7; oi++) {
+        const int P = l3_order[oi];
+        const L3Stage &S = A.st[l3_stage(P)];
+        const L3Class &K = A.cl[P];
+        int64_t first[3];
+        int cnt[3];
+        l3_region(P, blo, bhi, A.n, first, cnt);
+        const int total = cnt[0] * cnt[1] * cnt[2];
+        for (int i = tid; i < total; i += L3_THREADS) {
+          const int i2 = i % cnt[2], i01 = i / cnt[2];
+          const int i1 = i01 % cnt[1], i0 = i01 / cnt[1];
+          const int64_t c0 = first[0] + 2 * i0, c1 = first[1] + 2 * i1, c2 = first[2] + 2 * i2;
+          if (ENC) {
+            xs[off + i] = x[(uint32_t)((c0 * A.n[1] + c1) * A.n[2] + c2)];
+          } else {
+            const int64_t rank = K.offset + ((c0 - K.start[0]) >> 1) * K.rstr[0] +
+                                 ((c1 - K.start[1]) >> 1) * K.rstr[1] +
+                                 ((c2 - K.start[2]) >> 1) * K.rstr[2];
+            xs[off + i] = __uint_as_float((uint32_t)codes[S.code_base + rank]);
+          }
+        }
+        off += total;
+      }
+    }
     __syncthreads();
+    // --- compute phases
+    int off = 0;
     for (int phase = 0; phase < 3; phase++) {
-      for (int P = 1; P < 8; P++) {
-        const int w = __popc(P);
-        if (w != phase + 1) continue;
+      const int oi0 = phase == 0 ? 0 : phase == 1 ? 3 : 6;
+      const int oi1 = phase == 0 ? 3 : phase == 1 ? 6 : 7;
+      for (int oi = oi0; oi < oi1; oi++) {
+        const int P = l3_order[oi];
         const int si = l3_stage(P);
         const L3Stage &S = A.st[si];
         const L3Class &K = A.cl[P];
-        int64_t first[3], cnt[3];
-        for (int j = 0; j < 3; j++) {
-          const bool odd = (P >> j) & 1;
-          int64_t lo = odd ? blo[j] : blo[j] - 2;
-          int64_t hi = odd ? bhi[j] - 1 : bhi[j] + 2;  // inclusive
-          if (lo < 0) lo = 0;
-          if (hi > A.n[j] - 1) hi = A.n[j] - 1;
-          if ((lo & 1) != (odd ? 1 : 0)) lo++;
-          first[j] = lo;
-          cnt[j] = hi >= lo ? (hi - lo) / 2 + 1 : 0;
-        }
-        const int total = (int)(cnt[0] * cnt[1] * cnt[2]);
-        const int n1c = (int)cnt[1], n2c = (int)cnt[2];
+        int64_t first[3];
+        int cnt[3];
+        l3_region(P, blo, bhi, A.n, first, cnt);
+        const int total = cnt[0] * cnt[1] * cnt[2];
         GridArgs gq;
         gq.w = S.w;
         gq.inv_w = S.inv_w;
@@ -576,30 +619,31 @@ __global__ void __launch_bounds__(L3_THREADS) level3_kernel(
         for (int base = 0; base < total; base += L3_THREADS) {
           const int i = base + tid;
           const bool active = i < total;
-          int64_t coord[3] = {0, 0, 0};
           bool owned = false;
-          int l = 0;
-          int64_t f = 0, rank = 0;
+          int64_t rank = 0;
           uint32_t code = 0;
-          float xv = 0.f, rec = 0.f;
+          float xv = 0.f;
           if (active) {
-            const int i2 = i % n2c, i01 = i / n2c;
-            const int i1 = i01 % n1c, i0 = i01 / n1c;
+            const int i2 = i % cnt[2], i01 = i / cnt[2];
+            const int i1 = i01 % cnt[1], i0 = i01 / cnt[1];
+            int64_t coord[3];
             coord[0] = first[0] + 2 * i0;
             coord[1] = first[1] + 2 * i1;
             coord[2] = first[2] + 2 * i2;
-            owned = true;
-            for (int j = 0; j < 3; j++) owned = owned && coord[j] >= blo[j] && coord[j] < bhi[j];
-            l = (int)((coord[0] - wlo[0]) * lstr0 + (coord[1] - wlo[1]) * lstr1 + (coord[2] - wlo[2]));
-            f = (coord[0] * A.n[1] + coord[1]) * A.n[2] + coord[2];
-            rank = K.offset;
-            for (int j = 0; j < 3; j++) rank += ((coord[j] - K.start[j]) >> 1) * K.rstr[j];
+            owned = coord[0] >= blo[0] && coord[0] < bhi[0] && coord[1] >= blo[1] &&
+                    coord[1] < bhi[1] && coord[2] >= blo[2] && coord[2] < bhi[2];
+            const int l = (int)((coord[0] - wlo[0]) * lstr0 + (coord[1] - wlo[1]) * lstr1 +
+                                (coord[2] - wlo[2]));
+            rank = K.offset + ((coord[0] - K.start[0]) >> 1) * K.rstr[0] +
+                   ((coord[1] - K.start[1]) >> 1) * K.rstr[1] +
+                   ((coord[2] - K.start[2]) >> 1) * K.rstr[2];
             const float p = predict_point<3, int>(win, gp, P, l, coord);
+            float rec;
             if (ENC) {
-              xv = x[f];
+              xv = xs[off + i];
               code = quantize_one(xv, p, gq, &rec);
             } else {
-              code = (uint32_t)codes[S.code_base + rank];
+              code = __float_as_uint(xs[off + i]);
               if (code) {
                 rec = (float)((double)p + S.w * (double)((int)code - (int)S.R));
               } else {
@@ -615,7 +659,7 @@ __global__ void __launch_bounds__(L3_THREADS) level3_kernel(
               }
             }
             win[l] = rec;
-            if (owned) recon[f] = rec;
+            if (owned) recon[(uint32_t)((coord[0] * A.n[1] + coord[1]) * A.n[2] + coord[2])] = rec;
           }
           if (ENC) {
             const bool emit_ok = active && owned;
@@ -643,6 +687,7 @@ __global__ void __launch_bounds__(L3_THREADS) level3_kernel(
             }
           }
         }
+        off += total;
       }
       __syncthreads();
     }
@@ -1146,10 +1191,18 @@ static int launch_stage(const latec_plan *gp, const latec_plan *q, int js, int g
 
 // Level-fused launch for 3-D, one channel, u16 codes: all five sub-stages of
 // level k in one kernel (level3_kernel).  Returns 1 if not applicable.
-static const int L3_T[3] = {16, 16, 32};
+static const int L3_T[3] = {16, 16, 16};
+// largest number of points one tile computes (weight-1 classes over two
+// extended axes, weight-2 over one, weight-3 over the tile)
+static size_t level3_points() {
+  const size_t h0 = L3_T[0] / 2, h1 = L3_T[1] / 2, h2 = L3_T[2] / 2;
+  const size_t e0 = h0 + 3, e1 = h1 + 3, e2 = h2 + 3;
+  return h0 * e1 * e2 + e0 * h1 * e2 + e0 * e1 * h2 + h0 * h1 * e2 + h0 * e1 * h2 +
+         e0 * h1 * h2 + h0 * h1 * h2;
+}
 static size_t level3_smem() {
   return 5 * L3_HBINS * sizeof(uint32_t) +
-         sizeof(float) * (size_t)(L3_T[0] + 6) * (L3_T[1] + 6) * (L3_T[2] + 6);
+         sizeof(float) * ((size_t)(L3_T[0] + 6) * (L3_T[1] + 6) * (L3_T[2] + 6) + level3_points());
 }
 static bool level3_ok(const latec_plan *gp, const latec_plan *q, const latec_interp_opts *o) {
   static const bool on = getenv("LATEC_LEVEL3") != nullptr;  // opt-in until it beats the stage kernels
