@@ -403,16 +403,30 @@ __global__ void pair_scan_kernel(int32_t *__restrict__ counts, int64_t nblocks,
   }
 }
 
-// Writes the bidir and one-sided pair lists and slot[p] (message slot:
-// bidir position, or n_bidir + one-sided position; -1 for a dead pair).
-__global__ void pair_write_kernel(const int32_t *__restrict__ steps, int64_t n_pairs,
+// Per live pair, what the message rows read: the neighbour frame rows and the
+// two per-row scalars of the first layer (two-sided: log distances of the
+// negative and positive neighbour; one-sided: sign and log distance).
+struct __align__(16) PairRec {
+  int32_t ra, rb;
+  float za, zb;
+};
+
+// Writes the bidir and one-sided pair lists, their records (+ the line index,
+// with the one-sided side in bit 14), and slot[p] (message slot: bidir
+// position, or n_bidir + one-sided position; -1 for a dead pair).
+__global__ void pair_write_kernel(const int32_t *__restrict__ steps, int64_t n_pairs, int L,
+                                  const int64_t *__restrict__ qidx,
+                                  const int64_t *__restrict__ line_off,
+                                  const float *__restrict__ lognnz,
                                   const int32_t *__restrict__ offs,
                                   const int32_t *__restrict__ totals,
-                                  int32_t *__restrict__ list_bi, int32_t *__restrict__ list_dir,
+                                  PairRec *__restrict__ rec_bi, int16_t *__restrict__ lw_bi,
+                                  PairRec *__restrict__ rec_dir, int16_t *__restrict__ lw_dir,
                                   int32_t *__restrict__ slot) {
   __shared__ int sb[PAIR_BLOCK / 64], sd[PAIR_BLOCK / 64];
   const int64_t p = (int64_t)blockIdx.x * PAIR_BLOCK + threadIdx.x;
-  const int k = p < n_pairs ? pair_kind(steps[p]) : 0;
+  const int32_t stp = p < n_pairs ? steps[p] : 0;
+  const int k = pair_kind(stp);
   const uint64_t mb = __ballot(k == 2), md = __ballot(k == 1);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
@@ -426,12 +440,31 @@ __global__ void pair_write_kernel(const int32_t *__restrict__ steps, int64_t n_p
   pb += __popcll(mb & below);
   pd += __popcll(md & below);
   if (p < n_pairs) {
-    if (k == 2) {
-      list_bi[pb] = (int32_t)p;
-      slot[p] = pb;
-    } else if (k == 1) {
-      list_dir[pd] = (int32_t)p;
-      slot[p] = totals[0] + pd;
+    const int m = (int)(p / L), l = (int)(p % L);
+    const int sp = stp & 0xffff, sn = stp >> 16;
+    if (k) {
+      const int64_t q = qidx[m], off = line_off[l];
+      const float lnz = lognnz[l];
+      PairRec r;
+      if (k == 2) {
+        r.ra = (int32_t)(q - (int64_t)sn * off);
+        r.rb = (int32_t)(q + (int64_t)sp * off);
+        r.za = log2f((float)sn) + lnz;
+        r.zb = log2f((float)sp) + lnz;
+        rec_bi[pb] = r;
+        lw_bi[pb] = (int16_t)l;
+        slot[p] = pb;
+      } else {
+        const bool pos = sp > 0;
+        const int t = pos ? sp : sn;
+        r.ra = (int32_t)(q + (pos ? (int64_t)t : -(int64_t)t) * off);
+        r.rb = 0;
+        r.za = pos ? 1.0f : -1.0f;
+        r.zb = log2f((float)t) + lnz;
+        rec_dir[pd] = r;
+        lw_dir[pd] = (int16_t)(l | (pos ? 1 << 14 : 0));
+        slot[p] = totals[0] + pd;
+      }
     } else {
       slot[p] = -1;
     }
@@ -443,20 +476,16 @@ __global__ void pair_write_kernel(const int32_t *__restrict__ steps, int64_t n_p
 // ---------------------------------------------------------------------------
 
 struct RowArgs {
-  int32_t K, L, mode, pad;
-  const int32_t *n_rows_dev;   // rows = n_items * K, item count read on device
-  int64_t n_rows_host;         // used when n_rows_dev == nullptr
-  // messages
-  const int32_t *list;         // pair ids (tile-local m * L + l)
-  const int32_t *steps;        // tile steps (m * L + l)
-  const int64_t *qidx;         // tile query frame indices (m)
-  const int64_t *line_off;     // (L) frame offsets of the lines
-  const float *lognnz;         // (L)
-  const int8_t *sidx;          // (L, K, 2)
+  int32_t K, L, ns, pad;
+  const int32_t *n_items_dev;  // message items (pairs), read on device
+  int64_t n_rows_host;         // finalize rows (M * K)
+  const PairRec *rec;          // messages: per list entry
+  const int16_t *lw;           // messages: line (| side << 14)
+  const int8_t *sidx;          // (L, K, 2) state index per line, axis, side
   const void *E;               // (frame rows, K, D)
   void *out;                   // msg (slots, K, D) / E for finalize
   float *score;                // (slots, K) for messages
-  int64_t slot_base;           // slot of list[0] (0 for bidir, n_bidir for dir)
+  int64_t slot_base;           // slot of list[0] (0 for bidir)
   const int32_t *slot_base_dev;
   // finalize
   const void *ctx;             // (M, K, D)
@@ -470,14 +499,15 @@ template <int D, typename T>
 struct RowCfg {
   static constexpr int H = 2 * D;
   static constexpr bool BF = sizeof(T) == 2;
-  static constexpr int KS = BF ? 32 : 4;       // MFMA K per instruction
-  static constexpr int W1P = 2 * D + (BF ? 8 : 4);  // padded LDS row (elements)
+  static constexpr int KS = BF ? 32 : 4;            // MFMA K per instruction
+  static constexpr int W1P = 2 * D + (BF ? 8 : 4);  // padded LDS rows (elements)
   static constexpr int W2P = H + (BF ? 8 : 4);
   static constexpr int WKP = D + (BF ? 8 : 4);
-  static constexpr int HP = H + (BF ? 8 : 4);  // H tile row
-  static constexpr int MP = D + (BF ? 8 : 4);  // message tile row
-  static constexpr int WAVES = 4;
-  static constexpr int ROWS = 16;              // rows per wave per iteration
+  static constexpr int HP = H + (BF ? 8 : 4);
+  static constexpr int MP = D + (BF ? 8 : 4);
+  static constexpr int WAVES = 8;
+  static constexpr int ROWS = 16;                   // rows per wave per iteration
+  static constexpr int THREADS = 64 * WAVES;
 };
 
 template <typename T>
@@ -497,110 +527,173 @@ __device__ __forceinline__ void mfma_step<float>(f32x4 &acc, const float *a, con
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(*a, *b, acc, 0, 0, 0);
 }
 
+// GELU of the bf16 path: x * Phi(x) with erf from Abramowitz & Stegun
+// 7.1.26 (|error| < 1.5e-7, far below the bf16 rounding of the result).
+__device__ __forceinline__ float gelu_bf(float x) {
+  const float z = fabsf(x) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * z);
+  float y = 1.061405429f;
+  y = y * t - 1.453152027f;
+  y = y * t + 1.421413741f;
+  y = y * t - 0.284496736f;
+  y = y * t + 0.254829592f;
+  const float e = 1.0f - y * t * __expf(-z * z);  // erf(|x| / sqrt 2)
+  return 0.5f * x * (1.0f + copysignf(e, x));
+}
+
+template <typename T>
+__device__ __forceinline__ float gelu_t(float x) {
+  return sizeof(T) == 2 ? gelu_bf(x) : gelu(x);
+}
+
+// Wave-local LDS hand-off: every lane's LDS writes are complete before any
+// lane reads (one wave; no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Row MLP: for each row r (item i = r / K, axis k = r % K):
 //   x = [half0 | half1]  (D or 2D wide, T)
 //   h = gelu(W1 x + b1 + S0[st0] + S1[st1] + c0 * z0 + c1 * z1)   (H)
 //   y = W2 h + b2                                                  (D)
 // messages also get score = (wk2 . gelu(Wk1 y + bk1) + bk2) * scale.
-// MFMA fragment layouts (16x16 tiles, lane l):
-//   A: row l%16, k = KS/4*(l/16) .. +KS/4-1 ; B: col l%16, same k ;
-//   D: rows 4*(l/16) .. +3, col l%16.
+// Each wave walks its own 16-row groups (no workgroup barrier after the
+// weights are staged) and prefetches the next group's pair records.
+// MFMA fragment layouts (16x16 tiles, lane l): A row l%16, k = KS/4*(l/16)..;
+// B col l%16, same k; D rows 4*(l/16)..+3, col l%16.
 template <int D, typename T, int MODE>
-__global__ __launch_bounds__(256) void row_mlp_kernel(RowArgs a, MlpW<T> w,
-                                                      KeyW<T> pk) {
+__global__ __launch_bounds__(512) void row_mlp_kernel(RowArgs a, MlpW<T> w, KeyW<T> pk) {
   using C = RowCfg<D, T>;
-  constexpr int H = C::H, KS = C::KS, KL = KS / 4;  // elements per lane per step
+  constexpr int H = C::H, KS = C::KS, KL = KS / 4;
   constexpr int K1 = MODE == ROW_DIR ? D : 2 * D;
-  extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NW = C::BF ? H * C::W1P + D * C::W2P + D * C::WKP : 0;
-  T *sW1 = (T *)smem;                           // H x W1P   (bf16 only)
-  T *sW2 = sW1 + H * C::W1P;                    // D x W2P   (bf16 only)
-  T *sWk = sW2 + D * C::W2P;                    // D x WKP   (bf16 only)
-  T *sH = (T *)smem + NW;                       // WAVES x ROWS x HP
-  T *sM = sH + C::WAVES * C::ROWS * C::HP;      // WAVES x ROWS x MP
-  float *sAdd = (float *)(sM + C::WAVES * C::ROWS * C::MP);  // WAVES x ROWS x 4
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *sW1 = (T *)smem;
+  T *sW2 = sW1 + H * C::W1P;
+  T *sWk = sW2 + D * C::W2P;
+  float *tb = (float *)(smem + sizeof(T) * (size_t)NW);
+  float *tb1 = tb, *tc0 = tb1 + H, *tc1 = tc0 + H, *tb2 = tc1 + H, *tbk1 = tb2 + D;
+  float *twk2 = tbk1 + D, *ts0 = twk2 + D, *ts1 = ts0 + a.ns * H;
+  int8_t *tsx = (int8_t *)(ts1 + a.ns * H);                      // L*K*2, padded to 1024
+  unsigned char *pw = (unsigned char *)tsx + 1024;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr size_t PER_WAVE = sizeof(T) * (C::ROWS * C::HP + C::ROWS * C::MP) + 16 * 4 * 4;
+  T *sH = (T *)(pw + wv * PER_WAVE);
+  T *sM = sH + C::ROWS * C::HP;
+  float *add = (float *)(sM + C::ROWS * C::MP);
+  if constexpr (C::BF) {
+    for (int i = tid; i < H * K1; i += C::THREADS) sW1[(i / K1) * C::W1P + i % K1] = w.w1[i];
+    for (int i = tid; i < D * H; i += C::THREADS) sW2[(i / H) * C::W2P + i % H] = w.w2[i];
+    if constexpr (MODE != ROW_FIN)
+      for (int i = tid; i < D * D; i += C::THREADS) sWk[(i / D) * C::WKP + i % D] = pk.wk1[i];
+  }
+  for (int i = tid; i < H; i += C::THREADS) {
+    tb1[i] = w.b1[i];
+    tc0[i] = w.c0 ? w.c0[i] : 0.f;
+    tc1[i] = w.c1 ? w.c1[i] : 0.f;
+  }
+  for (int i = tid; i < D; i += C::THREADS) {
+    tb2[i] = w.b2[i];
+    if constexpr (MODE != ROW_FIN) {
+      tbk1[i] = pk.bk1[i];
+      twk2[i] = pk.wk2[i];
+    }
+  }
+  for (int i = tid; i < a.ns * H; i += C::THREADS) {
+    ts0[i] = w.s0 ? w.s0[i] : 0.f;
+    ts1[i] = w.s1 ? w.s1[i] : 0.f;
+  }
+  if constexpr (MODE != ROW_FIN)
+    for (int i = tid; i < a.L * a.K * 2; i += C::THREADS) tsx[i] = a.sidx[i];
+  __syncthreads();
 
-  // bf16: weights staged in padded LDS rows; fp32: read from global (L2)
   const T *gW1 = C::BF ? sW1 : w.w1;
   const T *gW2 = C::BF ? sW2 : w.w2;
   const T *gWk = C::BF ? sWk : pk.wk1;
   constexpr int P1 = C::BF ? C::W1P : K1, P2 = C::BF ? C::W2P : H, PK = C::BF ? C::WKP : D;
-  if constexpr (C::BF) {
-    for (int i = tid; i < H * K1; i += 256) sW1[(i / K1) * C::W1P + i % K1] = w.w1[i];
-    for (int i = tid; i < D * H; i += 256) sW2[(i / H) * C::W2P + i % H] = w.w2[i];
-    if constexpr (MODE != ROW_FIN) {
-      for (int i = tid; i < D * D; i += 256) sWk[(i / D) * C::WKP + i % D] = pk.wk1[i];
-    }
-  }
-  __syncthreads();
-
-  const int64_t n_rows = a.n_rows_dev ? (int64_t)a.n_rows_dev[0] * a.K : a.n_rows_host;
+  const int64_t n_rows = MODE == ROW_FIN ? a.n_rows_host : (int64_t)a.n_items_dev[0] * a.K;
   const int64_t slot_base = a.slot_base_dev ? (int64_t)a.slot_base_dev[0] : a.slot_base;
   const T *E = (const T *)a.E;
-  const int64_t per_iter = (int64_t)C::WAVES * C::ROWS;
-  for (int64_t base = (int64_t)blockIdx.x * per_iter; base < n_rows;
-       base += (int64_t)gridDim.x * per_iter) {
-    const int64_t r0 = base + wv * C::ROWS;
-    // --- per-row sources (lanes 0..15 own row r0 + lane%16 for the A operand)
-    const int ri = lane & 15;
+  const int64_t gstep = (int64_t)gridDim.x * C::WAVES;
+  const int ri = lane & 15, kq = (lane >> 4) * KL;
+  constexpr int NK1 = K1 / KS, NK2 = H / KS, NK3 = D / KS;
+
+  int64_t grp = (int64_t)blockIdx.x * C::WAVES + wv;
+  // metadata of this lane's row in group g
+  PairRec rc{0, 0, 0.f, 0.f};
+  int16_t lc = 0;
+  auto fetch = [&](int64_t g, PairRec &r, int16_t &lw) {
+    const int64_t row = g * C::ROWS + ri;
+    if constexpr (MODE != ROW_FIN) {
+      if (row < n_rows) {
+        const int64_t item = row / a.K;
+        r = a.rec[item];
+        lw = a.lw[item];
+      }
+    }
+  };
+  if (grp * C::ROWS < n_rows) fetch(grp, rc, lc);
+  for (; grp * C::ROWS < n_rows; grp += gstep) {
+    // prefetch the next group's records
+    PairRec rn{0, 0, 0.f, 0.f};
+    int16_t ln = 0;
+    if ((grp + gstep) * C::ROWS < n_rows) fetch(grp + gstep, rn, ln);
+    const int64_t r0 = grp * C::ROWS;
     const int64_t r = r0 + ri;
     const bool live = r < n_rows;
+    const int64_t item = live ? r / a.K : 0;
+    const int k = live ? (int)(r % a.K) : 0;
     const T *src0 = nullptr, *src1 = nullptr;
     float z0 = 0.f, z1 = 0.f;
     int st0 = 0, st1 = 0;
     if (live) {
-      const int64_t item = r / a.K;
-      const int k = (int)(r % a.K);
       if constexpr (MODE == ROW_FIN) {
         src0 = (const T *)a.ctx + (item * a.K + k) * D;
         src1 = (const T *)a.vemb + item * D;
+      } else if constexpr (MODE == ROW_BI) {
+        src0 = E + ((int64_t)rc.ra * a.K + k) * D;  // negative side first
+        src1 = E + ((int64_t)rc.rb * a.K + k) * D;
+        const int l = lc & 0x3fff;
+        st0 = tsx[(l * a.K + k) * 2 + 0];
+        st1 = tsx[(l * a.K + k) * 2 + 1];
+        z0 = rc.za;
+        z1 = rc.zb;
       } else {
-        const int32_t p = a.list[item];
-        const int m = p / a.L, l = p % a.L;
-        const int32_t stp = a.steps[p];
-        const int sp = stp & 0xffff, sn = stp >> 16;
-        const int64_t q = a.qidx[m], off = a.line_off[l];
-        const float lnz = a.lognnz[l];
-        if constexpr (MODE == ROW_BI) {
-          src0 = E + ((q - (int64_t)sn * off) * a.K + k) * D;  // negative side first
-          src1 = E + ((q + (int64_t)sp * off) * a.K + k) * D;
-          st0 = a.sidx[(l * a.K + k) * 2 + 0];
-          st1 = a.sidx[(l * a.K + k) * 2 + 1];
-          z0 = log2f((float)sn) + lnz;
-          z1 = log2f((float)sp) + lnz;
-        } else {
-          const bool pos = sp > 0;
-          const int t = pos ? sp : sn;
-          src0 = E + ((q + (pos ? (int64_t)t : -(int64_t)t) * off) * a.K + k) * D;
-          st0 = a.sidx[(l * a.K + k) * 2 + (pos ? 1 : 0)];
-          z0 = pos ? 1.0f : -1.0f;
-          z1 = log2f((float)t) + lnz;
-        }
+        src0 = E + ((int64_t)rc.ra * a.K + k) * D;
+        const int l = lc & 0x3fff, side = (lc >> 14) & 1;
+        st0 = tsx[(l * a.K + k) * 2 + side];
+        z0 = rc.za;
+        z1 = rc.zb;
       }
     }
-    // row-scalar table for the epilogue (rows are indexed by D-layout rows)
-    float *add = sAdd + (wv * C::ROWS) * 4;
     if (lane < 16) {
       add[ri * 4 + 0] = z0;
       add[ri * 4 + 1] = z1;
       add[ri * 4 + 2] = __int_as_float(st0);
       add[ri * 4 + 3] = __int_as_float(st1);
     }
-    // --- layer 1: A fragments straight from the sources
-    constexpr int NK1 = K1 / KS;
     T afr[NK1][KL];
-    const int kq = (lane >> 4) * KL;
 #pragma unroll
     for (int ks = 0; ks < NK1; ks++) {
       const int kk = ks * KS + kq;
-      const T *s = (MODE == ROW_DIR || kk < D) ? src0 : src1;
+      const T *sp_ = (MODE == ROW_DIR || kk < D) ? src0 : src1;
       const int ko = (MODE == ROW_DIR || kk < D) ? kk : kk - D;
+      if (live) {
+        if constexpr (C::BF) {
+          memcpy(&afr[ks][0], sp_ + ko, 16);
+        } else {
 #pragma unroll
-      for (int e = 0; e < KL; e++) afr[ks][e] = live ? s[ko + e] : T(0);
+          for (int e = 0; e < KL; e++) afr[ks][e] = sp_[ko + e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < KL; e++) afr[ks][e] = T(0);
+      }
     }
-    __syncthreads();  // add[] visible
-    T *hrow = sH + (wv * C::ROWS) * C::HP;
+    wave_lds_sync();
+    // --- layer 1
 #pragma unroll 1
     for (int cb = 0; cb < H / 16; cb++) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -608,30 +701,27 @@ __global__ __launch_bounds__(256) void row_mlp_kernel(RowArgs a, MlpW<T> w,
 #pragma unroll
       for (int ks = 0; ks < NK1; ks++) mfma_step<T>(acc, afr[ks], bw + ks * KS);
       const int col = cb * 16 + (lane & 15);
-      const float b1 = w.b1[col];
-      const float c0 = w.c0 ? w.c0[col] : 0.f, c1 = w.c1 ? w.c1[col] : 0.f;
+      const float b1 = tb1[col], c0 = tc0[col], c1 = tc1[col];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int row = (lane >> 4) * 4 + j;
         const float *ad = add + row * 4;
         float v = acc[j] + b1;
-        if (w.s0) v += w.s0[__float_as_int(ad[2]) * H + col];
-        if (w.s1) v += w.s1[__float_as_int(ad[3]) * H + col];
+        if (MODE != ROW_FIN) v += ts0[__float_as_int(ad[2]) * H + col];
+        if (MODE == ROW_BI) v += ts1[__float_as_int(ad[3]) * H + col];
         v += c0 * ad[0];
         v += c1 * ad[1];
-        hrow[row * C::HP + col] = st<T>(gelu(v));
+        sH[row * C::HP + col] = st<T>(gelu_t<T>(v));
       }
     }
-    __syncthreads();
-    // --- layer 2: A fragments from the H tile
-    constexpr int NK2 = H / KS;
+    wave_lds_sync();
+    // --- layer 2
     T hfr[NK2][KL];
-    const T *hsrc = hrow + ri * C::HP + kq;
+    const T *hsrc = sH + ri * C::HP + kq;
 #pragma unroll
     for (int ks = 0; ks < NK2; ks++)
 #pragma unroll
       for (int e = 0; e < KL; e++) hfr[ks][e] = hsrc[ks * KS + e];
-    T *mrow = sM + (wv * C::ROWS) * C::MP;
 #pragma unroll 1
     for (int cb = 0; cb < D / 16; cb++) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -639,14 +729,14 @@ __global__ __launch_bounds__(256) void row_mlp_kernel(RowArgs a, MlpW<T> w,
 #pragma unroll
       for (int ks = 0; ks < NK2; ks++) mfma_step<T>(acc, hfr[ks], bw + ks * KS);
       const int col = cb * 16 + (lane & 15);
-      const float b2 = w.b2[col];
+      const float b2 = tb2[col];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int row = (lane >> 4) * 4 + j;
-        mrow[row * C::MP + col] = st<T>(acc[j] + b2);
+        sM[row * C::MP + col] = st<T>(acc[j] + b2);
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     // --- coalesced store of the 16 output rows
     {
       T *out = (T *)a.out;
@@ -656,19 +746,18 @@ __global__ __launch_bounds__(256) void row_mlp_kernel(RowArgs a, MlpW<T> w,
         if (rr >= n_rows) continue;
         int64_t dst;
         if constexpr (MODE == ROW_FIN) {
-          const int64_t item = rr / a.K;
-          dst = (a.qout[item] * a.K + rr % a.K) * D + col;
+          const int64_t it = rr / a.K;
+          dst = (a.qout[it] * a.K + rr % a.K) * D + col;
         } else {
           dst = (slot_base * a.K + rr) * D + col;
         }
-        out[dst] = mrow[row * C::MP + col];
+        out[dst] = sM[row * C::MP + col];
       }
     }
     // --- attention key of each message row
     if constexpr (MODE != ROW_FIN) {
-      constexpr int NK3 = D / KS;
       T mfr[NK3][KL];
-      const T *msrc = mrow + ri * C::MP + kq;
+      const T *msrc = sM + ri * C::MP + kq;
 #pragma unroll
       for (int ks = 0; ks < NK3; ks++)
 #pragma unroll
@@ -681,11 +770,10 @@ __global__ __launch_bounds__(256) void row_mlp_kernel(RowArgs a, MlpW<T> w,
 #pragma unroll
         for (int ks = 0; ks < NK3; ks++) mfma_step<T>(acc, mfr[ks], bw + ks * KS);
         const int col = cb * 16 + (lane & 15);
-        const float bk1 = pk.bk1[col], wk2 = pk.wk2[col];
+        const float bk1 = tbk1[col], wk2 = twk2[col];
 #pragma unroll
-        for (int j = 0; j < 4; j++) part[j] += wk2 * gelu(acc[j] + bk1);
+        for (int j = 0; j < 4; j++) part[j] += wk2 * gelu_t<T>(acc[j] + bk1);
       }
-      // sum over the 16 columns held by lanes of the same row group
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         float v = part[j];
@@ -703,16 +791,19 @@ __global__ __launch_bounds__(256) void row_mlp_kernel(RowArgs a, MlpW<T> w,
         }
       }
     }
-    __syncthreads();
+    wave_lds_sync();
+    rc = rn;
+    lc = ln;
   }
 }
 
 template <int D, typename T>
-static size_t row_mlp_smem() {
+static size_t row_mlp_smem(int ns) {
   using C = RowCfg<D, T>;
   const size_t nw = C::BF ? (size_t)C::H * C::W1P + (size_t)D * C::W2P + (size_t)D * C::WKP : 0;
-  size_t n = nw + (size_t)C::WAVES * C::ROWS * C::HP + (size_t)C::WAVES * C::ROWS * C::MP;
-  return n * sizeof(T) + (size_t)C::WAVES * C::ROWS * 4 * sizeof(float);
+  const size_t tables = sizeof(float) * (3 * (size_t)C::H + 3 * (size_t)D + 2 * (size_t)ns * C::H);
+  const size_t per_wave = sizeof(T) * (C::ROWS * C::HP + C::ROWS * C::MP) + 16 * 4 * 4;
+  return sizeof(T) * nw + tables + 1024 + C::WAVES * per_wave;
 }
 
 // ---------------------------------------------------------------------------
@@ -1258,22 +1349,27 @@ int launch_rows(latec_gnn *g, int mode, RowArgs &a, hipStream_t s) {
     w = {(const T *)g->mx_w1, (const T *)g->mx_w2, fs(g, F_MX_B1), fs(g, F_MX_B2),
          nullptr, nullptr, nullptr, nullptr, 2 * D};
   }
-  const size_t smem = row_mlp_smem<D, T>();
-  a.mode = mode;
-  a.pad = 0;
-  const int grid = 2 * g->n_cu;
+  a.ns = g->lay.ns;
+  a.L = g->L;
+  a.sidx = g->d_sidx;
+  const size_t smem = row_mlp_smem<D, T>(a.ns);
+  const int grid = g->n_cu;
   const void *fn = mode == ROW_BI    ? (const void *)row_mlp_kernel<D, T, ROW_BI>
                    : mode == ROW_DIR ? (const void *)row_mlp_kernel<D, T, ROW_DIR>
                                      : (const void *)row_mlp_kernel<D, T, ROW_FIN>;
   int rc = hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
                   "hipFuncSetAttribute(row_mlp_kernel)");
   if (rc) return rc;
+  using C = RowCfg<D, T>;
   if (mode == ROW_BI)
-    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_BI>), dim3(grid), dim3(256), smem, s, a, w, pk);
+    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_BI>), dim3(grid), dim3(C::THREADS), smem, s, a,
+                       w, pk);
   else if (mode == ROW_DIR)
-    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_DIR>), dim3(grid), dim3(256), smem, s, a, w, pk);
+    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_DIR>), dim3(grid), dim3(C::THREADS), smem, s, a,
+                       w, pk);
   else
-    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_FIN>), dim3(grid), dim3(256), smem, s, a, w, pk);
+    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_FIN>), dim3(grid), dim3(C::THREADS), smem, s, a,
+                       w, pk);
   return hip_ok(hipGetLastError(), "row_mlp_kernel launch");
 }
 
@@ -1358,17 +1454,20 @@ int embed_t(latec_gnn *g, const void *E, const int64_t *frame_shape, const int32
   const int64_t nblk = (np + PAIR_BLOCK - 1) / PAIR_BLOCK;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t b_counts = al(sizeof(int32_t) * 2 * nblk), b_tot = al(16);
-  const size_t b_list = al(sizeof(int32_t) * np);
+  const size_t b_slot = al(sizeof(int32_t) * np);
+  const size_t b_rec = al(sizeof(PairRec) * np), b_lw = al(sizeof(int16_t) * np);
   const size_t b_msg = al((size_t)np * K * D * sizeof(T)), b_score = al(sizeof(float) * np * K);
-  rc = ensure_work(g, b_counts + b_tot + 3 * b_list + b_msg + b_score);
+  rc = ensure_work(g, b_counts + b_tot + b_slot + 2 * (b_rec + b_lw) + b_msg + b_score);
   if (rc) return rc;
   unsigned char *w = (unsigned char *)g->work;
   int32_t *counts = (int32_t *)w;
   int32_t *totals = (int32_t *)(w + b_counts);
-  int32_t *list_bi = (int32_t *)(w + b_counts + b_tot);
-  int32_t *list_dir = list_bi + b_list / sizeof(int32_t);
-  int32_t *slot = list_dir + b_list / sizeof(int32_t);
-  T *msg = (T *)((unsigned char *)slot + b_list);
+  int32_t *slot = (int32_t *)(w + b_counts + b_tot);
+  PairRec *rec_bi = (PairRec *)((unsigned char *)slot + b_slot);
+  PairRec *rec_dir = (PairRec *)((unsigned char *)rec_bi + b_rec);
+  int16_t *lw_bi = (int16_t *)((unsigned char *)rec_dir + b_rec);
+  int16_t *lw_dir = (int16_t *)((unsigned char *)lw_bi + b_lw);
+  T *msg = (T *)((unsigned char *)lw_dir + b_lw);
   float *score = (float *)((unsigned char *)msg + b_msg);
   const PoolW pw = pool_w(g, true);
   const float scale = 1.0f / std::sqrt((float)D);
@@ -1381,28 +1480,26 @@ int embed_t(latec_gnn *g, const void *E, const int64_t *frame_shape, const int32
                        n_pairs, counts);
     hipLaunchKernelGGL(pair_scan_kernel, dim3(1), dim3(1024), 0, s, counts, nb, totals);
     hipLaunchKernelGGL(pair_write_kernel, dim3((unsigned)nb), dim3(PAIR_BLOCK), 0, s, st_t,
-                       n_pairs, counts, totals, list_bi, list_dir, slot);
+                       n_pairs, L, qidx + m0, (const int64_t *)g->d_line_off,
+                       (const float *)g->d_lognnz, counts, totals, rec_bi, lw_bi, rec_dir,
+                       lw_dir, slot);
     rc = hip_ok(hipGetLastError(), "pair kernels");
     if (rc) return rc;
     RowArgs a{};
     a.K = K;
-    a.L = L;
-    a.steps = st_t;
-    a.qidx = qidx + m0;
-    a.line_off = g->d_line_off;
-    a.lognnz = g->d_lognnz;
-    a.sidx = g->d_sidx;
     a.E = E;
     a.out = msg;
     a.score = score;
     a.score_scale = scale;
-    a.list = list_bi;
-    a.n_rows_dev = totals;
+    a.rec = rec_bi;
+    a.lw = lw_bi;
+    a.n_items_dev = totals;
     a.slot_base = 0;
     rc = launch_rows<D, T>(g, ROW_BI, a, s);
     if (rc) return rc;
-    a.list = list_dir;
-    a.n_rows_dev = totals + 1;
+    a.rec = rec_dir;
+    a.lw = lw_dir;
+    a.n_items_dev = totals + 1;
     a.slot_base_dev = totals;
     rc = launch_rows<D, T>(g, ROW_DIR, a, s);
     if (rc) return rc;
@@ -1460,7 +1557,6 @@ int finalize_t(latec_gnn *g, void *E, const void *ctx, const int64_t *qidx,
   if (rc) return rc;
   RowArgs a{};
   a.K = g->ndim;
-  a.L = g->L;
   a.n_rows_host = M * g->ndim;
   a.ctx = ctx;
   a.vemb = vemb;
@@ -1532,6 +1628,11 @@ int latec_gnn_create(int d, int agg_level, int ndim, int numerics, const float *
   g->numerics = numerics;
   g->lay = ly;
   build_lines(g);
+  if (g->L * ndim * 2 > 1024) {
+    latec::set_error("too many (line, axis) state entries for this rank and aggregation level");
+    delete g;
+    return LATEC_E_ARG;
+  }
   int dev = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&g->n_cu, hipDeviceAttributeMultiprocessorCount, dev);

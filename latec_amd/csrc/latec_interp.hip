@@ -804,6 +804,272 @@ __global__ void __launch_bounds__(256) code_hist_kernel(const CodeT *__restrict_
   flush_hist(lhist, ghist, R);
 }
 
+// ---------------------------------------------------------------------------
+// Marching class kernels (3-D level grids of the lattice pyramid, one
+// channel, 16-bit codes).
+//
+// On a level's compact grid (stride 1) parity class P holds the points whose
+// coordinate is odd exactly on the axes of P.  A thread owns a column of
+// MARCH_M consecutive class points along its marching axis (axis 0 when odd
+// in P, else axis 1 when odd, else axis 0) and walks it, carrying the taps
+// c-3, c-1, c+1, c+3 of that axis in registers: one new tap load per point
+// instead of four.  Lanes run along the last axis, so a wave's x, recon and
+// code accesses are row segments.  One launch takes up to three classes,
+// blockIdx.y picking one: the three weight-1 stages of a level are
+// independent of each other and share a launch.  P is a template argument, so
+// the axis loop, the centre combine and the tap validity are compiled per
+// class.  The arithmetic is interp_axis / predict_point / quantize_one's,
+// operation for operation, so codes, outliers and recon are bit-identical to
+// the stage kernels'.
+// ---------------------------------------------------------------------------
+#define MARCH_M 8
+
+struct MClass {
+  int32_t parity, stage;  // stage: index into MArgs::st
+  uint32_t m[3];          // class extents
+  uint32_t e[3];          // thread extents (the marching axis divided by MARCH_M)
+  uint32_t nthreads, pad;
+  int64_t offset;         // stage rank of the class's first point
+  int64_t rstr[3];        // stage-rank strides per class index
+};
+
+struct MArgs {
+  int32_t n[3];
+  int32_t cubic, center, end_mode, round_mode, ncls;
+  double span, offset;
+  int64_t hist_words;
+  L3Stage st[3];
+  MClass cl[3];
+};
+
+// interp_axis on taps already read (A = c-3s, B = c-s, C = c+s, D = c+3s).
+__device__ __forceinline__ double interp4(double A, double B, double C, double D, bool vm3,
+                                          bool vp1, bool vp3, int cubic, int end_mode) {
+  if (!vp1) {
+    if ((end_mode & 2) && vm3) return 1.5 * B - 0.5 * A;
+    return B;
+  }
+  if (cubic) {
+    if (vm3 && vp3) return (((-A + 9.0 * B) + 9.0 * C) - D) * 0.0625;
+    if (end_mode & 1) {
+      if (vp3) return ((3.0 * B + 6.0 * C) - D) * 0.125;
+      if (vm3) return ((-A + 6.0 * B) + 3.0 * C) * 0.125;
+    }
+  }
+  return 0.5 * (B + C);
+}
+
+__device__ __forceinline__ void hist_add(uint32_t *lhist, uint32_t *__restrict__ ghist,
+                                         uint32_t win_lo, uint32_t code, uint32_t k) {
+  const uint32_t b = code - win_lo;
+  if (code != 0 && b < HIST_WIN)
+    atomicAdd(&lhist[b], k);
+  else
+    atomicAdd(&ghist[code], k);
+}
+
+// emit() with the histogram counted per wave: the wave's most common codes
+// (up to four distinct values, found by ballot) take one atomic each, the
+// remaining lanes one atomic per lane.
+__device__ __forceinline__ void emit_wave(uint16_t *__restrict__ codes, int64_t slot,
+                                          uint32_t code, float x, uint32_t rank,
+                                          uint32_t *__restrict__ out_rank,
+                                          float *__restrict__ out_val,
+                                          uint32_t *__restrict__ out_count, uint32_t *lhist,
+                                          uint32_t *__restrict__ ghist, uint32_t win_lo,
+                                          bool active) {
+  const int lane = __lane_id();
+  if (active) codes[slot] = (uint16_t)code;
+  bool todo = active;
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    const uint64_t m = __ballot(todo);
+    if (!m) break;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)code, leader);
+    const bool mine = todo && code == c0;
+    const uint64_t mm = __ballot(mine);
+    if (lane == leader) hist_add(lhist, ghist, win_lo, c0, (uint32_t)__popcll(mm));
+    todo = todo && !mine;
+  }
+  if (todo) hist_add(lhist, ghist, win_lo, code, 1u);
+  const bool out = active && code == 0;
+  const uint64_t m = __ballot(out);
+  if (m) {
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(out_count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (out) {
+      const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      out_rank[k] = rank;
+      out_val[k] = x;
+    }
+  }
+}
+
+template <int P, bool ENC>
+__device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
+                                            const float *__restrict__ x,
+                                            float *__restrict__ rec,
+                                            uint16_t *__restrict__ codes,
+                                            uint32_t *__restrict__ hist, uint32_t *lhist,
+                                            uint32_t *__restrict__ out_rank,
+                                            float *__restrict__ out_val,
+                                            uint32_t *__restrict__ out_count) {
+  constexpr int MA = (P & 1) ? 0 : (P & 2) ? 1 : 0;
+  constexpr bool WIN = (P >> MA) & 1;
+  constexpr int NODD = (P & 1) + ((P >> 1) & 1) + ((P >> 2) & 1);
+  const L3Stage &S = A.st[K.stage];
+  const uint32_t gstr[3] = {(uint32_t)A.n[1] * (uint32_t)A.n[2], (uint32_t)A.n[2], 1u};
+  const uint32_t sMA = gstr[MA];
+  const int nMA = A.n[MA];
+  GridArgs g;
+  g.w = S.w;
+  g.inv_w = S.inv_w;
+  g.R = S.R;
+  g.feb = S.feb;
+  g.round_mode = A.round_mode;
+  g.span = A.span;
+  g.offset = A.offset;
+  const uint32_t win_lo = S.R > HIST_WIN / 2 ? S.R - HIST_WIN / 2 : 0;
+  uint32_t *ghist = ENC ? hist + (int64_t)S.gs * A.hist_words : nullptr;
+  uint32_t *orank = out_rank + S.out_lo;
+  float *oval = out_val + S.out_lo;
+  uint32_t *ocnt = out_count + S.gs;
+  const int64_t nout = ENC ? 0 : (int64_t)out_count[S.gs];
+  const uint32_t G = gridDim.x;
+  const uint32_t lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  for (uint32_t base = lb * 256u; base < K.nthreads; base += G * 256u) {
+    const uint32_t tid = base + threadIdx.x;
+    const bool tv = tid < K.nthreads;
+    uint32_t u = tv ? tid : 0u;
+    uint32_t ci[3];
+    ci[2] = u % K.e[2];
+    u /= K.e[2];
+    ci[1] = u % K.e[1];
+    ci[0] = u / K.e[1];
+    ci[MA] *= MARCH_M;
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) c[a] = ((P >> a) & 1) + 2 * (int)ci[a];
+    uint32_t f = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
+    int64_t rank = K.offset + (int64_t)ci[0] * K.rstr[0] + (int64_t)ci[1] * K.rstr[1] +
+                   (int64_t)ci[2] * K.rstr[2];
+    const uint32_t cnt = tv ? min((uint32_t)MARCH_M, K.m[MA] - ci[MA]) : 0u;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;  // taps c-3, c-1, c+1, c+3 along MA
+    if (WIN && cnt) {
+      const int cm = c[MA];
+      if (cm >= 3) w0 = (double)rec[f - 3 * sMA];
+      w1 = (double)rec[f - sMA];
+      if (cm + 1 < nMA) w2 = (double)rec[f + sMA];
+    }
+    for (uint32_t t = 0; t < (uint32_t)MARCH_M; t++) {
+      const bool active = t < cnt;
+      uint32_t code = 0;
+      float xv = 0.f;
+      if (active) {
+        const int cm = c[MA];
+        if (WIN && cm + 3 < nMA) w3 = (double)rec[f + 3 * sMA];
+        double total = 0.0, v_first = 0.0, v_last = 0.0;
+        int count = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          if (!((P >> a) & 1)) continue;
+          const int ca = c[a], na = A.n[a];
+          const bool vm3 = ca >= 3, vp1 = ca + 1 < na, vp3 = ca + 3 < na;
+          double v;
+          if (WIN && a == MA) {
+            v = interp4(w0, w1, w2, w3, vm3, vp1, vp3, A.cubic, A.end_mode);
+          } else {
+            const uint32_t sa = gstr[a];
+            const double B = (double)rec[f - sa];
+            const double C = vp1 ? (double)rec[f + sa] : 0.0;
+            const double Av = vm3 ? (double)rec[f - 3 * sa] : 0.0;
+            const double D = vp3 ? (double)rec[f + 3 * sa] : 0.0;
+            v = interp4(Av, B, C, D, vm3, vp1, vp3, A.cubic, A.end_mode);
+          }
+          total = count ? total + v : v;
+          if (!count) v_first = v;
+          v_last = v;
+          count++;
+        }
+        double res;
+        if (A.center == 0 || NODD == 1)
+          res = NODD == 1 ? total : total / (double)NODD;
+        else
+          res = A.center == 1 ? v_first : v_last;
+        const float p = (float)res;
+        float r;
+        if (ENC) {
+          xv = x[f];
+          code = quantize_one(xv, p, g, &r);
+        } else {
+          code = codes[S.code_base + rank];
+          if (code) {
+            r = (float)((double)p + S.w * (double)((int)code - (int)S.R));
+          } else {
+            int64_t lo = 0, hi = nout;
+            while (lo < hi) {
+              const int64_t mid = (lo + hi) >> 1;
+              if ((int64_t)orank[mid] < rank) lo = mid + 1; else hi = mid;
+            }
+            r = (lo < nout && (int64_t)orank[lo] == rank) ? oval[lo]
+                                                          : __int_as_float(0x7fc00000);
+          }
+        }
+        rec[f] = r;
+      }
+      if (ENC)
+        emit_wave(codes, S.code_base + rank, code, xv, (uint32_t)rank, orank, oval, ocnt, lhist,
+                  ghist, win_lo, active);
+      if (WIN) {
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+      }
+      f += 2 * sMA;
+      rank += K.rstr[MA];
+      c[MA] += 2;
+    }
+  }
+}
+
+template <bool ENC>
+__global__ void __launch_bounds__(256) march_kernel(MArgs A, const float *__restrict__ x,
+                                                    float *__restrict__ rec,
+                                                    uint16_t *__restrict__ codes,
+                                                    uint32_t *__restrict__ hist,
+                                                    uint32_t *__restrict__ out_rank,
+                                                    float *__restrict__ out_val,
+                                                    uint32_t *__restrict__ out_count) {
+  __shared__ uint32_t lhist[ENC ? HIST_WIN : 1];
+  if (ENC) {
+    for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x) lhist[i] = 0;
+    __syncthreads();
+  }
+  const MClass &K = A.cl[blockIdx.y];
+  switch (K.parity) {
+#define LATEC_MARCH_CASE(PP)                                                                 \
+  case PP:                                                                                   \
+    march_class<PP, ENC>(A, K, x, rec, codes, hist, lhist, out_rank, out_val, out_count); \
+    break;
+    LATEC_MARCH_CASE(1)
+    LATEC_MARCH_CASE(2)
+    LATEC_MARCH_CASE(3)
+    LATEC_MARCH_CASE(4)
+    LATEC_MARCH_CASE(5)
+    LATEC_MARCH_CASE(6)
+    LATEC_MARCH_CASE(7)
+#undef LATEC_MARCH_CASE
+    default: break;
+  }
+  if (ENC) {
+    __syncthreads();
+    flush_hist(lhist, hist + (int64_t)A.st[K.stage].gs * A.hist_words, A.st[K.stage].R);
+  }
+}
+
 // Lattice gather / scatter between a grid (shape `fine`) and its every-other
 // point lattice (shape ceil(fine / 2)): gather coarse[m] = fine[2m],
 // scatter fine[2m] = coarse[m].  One thread per coarse point, C channels.
@@ -1268,6 +1534,91 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
   return hip_check(hipGetLastError(), "level3_kernel");
 }
 
+// Marching class launches for a 3-D level (one channel, u16 codes): the
+// three weight-1 stages together, then the weight-2 stage, then weight 3.
+// Returns 1 if the level's layout is not the one the kernel assumes.
+static bool march_ok(const latec_plan *gp, const latec_plan *q, const latec_interp_opts *o) {
+  static const bool off = getenv("LATEC_NO_MARCH") != nullptr;
+  if (off) return false;
+  return gp->ndim == 3 && gp->channels == 1 && o->code_bytes == 2 && q->stages.size() == 6 &&
+         q->npoints < 0x7fffffffll;
+}
+
+static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool encode,
+                        const float *x, float *recon, void *codes, uint32_t *hist,
+                        uint32_t *out_rank, float *out_val, uint32_t *out_count,
+                        const double *ebs, const latec_interp_opts *o, hipStream_t stream) {
+  static const int groups[3][3] = {{1, 2, 3}, {4, 0, 0}, {5, 0, 0}};  // sub-plan stages
+  MArgs A[3];
+  for (int gi = 0; gi < 3; gi++) {
+    MArgs &a = A[gi];
+    memset(&a, 0, sizeof a);
+    for (int j = 0; j < 3; j++) a.n[j] = (int32_t)q->shape[j];
+    a.cubic = o->cubic;
+    a.center = o->center;
+    a.end_mode = o->end_mode;
+    a.round_mode = o->round_mode;
+    a.span = o->span;
+    a.offset = o->offset;
+    a.hist_words = hist_words(o->radius);
+    int ns = 0;
+    for (int si = 0; si < 3 && groups[gi][si]; si++) {
+      const int js = groups[gi][si], gs = g0 + js - 1;
+      const StageDesc &st = q->stages[js];
+      const GridArgs g = make_args(q, js, ebs[gs], o);
+      L3Stage &S = a.st[ns];
+      S.w = g.w;
+      S.inv_w = g.inv_w;
+      S.feb = g.feb;
+      S.R = g.R;
+      S.code_base = (int64_t)gp->channels * gp->stages[gs].offset;
+      S.out_lo = S.code_base;
+      S.gs = gs;
+      for (const ClassDesc &c : st.classes) {
+        if (c.parity < 1 || c.parity > 7 || c.ndim != 3 || c.step != 2 || (c.flags & 1))
+          return 1;
+        for (int j = 0; j < 3; j++)
+          if (c.starts[j] != ((c.parity >> j) & 1)) return 1;
+        if (!c.size) continue;
+        if (a.ncls == 3) return 1;
+        MClass &K = a.cl[a.ncls++];
+        K.parity = c.parity;
+        K.stage = ns;
+        const int ma = (c.parity & 1) ? 0 : (c.parity & 2) ? 1 : 0;
+        for (int j = 0; j < 3; j++) {
+          K.m[j] = c.sizes[j];
+          K.e[j] = j == ma ? (c.sizes[j] + MARCH_M - 1) / MARCH_M : c.sizes[j];
+          K.rstr[j] = c.rstr[j];
+        }
+        const int64_t nt = (int64_t)K.e[0] * K.e[1] * K.e[2];
+        if (nt >= 0x7fffffffll) return 1;
+        K.nthreads = (uint32_t)nt;
+        K.offset = c.offset;
+      }
+      ns++;
+    }
+  }
+  for (int gi = 0; gi < 3; gi++) {
+    const MArgs &a = A[gi];
+    if (!a.ncls) continue;
+    int64_t blocks = 1;
+    for (int i = 0; i < a.ncls; i++)
+      blocks = std::max<int64_t>(blocks, ((int64_t)a.cl[i].nthreads + 255) / 256);
+    blocks = std::min<int64_t>(blocks, 4096);
+    blocks = (blocks + 7) / 8 * 8;
+    const dim3 grid((unsigned)blocks, (unsigned)a.ncls);
+    if (encode)
+      hipLaunchKernelGGL(march_kernel<true>, grid, dim3(256), 0, stream, a, x, recon,
+                         (uint16_t *)codes, hist, out_rank, out_val, out_count);
+    else
+      hipLaunchKernelGGL(march_kernel<false>, grid, dim3(256), 0, stream, a, nullptr, recon,
+                         (uint16_t *)codes, nullptr, out_rank, out_val, out_count);
+    const int rc = hip_check(hipGetLastError(), "march_kernel");
+    if (rc) return rc;
+  }
+  return LATEC_OK;
+}
+
 // Slab wavefront (opt-in, LATEC_SLAB_MB=<MiB per slab>): a level whose grid
 // exceeds four slabs runs its sub-stages slab by slab along axis 0 (weight-w
 // stages trail by w-1 slabs) so that each slab stays cache resident.
@@ -1306,6 +1657,12 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
     if (nslab == 1 && level3_ok(p, q, o)) {
       rc = launch_level3(p, q, g0, encode, X(k), R(k), codes, hist, out_rank, out_val, out_count,
                          ebs, o, stream);
+      if (rc < 0) return rc;
+      if (rc == 0) continue;
+    }
+    if (nslab == 1 && march_ok(p, q, o)) {
+      rc = launch_march(p, q, g0, encode, X(k), R(k), codes, hist, out_rank, out_val, out_count,
+                        ebs, o, stream);
       if (rc < 0) return rc;
       if (rc == 0) continue;
     }
