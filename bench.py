@@ -204,6 +204,22 @@ def main():
         torch.cuda.synchronize()
         return d
 
+    from latec_amd import mlrans
+
+    header5 = Header(**{**header.__dict__, "version": mlrans.VERSION_MULTILANE})
+
+    def compress_once_v5():
+        res = codec.encode_device(xd, plan, ebs, o, graph=True, ws=ws)
+        fits = codec.fit_device(res)
+        payload, _, _ = mlrans.pack_payload(res, fits)
+        return write_stream(header5, payload)
+
+    def decompress_once_v5(stream):
+        h, payload = read_stream(stream)
+        d = codec.decode_device_v5(payload, plan, ebs, o)
+        torch.cuda.synchronize()
+        return d
+
     for _ in range(args.warmup):
         stream, n_out = compress_once()
         if not args.skip_decode:
@@ -228,6 +244,31 @@ def main():
     print(f"[bench] decompress {t_dec * 1e3:.1f} ms/step, max err {max_err:.3e} <= {eb:.3e}",
           file=sys.stderr, flush=True)
     assert max_err <= eb
+
+    # stream version 5: the same encode, multi-lane rANS on the GPU, no outer zstd
+    v5 = None
+    if not args.skip_decode:
+        recon4 = recon.clone()
+        s5 = compress_once_v5()
+        decompress_once_v5(s5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s5 = compress_once_v5()
+        torch.cuda.synchronize()
+        t_c5 = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r5 = decompress_once_v5(s5)
+        torch.cuda.synchronize()
+        t_d5 = (time.perf_counter() - t0) / args.steps
+        assert torch.equal(r5, recon4), "version-5 decode differs from version 4's"
+        v5 = {"compress_GBps": in_bytes / t_c5 / 1e9, "decompress_GBps": in_bytes / t_d5 / 1e9,
+              "ms_per_step": t_c5 * 1e3, "decompress_ms": t_d5 * 1e3,
+              "bits_per_value": 8 * len(s5) / n_values, "stream_bytes": len(s5)}
+        print(f"[bench] v5 multi-lane: compress {t_c5 * 1e3:.1f} ms, decompress "
+              f"{t_d5 * 1e3:.1f} ms, {v5['bits_per_value']:.4f} bits/value",
+              file=sys.stderr, flush=True)
 
     # kernel time: HIP events around the fused stage kernels (graph replay)
     stream_t = torch.cuda.current_stream()
@@ -317,6 +358,7 @@ def main():
                                "(10 B/value + 4 B/outlier)",
                      "decode": {"achieved": ach_dec / 1e9, "frac": ach_dec / HBM_PEAK}},
         "cpu_baseline": cpu,
+        "stream_v5": v5,
     }
     if not args.no_gnn and os.path.exists(args.gnn_ckpt):
         line["gnn"] = gnn_leg(args)

@@ -401,6 +401,48 @@ int latec_train_nll(int64_t M, const float *d_o, const float *d_t,
                     const float *d_eb, float mse_w, float *d_bits, float *d_dO,
                     float *d_mu, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Multi-lane rANS stage blobs (LATEC001 version 5).  No reference     */
+/* counterpart: version 5 is added beside version 4 (whose serial      */
+/* stack-rANS blobs latec/rans.py:466-535 writes) so that entropy      */
+/* coding runs on the GPU.  Records keep version 4's framing           */
+/* (latec/bitstream.py:178-285) and stage-constant model fit; a blob   */
+/* is [L][L + 1 lane word offsets][lane words], symbol i in lane i % L,*/
+/* each lane a range-ANS coder with 64-bit state in [2^31, 2^63),      */
+/* 32-bit words and the stage's 2^24 CDF row as the model.             */
+/* ------------------------------------------------------------------ */
+/* Lanes of an n-symbol stage: clamp(ceil(n / 4096), 1, 65536). */
+int64_t latec_mlrans_lanes(int64_t n);
+/* Device scratch (u32 words) latec_mlrans_encode needs for these stages. */
+int64_t latec_mlrans_scratch_words(int ns, const int64_t *n);
+/* Encode ns stages of device codes (u16/u32) with per-stage kexp and CDF
+ * rows (device, row_off into d_rows, host arrays for the rest); writes the
+ * blobs back to back into d_out (4-byte aligned) and their host-side
+ * offsets / lengths in bytes.  Synchronizes the stream. */
+int latec_mlrans_encode(int ns, const void *d_codes, int code_bytes,
+                        const int64_t *code_off, const int64_t *n,
+                        const int32_t *kexp, uint32_t radius,
+                        const uint32_t *d_rows, const int64_t *row_off,
+                        uint32_t *d_scratch, int64_t scratch_words,
+                        uint8_t *d_out, int64_t out_cap, int64_t *blob_off,
+                        int64_t *blob_len, void *stream);
+/* Decode the blobs at blob_off / blob_len (bytes, in the device payload)
+ * into d_codes at code_off; LATEC_E_FORMAT for a malformed blob.
+ * Synchronizes the stream. */
+int latec_mlrans_decode(int ns, const uint8_t *d_payload, int64_t payload_len,
+                        const int64_t *blob_off, const int64_t *blob_len,
+                        const int64_t *n, const int32_t *kexp, uint32_t radius,
+                        const uint32_t *d_rows, const int64_t *row_off,
+                        void *d_codes, int code_bytes, const int64_t *code_off,
+                        uint32_t *d_scratch, int64_t scratch_words, void *stream);
+/* Ordered ranks of the zero (outlier) codes of each stage (the positions
+ * latec/bitstream.py:288-344 recovers on the host) into d_out_rank at
+ * code_off, counts into d_out_count[ns]. */
+int latec_zero_ranks(int ns, const void *d_codes, int code_bytes,
+                     const int64_t *code_off, const int64_t *n,
+                     uint32_t *d_out_rank, uint32_t *d_out_count,
+                     uint32_t *d_scratch, int64_t scratch_words, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

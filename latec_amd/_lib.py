@@ -123,6 +123,11 @@ SIGNATURES = {
     "latec_train_attn": (i32, [i64, p, p, p, i32, f32, p, p, p, p, p]),
     "latec_train_attn_bwd": (i32, [i64, p, p, i32, p, p, p, p, p, p, p, p, p]),
     "latec_train_nll": (i32, [i64, p, p, p, f32, p, p, p, p]),
+    "latec_mlrans_lanes": (i64, [i64]),
+    "latec_mlrans_scratch_words": (i64, [i32, p]),
+    "latec_mlrans_encode": (i32, [i32, p, i32, p, p, p, u32, p, p, p, i64, p, i64, p, p, p]),
+    "latec_mlrans_decode": (i32, [i32, p, i64, p, p, p, p, u32, p, p, p, i32, p, p, i64, p]),
+    "latec_zero_ranks": (i32, [i32, p, i32, p, p, p, p, p, i64, p]),
 }
 
 _lib = None

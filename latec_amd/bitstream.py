@@ -22,6 +22,9 @@ from .quantizer import DEFAULT_RADIUS
 
 MAGIC = b"LATEC001"
 VERSION = 4
+# Version 5 (this package's addition): multi-lane rANS blobs coded on the GPU
+# and the payload stored without the outer zstd frame (latec_amd.mlrans).
+VERSION_MULTILANE = 5
 
 FLAG_MOCK = 1 << 0
 FLAG_GRAY = 1 << 1
@@ -85,7 +88,7 @@ class Header:
          eb_ratio) = struct.unpack_from(_HEADER_FMT, buf, 0)
         if magic != MAGIC:
             raise ValueError(f"not a LATEC stream (bad magic {magic!r})")
-        if version != VERSION:
+        if version not in (VERSION, VERSION_MULTILANE):
             raise ValueError(f"unsupported version {version}")
         (ndim,) = struct.unpack_from("<B", buf, _HEADER_SIZE)
         if not ndim:
@@ -118,12 +121,16 @@ def zstd_decompress(data: bytes) -> bytes:
 
 
 def write_stream(header: Header, payload: bytes, zstd_level: int = 9) -> bytes:
+    if header.version == VERSION_MULTILANE:
+        return header.pack() + payload
     return header.pack() + zstd_compress(payload, zstd_level)
 
 
 def read_stream(data: bytes):
     header = Header.unpack(data)
     off = _HEADER_SIZE + 1 + 4 * len(header.spatial)
+    if header.version == VERSION_MULTILANE:
+        return header, bytes(data[off:])
     return header, zstd_decompress(data[off:])
 
 

@@ -23,8 +23,9 @@ from . import _device as D
 from . import rans as R
 from ._lib import Fit, check, lib, ptr
 from .bitstream import (DTYPE_CODES, DTYPE_IDS, END_MODE_SHIFT, FLAG_CUBIC, FLAG_INTERP,
-                        FLAG_RANS, Header, pack_record, pack_stage, parse_record,
-                        read_stream, unpack_record, unpack_stage, write_stream)
+                        FLAG_RANS, VERSION, VERSION_MULTILANE, Header, pack_record,
+                        pack_stage, parse_record, read_stream, unpack_record, unpack_stage,
+                        write_stream)
 from .levels import mask_tiles, stage_ebs, stage_masks
 from .predictor import (END_MODE_MAX, InterpPredictor, default_interp_center,
                         default_interp_end_mode, get_plan)
@@ -55,6 +56,7 @@ class Workspace:
     def __init__(self, plan, radius):
         C, N, ns = plan.channels, plan.npoints, plan.n_stages
         cb = 2 if radius <= 32768 else 4
+        self.radius = radius
         self.recon = D.empty(C * N, np.float32)
         self.codes = D.empty(C * N, np.uint16 if cb == 2 else np.uint32)
         self.hist = D.empty(ns * lib().latec_hist_words(radius), np.uint32)
@@ -192,32 +194,6 @@ def _interp_opts(predictor, ndim, radius, round_output):
     center, end = predictor.resolved(ndim)
     return opts_for(radius, round_output, cubic=predictor.order == "cubic", center=center,
                     end_mode=end)
-
-
-def _compress_region_gpu(field, ebs, predictor, radius, round_output, stats, zstd_level):
-    C = field.shape[0]
-    region = field.shape[1:]
-    plan = get_plan(region, C, predictor.levels, predictor.anchor_stride,
-                    predictor.anchor_block)
-    o = _interp_opts(predictor, len(region), radius, round_output)
-    t0 = time.time()
-    d_x = field if hasattr(field, "data_ptr") else D.to_device(field.reshape(-1))
-    res = encode_device(d_x.reshape(-1), plan, ebs, o)
-    fits = fit_device(res)
-    stats["quantize_s"] += time.time() - t0
-    t0 = time.time()
-    payload, rec_len, n_out = pack_device(res, fits, zstd_level)
-    stats["entropy_s"] += time.time() - t0
-    recon = D.to_host(res.d_recon).reshape(C, *region)
-    x = field if isinstance(field, np.ndarray) else D.to_host(field).reshape(C, *region)
-    for s in range(plan.n_stages):
-        stats["stage_codes"][s] += C * plan.counts[s]
-        stats["stage_outliers"][s] += int(n_out[s])
-        stats["stage_payload_bytes"][s] += int(rec_len[s])
-        stats["stage_model_bits"][s] += float(fits[s].bits)
-    stats["outliers"] += int(n_out.sum())
-    _region_error_stats(stats, plan, x, recon)
-    return payload, recon
 
 
 def _region_error_stats(stats, plan, x, recon):
@@ -388,8 +364,14 @@ def _fast_path(predictor) -> bool:
 
 
 def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, radius=1 << 15,
-             zstd_level=9, eb_ratio=None, tune="fast", tune_size_slack=1.05, verbose=False):
-    """codec.py:45-285: returns (stream, stats)."""
+             zstd_level=9, eb_ratio=None, tune="fast", tune_size_slack=1.05, verbose=False,
+             *, stream_version=VERSION):
+    """codec.py:45-285: returns (stream, stats).  ``stream_version=5`` (an
+    addition) writes the multi-lane rANS stream version (latec_amd.mlrans)
+    with the same codes, outliers and model fit; the default 4 is the
+    reference's layout."""
+    if stream_version not in (VERSION, VERSION_MULTILANE):
+        raise ValueError(f"unsupported version {stream_version}")
     is_image = img.ndim <= 2 or (img.ndim == 3 and img.shape[-1] in (1, 3))
     if is_image:
         if img.ndim == 2:
@@ -420,6 +402,8 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
     flags |= FLAG_RANS if use_rans else 0
     round_output = np.issubdtype(np.dtype(img.dtype), np.integer)
     fast = _fast_path(predictor)
+    if stream_version == VERSION_MULTILANE and not fast:
+        raise ValueError("stream version 5 needs the package's InterpPredictor")
     d_field = D.to_device(field.reshape(-1)) if fast else None
 
     def new_stats():
@@ -434,8 +418,9 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
     def encode(ebs):
         st = new_stats()
         if fast:
-            payload, recon = _compress_region_gpu_dev(field, d_field, ebs, predictor, radius,
-                                                      round_output, st, zstd_level)
+            payload, recon = _compress_region_gpu_impl(field, d_field, ebs, predictor, radius,
+                                                       round_output, st, zstd_level,
+                                                       stream_version)
         else:
             payload, recon = _compress_region(field, masks, ebs, predictor, radius,
                                               round_output, st, zstd_level)
@@ -453,7 +438,7 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
                       vmin=vmin, vmax=vmax,
                       ckpt_hash=getattr(predictor, "checkpoint_hash", b"\0" * 16),
                       flags=flags | (int(end_mode) << END_MODE_SHIFT), interp_center=center,
-                      eb_ratio=ratio)
+                      eb_ratio=ratio, version=stream_version)
 
     if tune not in ("fast", "size", "rd"):
         raise ValueError("tune must be 'fast', 'size', or 'rd'")
@@ -544,14 +529,8 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
     return stream, stats
 
 
-def _compress_region_gpu_dev(field, d_field, ebs, predictor, radius, round_output, stats,
-                             zstd_level):
-    return _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_output,
-                                     stats, zstd_level)
-
-
 def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_output, stats,
-                              zstd_level):
+                              zstd_level, stream_version=VERSION):
     C = field.shape[0]
     region = field.shape[1:]
     plan = get_plan(region, C, predictor.levels, predictor.anchor_stride,
@@ -562,7 +541,12 @@ def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_outp
     fits = fit_device(res)
     stats["quantize_s"] += time.time() - t0
     t0 = time.time()
-    payload, rec_len, n_out = pack_device(res, fits, zstd_level)
+    if stream_version == VERSION_MULTILANE:
+        from . import mlrans
+
+        payload, rec_len, n_out = mlrans.pack_payload(res, fits)
+    else:
+        payload, rec_len, n_out = pack_device(res, fits, zstd_level)
     stats["entropy_s"] += time.time() - t0
     recon = D.to_host(res.d_recon).reshape(C, *region)
     for s in range(plan.n_stages):
@@ -573,6 +557,20 @@ def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_outp
     stats["outliers"] += int(n_out.sum())
     _region_error_stats(stats, plan, field, recon)
     return payload, recon
+
+
+def decode_device_v5(payload: bytes, plan, ebs, o):
+    """decode_device for a version-5 payload: lanes decoded on the GPU."""
+    from . import mlrans
+
+    ws = workspace(plan, o.radius)
+    mlrans.decode_payload(payload, plan, ws)
+    ws.recon.zero_()
+    ebs_arr = np.ascontiguousarray(ebs, np.float64)
+    check(lib().latec_interp_decode(plan.handle, 0, plan.n_stages, ptr(ws.codes), ptr(ws.rank),
+                                    ptr(ws.val), ptr(ws.cnt), ptr(ws.recon), ptr(ebs_arr),
+                                    _optr(o), ptr(None)))
+    return ws.recon
 
 
 def decompress(stream: bytes, predictor_factory=None) -> np.ndarray:
@@ -598,9 +596,14 @@ def decompress(stream: bytes, predictor_factory=None) -> np.ndarray:
                         header.anchor_block)
         round_output = np.issubdtype(DTYPE_CODES[header.src_dtype], np.integer)
         o = _interp_opts(predictor, len(spatial), header.radius, round_output)
-        d_recon = decode_device(payload, plan, ebs, o)
+        if header.version == VERSION_MULTILANE:
+            d_recon = decode_device_v5(payload, plan, ebs, o)
+        else:
+            d_recon = decode_device(payload, plan, ebs, o)
         canvas = D.to_host(d_recon).reshape(header.channels, *spatial)
         return _finalize(canvas, header)
+    if header.version == VERSION_MULTILANE:
+        raise ValueError("stream version 5 needs the package's InterpPredictor")
     make_masks = getattr(predictor, "stage_masks", stage_masks)
     masks = make_masks(spatial, header.levels, header.anchor_stride, header.anchor_block)
     canvas = _decompress_region(payload, masks, ebs, header, predictor)

@@ -1012,3 +1012,97 @@ int64_t orc_zstd_decompress(const uint8_t *src, int64_t n, uint8_t *dst,
   size_t r = ZSTD_decompress(dst, cap, src, n);
   return ZSTD_isError(r) ? -1 : (int64_t)r;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Multi-lane rANS stage blobs (LATEC001 version 5).  Version 5 is this      */
+/* project's own stream version beside v4 (no reference counterpart); this   */
+/* is its CPU specification, written independently of the GPU kernels:      */
+/*   lanes L = clamp(ceil(n / 4096), 1, 65536); symbol i -> lane i % L;      */
+/*   per lane a range-ANS coder (Duda 2013) with 64-bit state in            */
+/*   [2^31, 2^63), 32-bit words, and the stage's CDF row (total 2^24,       */
+/*   every symbol >= 1) as the model;                                        */
+/*   blob = [L][L + 1 word offsets][lane words], a lane's words being its   */
+/*   final state (low, high) then its words in decoder order.               */
+/* ------------------------------------------------------------------------ */
+int64_t orc_mlrans_lanes(int64_t n) {
+  if (n <= 0) return 0;
+  int64_t L = (n + 4095) / 4096;
+  return L > 65536 ? 65536 : L;
+}
+
+/* Returns the blob length in words, or -1 (capacity / symbol outside A). */
+int64_t orc_mlrans_encode(const uint32_t *syms, int64_t n, const uint32_t *cdf, int64_t A,
+                          uint32_t *out, int64_t cap) {
+  if (n <= 0) return 0;
+  const int64_t L = orc_mlrans_lanes(n);
+  if (cap < 2 + L) return -1;
+  out[0] = (uint32_t)L;
+  uint32_t *offs = out + 1;
+  uint32_t *words = out + 2 + L;
+  int64_t room = cap - 2 - L;
+  uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)((n + L - 1) / L + 1));
+  int64_t at = 0;
+  for (int64_t l = 0; l < L; l++) {
+    offs[l] = (uint32_t)at;
+    int64_t cnt = n > l ? (n - l + L - 1) / L : 0;
+    uint64_t x = 1ull << 31;
+    int64_t k = 0;
+    for (int64_t t = cnt - 1; t >= 0; t--) {
+      uint32_t s = syms[l + t * L];
+      if ((int64_t)s >= A) {
+        free(tmp);
+        return -1;
+      }
+      uint64_t c = cdf[s], f = cdf[s + 1] - cdf[s];
+      if (x >= (f << 39)) {
+        tmp[k++] = (uint32_t)x;
+        x >>= 32;
+      }
+      x = ((x / f) << 24) + (x % f) + c;
+    }
+    if (at + 2 + k > room) {
+      free(tmp);
+      return -1;
+    }
+    words[at++] = (uint32_t)x;
+    words[at++] = (uint32_t)(x >> 32);
+    for (int64_t j = k - 1; j >= 0; j--) words[at++] = tmp[j];
+  }
+  offs[L] = (uint32_t)at;
+  free(tmp);
+  return 2 + L + at;
+}
+
+/* Returns 0, or -1 for a malformed blob. */
+int orc_mlrans_decode(const uint32_t *blob, int64_t nw, int64_t n, const uint32_t *cdf,
+                      int64_t A, uint32_t *syms) {
+  if (n <= 0) return nw == 0 ? 0 : -1;
+  const int64_t L = orc_mlrans_lanes(n);
+  if (nw < 2 + L || blob[0] != (uint32_t)L) return -1;
+  const uint32_t *offs = blob + 1, *words = blob + 2 + L;
+  if (offs[0] != 0 || 2 + L + (int64_t)offs[L] != nw) return -1;
+  for (int64_t l = 0; l < L; l++) {
+    if (offs[l + 1] < offs[l] + 2) return -1;
+    const uint32_t *w = words + offs[l];
+    int64_t m = offs[l + 1] - offs[l], k = 2;
+    uint64_t x = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+    int64_t cnt = n > l ? (n - l + L - 1) / L : 0;
+    for (int64_t t = 0; t < cnt; t++) {
+      uint32_t slot = (uint32_t)(x & ((1u << 24) - 1));
+      int64_t lo = 0, hi = A - 1; /* largest s with cdf[s] <= slot */
+      while (lo < hi) {
+        int64_t mid = (lo + hi + 1) / 2;
+        if (cdf[mid] <= slot) lo = mid; else hi = mid - 1;
+      }
+      uint64_t c = cdf[lo], f = cdf[lo + 1] - cdf[lo];
+      x = f * (x >> 24) + (slot - c);
+      if (x < (1ull << 31)) {
+        if (k >= m) return -1;
+        x = (x << 32) | w[k++];
+      }
+      syms[l + t * L] = (uint32_t)lo;
+    }
+    if (k != m || x != (1ull << 31)) return -1;
+  }
+  return 0;
+}

@@ -73,6 +73,9 @@ def lib():
             "orc_zstd_bound": (i64, [i64]),
             "orc_zstd_content_size": (i64, [p, i64]),
             "orc_zstd_decompress": (i64, [p, i64, p, i64]),
+            "orc_mlrans_lanes": (i64, [i64]),
+            "orc_mlrans_encode": (i64, [p, i64, p, i64, p, i64]),
+            "orc_mlrans_decode": (i32, [p, i64, i64, p, i64, p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -380,3 +383,154 @@ def decompress(stream):
     if rc:
         raise ValueError(f"bad stage record {-rc - 1}")
     return recon
+
+
+# ---- multi-lane rANS stream version 5 (this project's own format) ----------
+#
+# Version 5 has no reference counterpart: the functions below are its CPU
+# specification (latec_oracle.c, orc_mlrans_*), used to check the GPU coder.
+# Records keep version 4's framing and model fit; the blob is the multi-lane
+# word stream; the payload is stored without the outer zstd frame.
+
+VERSION_MULTILANE = 5
+_HLEN_BITS, _KEXP_SHIFT, _SHAPE_SHIFT, _LEVEL_SHIFT, _OWX_SHIFT = 45, 45, 49, 54, 60
+
+
+def mlrans_lanes(n):
+    return int(lib().orc_mlrans_lanes(int(n)))
+
+
+def mlrans_encode_syms(syms, cdf):
+    syms = np.ascontiguousarray(syms, np.uint32)
+    cdf = np.ascontiguousarray(cdf, np.uint32)
+    cap = len(syms) + 3 * mlrans_lanes(len(syms)) + 8
+    out = np.empty(cap, np.uint32)
+    nw = lib().orc_mlrans_encode(_p(syms), len(syms), _p(cdf), len(cdf) - 1, _p(out), cap)
+    if nw < 0:
+        raise ValueError("symbol outside the CDF row")
+    return out[:nw].copy()
+
+
+def mlrans_decode_syms(words, n, cdf):
+    words = np.ascontiguousarray(words, np.uint32)
+    cdf = np.ascontiguousarray(cdf, np.uint32)
+    out = np.empty(max(int(n), 1), np.uint32)
+    if lib().orc_mlrans_decode(_p(words), len(words), int(n), _p(cdf), len(cdf) - 1, _p(out)):
+        raise ValueError("corrupt multi-lane rANS blob")
+    return out[: int(n)].copy()
+
+
+def _narrow(codes, radius, kexp):
+    codes = np.asarray(codes, np.int64)
+    return np.where(codes == 0, 0, codes - int(radius) + (1 << kexp)).astype(np.uint32)
+
+
+def _widen(syms, radius, kexp):
+    syms = np.asarray(syms, np.int64)
+    return np.where(syms == 0, 0, syms - (1 << kexp) + int(radius)).astype(np.uint32)
+
+
+def pack_stage_v5(codes, outliers, radius=1 << 15, precision=24):
+    """Version-5 stage record: version 4's fit (kexp, level, shape, owx), the
+    multi-lane blob of the narrowed codes under that CDF row."""
+    codes = np.ascontiguousarray(codes, np.uint32).ravel()
+    outliers = np.ascontiguousarray(outliers, np.float32).ravel()
+    n = len(codes)
+    if n:
+        kexp = choose_kexp(codes, radius)
+        level, shape, owx, _ = choose_const_level(codes, radius, kexp, precision)
+        cdf = build_cdfs(kexp, shape, precision, owx)[level]
+        blob = mlrans_encode_syms(_narrow(codes, radius, kexp), cdf).astype("<u4").tobytes()
+    else:
+        kexp, level, shape, owx, blob = 1, 0, 0, 0, b""
+    framed = (len(blob) | (kexp << _KEXP_SHIFT) | (shape << _SHAPE_SHIFT)
+              | (level << _LEVEL_SHIFT) | (owx << _OWX_SHIFT))
+    return (struct.pack("<IQ", n, framed) + blob + struct.pack("<I", len(outliers))
+            + outliers.tobytes())
+
+
+def unpack_stage_v5(buf, off, n_codes, radius=1 << 15, precision=24):
+    n, framed = struct.unpack_from("<IQ", buf, off)
+    if n != n_codes:
+        raise ValueError("stage code count mismatch")
+    hlen = framed & ((1 << _HLEN_BITS) - 1)
+    kexp = (framed >> _KEXP_SHIFT) & 15
+    shape = (framed >> _SHAPE_SHIFT) & 31
+    level = (framed >> _LEVEL_SHIFT) & 63
+    owx = (framed >> _OWX_SHIFT) & 7
+    o = off + 12
+    if n:
+        words = np.frombuffer(buf, "<u4", count=hlen // 4, offset=o)
+        cdf = build_cdfs(kexp, shape, precision, owx)[level]
+        codes = _widen(mlrans_decode_syms(words, n, cdf), radius, kexp)
+    else:
+        codes = np.zeros(0, np.uint32)
+    o += hlen
+    (n_out,) = struct.unpack_from("<I", buf, o)
+    o += 4
+    outl = np.frombuffer(buf, np.float32, count=n_out, offset=o).copy()
+    return codes, outl, o + 4 * n_out
+
+
+def _split_v4_payload(payload, counts, channels, radius):
+    """(codes, outliers) per stage of a version-4 payload."""
+    out, off = [], 0
+    for n in counts:
+        c, o, off = unpack_stage(payload, off, channels * n, radius)
+        out.append((c, o))
+    return out
+
+
+def compress_v5(img, eb, order="cubic", levels=4, stride=16, block=1, center=None,
+                end_mode=1, eb_ratio=0.9, radius=1 << 15):
+    """The version-5 stream of one fast-mode encode: the same codes and
+    outliers as ``compress``, version-5 records, no outer zstd frame."""
+    is_image = img.ndim <= 2 or (img.ndim == 3 and img.shape[-1] in (1, 3))
+    src_dtype = {np.dtype(np.uint8): 0, np.dtype(np.float32): 1}[np.dtype(img.dtype)]
+    if is_image:
+        if img.ndim == 2:
+            img = img[..., None]
+        spatial, c = img.shape[:-1], img.shape[-1]
+        fimg = np.asarray(img, np.float32)
+    else:
+        spatial, c = img.shape, 1
+        fimg = np.asarray(img, np.float32)[..., None]
+    vmin, vmax = float(fimg.min()), float(fimg.max())
+    if vmax <= vmin:
+        vmax = vmin + 1.0
+    field = np.ascontiguousarray(np.moveaxis(fimg, -1, 0))
+    if center is None:
+        center = 1 if len(spatial) >= 4 else 0
+    ebs = stage_ebs_ref(len(spatial), levels, stride, eb, eb_ratio)
+    round_mode = 1 if src_dtype == 0 else 0
+    payload, recon, _ = compress_payload(field, ebs, levels, stride, block, order, center,
+                                         end_mode, radius, round_mode, 1)
+    counts = [int(m.sum()) for m, *_ in stage_plan(spatial, levels, stride, block)]
+    parts = [pack_stage_v5(cd, ol, radius)
+             for cd, ol in _split_v4_payload(payload, counts, c, radius)]
+    flags = FLAG_INTERP | (FLAG_CUBIC if order == "cubic" else 0) | (end_mode << END_MODE_SHIFT)
+    head = header_bytes(c, src_dtype, tuple(spatial), eb, levels, stride, block, radius,
+                        vmin, vmax, flags, center, eb_ratio)
+    head = head[:8] + struct.pack("<H", VERSION_MULTILANE) + head[10:]
+    return head + b"".join(parts), recon
+
+
+def decompress_v5(stream):
+    """Decode a version-5 stream: records are re-expressed as version-4
+    records and decoded by ``decompress``'s region decoder."""
+    hsize = struct.calcsize(HEADER_FMT)
+    fields = struct.unpack_from(HEADER_FMT, stream, 0)
+    if fields[1] != VERSION_MULTILANE:
+        raise ValueError(f"unsupported version {fields[1]}")
+    channels, levels, stride, block, radius = fields[3], fields[5], fields[6], fields[7], fields[9]
+    (ndim,) = struct.unpack_from("<B", stream, hsize)
+    spatial = struct.unpack_from(f"<{ndim}I", stream, hsize + 1)
+    off = hsize + 1 + 4 * ndim
+    v4 = []
+    for m, *_ in stage_plan(spatial, levels, stride, block):
+        codes, outl, off = unpack_stage_v5(stream, off, channels * int(m.sum()), radius)
+        v4.append(pack_stage(codes, outl, radius, 1))
+    if off != len(stream):
+        raise ValueError("trailing bytes in LATEC payload")
+    head = stream[:8] + struct.pack("<H", VERSION) + stream[10:hsize + 1 + 4 * ndim]
+    return decompress(head + _zstd(b"".join(v4), 1))
