@@ -869,8 +869,9 @@ __device__ __forceinline__ void hist_add(uint32_t *lhist, uint32_t *__restrict__
 }
 
 // emit() with the histogram counted per wave: the wave's most common codes
-// (up to four distinct values, found by ballot) take one atomic each, the
+// (up to HIST_PEEL distinct values, found by ballot) take one atomic each, the
 // remaining lanes one atomic per lane.
+#define HIST_PEEL 2
 __device__ __forceinline__ void emit_wave(uint16_t *__restrict__ codes, int64_t slot,
                                           uint32_t code, float x, uint32_t rank,
                                           uint32_t *__restrict__ out_rank,
@@ -882,7 +883,7 @@ __device__ __forceinline__ void emit_wave(uint16_t *__restrict__ codes, int64_t 
   if (active) codes[slot] = (uint16_t)code;
   bool todo = active;
 #pragma unroll
-  for (int it = 0; it < 4; it++) {
+  for (int it = 0; it < HIST_PEEL; it++) {
     const uint64_t m = __ballot(todo);
     if (!m) break;
     const int leader = __ffsll((unsigned long long)m) - 1;
@@ -920,6 +921,11 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
   constexpr int MA = (P & 1) ? 0 : (P & 2) ? 1 : 0;
   constexpr bool WIN = (P >> MA) & 1;
   constexpr int NODD = (P & 1) + ((P >> 1) & 1) + ((P >> 2) & 1);
+  constexpr int NDIR = NODD - (WIN ? 1 : 0);  // odd axes read directly (not the window)
+  // the direct axes, ascending
+  constexpr int DA0 = (P & 1) && (!WIN || MA != 0) ? 0
+                      : ((P >> 1) & 1) && (!WIN || MA != 1) ? 1 : 2;
+  constexpr int DA1 = 2;
   const L3Stage &S = A.st[K.stage];
   const uint32_t gstr[3] = {(uint32_t)A.n[1] * (uint32_t)A.n[2], (uint32_t)A.n[2], 1u};
   const uint32_t sMA = gstr[MA];
@@ -938,6 +944,9 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
   float *oval = out_val + S.out_lo;
   uint32_t *ocnt = out_count + S.gs;
   const int64_t nout = ENC ? 0 : (int64_t)out_count[S.gs];
+  // stage ranks and code slots fit 32 bits (the pyramid requires C*N < 2^32)
+  uint16_t *scodes = codes + S.code_base;
+  const uint32_t rMA = (uint32_t)K.rstr[MA];
   const uint32_t G = gridDim.x;
   const uint32_t lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
   for (uint32_t base = lb * 256u; base < K.nthreads; base += G * 256u) {
@@ -953,84 +962,117 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
     int c[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) c[a] = ((P >> a) & 1) + 2 * (int)ci[a];
-    uint32_t f = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
-    int64_t rank = K.offset + (int64_t)ci[0] * K.rstr[0] + (int64_t)ci[1] * K.rstr[1] +
-                   (int64_t)ci[2] * K.rstr[2];
+    const uint32_t f0 = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
+    const uint32_t rk0 = (uint32_t)(K.offset + (int64_t)ci[0] * K.rstr[0] +
+                                    (int64_t)ci[1] * K.rstr[1] + (int64_t)ci[2] * K.rstr[2]);
     const uint32_t cnt = tv ? min((uint32_t)MARCH_M, K.m[MA] - ci[MA]) : 0u;
-    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;  // taps c-3, c-1, c+1, c+3 along MA
-    if (WIN && cnt) {
-      const int cm = c[MA];
-      if (cm >= 3) w0 = (double)rec[f - 3 * sMA];
-      w1 = (double)rec[f - sMA];
-      if (cm + 1 < nMA) w2 = (double)rec[f + sMA];
-    }
-    for (uint32_t t = 0; t < (uint32_t)MARCH_M; t++) {
-      const bool active = t < cnt;
-      uint32_t code = 0;
-      float xv = 0.f;
-      if (active) {
-        const int cm = c[MA];
-        if (WIN && cm + 3 < nMA) w3 = (double)rec[f + 3 * sMA];
-        double total = 0.0, v_first = 0.0, v_last = 0.0;
-        int count = 0;
+    // direct axes: validity is constant along the march; invalid taps read
+    // the point itself (in bounds, value unused)
+    bool dv[2][3];      // vm3, vp1, vp3
+    int32_t doff[2][4];  // offsets of taps A (-3s), B (-s), C (+s), D (+3s)
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-          if (!((P >> a) & 1)) continue;
-          const int ca = c[a], na = A.n[a];
-          const bool vm3 = ca >= 3, vp1 = ca + 1 < na, vp3 = ca + 3 < na;
-          double v;
-          if (WIN && a == MA) {
-            v = interp4(w0, w1, w2, w3, vm3, vp1, vp3, A.cubic, A.end_mode);
-          } else {
-            const uint32_t sa = gstr[a];
-            const double B = (double)rec[f - sa];
-            const double C = vp1 ? (double)rec[f + sa] : 0.0;
-            const double Av = vm3 ? (double)rec[f - 3 * sa] : 0.0;
-            const double D = vp3 ? (double)rec[f + 3 * sa] : 0.0;
-            v = interp4(Av, B, C, D, vm3, vp1, vp3, A.cubic, A.end_mode);
-          }
-          total = count ? total + v : v;
-          if (!count) v_first = v;
-          v_last = v;
-          count++;
-        }
-        double res;
-        if (A.center == 0 || NODD == 1)
-          res = NODD == 1 ? total : total / (double)NODD;
-        else
-          res = A.center == 1 ? v_first : v_last;
-        const float p = (float)res;
-        float r;
-        if (ENC) {
-          xv = x[f];
-          code = quantize_one(xv, p, g, &r);
-        } else {
-          code = codes[S.code_base + rank];
-          if (code) {
-            r = (float)((double)p + S.w * (double)((int)code - (int)S.R));
-          } else {
-            int64_t lo = 0, hi = nout;
-            while (lo < hi) {
-              const int64_t mid = (lo + hi) >> 1;
-              if ((int64_t)orank[mid] < rank) lo = mid + 1; else hi = mid;
-            }
-            r = (lo < nout && (int64_t)orank[lo] == rank) ? oval[lo]
-                                                          : __int_as_float(0x7fc00000);
-          }
-        }
-        rec[f] = r;
-      }
+    for (int j = 0; j < NDIR; j++) {
+      const int a = j == 0 ? DA0 : DA1;
+      const int ca = c[a], na = A.n[a];
+      const int32_t sa = (int32_t)gstr[a];
+      dv[j][0] = ca >= 3;
+      dv[j][1] = ca + 1 < na;
+      dv[j][2] = ca + 3 < na;
+      doff[j][0] = dv[j][0] ? -3 * sa : 0;
+      doff[j][1] = -sa;
+      doff[j][2] = dv[j][1] ? sa : 0;
+      doff[j][3] = dv[j][2] ? 3 * sa : 0;
+    }
+    // the loads of point t: x (or its code), the window's new tap c+3, and
+    // the direct taps -- all unconditional, issued one point ahead
+    struct Loads {
+      float xin;
+      uint32_t cin;
+      float w3;
+      float dt[NDIR > 0 ? NDIR : 1][4];
+    };
+    auto load = [&](uint32_t t, Loads &L) {
+      const bool has = t < cnt;
+      const uint32_t f = has ? f0 + 2 * t * sMA : f0;
       if (ENC)
-        emit_wave(codes, S.code_base + rank, code, xv, (uint32_t)rank, orank, oval, ocnt, lhist,
-                  ghist, win_lo, active);
+        L.xin = x[f];
+      else
+        L.cin = scodes[has ? rk0 + t * rMA : rk0];
+      if (WIN) L.w3 = rec[has && c[MA] + 2 * (int)t + 3 < nMA ? f + 3 * sMA : f];
+#pragma unroll
+      for (int j = 0; j < NDIR; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) L.dt[j][k] = rec[(uint32_t)((int32_t)f + doff[j][k])];
+    };
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0;  // window taps c-3, c-1, c+1
+    if (WIN) {
+      const int cm = c[MA];
+      w0 = (double)rec[cm >= 3 ? f0 - 3 * sMA : f0];
+      w1 = (double)rec[f0 - sMA];
+      w2 = (double)rec[cm + 1 < nMA ? f0 + sMA : f0];
+    }
+    Loads cur, nxt;
+    load(0, cur);
+#pragma unroll
+    for (uint32_t t = 0; t < (uint32_t)MARCH_M; t++) {
+      if (t + 1 < (uint32_t)MARCH_M) load(t + 1, nxt);
+      const bool active = t < cnt;
+      const uint32_t f = f0 + 2 * t * sMA;
+      const uint32_t rank = rk0 + t * rMA;
+      const double w3 = WIN ? (double)cur.w3 : 0.0;
+      double total = 0.0, v_first = 0.0, v_last = 0.0;
+      int count = 0;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (!((P >> a) & 1)) continue;
+        double v;
+        if (WIN && a == MA) {
+          const int cm = c[MA] + 2 * (int)t;
+          v = interp4(w0, w1, w2, w3, cm >= 3, cm + 1 < nMA, cm + 3 < nMA, A.cubic, A.end_mode);
+        } else {
+          const int j = a == DA0 ? 0 : 1;
+          v = interp4((double)cur.dt[j][0], (double)cur.dt[j][1], (double)cur.dt[j][2],
+                      (double)cur.dt[j][3], dv[j][0], dv[j][1], dv[j][2], A.cubic, A.end_mode);
+        }
+        total = count ? total + v : v;
+        if (!count) v_first = v;
+        v_last = v;
+        count++;
+      }
+      double res;
+      if (A.center == 0 || NODD == 1)
+        res = NODD == 1 ? total : total / (double)NODD;
+      else
+        res = A.center == 1 ? v_first : v_last;
+      const float p = (float)res;
+      uint32_t code = 0;
+      float xv = 0.f, r;
+      if (ENC) {
+        xv = cur.xin;
+        code = quantize_one(xv, p, g, &r);
+      } else {
+        code = cur.cin;
+        if (code) {
+          r = (float)((double)p + S.w * (double)((int)code - (int)S.R));
+        } else if (active) {
+          int64_t lo = 0, hi = nout;
+          while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)orank[mid] < (int64_t)rank) lo = mid + 1; else hi = mid;
+          }
+          r = (lo < nout && (int64_t)orank[lo] == (int64_t)rank) ? oval[lo]
+                                                                 : __int_as_float(0x7fc00000);
+        }
+      }
+      if (active) rec[f] = r;
+      if (ENC)
+        emit_wave(scodes, rank, code, xv, rank, orank, oval, ocnt, lhist, ghist, win_lo, active);
       if (WIN) {
         w0 = w1;
         w1 = w2;
         w2 = w3;
       }
-      f += 2 * sMA;
-      rank += K.rstr[MA];
-      c[MA] += 2;
+      cur = nxt;
     }
   }
 }
