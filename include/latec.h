@@ -44,6 +44,15 @@ const char *latec_last_error(void);
 /* ABI version (bumped on any signature change). */
 int latec_abi_version(void);
 
+/* Launch-shape knobs of the plain-path kernels, read and set by the
+ * autotuner (latec_amd/autotune.py).  No reference counterpart: the
+ * reference has no GPU kernels to shape.  Knobs of the 3-D level kernels:
+ * "march_blocks" (workgroups per parity class, 8..2^20, rounded up to a
+ * multiple of 8) and "march_hist_peel" (0..8 distinct codes per wave
+ * counted with one histogram atomic each).  Process-wide. */
+int latec_tune_set(const char *name, int64_t value);
+int64_t latec_tune_get(const char *name);
+
 /* ------------------------------------------------------------------ */
 /* Schedule plan: the dyadic stage schedule of one region shape.       */
 /* latec/levels.py:100-147 (iter_stage_plan), 221-282 (stage_counts),  */

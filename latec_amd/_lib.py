@@ -66,6 +66,8 @@ i64, i32, u32, f64, f32 = C.c_int64, C.c_int, C.c_uint32, C.c_double, C.c_float
 SIGNATURES = {
     "latec_last_error": (C.c_char_p, []),
     "latec_abi_version": (i32, []),
+    "latec_tune_set": (i32, [C.c_char_p, i64]),
+    "latec_tune_get": (i64, [C.c_char_p]),
     "latec_plan_create": (i32, [i32, p, i32, i32, i32, i32, C.POINTER(p)]),
     "latec_plan_destroy": (None, [p]),
     "latec_plan_n_stages": (i32, [p]),

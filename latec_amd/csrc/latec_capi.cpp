@@ -63,6 +63,8 @@ int64_t pack_stages(int, const void *, int, const int64_t *, const int64_t *, co
 int64_t zstd_compress(const uint8_t *, int64_t, int, uint8_t *, int64_t);
 int64_t zstd_content_size(const uint8_t *, int64_t);
 int64_t zstd_decompress(const uint8_t *, int64_t, uint8_t *, int64_t);
+int tune_set(const char *, int64_t);
+int64_t tune_get(const char *);
 
 }  // namespace latec
 
@@ -74,6 +76,22 @@ extern "C" {
 
 const char *latec_last_error(void) { return g_err.c_str(); }
 int latec_abi_version(void) { return 1; }
+
+int latec_tune_set(const char *name, int64_t value) {
+  if (!name) {
+    set_error("latec_tune_set: null name");
+    return LATEC_E_ARG;
+  }
+  return tune_set(name, value);
+}
+
+int64_t latec_tune_get(const char *name) {
+  if (!name) {
+    set_error("latec_tune_get: null name");
+    return LATEC_E_ARG;
+  }
+  return tune_get(name);
+}
 
 int latec_plan_create(int ndim, const int64_t *shape, int channels, int levels,
                       int anchor_stride, int anchor_block, latec_plan **out) {

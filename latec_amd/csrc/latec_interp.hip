@@ -835,7 +835,7 @@ struct MClass {
 
 struct MArgs {
   int32_t n[3];
-  int32_t cubic, center, end_mode, round_mode, ncls;
+  int32_t cubic, center, end_mode, round_mode, ncls, peel;
   double span, offset;
   int64_t hist_words;
   L3Stage st[3];
@@ -869,21 +869,19 @@ __device__ __forceinline__ void hist_add(uint32_t *lhist, uint32_t *__restrict__
 }
 
 // emit() with the histogram counted per wave: the wave's most common codes
-// (up to HIST_PEEL distinct values, found by ballot) take one atomic each, the
-// remaining lanes one atomic per lane.
-#define HIST_PEEL 2
-__device__ __forceinline__ void emit_wave(uint16_t *__restrict__ codes, int64_t slot,
+// (up to `peel` distinct values, found by ballot) take one atomic each, the
+// remaining lanes one atomic per lane.  Stores through 32-bit byte offsets.
+__device__ __forceinline__ void emit_wave(uint16_t *__restrict__ codes, uint32_t slot,
                                           uint32_t code, float x, uint32_t rank,
                                           uint32_t *__restrict__ out_rank,
                                           float *__restrict__ out_val,
                                           uint32_t *__restrict__ out_count, uint32_t *lhist,
                                           uint32_t *__restrict__ ghist, uint32_t win_lo,
-                                          bool active) {
+                                          bool active, int peel) {
   const int lane = __lane_id();
-  if (active) codes[slot] = (uint16_t)code;
+  if (active) *(uint16_t *)((char *)codes + 2u * slot) = (uint16_t)code;
   bool todo = active;
-#pragma unroll
-  for (int it = 0; it < HIST_PEEL; it++) {
+  for (int it = 0; it < peel; it++) {  // peel is uniform: a scalar loop
     const uint64_t m = __ballot(todo);
     if (!m) break;
     const int leader = __ffsll((unsigned long long)m) - 1;
@@ -909,6 +907,22 @@ __device__ __forceinline__ void emit_wave(uint16_t *__restrict__ codes, int64_t 
   }
 }
 
+// Loads and stores at a 32-bit byte offset from a uniform base (SGPR base +
+// VGPR offset addressing; every level grid here is below 2^30 values).
+template <typename T>
+__device__ __forceinline__ T ldo(const T *__restrict__ base, uint32_t idx) {
+  return *(const T *)((const char *)base + idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void sto(T *__restrict__ base, uint32_t idx, T v) {
+  *(T *)((char *)base + idx * (uint32_t)sizeof(T)) = v;
+}
+
+// Cubic interior stencil of interp4 (all four taps valid).
+__device__ __forceinline__ double cubic4(double A, double B, double C, double D) {
+  return (((-A + 9.0 * B) + 9.0 * C) - D) * 0.0625;
+}
+
 template <int P, bool ENC>
 __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
                                             const float *__restrict__ x,
@@ -918,6 +932,7 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
                                             uint32_t *__restrict__ out_rank,
                                             float *__restrict__ out_val,
                                             uint32_t *__restrict__ out_count) {
+  constexpr int M = MARCH_M;
   constexpr int MA = (P & 1) ? 0 : (P & 2) ? 1 : 0;
   constexpr bool WIN = (P >> MA) & 1;
   constexpr int NODD = (P & 1) + ((P >> 1) & 1) + ((P >> 2) & 1);
@@ -930,20 +945,17 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
   const uint32_t gstr[3] = {(uint32_t)A.n[1] * (uint32_t)A.n[2], (uint32_t)A.n[2], 1u};
   const uint32_t sMA = gstr[MA];
   const int nMA = A.n[MA];
-  GridArgs g;
-  g.w = S.w;
-  g.inv_w = S.inv_w;
-  g.R = S.R;
-  g.feb = S.feb;
-  g.round_mode = A.round_mode;
-  g.span = A.span;
-  g.offset = A.offset;
+  const int cubic = A.cubic, end_mode = A.end_mode, center = A.center;
+  const int round_mode = A.round_mode;
+  const double w = S.w, inv_w = S.inv_w, Rd = (double)S.R;
+  const float feb = S.feb;
   const uint32_t win_lo = S.R > HIST_WIN / 2 ? S.R - HIST_WIN / 2 : 0;
   uint32_t *ghist = ENC ? hist + (int64_t)S.gs * A.hist_words : nullptr;
   uint32_t *orank = out_rank + S.out_lo;
   float *oval = out_val + S.out_lo;
   uint32_t *ocnt = out_count + S.gs;
   const int64_t nout = ENC ? 0 : (int64_t)out_count[S.gs];
+  const int peel = A.peel;
   // stage ranks and code slots fit 32 bits (the pyramid requires C*N < 2^32)
   uint16_t *scodes = codes + S.code_base;
   const uint32_t rMA = (uint32_t)K.rstr[MA];
@@ -958,18 +970,19 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
     u /= K.e[2];
     ci[1] = u % K.e[1];
     ci[0] = u / K.e[1];
-    ci[MA] *= MARCH_M;
+    ci[MA] *= M;
     int c[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) c[a] = ((P >> a) & 1) + 2 * (int)ci[a];
     const uint32_t f0 = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
     const uint32_t rk0 = (uint32_t)(K.offset + (int64_t)ci[0] * K.rstr[0] +
                                     (int64_t)ci[1] * K.rstr[1] + (int64_t)ci[2] * K.rstr[2]);
-    const uint32_t cnt = tv ? min((uint32_t)MARCH_M, K.m[MA] - ci[MA]) : 0u;
+    const uint32_t cnt = tv ? min((uint32_t)M, K.m[MA] - ci[MA]) : 0u;
     // direct axes: validity is constant along the march; invalid taps read
     // the point itself (in bounds, value unused)
-    bool dv[2][3];      // vm3, vp1, vp3
+    bool dv[2][3];       // vm3, vp1, vp3
     int32_t doff[2][4];  // offsets of taps A (-3s), B (-s), C (+s), D (+3s)
+    bool dir_in = true;  // every direct axis has all four taps
 #pragma unroll
     for (int j = 0; j < NDIR; j++) {
       const int a = j == 0 ? DA0 : DA1;
@@ -978,6 +991,7 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
       dv[j][0] = ca >= 3;
       dv[j][1] = ca + 1 < na;
       dv[j][2] = ca + 3 < na;
+      dir_in = dir_in && dv[j][0] && dv[j][2];
       doff[j][0] = dv[j][0] ? -3 * sa : 0;
       doff[j][1] = -sa;
       doff[j][2] = dv[j][1] ? sa : 0;
@@ -995,78 +1009,116 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
       const bool has = t < cnt;
       const uint32_t f = has ? f0 + 2 * t * sMA : f0;
       if (ENC)
-        L.xin = x[f];
+        L.xin = ldo(x, f);
       else
-        L.cin = scodes[has ? rk0 + t * rMA : rk0];
-      if (WIN) L.w3 = rec[has && c[MA] + 2 * (int)t + 3 < nMA ? f + 3 * sMA : f];
+        L.cin = ldo(scodes, has ? rk0 + t * rMA : rk0);
+      if (WIN) L.w3 = ldo((const float *)rec, has && c[MA] + 2 * (int)t + 3 < nMA ? f + 3 * sMA : f);
 #pragma unroll
       for (int j = 0; j < NDIR; j++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) L.dt[j][k] = rec[(uint32_t)((int32_t)f + doff[j][k])];
+        for (int k = 0; k < 4; k++)
+          L.dt[j][k] = ldo((const float *)rec, (uint32_t)((int32_t)f + doff[j][k]));
     };
     double w0 = 0.0, w1 = 0.0, w2 = 0.0;  // window taps c-3, c-1, c+1
     if (WIN) {
       const int cm = c[MA];
-      w0 = (double)rec[cm >= 3 ? f0 - 3 * sMA : f0];
-      w1 = (double)rec[f0 - sMA];
-      w2 = (double)rec[cm + 1 < nMA ? f0 + sMA : f0];
+      w0 = (double)ldo((const float *)rec, cm >= 3 ? f0 - 3 * sMA : f0);
+      w1 = (double)ldo((const float *)rec, f0 - sMA);
+      w2 = (double)ldo((const float *)rec, cm + 1 < nMA ? f0 + sMA : f0);
     }
     Loads cur, nxt;
     load(0, cur);
 #pragma unroll
-    for (uint32_t t = 0; t < (uint32_t)MARCH_M; t++) {
-      if (t + 1 < (uint32_t)MARCH_M) load(t + 1, nxt);
+    for (uint32_t t = 0; t < (uint32_t)M; t++) {
+      if (t + 1 < (uint32_t)M) load(t + 1, nxt);
       const bool active = t < cnt;
       const uint32_t f = f0 + 2 * t * sMA;
       const uint32_t rank = rk0 + t * rMA;
       const double w3 = WIN ? (double)cur.w3 : 0.0;
-      double total = 0.0, v_first = 0.0, v_last = 0.0;
-      int count = 0;
+      const int cm = c[MA] + 2 * (int)t;
+      // --- prediction: the cubic interior stencil when the whole wave is
+      // interior, else interp4's general end handling (same values)
+      double v[3];
+      const bool win_in = !WIN || (cm >= 3 && cm + 3 < nMA);
+      if (cubic && __all(win_in && dir_in)) {
 #pragma unroll
-      for (int a = 0; a < 3; a++) {
-        if (!((P >> a) & 1)) continue;
-        double v;
-        if (WIN && a == MA) {
-          const int cm = c[MA] + 2 * (int)t;
-          v = interp4(w0, w1, w2, w3, cm >= 3, cm + 1 < nMA, cm + 3 < nMA, A.cubic, A.end_mode);
-        } else {
-          const int j = a == DA0 ? 0 : 1;
-          v = interp4((double)cur.dt[j][0], (double)cur.dt[j][1], (double)cur.dt[j][2],
-                      (double)cur.dt[j][3], dv[j][0], dv[j][1], dv[j][2], A.cubic, A.end_mode);
+        for (int a = 0, i = 0; a < 3; a++) {
+          if (!((P >> a) & 1)) continue;
+          if (WIN && a == MA) {
+            v[i++] = cubic4(w0, w1, w2, w3);
+          } else {
+            const int j = a == DA0 ? 0 : 1;
+            v[i++] = cubic4((double)cur.dt[j][0], (double)cur.dt[j][1], (double)cur.dt[j][2],
+                            (double)cur.dt[j][3]);
+          }
         }
-        total = count ? total + v : v;
-        if (!count) v_first = v;
-        v_last = v;
-        count++;
+      } else {
+#pragma unroll
+        for (int a = 0, i = 0; a < 3; a++) {
+          if (!((P >> a) & 1)) continue;
+          if (WIN && a == MA) {
+            v[i++] = interp4(w0, w1, w2, w3, cm >= 3, cm + 1 < nMA, cm + 3 < nMA, cubic, end_mode);
+          } else {
+            const int j = a == DA0 ? 0 : 1;
+            v[i++] = interp4((double)cur.dt[j][0], (double)cur.dt[j][1], (double)cur.dt[j][2],
+                             (double)cur.dt[j][3], dv[j][0], dv[j][1], dv[j][2], cubic, end_mode);
+          }
+        }
       }
       double res;
-      if (A.center == 0 || NODD == 1)
-        res = NODD == 1 ? total : total / (double)NODD;
-      else
-        res = A.center == 1 ? v_first : v_last;
+      if (NODD == 1) {
+        res = v[0];
+      } else if (center == 0) {
+        double total = v[0] + v[1];
+        if (NODD == 3) total = total + v[2];
+        res = total / (double)NODD;
+      } else {
+        res = center == 1 ? v[0] : v[NODD - 1];
+      }
       const float p = (float)res;
       uint32_t code = 0;
-      float xv = 0.f, r;
+      float xv = 0.f, r = 0.f;
       if (ENC) {
+        // quantize_one, branch-free but for the rare exact-division fallback
         xv = cur.xin;
-        code = quantize_one(xv, p, g, &r);
+        const double pd = (double)p;
+        const double d = (double)xv - pd;
+        const double tq = d * inv_w;
+        double qd = rint(tq);
+        const bool fb = !(0.5 - fabs(tq - qd) > 1e-12 * fmax(fabs(tq), 1.0));
+        if (__any(fb)) {
+          if (fb) qd = rint(d / w);
+        }
+        const bool inr = fabs(qd) < Rd;
+        const int qi = inr ? (int)qd : 0;
+        const float cand = (float)(pd + w * (double)qi);
+        float chk = cand;
+        if (round_mode == 1)
+          chk = rintf(cand);
+        else if (round_mode == 2)
+          chk = (float)((rint((double)cand * A.span + A.offset) - A.offset) / A.span);
+        const bool ok = inr && fabsf(xv - chk) <= feb;
+        code = ok ? (uint32_t)(qi + (int)S.R) : 0u;
+        r = ok ? cand : xv;
       } else {
         code = cur.cin;
-        if (code) {
-          r = (float)((double)p + S.w * (double)((int)code - (int)S.R));
-        } else if (active) {
-          int64_t lo = 0, hi = nout;
-          while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if ((int64_t)orank[mid] < (int64_t)rank) lo = mid + 1; else hi = mid;
+        r = (float)((double)p + w * (double)((int)code - (int)S.R));
+        if (__any(active && code == 0)) {
+          if (active && code == 0) {
+            int64_t lo = 0, hi = nout;
+            while (lo < hi) {
+              const int64_t mid = (lo + hi) >> 1;
+              if ((int64_t)orank[mid] < (int64_t)rank) lo = mid + 1; else hi = mid;
+            }
+            r = (lo < nout && (int64_t)orank[lo] == (int64_t)rank) ? oval[lo]
+                                                                   : __int_as_float(0x7fc00000);
           }
-          r = (lo < nout && (int64_t)orank[lo] == (int64_t)rank) ? oval[lo]
-                                                                 : __int_as_float(0x7fc00000);
         }
       }
-      if (active) rec[f] = r;
+      if (active) sto(rec, f, r);
       if (ENC)
-        emit_wave(scodes, rank, code, xv, rank, orank, oval, ocnt, lhist, ghist, win_lo, active);
+        emit_wave(scodes, rank, code, xv, rank, orank, oval, ocnt, lhist, ghist, win_lo, active,
+                  peel);
       if (WIN) {
         w0 = w1;
         w1 = w2;
@@ -1576,6 +1628,41 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
   return hip_check(hipGetLastError(), "level3_kernel");
 }
 
+// Launch-shape knobs of the march kernels (set by the autotuner through
+// latec_tune_set): at most march_blocks workgroups per class (a multiple of
+// 8, so the XCD remap stays exact; the threads walk the rest of the class),
+// and march_hist_peel, the distinct codes per wave counted with one atomic.
+static int64_t g_march_blocks = 4096;
+static int64_t g_march_peel = 2;  // histogram: distinct codes per wave counted by ballot
+
+int tune_set(const char *name, int64_t v) {
+  if (!strcmp(name, "march_blocks")) {
+    if (v < 8 || v > (1 << 20)) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_march_blocks = (v + 7) / 8 * 8;
+    return LATEC_OK;
+  }
+  if (!strcmp(name, "march_hist_peel")) {
+    if (v < 0 || v > 8) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_march_peel = v;
+    return LATEC_OK;
+  }
+  set_error(std::string("unknown tuning knob ") + name);
+  return LATEC_E_ARG;
+}
+
+int64_t tune_get(const char *name) {
+  if (!strcmp(name, "march_blocks")) return g_march_blocks;
+  if (!strcmp(name, "march_hist_peel")) return g_march_peel;
+  set_error(std::string("unknown tuning knob ") + name);
+  return LATEC_E_ARG;
+}
+
 // Marching class launches for a 3-D level (one channel, u16 codes): the
 // three weight-1 stages together, then the weight-2 stage, then weight 3.
 // Returns 1 if the level's layout is not the one the kernel assumes.
@@ -1603,6 +1690,7 @@ static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool 
     a.span = o->span;
     a.offset = o->offset;
     a.hist_words = hist_words(o->radius);
+    a.peel = (int32_t)g_march_peel;
     int ns = 0;
     for (int si = 0; si < 3 && groups[gi][si]; si++) {
       const int js = groups[gi][si], gs = g0 + js - 1;
@@ -1646,7 +1734,7 @@ static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool 
     int64_t blocks = 1;
     for (int i = 0; i < a.ncls; i++)
       blocks = std::max<int64_t>(blocks, ((int64_t)a.cl[i].nthreads + 255) / 256);
-    blocks = std::min<int64_t>(blocks, 4096);
+    blocks = std::min<int64_t>(blocks, g_march_blocks);
     blocks = (blocks + 7) / 8 * 8;
     const dim3 grid((unsigned)blocks, (unsigned)a.ncls);
     if (encode)
