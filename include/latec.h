@@ -452,6 +452,36 @@ int latec_zero_ranks(int ns, const void *d_codes, int code_bytes,
                      uint32_t *d_out_rank, uint32_t *d_out_count,
                      uint32_t *d_scratch, int64_t scratch_words, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Batched uneven workload: many independent fields (anchor_block 1,   */
+/* one channel, radius <= 32768) coded together.  No reference         */
+/* counterpart as one call: the reference codes tensors one at a time  */
+/* (latec/codec.py:45-322 per tensor, scripts/eval_tensor.py:145-147   */
+/* casting float64 to float32).  Step s runs stage s of every field    */
+/* that has one as ONE persistent launch over (field-stage, class,     */
+/* rows) items; the steps are recorded once into a hipGraph.  Each     */
+/* field's codes/outliers/recon equal its single-field encode.         */
+/* ------------------------------------------------------------------ */
+typedef struct latec_batch latec_batch;
+/* plans[f], d_x[f] / d_recon[f] (device, N_f floats), code_base[f] (first
+ * code of field f in d_codes / d_out_rank / d_out_val), hist_base[f]
+ * (words, n_stages_f * latec_hist_words each), cnt_base[f] (first of its
+ * n_stages_f outlier counters), ebs[f] (host, per stage).  encode = 0
+ * builds the decoder (d_x unused; codes, sorted outlier ranks, values and
+ * counts are inputs).  o[f]: field f's options (one radius for all). */
+int latec_batch_create(int nf, latec_plan *const *plans, const float *const *d_x,
+                       float *const *d_recon, const int64_t *code_base,
+                       const int64_t *hist_base, const int64_t *cnt_base,
+                       const double *const *ebs, const latec_interp_opts *o,
+                       int encode, void *d_codes, uint32_t *d_hist,
+                       uint32_t *d_out_rank, float *d_out_val,
+                       uint32_t *d_out_count, latec_batch **out);
+int latec_batch_n_steps(const latec_batch *b);
+/* Run every step (use_graph: record once, then replay the hipGraph).
+ * Histograms and outlier counters must be zero before an encode run. */
+int latec_batch_run(latec_batch *b, int use_graph, void *stream);
+void latec_batch_destroy(latec_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

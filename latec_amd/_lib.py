@@ -130,6 +130,11 @@ SIGNATURES = {
     "latec_mlrans_encode": (i32, [i32, p, i32, p, p, p, u32, p, p, p, i64, p, i64, p, p, p]),
     "latec_mlrans_decode": (i32, [i32, p, i64, p, p, p, p, u32, p, p, p, i32, p, p, i64, p]),
     "latec_zero_ranks": (i32, [i32, p, i32, p, p, p, p, p, i64, p]),
+    "latec_batch_create": (i32, [i32, p, p, p, p, p, p, p, p, i32, p, p, p, p, p,
+                                 C.POINTER(p)]),
+    "latec_batch_n_steps": (i32, [p]),
+    "latec_batch_run": (i32, [p, i32, p]),
+    "latec_batch_destroy": (None, [p]),
 }
 
 _lib = None

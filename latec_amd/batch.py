@@ -1,0 +1,371 @@
+"""Batched uneven workload: many independent fields compressed together.
+
+The reference codes one tensor per ``compress`` call (latec/codec.py:45-322;
+its tensor evaluation script casts float64 inputs to float32 first,
+scripts/eval_tensor.py:145-147).  ``compress_batch`` produces, for every
+field, exactly the stream ``compress(field, eb, InterpPredictor(...),
+stream_version=...)`` would, but runs the whole batch at once on the GPU:
+
+* every field, reconstruction, code and outlier array lives in one set of
+  device buffers;
+* schedule step s (stage s of each field that has one) is ONE persistent
+  launch over (field-stage, parity class, rows) work items, and the step
+  sequence is recorded once into a hipGraph (``latec_batch_run``);
+* the per-stage model fits run per field on the GPU, and with stream version
+  5 the multi-lane rANS coder codes every stage of every field in one call.
+
+``decompress_batch`` is the mirror: one upload of all payloads, one
+multi-lane decode of all stages, the outlier ranks on the GPU, one graph of
+persistent decode launches.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import struct
+
+import numpy as np
+
+from . import _device as D
+from . import rans as R
+from ._lib import check, lib, ptr
+from .bitstream import (DTYPE_IDS, END_MODE_SHIFT, VERSION, VERSION_MULTILANE, Header,
+                        _pack_hlen, parse_record, read_stream, write_stream)
+from .codec import fit_device, pack_device
+from .levels import stage_ebs
+from .predictor import InterpPredictor, get_plan
+from .quantizer import opts_for
+
+_P = ctypes.c_void_p
+
+
+def _ptr_array(values):
+    return (_P * max(len(values), 1))(*[_P(int(v)) for v in values])
+
+
+def _opts_array(opts):
+    """A contiguous InterpOpts[] (keep it referenced across the call)."""
+    from ._lib import InterpOpts
+
+    return (InterpOpts * max(len(opts), 1))(*opts)
+
+
+class _Layout:
+    """Offsets of every field's arrays in the batch buffers."""
+
+    def __init__(self, plans, radius):
+        self.hw = int(lib().latec_hist_words(radius))
+        self.x_off, self.code_base, self.hist_base, self.cnt_base = [], [], [], []
+        x = c = h = k = 0
+        for p in plans:
+            self.x_off.append(x)
+            self.code_base.append(c)
+            self.hist_base.append(h)
+            self.cnt_base.append(k)
+            x += p.npoints
+            c += p.npoints
+            h += p.n_stages * self.hw
+            k += p.n_stages
+        self.n_points, self.n_codes, self.n_hist, self.n_cnt = x, c, h, k
+
+
+class _BatchBuffers:
+    def __init__(self, lay: _Layout):
+        self.x = D.empty(max(lay.n_points, 1), np.float32)
+        self.recon = D.empty(max(lay.n_points, 1), np.float32)
+        self.codes = D.empty(max(lay.n_codes, 1), np.uint16)
+        self.rank = D.empty(max(lay.n_codes, 1), np.uint32)
+        self.val = D.empty(max(lay.n_codes, 1), np.float32)
+        self.hist = D.empty(max(lay.n_hist, 1), np.uint32)
+        self.cnt = D.empty(max(lay.n_cnt, 1), np.uint32)
+
+
+class _FieldView:
+    """The GpuEncodeResult interface of one field inside the batch buffers
+    (what codec.fit_device / pack_device / mlrans read)."""
+
+    def __init__(self, plan, buf, lay, f, opts):
+        n = plan.npoints
+        lo, hb, cb = lay.code_base[f], lay.hist_base[f], lay.cnt_base[f]
+        self.plan, self.opts = plan, opts
+        self.d_recon = buf.recon[lay.x_off[f]:lay.x_off[f] + n]
+        self.d_codes = buf.codes[lo:lo + n]
+        self.d_rank = buf.rank[lo:lo + n]
+        self.d_val = buf.val[lo:lo + n]
+        self.d_hist = buf.hist[hb:hb + plan.n_stages * lay.hw]
+        self.d_cnt = buf.cnt[cb:cb + plan.n_stages]
+
+
+def _prep(x):
+    """(float32 field, spatial shape, src dtype id) with the reference's
+    image rule: 2-D arrays and (H, W, 1|3) are images (one or three channels)."""
+    x = np.asarray(x)
+    if x.dtype == np.float64:
+        x = x.astype(np.float32)
+    if x.dtype not in (np.float32, np.uint8):
+        raise ValueError(f"unsupported dtype {x.dtype}")
+    if x.ndim == 3 and x.shape[-1] in (1, 3):
+        raise ValueError("the batch path codes single-channel fields")
+    if not 2 <= x.ndim <= 4:
+        raise ValueError("batch fields are 2-D, 3-D or 4-D arrays")
+    return np.ascontiguousarray(x), tuple(x.shape), DTYPE_IDS[np.dtype(x.dtype)]
+
+
+def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cubic",
+                   radius=1 << 15, stream_version=VERSION_MULTILANE, zstd_level=9,
+                   use_graph=True, timings=None):
+    """Streams of ``fields`` (list of arrays, any mix of ranks 2..4 and
+    shapes; float64 is cast to float32), each at absolute bound
+    ``rel_eb * (max - min)`` of its own field, tune 'fast'.  Every stream
+    equals the single-field ``compress`` stream of that field."""
+    import time
+
+    import torch
+
+    if stream_version not in (VERSION, VERSION_MULTILANE):
+        raise ValueError(f"unsupported version {stream_version}")
+    t0 = time.perf_counter()
+    prepped = [_prep(x) for x in fields]
+    plans, ebs_list, heads, opts = [], [], [], []
+    for x, shape, src in prepped:
+        pred = InterpPredictor(order, levels, anchor_stride, 1)
+        center, end_mode = pred.resolved(len(shape))
+        opts.append(opts_for(radius, src == 0, cubic=order == "cubic", center=center,
+                             end_mode=end_mode))
+        vmin, vmax = float(x.min()), float(x.max())
+        if vmax <= vmin:
+            vmax = vmin + 1.0
+        eb = float(rel_eb) * (vmax - vmin)
+        plans.append(get_plan(shape, 1, levels, anchor_stride, 1))
+        ebs_list.append(np.ascontiguousarray(
+            stage_ebs(shape, levels, anchor_stride, 1, eb, pred.fast_eb_ratio), np.float64))
+        heads.append(Header(channels=1, src_dtype=src, spatial=shape, eb=eb, levels=levels,
+                            anchor_stride=anchor_stride, anchor_block=1, radius=radius,
+                            max_radius=0, agg_level=-1, vmin=vmin, vmax=vmax,
+                            flags=pred.stream_flag | (end_mode << END_MODE_SHIFT),
+                            interp_center=center, eb_ratio=pred.fast_eb_ratio,
+                            version=stream_version))
+    lay = _Layout(plans, radius)
+    buf = _BatchBuffers(lay)
+    host_x = np.concatenate([x.reshape(-1).astype(np.float32) for x, _, _ in prepped]) \
+        if prepped else np.zeros(1, np.float32)
+    buf.x[:lay.n_points].copy_(torch.from_numpy(host_x[:lay.n_points]))
+    buf.recon.zero_()
+    buf.hist.zero_()
+    buf.cnt.zero_()
+    base_x, base_r = buf.x.data_ptr(), buf.recon.data_ptr()
+    oa = _opts_array(opts)
+    h = _P()
+    check(lib().latec_batch_create(
+        len(plans), _ptr_array([p.handle.value for p in plans]),
+        _ptr_array([base_x + 4 * o_ for o_ in lay.x_off]),
+        _ptr_array([base_r + 4 * o_ for o_ in lay.x_off]),
+        ptr(np.asarray(lay.code_base, np.int64)), ptr(np.asarray(lay.hist_base, np.int64)),
+        ptr(np.asarray(lay.cnt_base, np.int64)), _ptr_array([e.ctypes.data for e in ebs_list]),
+        ctypes.cast(oa, _P), 1, ptr(buf.codes), ptr(buf.hist), ptr(buf.rank), ptr(buf.val),
+        ptr(buf.cnt), ctypes.byref(h)))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    try:
+        check(lib().latec_batch_run(h, int(bool(use_graph)), ptr(None)))
+        torch.cuda.synchronize()
+    finally:
+        lib().latec_batch_destroy(h)
+    t2 = time.perf_counter()
+    views = [_FieldView(p, buf, lay, f, o) for f, (p, o) in enumerate(zip(plans, opts))]
+    fits = [fit_device(v) for v in views]
+    t3 = time.perf_counter()
+    if timings is not None:
+        timings["fit_s"] = t3 - t2
+    if stream_version == VERSION_MULTILANE:
+        payloads = _pack_v5_all(views, fits, lay, buf, radius)
+    else:
+        payloads = [pack_device(v, ft, zstd_level)[0] for v, ft in zip(views, fits)]
+    streams = [write_stream(hd, pl, zstd_level) for hd, pl in zip(heads, payloads)]
+    t4 = time.perf_counter()
+    if timings is not None:
+        timings.update({"setup_s": t1 - t0, "kernels_s": t2 - t1, "fit_s": t3 - t2,
+                        "entropy_s": t4 - t3, "fields": len(plans),
+                        "values": int(lay.n_points)})
+    return streams
+
+
+def _pack_v5_all(views, fits, lay, buf, radius):
+    """Version-5 payloads of every field, all stages coded in one
+    multi-lane call."""
+    from .mlrans import sorted_outliers
+
+    n, code_off, params = [], [], []
+    for f, (v, ft) in enumerate(zip(views, fits)):
+        p = v.plan
+        for s in range(p.n_stages):
+            k = p.counts[s]
+            n.append(k)
+            code_off.append(lay.code_base[f] + p.offsets[s])
+            params.append((int(ft[s].kexp) if k else 1, int(ft[s].shape), int(ft[s].owx),
+                           int(ft[s].level)))
+    n = np.asarray(n, np.int64)
+    code_off = np.asarray(code_off, np.int64)
+    ns = len(n)
+    rows, row_off, at = [], np.zeros(ns, np.int64), 0
+    for s, (kx, shape, owx, level) in enumerate(params):
+        row_off[s] = at
+        if n[s]:
+            r = R.cdf_row(kx, shape, owx, level)
+            rows.append(r)
+            at += len(r)
+    d_rows = D.to_device(np.concatenate(rows) if rows else np.zeros(1, np.uint32))
+    kexp = np.asarray([p[0] for p in params], np.int32)
+    sw = int(lib().latec_mlrans_scratch_words(ns, ptr(n)))
+    scratch = D.empty(sw, np.uint32)
+    lanes = sum(int(lib().latec_mlrans_lanes(int(k))) for k in n)
+    cap = 4 * (int(n.sum()) + 3 * lanes + 4 * ns + 16)
+    out = D.empty(cap, np.uint8)
+    blob_off = np.zeros(ns, np.int64)
+    blob_len = np.zeros(ns, np.int64)
+    check(lib().latec_mlrans_encode(ns, ptr(buf.codes), 2, ptr(code_off), ptr(n), ptr(kexp),
+                                    radius, ptr(d_rows), ptr(row_off), ptr(scratch), sw,
+                                    ptr(out), cap, ptr(blob_off), ptr(blob_len), ptr(None)))
+    total = int((blob_off + blob_len).max(initial=0))
+    blobs = D.to_host(out[:total], np.uint8) if total else np.zeros(0, np.uint8)
+    payloads, s0 = [], 0
+    for f, v in enumerate(views):
+        p = v.plan
+        n_out = D.to_host(v.d_cnt, np.uint32).astype(np.int64)
+        outl, out_off = sorted_outliers(v, p, n_out)
+        parts = []
+        for s in range(p.n_stages):
+            g = s0 + s
+            kx, shape, owx, level = params[g]
+            blob = blobs[blob_off[g]:blob_off[g] + blob_len[g]].tobytes()
+            k = int(n_out[s])
+            parts.append(struct.pack("<IQ", int(n[g]), _pack_hlen(len(blob), kx, shape, level,
+                                                                  owx)))
+            parts.append(blob)
+            parts.append(struct.pack("<I", k))
+            parts.append(outl[out_off[s]:out_off[s] + k].tobytes())
+        payloads.append(b"".join(parts))
+        s0 += p.n_stages
+    return payloads
+
+
+def decompress_batch(streams, *, use_graph=True, timings=None):
+    """Arrays of version-5 streams written by ``compress_batch`` or
+    ``compress(..., stream_version=5)`` (float32; uint8 sources rounded)."""
+    import time
+
+    import torch
+
+    from .codec import _finalize, _interp_opts
+
+    t0 = time.perf_counter()
+    heads, payloads, plans, ebs_list, opts = [], [], [], [], []
+    for st in streams:
+        hd, pl = read_stream(st)
+        if hd.version != VERSION_MULTILANE:
+            raise ValueError("decompress_batch reads stream version 5")
+        if hd.channels != 1 or hd.anchor_block != 1:
+            raise ValueError("the batch path codes single-channel anchor_block-1 streams")
+        pred = InterpPredictor("cubic" if hd.flags & (1 << 4) else "linear", hd.levels,
+                               hd.anchor_stride, 1)
+        pred.center = hd.interp_center
+        pred.end_mode = (hd.flags >> END_MODE_SHIFT) & 3
+        o = _interp_opts(pred, len(hd.spatial), hd.radius, hd.src_dtype == 0)
+        if opts and o.radius != opts[0].radius:
+            raise ValueError("a batch needs one quantizer radius")
+        opts.append(o)
+        plans.append(get_plan(hd.spatial, 1, hd.levels, hd.anchor_stride, 1))
+        ebs_list.append(np.ascontiguousarray(
+            stage_ebs(hd.spatial, hd.levels, hd.anchor_stride, 1, hd.eb, hd.eb_ratio),
+            np.float64))
+        heads.append(hd)
+        payloads.append(pl)
+    radius = opts[0].radius if opts else 1 << 15
+    lay = _Layout(plans, radius)
+    buf = _BatchBuffers(lay)
+    # parse every record, one upload of all payloads
+    blob_off, blob_len, n, code_off, params, outs = [], [], [], [], [], []
+    pay_base, at = [], 0
+    for f, (pl, p) in enumerate(zip(payloads, plans)):
+        pay_base.append(at)
+        off = 0
+        for s in range(p.n_stages):
+            n_codes, hlen, kexp, shape, level, owx, literal, rlen = parse_record(pl, off)
+            if n_codes != p.counts[s]:
+                raise ValueError("stage code count mismatch")
+            if literal:
+                raise ValueError("version-5 stage record has the literal bit set")
+            params.append((int(kexp) if n_codes else 1, int(shape), int(owx), int(level)))
+            blob_off.append(at + off + 12)
+            blob_len.append(hlen)
+            n.append(n_codes)
+            code_off.append(lay.code_base[f] + p.offsets[s])
+            (k,) = struct.unpack_from("<I", pl, off + 12 + hlen)
+            outs.append(np.frombuffer(pl, np.float32, count=k, offset=off + 16 + hlen))
+            off += rlen
+        if off != len(pl):
+            raise ValueError("trailing bytes in LATEC payload")
+        at += (len(pl) + 3) & ~3
+    allpay = np.zeros(max(at, 4), np.uint8)
+    for b0, pl in zip(pay_base, payloads):
+        allpay[b0:b0 + len(pl)] = np.frombuffer(pl, np.uint8)
+    d_pay = D.to_device(allpay)
+    n = np.asarray(n, np.int64)
+    code_off = np.asarray(code_off, np.int64)
+    blob_off = np.asarray(blob_off, np.int64)
+    blob_len = np.asarray(blob_len, np.int64)
+    ns = len(n)
+    rows, row_off, rat = [], np.zeros(ns, np.int64), 0
+    for s, (kx, shape, owx, level) in enumerate(params):
+        row_off[s] = rat
+        if n[s]:
+            r = R.cdf_row(kx, shape, owx, level)
+            rows.append(r)
+            rat += len(r)
+    d_rows = D.to_device(np.concatenate(rows) if rows else np.zeros(1, np.uint32))
+    kexp = np.asarray([p[0] for p in params], np.int32)
+    lanes = [int(lib().latec_mlrans_lanes(int(k))) for k in n]
+    sw = max(16 * ns + 2 * sum((L + 255) // 256 for L in lanes) + 64,
+             8 * ns + 2 * sum((int(k) + 4095) // 4096 for k in n) + 64)
+    scratch = D.empty(sw, np.uint32)
+    check(lib().latec_mlrans_decode(ns, ptr(d_pay), len(allpay), ptr(blob_off), ptr(blob_len),
+                                    ptr(n), ptr(kexp), radius, ptr(d_rows), ptr(row_off),
+                                    ptr(buf.codes), 2, ptr(code_off), ptr(scratch), sw,
+                                    ptr(None)))
+    check(lib().latec_zero_ranks(ns, ptr(buf.codes), 2, ptr(code_off), ptr(n), ptr(buf.rank),
+                                 ptr(buf.cnt), ptr(scratch), sw, ptr(None)))
+    cnt = D.to_host(buf.cnt[:ns], np.uint32)
+    host_val = np.zeros(max(lay.n_codes, 1), np.float32)
+    for s in range(ns):
+        if int(cnt[s]) != len(outs[s]):
+            raise ValueError("outlier count mismatch")
+        if len(outs[s]):
+            host_val[code_off[s]:code_off[s] + len(outs[s])] = outs[s]
+    buf.val.copy_(torch.from_numpy(host_val))
+    buf.recon.zero_()
+    base_r = buf.recon.data_ptr()
+    oa = _opts_array(opts)
+    h = _P()
+    check(lib().latec_batch_create(
+        len(plans), _ptr_array([p.handle.value for p in plans]), _ptr_array([0] * len(plans)),
+        _ptr_array([base_r + 4 * o_ for o_ in lay.x_off]),
+        ptr(np.asarray(lay.code_base, np.int64)), ptr(np.asarray(lay.hist_base, np.int64)),
+        ptr(np.asarray(lay.cnt_base, np.int64)), _ptr_array([e.ctypes.data for e in ebs_list]),
+        ctypes.cast(oa, _P), 0, ptr(buf.codes), ptr(buf.hist), ptr(buf.rank), ptr(buf.val),
+        ptr(buf.cnt), ctypes.byref(h)))
+    t1 = time.perf_counter()
+    try:
+        check(lib().latec_batch_run(h, int(bool(use_graph)), ptr(None)))
+        torch.cuda.synchronize()
+    finally:
+        lib().latec_batch_destroy(h)
+    t2 = time.perf_counter()
+    host = D.to_host(buf.recon[:max(lay.n_points, 1)])
+    out = []
+    for f, (hd, p) in enumerate(zip(heads, plans)):
+        canvas = host[lay.x_off[f]:lay.x_off[f] + p.npoints].reshape(1, *hd.spatial)
+        out.append(_finalize(canvas, hd))
+    if timings is not None:
+        timings.update({"setup_s": t1 - t0, "kernels_s": t2 - t1,
+                        "finish_s": time.perf_counter() - t2})
+    return out

@@ -1635,7 +1635,11 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
 static int64_t g_march_blocks = 4096;
 static int64_t g_march_peel = 2;  // histogram: distinct codes per wave counted by ballot
 
+static int64_t g_tune_epoch = 0;  // bumped by every knob change
+static int64_t tune_epoch() { return g_tune_epoch; }
+
 int tune_set(const char *name, int64_t v) {
+  g_tune_epoch++;
   if (!strcmp(name, "march_blocks")) {
     if (v < 8 || v > (1 << 20)) {
       set_error(std::string("tuning value out of range for ") + name);
@@ -1817,6 +1821,8 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
   return LATEC_OK;
 }
 
+static int64_t tune_epoch();
+
 int interp_encode_graph(latec_plan *p, const float *x, float *recon, void *codes,
                         uint32_t *hist, uint32_t *out_rank, float *out_val,
                         uint32_t *out_count, const double *ebs,
@@ -1835,6 +1841,8 @@ int interp_encode_graph(latec_plan *p, const float *x, float *recon, void *codes
   };
   mix(ebs, sizeof(double) * p->stages.size());
   mix(o, sizeof *o);
+  const int64_t knobs = tune_epoch();  // launch shapes are baked into the graph too
+  mix(&knobs, sizeof knobs);
   key.a[7] = (const void *)(uintptr_t)h;
   auto it = p->dev->graphs.find(key);
   if (it == p->dev->graphs.end()) {
@@ -1938,3 +1946,322 @@ int dequantize(const float *pred, const void *codes, int64_t n,
 }
 
 }  // namespace latec
+
+// ---------------------------------------------------------------------------
+// Batched uneven workload: many independent fields (any rank <= 4, any
+// shape, anchor_block 1) coded together.  Step s of the batch runs stage s
+// of every field that has one, as ONE persistent launch over a work list of
+// (field-stage, parity class, row range) items of about BATCH_ITEM points;
+// workgroups take items round-robin.  The whole step sequence is recorded
+// once into a hipGraph and replayed.  The per-point arithmetic is the stage
+// kernels' (predict_point / quantize_one / emit), so each field's codes,
+// outliers and reconstruction equal its own single-field encode.
+// ---------------------------------------------------------------------------
+#define BATCH_ITEM 8192
+
+namespace latec {
+
+struct BDesc {  // one (field, stage)
+  GridArgs g;   // code_base = the stage's first code in the batch arrays
+  const ClassDesc *classes;
+  const float *x;
+  float *recon;
+  int64_t hist_off;  // words: the stage's histogram in the batch array
+  int64_t cnt_idx;   // its outlier counter
+  int32_t anchors, pad;
+};
+struct BItem {
+  int32_t d, cls;
+  uint32_t row0, nrows;
+};
+
+// Rows [row0, row0 + nrows) of class c by one workgroup (anchor_block 1).
+template <int ND, typename F>
+__device__ __forceinline__ void walk_item(const ClassDesc &c, const GridArgs &g, uint32_t row0,
+                                          uint32_t nrows, F &&fn) {
+  constexpr int NMAX = ND > 0 ? ND : LATEC_MAX_DIM;
+  const int nd = ND > 0 ? ND : g.ndim;
+  const int tr = c.tr, rb = 256 / tr;
+  const int lrow = threadIdx.x / tr, lcol = threadIdx.x % tr;
+  const uint32_t L = c.row_len;
+  const int64_t start_last = c.starts[nd - 1];
+  for (uint32_t base = 0; base < nrows; base += rb) {
+    const uint32_t row = row0 + base + lrow;
+    const bool rv = base + lrow < nrows;
+    int64_t coord[NMAX];
+    uint32_t fbase = 0;
+    int64_t rbase = c.offset;
+    uint32_t idx = rv ? row : row0;
+#pragma unroll
+    for (int j = NMAX - 2; j >= 0; j--) {
+      if (ND == 0 && j > nd - 2) continue;
+      const uint32_t q = fdiv(idx, c.magic[j], c.shift[j]);
+      const uint32_t m = idx - q * c.sizes[j];
+      idx = q;
+      const int64_t xj = c.starts[j] + (int64_t)c.step * m;
+      coord[j] = xj;
+      fbase += (uint32_t)xj * (uint32_t)g.gstr[j];
+      rbase += (int64_t)m * c.rstr[j];
+    }
+    for (uint32_t col0 = 0; col0 < L; col0 += tr) {
+      const uint32_t col = col0 + lcol;
+      const bool active = rv && col < L;
+      const int64_t xl = start_last + (int64_t)c.step * col;
+      coord[nd - 1] = xl;
+      fn(active, fbase + (uint32_t)xl, rbase + col, (const int64_t *)coord);
+    }
+  }
+}
+
+template <int ND, bool ENC>
+__device__ __forceinline__ void batch_item(const BDesc &D, const ClassDesc &c, const BItem &it,
+                                           uint16_t *__restrict__ codes,
+                                           uint32_t *__restrict__ hist,
+                                           uint32_t *__restrict__ out_rank,
+                                           float *__restrict__ out_val,
+                                           uint32_t *__restrict__ out_count, uint32_t *lhist) {
+  const GridArgs &g = D.g;
+  const int parity = c.parity;
+  const uint32_t win_lo = g.R > HIST_WIN / 2 ? g.R - HIST_WIN / 2 : 0;
+  uint32_t *ghist = hist + D.hist_off;
+  const int64_t nout = ENC ? 0 : (int64_t)out_count[D.cnt_idx];
+  walk_item<ND>(c, g, it.row0, it.nrows, [&](bool active, uint32_t flat, int64_t rank,
+                                             const int64_t *coord) {
+    uint32_t code = 0;
+    float xv = 0.f;
+    if (active) {
+      const float p = D.anchors ? 0.0f
+                                : predict_point<ND, uint32_t>(D.recon, g, parity, flat, coord);
+      float r;
+      if (ENC) {
+        xv = D.x[flat];
+        code = quantize_one(xv, p, g, &r);
+      } else {
+        code = codes[g.code_base + rank];
+        if (code) {
+          r = (float)((double)p + g.w * (double)((int)code - (int)g.R));
+        } else {
+          const uint32_t *orr = out_rank + g.code_base;
+          int64_t lo = 0, hi = nout;
+          while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)orr[mid] < rank) lo = mid + 1; else hi = mid;
+          }
+          r = (lo < nout && (int64_t)orr[lo] == rank) ? out_val[g.code_base + lo]
+                                                      : __int_as_float(0x7fc00000);
+        }
+      }
+      D.recon[flat] = r;
+    }
+    if (ENC)
+      emit<uint16_t>(codes, g.code_base + rank, code, xv, (uint32_t)rank,
+                     out_rank + g.code_base, out_val + g.code_base, out_count + D.cnt_idx, lhist,
+                     ghist, win_lo, active);
+  });
+}
+
+template <bool ENC>
+__global__ void __launch_bounds__(256) batch_kernel(const BDesc *__restrict__ descs,
+                                                    const BItem *__restrict__ items,
+                                                    int32_t n_items, uint16_t *__restrict__ codes,
+                                                    uint32_t *__restrict__ hist,
+                                                    uint32_t *__restrict__ out_rank,
+                                                    float *__restrict__ out_val,
+                                                    uint32_t *__restrict__ out_count) {
+  __shared__ uint32_t lhist[ENC ? HIST_WIN : 1];
+  for (int32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+    const BItem it = items[i];
+    const BDesc &D = descs[it.d];
+    const ClassDesc &c = D.classes[it.cls];
+    if (ENC) {
+      for (int k = threadIdx.x; k < HIST_WIN; k += blockDim.x) lhist[k] = 0;
+      __syncthreads();
+    }
+    switch (D.g.ndim) {
+      case 1: batch_item<1, ENC>(D, c, it, codes, hist, out_rank, out_val, out_count, lhist); break;
+      case 2: batch_item<2, ENC>(D, c, it, codes, hist, out_rank, out_val, out_count, lhist); break;
+      case 3: batch_item<3, ENC>(D, c, it, codes, hist, out_rank, out_val, out_count, lhist); break;
+      default: batch_item<4, ENC>(D, c, it, codes, hist, out_rank, out_val, out_count, lhist); break;
+    }
+    if (ENC) {
+      __syncthreads();
+      const uint32_t R = D.g.R;
+      uint32_t *gwin = hist + D.hist_off + 2 * (size_t)R + (size_t)(i % HIST_COPIES) * HIST_WIN;
+      for (int k = threadIdx.x; k < HIST_WIN; k += blockDim.x)
+        if (lhist[k]) atomicAdd(&gwin[k], lhist[k]);
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace latec
+
+struct latec_batch {
+  bool encode = true;
+  int n_steps = 0;
+  std::vector<int32_t> step_items;  // items per step
+  std::vector<int64_t> step_first;  // first item of each step
+  latec::BDesc *d_desc = nullptr;
+  latec::BItem *d_items = nullptr;
+  uint16_t *codes = nullptr;
+  uint32_t *hist = nullptr, *out_rank = nullptr, *out_count = nullptr;
+  float *out_val = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+using namespace latec;
+
+static int batch_launch_steps(latec_batch *b, hipStream_t s) {
+  for (int st = 0; st < b->n_steps; st++) {
+    const int32_t n = b->step_items[st];
+    if (!n) continue;
+    const unsigned grid = (unsigned)std::min<int64_t>(n, 4096);
+    const BItem *items = b->d_items + b->step_first[st];
+    if (b->encode)
+      hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(256), 0, s, b->d_desc, items, n,
+                         b->codes, b->hist, b->out_rank, b->out_val, b->out_count);
+    else
+      hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(256), 0, s, b->d_desc, items, n,
+                         b->codes, b->hist, b->out_rank, b->out_val, b->out_count);
+    const int rc = hip_check(hipGetLastError(), "batch_kernel");
+    if (rc) return rc;
+  }
+  return LATEC_OK;
+}
+
+extern "C" {
+
+int latec_batch_create(int nf, latec_plan *const *plans, const float *const *d_x,
+                       float *const *d_recon, const int64_t *code_base, const int64_t *hist_base,
+                       const int64_t *cnt_base, const double *const *ebs,
+                       const latec_interp_opts *o, int encode, void *d_codes, uint32_t *d_hist,
+                       uint32_t *d_out_rank, float *d_out_val, uint32_t *d_out_count,
+                       latec_batch **out) {
+  if (!out || nf < 0 || (nf && !o)) {
+    set_error("latec_batch_create: bad arguments");
+    return LATEC_E_ARG;
+  }
+  int rc;
+  for (int f = 0; f < nf; f++) {
+    if ((rc = check_opts(&o[f]))) return rc;
+    if (o[f].code_bytes != 2 || o[f].radius != o[0].radius) {
+      set_error("a batch codes one radius <= 32768 (16-bit codes)");
+      return LATEC_E_ARG;
+    }
+  }
+  const int64_t twoR = nf ? hist_words(o[0].radius) : 0;
+  std::vector<BDesc> descs;
+  std::vector<std::vector<BItem>> steps;
+  for (int f = 0; f < nf; f++) {
+    latec_plan *p = plans[f];
+    if (!p || p->block != 1 || p->channels != 1 || p->npoints >= 0x7fffffffll) {
+      set_error("batch fields need anchor_block 1, one channel and < 2^31 points");
+      return LATEC_E_ARG;
+    }
+    if ((rc = plan_upload(p))) return rc;
+    for (int s = 0; s < (int)p->stages.size(); s++) {
+      const StageDesc &st = p->stages[s];
+      if (!st.count) continue;
+      if (encode && !(ebs[f][s] > 0)) {
+        set_error("error bound must be > 0");
+        return LATEC_E_ARG;
+      }
+      BDesc D;
+      memset(&D, 0, sizeof D);
+      D.g = make_args(p, s, ebs[f][s], &o[f]);
+      D.g.code_base = code_base[f] + st.offset;
+      D.classes = p->dev->classes + p->dev->class_off[s];
+      D.x = encode ? d_x[f] : nullptr;
+      D.recon = d_recon[f];
+      D.hist_off = hist_base[f] + (int64_t)s * twoR;
+      D.cnt_idx = cnt_base[f] + s;
+      D.anchors = s == 0;
+      const int32_t di = (int32_t)descs.size();
+      descs.push_back(D);
+      if ((int)steps.size() <= s) steps.resize(s + 1);
+      for (int ci = 0; ci < (int)st.classes.size(); ci++) {
+        const ClassDesc &c = st.classes[ci];
+        if (!c.size) continue;
+        const uint32_t per = (uint32_t)std::max<int64_t>(1, BATCH_ITEM / std::max<uint32_t>(c.row_len, 1));
+        for (uint32_t r0 = 0; r0 < c.nrows; r0 += per)
+          steps[s].push_back({di, ci, r0, std::min(per, c.nrows - r0)});
+      }
+    }
+  }
+  auto *b = new latec_batch();
+  b->encode = encode != 0;
+  b->n_steps = (int)steps.size();
+  std::vector<BItem> all;
+  for (auto &v : steps) {
+    b->step_first.push_back((int64_t)all.size());
+    b->step_items.push_back((int32_t)v.size());
+    all.insert(all.end(), v.begin(), v.end());
+  }
+  b->codes = (uint16_t *)d_codes;
+  b->hist = d_hist;
+  b->out_rank = d_out_rank;
+  b->out_val = d_out_val;
+  b->out_count = d_out_count;
+  if ((rc = hip_check(hipMalloc(&b->d_desc, sizeof(BDesc) * std::max<size_t>(descs.size(), 1)),
+                      "hipMalloc")) ||
+      (rc = hip_check(hipMalloc(&b->d_items, sizeof(BItem) * std::max<size_t>(all.size(), 1)),
+                      "hipMalloc")) ||
+      (rc = hip_check(hipMemcpy(b->d_desc, descs.data(), sizeof(BDesc) * descs.size(),
+                                hipMemcpyHostToDevice), "hipMemcpy")) ||
+      (rc = hip_check(hipMemcpy(b->d_items, all.data(), sizeof(BItem) * all.size(),
+                                hipMemcpyHostToDevice), "hipMemcpy"))) {
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_items) (void)hipFree(b->d_items);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return LATEC_OK;
+}
+
+int latec_batch_n_steps(const latec_batch *b) { return b ? b->n_steps : LATEC_E_ARG; }
+
+int latec_batch_run(latec_batch *b, int use_graph, void *stream) {
+  if (!b) {
+    set_error("latec_batch_run: null batch");
+    return LATEC_E_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (!use_graph) return batch_launch_steps(b, s);
+  int rc;
+  if (!b->graph) {
+    // capture on a private stream (capturing the legacy default stream is not allowed)
+    hipStream_t cs;
+    if ((rc = hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking),
+                        "hipStreamCreateWithFlags")))
+      return rc;
+    hipGraph_t gr;
+    if ((rc = hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal),
+                        "hipStreamBeginCapture"))) {
+      (void)hipStreamDestroy(cs);
+      return rc;
+    }
+    rc = batch_launch_steps(b, cs);
+    const hipError_t e = hipStreamEndCapture(cs, &gr);
+    if (!rc) rc = hip_check(e, "hipStreamEndCapture");
+    if (!rc) {
+      rc = hip_check(hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0),
+                     "hipGraphInstantiate");
+      (void)hipGraphDestroy(gr);
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc) return rc;
+  }
+  return hip_check(hipGraphLaunch(b->graph, s), "hipGraphLaunch");
+}
+
+void latec_batch_destroy(latec_batch *b) {
+  if (!b) return;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_items) (void)hipFree(b->d_items);
+  delete b;
+}
+
+}  // extern "C"

@@ -219,3 +219,24 @@ def training_batch(batch, shape, corr_range, gen, device, turbulence_frac=0.5, n
     t = _torch()
     return t.stack([training_field(shape, corr_range, gen, device, turbulence_frac,
                                    n_jumps).reshape(-1) for _ in range(batch)])
+
+
+def batch_fields(n=256, seed=4, scale=1.0, device="cpu"):
+    """The batched uneven workload: ``n`` fields, half 2-D with sides uniform
+    in [64, 1024], half 3-D with sides uniform in [32, 256] (each axis drawn
+    on its own, times ``scale``), 70 % float32 / 30 % float64, generator
+    smooth / turbulent / discontinuous uniformly.  Returns a list of host
+    arrays (NumPy)."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for i in range(n):
+        if i % 2 == 0:
+            shape = tuple(max(8, int(rng.randint(64, 1025) * scale)) for _ in range(2))
+        else:
+            shape = tuple(max(8, int(rng.randint(32, 257) * scale)) for _ in range(3))
+        kind = ("smooth", "turbulent", "discontinuous")[rng.randint(3)]
+        f64 = rng.rand() < 0.3
+        x = mixed_field(shape, kind, seed * 100003 + i, device)
+        x = x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
+        out.append(x.astype(np.float64 if f64 else np.float32))
+    return out
