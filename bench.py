@@ -57,7 +57,86 @@ def parse():
     ap.add_argument("--gnn-shape", default="64,64,64,64")
     ap.add_argument("--gnn-levels", type=int, default=3)
     ap.add_argument("--gnn-steps", type=int, default=2)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--knob", action="append", default=[],
+                    help="name=value launch-shape knob (latec_tune_set), skips the autotuner")
+    ap.add_argument("--batch-n", type=int, default=256)
+    ap.add_argument("--batch-scale", type=float, default=1.0)
     return ap.parse_args()
+
+
+def batch_leg(args):
+    """Batched uneven workload: ``--batch-n`` fields (half 2-D sides
+    U[64,1024], half 3-D sides U[32,256], 70 % f32 / 30 % f64, smooth /
+    turbulent / discontinuous), relative eb 1e-3, cubic levels 4 stride 16,
+    stream version 5: one persistent launch per schedule step in a hipGraph.
+    Compared with coding the same fields one ``compress`` call at a time."""
+    import numpy as np
+    import torch
+
+    import latec_amd
+    from latec_amd.batch import compress_batch, decompress_batch
+    from latec_amd.predictor import InterpPredictor
+    from latec_amd.synthetic import batch_fields
+
+    def note(msg):
+        print(f"[bench] batch: {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    fields = batch_fields(args.batch_n, seed=4, scale=args.batch_scale, device="cuda")
+    n_values = sum(f.size for f in fields)
+    in_bytes = sum(f.astype(np.float32).nbytes for f in fields)
+    note(f"generated {len(fields)} fields, {n_values / 1e6:.1f} M values "
+         f"({time.perf_counter() - t0:.1f} s)")
+    kw = dict(levels=4, anchor_stride=16)
+    t0 = time.perf_counter()
+    compress_batch(fields[:8], 1e-3, **kw)  # warm-up (plans, tables, kernels)
+    torch.cuda.synchronize()
+    note(f"warm-up {time.perf_counter() - t0:.1f} s")
+    tm = {}
+    t0 = time.perf_counter()
+    streams = compress_batch(fields, 1e-3, timings=tm, **kw)
+    t_c = time.perf_counter() - t0
+    note(f"compress {t_c:.2f} s {tm}")
+    td = {}
+    t0 = time.perf_counter()
+    outs = decompress_batch(streams, timings=td)
+    t_d = time.perf_counter() - t0
+    note(f"decompress {t_d:.2f} s {td}")
+    worst = 0.0
+    for x, y, s in zip(fields, outs, streams):
+        x32 = x.astype(np.float32)
+        eb = 1e-3 * (float(x32.max()) - float(x32.min()))
+        err = float(np.abs(y - x32).max())
+        assert err <= eb
+        worst = max(worst, err / eb)
+    # the same fields one compress call at a time (first 32)
+    k = min(32, len(fields))
+    t0 = time.perf_counter()
+    for x in fields[:k]:
+        x32 = x.astype(np.float32)
+        eb = 1e-3 * (float(x32.max()) - float(x32.min()))
+        s, _ = latec_amd.compress(x32, eb, InterpPredictor("cubic", 4, 16, 1), anchor_block=1,
+                                  stream_version=5, **kw)
+    t_one = time.perf_counter() - t0
+    in_k = sum(f.astype(np.float32).nbytes for f in fields[:k])
+    out = {"workload": f"{len(fields)} fields (half 2-D sides U[64,1024], half 3-D sides "
+                       f"U[32,256], x{args.batch_scale:g}), 70% f32 / 30% f64, relative eb 1e-3, "
+                       "cubic L4 stride 16, stream version 5",
+           "fields": len(fields), "values": int(n_values),
+           "compress_GBps": in_bytes / t_c / 1e9, "decompress_GBps": in_bytes / t_d / 1e9,
+           "compress_s": t_c, "decompress_s": t_d, "kernels_s": tm["kernels_s"],
+           "fit_s": tm["fit_s"], "entropy_s": tm["entropy_s"], "setup_s": tm["setup_s"],
+           "decode_kernels_s": td["kernels_s"],
+           "bits_per_value": 8 * sum(len(s) for s in streams) / n_values,
+           "max_err_over_eb": worst,
+           "one_at_a_time_GBps_first32": in_k / t_one / 1e9}
+    print(f"[bench] batch: {len(fields)} fields, {n_values / 1e6:.1f} M values, compress "
+          f"{t_c:.2f} s ({out['compress_GBps']:.2f} GB/s, kernels {tm['kernels_s'] * 1e3:.1f} ms), "
+          f"decompress {t_d:.2f} s; one at a time {out['one_at_a_time_GBps_first32']:.2f} GB/s",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def gnn_leg(args):
@@ -191,6 +270,19 @@ def main():
                     flags=pred.stream_flag | (pred.end_mode << 9), interp_center=pred.center,
                     eb_ratio=pred.fast_eb_ratio)
     ws = codec.workspace(plan, o.radius)
+    tuned = None
+    if args.knob:
+        from latec_amd import autotune
+
+        autotune.set_knobs(**{k: int(v) for k, v in (kv.split("=") for kv in args.knob)})
+        print(f"[bench] knobs: {autotune.get()}", file=sys.stderr, flush=True)
+    elif not args.no_autotune and len(shape) == 3:
+        # launch shapes picked on this very field (outputs identical for all)
+        from latec_amd import autotune
+
+        tuned = autotune.tune(xd, shape, levels, stride, ebs, o, reps=2)
+        print(f"[bench] autotune: {tuned['knobs']} ({tuned['ms']:.3f} ms, "
+              f"{len(tuned['table'])} candidates)", file=sys.stderr, flush=True)
 
     def compress_once():
         res = codec.encode_device(xd, plan, ebs, o, graph=True, ws=ws)
@@ -359,10 +451,15 @@ def main():
                                "(10 B/value + 4 B/outlier)",
                      "decode": {"achieved": ach_dec / 1e9, "frac": ach_dec / HBM_PEAK}},
         "cpu_baseline": cpu,
+        "autotune": None if tuned is None else {"knobs": tuned["knobs"], "ms": tuned["ms"],
+                                                "candidates": len(tuned["table"]),
+                                                "table": tuned["table"]},
         "stream_v5": v5,
     }
     if not args.no_gnn and os.path.exists(args.gnn_ckpt):
         line["gnn"] = gnn_leg(args)
+    if not args.no_batch:
+        line["batch"] = batch_leg(args)
     print(json.dumps(line))
 
 

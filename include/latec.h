@@ -48,8 +48,10 @@ int latec_abi_version(void);
  * autotuner (latec_amd/autotune.py).  No reference counterpart: the
  * reference has no GPU kernels to shape.  Knobs of the 3-D level kernels:
  * "march_blocks" (workgroups per parity class, 8..2^20, rounded up to a
- * multiple of 8) and "march_hist_peel" (0..8 distinct codes per wave
- * counted with one histogram atomic each).  Process-wide. */
+ * multiple of 8), "march_hist_peel" (0..8 distinct codes per wave counted
+ * with one histogram atomic each) and "slab" (1: large levels run as one
+ * level-fused launch; 0: three class launches per level).  Process-wide;
+ * no knob changes a code or a reconstructed value. */
 int latec_tune_set(const char *name, int64_t value);
 int64_t latec_tune_get(const char *name);
 
@@ -183,6 +185,12 @@ typedef struct {
 int latec_fit_const(latec_plan *p, int s0, int s1, const uint32_t *d_hist,
                     const void *d_codes, uint32_t radius, int code_bytes,
                     int precision, latec_fit *fits, void *stream);
+/* The same fit for any list of stages whose histograms are consecutive
+ * (stage s at d_hist + s * latec_hist_words), codes at code_off[s] (n_codes[s]
+ * of them): e.g. every stage of every field of a batch in one call. */
+int latec_fit_stages(int nst, const uint32_t *d_hist, const void *d_codes,
+                     const int64_t *code_off, const int64_t *n_codes, uint32_t radius,
+                     int code_bytes, int precision, latec_fit *fits, void *stream);
 /* The same fit for one code array (no plan): n codes. */
 int latec_fit_codes(const uint32_t *d_hist, const void *d_codes, int64_t n,
                     uint32_t radius, int code_bytes, int precision,

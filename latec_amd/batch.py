@@ -31,7 +31,7 @@ from . import rans as R
 from ._lib import check, lib, ptr
 from .bitstream import (DTYPE_IDS, END_MODE_SHIFT, VERSION, VERSION_MULTILANE, Header,
                         _pack_hlen, parse_record, read_stream, write_stream)
-from .codec import fit_device, pack_device
+from .codec import pack_device
 from .levels import stage_ebs
 from .predictor import InterpPredictor, get_plan
 from .quantizer import opts_for
@@ -173,7 +173,7 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
         lib().latec_batch_destroy(h)
     t2 = time.perf_counter()
     views = [_FieldView(p, buf, lay, f, o) for f, (p, o) in enumerate(zip(plans, opts))]
-    fits = [fit_device(v) for v in views]
+    fits = _fit_all(plans, lay, buf, radius)
     t3 = time.perf_counter()
     if timings is not None:
         timings["fit_s"] = t3 - t2
@@ -188,6 +188,29 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
                         "entropy_s": t4 - t3, "fields": len(plans),
                         "values": int(lay.n_points)})
     return streams
+
+
+def _fit_all(plans, lay, buf, radius):
+    """Per-field stage fits, every stage of every field in one call (the
+    fields' histograms are consecutive in the batch array)."""
+    from ._lib import Fit
+
+    off, n = [], []
+    for f, p in enumerate(plans):
+        for s in range(p.n_stages):
+            off.append(lay.code_base[f] + p.offsets[s])
+            n.append(p.counts[s])
+    ns = len(n)
+    fits = (Fit * max(ns, 1))()
+    if ns:
+        check(lib().latec_fit_stages(ns, ptr(buf.hist), ptr(buf.codes),
+                                     ptr(np.asarray(off, np.int64)), ptr(np.asarray(n, np.int64)),
+                                     radius, 2, 24, ctypes.cast(fits, ctypes.c_void_p), ptr(None)))
+    out, at = [], 0
+    for p in plans:
+        out.append((Fit * p.n_stages)(*[fits[at + s] for s in range(p.n_stages)]))
+        at += p.n_stages
+    return out
 
 
 def _pack_v5_all(views, fits, lay, buf, radius):
@@ -228,11 +251,15 @@ def _pack_v5_all(views, fits, lay, buf, radius):
                                     ptr(out), cap, ptr(blob_off), ptr(blob_len), ptr(None)))
     total = int((blob_off + blob_len).max(initial=0))
     blobs = D.to_host(out[:total], np.uint8) if total else np.zeros(0, np.uint8)
+    all_out = D.to_host(buf.cnt[:max(lay.n_cnt, 1)], np.uint32).astype(np.int64)
     payloads, s0 = [], 0
     for f, v in enumerate(views):
         p = v.plan
-        n_out = D.to_host(v.d_cnt, np.uint32).astype(np.int64)
-        outl, out_off = sorted_outliers(v, p, n_out)
+        n_out = all_out[lay.cnt_base[f]:lay.cnt_base[f] + p.n_stages]
+        if n_out.any():
+            outl, out_off = sorted_outliers(v, p, n_out)
+        else:
+            outl, out_off = np.zeros(1, np.float32), np.zeros(p.n_stages, np.int64)
         parts = []
         for s in range(p.n_stages):
             g = s0 + s

@@ -242,6 +242,17 @@ int latec_fit_const(latec_plan *p, int s0, int s1, const uint32_t *d_hist, const
                    radius, code_bytes, precision, fits, S(stream));
 }
 
+int latec_fit_stages(int nst, const uint32_t *d_hist, const void *d_codes,
+                     const int64_t *code_off, const int64_t *n_codes, uint32_t radius,
+                     int code_bytes, int precision, latec_fit *fits, void *stream) {
+  if (nst < 0 || (nst && (!d_hist || !code_off || !n_codes || !fits))) {
+    set_error("latec_fit_stages: bad arguments");
+    return LATEC_E_ARG;
+  }
+  return fit_hists(nst, d_hist, d_codes, code_off, n_codes, radius, code_bytes, precision, fits,
+                   S(stream));
+}
+
 int latec_fit_codes(const uint32_t *d_hist, const void *d_codes, int64_t n, uint32_t radius,
                     int code_bytes, int precision, latec_fit *fit, void *stream) {
   int64_t off = 0;

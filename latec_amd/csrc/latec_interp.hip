@@ -26,6 +26,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "latec_plan.h"
@@ -1164,6 +1165,376 @@ __global__ void __launch_bounds__(256) march_kernel(MArgs A, const float *__rest
   }
 }
 
+// ---------------------------------------------------------------------------
+// Level-fused slab kernel (3-D level grids, one channel, 16-bit codes).
+//
+// A workgroup owns a tile of SL_TJ x SL_TK columns (axes 1, 2) of the level
+// grid and a segment of SL_SI planes along axis 0, and marches through the
+// segment plane by plane.  Its LDS holds the tile plus a halo of 3 columns
+// on each side for: a ring of the last four even planes (axis-0 taps of the
+// odd planes), the current odd plane, and both planes' input x.  Each step
+// takes even plane e and odd plane o = e - 3 (whose axis-0 taps o +- 1,
+// o +- 3 are then in the ring) through three compute phases: {010, 001 on e;
+// 100 on o}, {011 on e; 110, 101 on o}, {111 on o}; the next step's coarse
+// points and x are loaded into registers while these run.  A class is computed on the halo only where a later
+// class of the tile reads it (100 on the whole region, 001 / 101 on the
+// halo rows, 010 / 110 on the halo columns); owned points emit their code,
+// histogram count and outliers; every owned point of a finished plane --
+// including the coarse points -- is written back to recon as full rows.  x
+// is read once per plane as full region rows.  Arithmetic is interp4 /
+// quantize_one's, so codes, outliers and recon equal the stage kernels'.
+// ---------------------------------------------------------------------------
+#define SL_TJ 32
+#define SL_TK 64
+#define SL_SI 32
+#define SL_THREADS 512
+#define SL_RJ (SL_TJ + 6)
+#define SL_RK (SL_TK + 6)
+#define SL_PLANE (SL_RJ * SL_RK)
+#define SL_HB 256
+
+struct SlabStage {
+  double w, inv_w;
+  float feb;
+  uint32_t R;
+  int64_t code_base;
+  int32_t gs, pad;
+};
+struct SlabClass {
+  int64_t offset;
+  int64_t rstr[3];
+};
+struct SlabArgs {
+  int32_t n[3], cn[3];  // level grid and its coarse (even-point) grid
+  int32_t tiles_j, tiles_k, segs, nblocks;
+  int32_t cubic, center, end_mode, round_mode;
+  double span, offset;
+  int64_t hist_words;
+  SlabStage st[5];
+  SlabClass cl[8];  // by parity 1..7
+};
+
+__device__ __forceinline__ int slab_stage(int P) {
+  return P == 1 ? 0 : P == 2 ? 1 : P == 4 ? 2 : P == 7 ? 4 : 3;
+}
+
+template <bool ENC>
+__global__ void __launch_bounds__(SL_THREADS) slab_kernel(SlabArgs A, const float *__restrict__ x,
+                                                          const float *__restrict__ coarse,
+                                                          float *__restrict__ rec,
+                                                          uint16_t *__restrict__ codes,
+                                                          uint32_t *__restrict__ hist,
+                                                          uint32_t *__restrict__ out_rank,
+                                                          float *__restrict__ out_val,
+                                                          uint32_t *__restrict__ out_count) {
+  extern __shared__ __align__(16) unsigned char sl_smem[];
+  float(*ring)[SL_PLANE] = (float(*)[SL_PLANE])sl_smem;
+  float *odd = (float *)sl_smem + 4 * SL_PLANE;
+  float *xe = odd + SL_PLANE;             // x of the even plane (encode)
+  float *xo = xe + SL_PLANE;              // x of the odd plane (encode)
+  uint32_t *lh = (uint32_t *)(xo + SL_PLANE);
+  const int tid = threadIdx.x;
+  const int n0 = A.n[0], n1 = A.n[1], n2 = A.n[2];
+  // XCD-aware block order: each XCD walks a contiguous range of tiles
+  const uint32_t G = gridDim.x;
+  const uint32_t b = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  if ((int)b >= A.nblocks) return;
+  const int tk = b % A.tiles_k, tj = (b / A.tiles_k) % A.tiles_j;
+  const int seg = b / (A.tiles_k * A.tiles_j);
+  const int j0 = tj * SL_TJ, k0 = tk * SL_TK, I0 = seg * SL_SI;
+  const int I1 = min(I0 + SL_SI, n0);
+  const int j1 = min(j0 + SL_TJ, n1), k1 = min(k0 + SL_TK, n2);      // owned
+  const int rj0 = max(j0 - 3, 0), rj1 = min(j0 + SL_TJ + 3, n1);     // region
+  const int rk0 = max(k0 - 3, 0), rk1 = min(k0 + SL_TK + 3, n2);
+  if (ENC) {
+    for (int i = tid; i < 5 * SL_HB; i += SL_THREADS) lh[i] = 0;
+  }
+  auto L = [&](int j, int k) { return (j - j0 + 3) * SL_RK + (k - k0 + 3); };
+  // --- one class on one plane over rows [ja, jb) x cols [ka, kb) of parity P
+  auto run_class = [&](auto PC, int i, float *self, const float *xin, int ja, int jb, int ka,
+                       int kb) {
+    constexpr int P = decltype(PC)::value;  // the class parity, compile-time
+    const int pj = (P >> 1) & 1, pk = (P >> 2) & 1;
+    ja += (ja & 1) != pj;
+    ka += (ka & 1) != pk;
+    const int nr = jb > ja ? (jb - ja + 1) / 2 : 0, nc = kb > ka ? (kb - ka + 1) / 2 : 0;
+    const int cnt = nr * nc;
+    const int si = slab_stage(P);
+    const SlabStage &S = A.st[si];
+    const SlabClass &K = A.cl[P];
+    const bool own_i = i >= I0 && i < I1;
+    const uint32_t win_lo = S.R > SL_HB / 2 ? S.R - SL_HB / 2 : 0;
+    const int64_t nout = ENC ? 0 : (int64_t)out_count[S.gs];
+    constexpr int nodd = (P & 1) + ((P >> 1) & 1) + ((P >> 2) & 1);
+    for (int base = 0; base < cnt; base += SL_THREADS) {
+      const int q = base + tid;
+      const bool act = q < cnt;
+      const int r = act ? q / nc : 0, c = act ? q - r * nc : 0;
+      const int j = ja + 2 * r, k = ka + 2 * c;
+      const int l = L(j, k);
+      // taps per odd axis (always inside the LDS planes; invalid ones unused)
+      double v[3];
+      bool interior = true;
+      int na = 0;
+      float tap[3][4];
+      bool vm3[3], vp1[3], vp3[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (!((P >> a) & 1)) continue;
+        const int ca = a == 0 ? i : a == 1 ? j : k;
+        const int nn = a == 0 ? n0 : a == 1 ? n1 : n2;
+        vm3[na] = ca >= 3;
+        vp1[na] = ca + 1 < nn;
+        vp3[na] = ca + 3 < nn;
+        interior = interior && vm3[na] && vp3[na];
+        if (a == 0) {
+          tap[na][0] = ring[(((i - 3) >> 1) & 3)][l];
+          tap[na][1] = ring[(((i - 1) >> 1) & 3)][l];
+          tap[na][2] = ring[(((i + 1) >> 1) & 3)][l];
+          tap[na][3] = ring[(((i + 3) >> 1) & 3)][l];
+        } else {
+          const int s = a == 1 ? SL_RK : 1;
+          tap[na][0] = self[l - 3 * s];
+          tap[na][1] = self[l - s];
+          tap[na][2] = self[l + s];
+          tap[na][3] = self[l + 3 * s];
+        }
+        na++;
+      }
+      if (A.cubic && __all(interior || !act)) {
+#pragma unroll
+        for (int m = 0; m < nodd; m++)
+          v[m] = cubic4((double)tap[m][0], (double)tap[m][1], (double)tap[m][2],
+                        (double)tap[m][3]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < nodd; m++)
+          v[m] = interp4((double)tap[m][0], (double)tap[m][1], (double)tap[m][2],
+                         (double)tap[m][3], vm3[m], vp1[m], vp3[m], A.cubic, A.end_mode);
+      }
+      double res;
+      if (nodd == 1) {
+        res = v[0];
+      } else if (A.center == 0) {
+        double total = v[0] + v[1];
+        if (nodd == 3) total = total + v[2];
+        res = total / (double)nodd;
+      } else {
+        res = A.center == 1 ? v[0] : v[nodd - 1];
+      }
+      const float p = (float)res;
+      const bool owned = act && own_i && j >= j0 && j < j1 && k >= k0 && k < k1;
+      const uint32_t rank = (uint32_t)(K.offset + (int64_t)(i >> 1) * K.rstr[0] +
+                                       (int64_t)(j >> 1) * K.rstr[1] +
+                                       (int64_t)(k >> 1) * K.rstr[2]);
+      uint16_t *scodes = codes + S.code_base;
+      uint32_t code = 0;
+      float xv = 0.f, rv = 0.f;
+      if (ENC) {
+        xv = xin[l];
+        const double pd = (double)p;
+        const double d = (double)xv - pd;
+        const double tq = d * S.inv_w;
+        double qd = rint(tq);
+        const bool fb = !(0.5 - fabs(tq - qd) > 1e-12 * fmax(fabs(tq), 1.0));
+        if (__any(fb && act)) {
+          if (fb) qd = rint(d / S.w);
+        }
+        const bool inr = fabs(qd) < (double)S.R;
+        const int qi = inr ? (int)qd : 0;
+        const float cand = (float)(pd + S.w * (double)qi);
+        float chk = cand;
+        if (A.round_mode == 1)
+          chk = rintf(cand);
+        else if (A.round_mode == 2)
+          chk = (float)((rint((double)cand * A.span + A.offset) - A.offset) / A.span);
+        const bool ok = inr && fabsf(xv - chk) <= S.feb;
+        code = ok ? (uint32_t)(qi + (int)S.R) : 0u;
+        rv = ok ? cand : xv;
+      } else {
+        code = act ? (uint32_t)scodes[rank] : 1u;
+        rv = (float)((double)p + S.w * (double)((int)code - (int)S.R));
+        if (__any(act && code == 0)) {
+          if (act && code == 0) {
+            const uint32_t *orr = out_rank + S.code_base;
+            int64_t lo = 0, hi = nout;
+            while (lo < hi) {
+              const int64_t mid = (lo + hi) >> 1;
+              if ((int64_t)orr[mid] < (int64_t)rank) lo = mid + 1; else hi = mid;
+            }
+            rv = (lo < nout && (int64_t)orr[lo] == (int64_t)rank) ? out_val[S.code_base + lo]
+                                                                 : __int_as_float(0x7fc00000);
+          }
+        }
+      }
+      if (act) self[l] = rv;
+      if (ENC) {
+        if (owned) {
+          scodes[rank] = (uint16_t)code;
+          const uint32_t bb = code - win_lo;
+          if (code != 0 && bb < SL_HB)
+            atomicAdd(&lh[si * SL_HB + bb], 1u);
+          else
+            atomicAdd(&hist[(int64_t)S.gs * A.hist_words + code], 1u);
+        }
+        const bool out = owned && code == 0;
+        const uint64_t m = __ballot(out);
+        if (m) {
+          const int lane = __lane_id();
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          uint32_t ob = 0;
+          if (lane == leader) ob = atomicAdd(out_count + S.gs, (uint32_t)__popcll(m));
+          ob = __shfl(ob, leader);
+          if (out) {
+            const uint32_t kk = ob + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+            out_rank[S.code_base + kk] = rank;
+            out_val[S.code_base + kk] = xv;
+          }
+        }
+      }
+    }
+  };
+  // --- write the owned part of a finished plane back as full rows
+  auto write_plane = [&](int i, const float *self) {
+    const int w = k1 - k0;
+    for (int q = tid; q < (j1 - j0) * w; q += SL_THREADS) {
+      const int j = j0 + q / w, k = k0 + q % w;
+      rec[((uint32_t)i * (uint32_t)n1 + (uint32_t)j) * (uint32_t)n2 + (uint32_t)k] = self[L(j, k)];
+    }
+  };
+  // --- the loads of one step (coarse points of even plane e, x of e and of
+  // o = e - 3), held in registers until the step's LDS buffers are free
+  constexpr int PFC = (((SL_RJ + 1) / 2) * ((SL_RK + 1) / 2) + SL_THREADS - 1) / SL_THREADS;
+  constexpr int PFX = (SL_PLANE + SL_THREADS - 1) / SL_THREADS;
+  struct Pref {
+    float c[PFC], xe[ENC ? PFX : 1], xo[ENC ? PFX : 1];
+  };
+  const int cn1 = A.cn[1], cn2 = A.cn[2];
+  const int cj0 = (rj0 + 1) >> 1, cj1 = (rj1 + 1) >> 1;
+  const int ck0 = (rk0 + 1) >> 1, ck1 = (rk1 + 1) >> 1;
+  const int cw = ck1 - ck0, ccount = (cj1 - cj0) * cw;
+  const int xw = rk1 - rk0, xcount = (rj1 - rj0) * xw;
+  auto odd_of = [&](int e) { const int o = e - 3; return o >= I0 + 1 && o < I1 && o < n0; };
+  auto issue = [&](int e, Pref &P) {
+    if (e >= 0 && e < n0) {
+#pragma unroll
+      for (int m = 0; m < PFC; m++) {
+        const int q = tid + m * SL_THREADS;
+        if (q < ccount) {
+          const int cj = cj0 + q / cw, ck = ck0 + q % cw;
+          P.c[m] = coarse[((uint32_t)(e >> 1) * (uint32_t)cn1 + (uint32_t)cj) * (uint32_t)cn2 +
+                          (uint32_t)ck];
+        }
+      }
+      if (ENC) {
+#pragma unroll
+        for (int m = 0; m < PFX; m++) {
+          const int q = tid + m * SL_THREADS;
+          if (q < xcount) {
+            const int j = rj0 + q / xw, k = rk0 + q % xw;
+            P.xe[m] = x[((uint32_t)e * (uint32_t)n1 + (uint32_t)j) * (uint32_t)n2 + (uint32_t)k];
+          }
+        }
+      }
+    }
+    if (ENC && odd_of(e)) {
+      const int o = e - 3;
+#pragma unroll
+      for (int m = 0; m < PFX; m++) {
+        const int q = tid + m * SL_THREADS;
+        if (q < xcount) {
+          const int j = rj0 + q / xw, k = rk0 + q % xw;
+          P.xo[m] = x[((uint32_t)o * (uint32_t)n1 + (uint32_t)j) * (uint32_t)n2 + (uint32_t)k];
+        }
+      }
+    }
+  };
+  auto commit = [&](int e, const Pref &P) {
+    if (e >= 0 && e < n0) {
+      float *slot = ring[(e >> 1) & 3];
+#pragma unroll
+      for (int m = 0; m < PFC; m++) {
+        const int q = tid + m * SL_THREADS;
+        if (q < ccount) slot[L(2 * (cj0 + q / cw), 2 * (ck0 + q % cw))] = P.c[m];
+      }
+      if (ENC) {
+#pragma unroll
+        for (int m = 0; m < PFX; m++) {
+          const int q = tid + m * SL_THREADS;
+          if (q < xcount) xe[L(rj0 + q / xw, rk0 + q % xw)] = P.xe[m];
+        }
+      }
+    }
+    if (ENC && odd_of(e)) {
+#pragma unroll
+      for (int m = 0; m < PFX; m++) {
+        const int q = tid + m * SL_THREADS;
+        if (q < xcount) xo[L(rj0 + q / xw, rk0 + q % xw)] = P.xo[m];
+      }
+    }
+  };
+  __syncthreads();
+  Pref cur;
+  issue(I0 - 2, cur);
+  int prev_o = -1;
+  for (int e = I0 - 2; e <= I1 + 2; e += 2) {
+    const bool doE = e >= 0 && e < n0;
+    const int o = e - 3;
+    const bool doO = odd_of(e);
+    float *slot = ring[((e >> 1) & 3)];
+    // load phase: the previous odd plane goes out, this step's loads go in
+    if (prev_o >= 0) write_plane(prev_o, odd);
+    commit(e, cur);
+    __syncthreads();
+    Pref nxt;
+    if (e + 2 <= I1 + 2) issue(e + 2, nxt);
+    using C1 = std::integral_constant<int, 1>;
+    using C2 = std::integral_constant<int, 2>;
+    using C3 = std::integral_constant<int, 3>;
+    using C4 = std::integral_constant<int, 4>;
+    using C5 = std::integral_constant<int, 5>;
+    using C6 = std::integral_constant<int, 6>;
+    using C7 = std::integral_constant<int, 7>;
+    if (doE) {
+      run_class(C2{}, e, slot, xe, j0, j1, rk0, rk1);    // 010: owned rows, all columns
+      run_class(C4{}, e, slot, xe, rj0, rj1, k0, k1);    // 001: all rows, owned columns
+    }
+    if (doO) run_class(C1{}, o, odd, xo, rj0, rj1, rk0, rk1);  // 100: the whole region
+    __syncthreads();
+    if (doE) run_class(C6{}, e, slot, xe, j0, j1, k0, k1);      // 011
+    if (doO) {
+      run_class(C3{}, o, odd, xo, j0, j1, rk0, rk1);     // 110: owned rows, all columns
+      run_class(C5{}, o, odd, xo, rj0, rj1, k0, k1);     // 101: all rows, owned columns
+    }
+    __syncthreads();
+    if (doO) run_class(C7{}, o, odd, xo, j0, j1, k0, k1);       // 111
+    if (doE && e >= I0 && e < I1) write_plane(e, slot);
+    __syncthreads();
+    prev_o = doO ? o : -1;
+    cur = nxt;
+  }
+  if (prev_o >= 0) write_plane(prev_o, odd);
+  if (ENC) {  // flush the stage windows into the striped histogram copies
+    __syncthreads();
+    for (int si = 0; si < 5; si++) {
+      const SlabStage &S = A.st[si];
+      const uint32_t win_lo = S.R > SL_HB / 2 ? S.R - SL_HB / 2 : 0;
+      const uint32_t glo = S.R > HIST_WIN / 2 ? S.R - HIST_WIN / 2 : 0;
+      uint32_t *gh = hist + (int64_t)S.gs * A.hist_words;
+      uint32_t *gwin = gh + 2 * (int64_t)S.R + (int64_t)(b % HIST_COPIES) * HIST_WIN;
+      for (int i = tid; i < SL_HB; i += SL_THREADS) {
+        const uint32_t v = lh[si * SL_HB + i];
+        const uint32_t code = win_lo + i;
+        if (!v) continue;
+        if (code >= glo && code - glo < HIST_WIN)
+          atomicAdd(&gwin[code - glo], v);
+        else
+          atomicAdd(&gh[code], v);
+      }
+    }
+  }
+}
+
 // Lattice gather / scatter between a grid (shape `fine`) and its every-other
 // point lattice (shape ceil(fine / 2)): gather coarse[m] = fine[2m],
 // scatter fine[2m] = coarse[m].  One thread per coarse point, C channels.
@@ -1434,6 +1805,7 @@ static bool pyramid_ok(const latec_plan *p) {
 }
 
 static size_t level3_smem();
+static size_t slab_smem(bool encode);
 
 static int pyramid_setup(latec_plan *p) {
   int rc = plan_upload(p);
@@ -1470,6 +1842,15 @@ static int pyramid_setup(latec_plan *p) {
     }
   }
   d->K = K;
+  if (p->ndim == 3) {  // the slab kernel's planes exceed the default LDS limit
+    if ((rc = hip_check(hipFuncSetAttribute((const void *)slab_kernel<true>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)slab_smem(true)), "hipFuncSetAttribute")) ||
+        (rc = hip_check(hipFuncSetAttribute((const void *)slab_kernel<false>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)slab_smem(false)), "hipFuncSetAttribute")))
+      return rc;
+  }
   if (p->ndim == 3) {  // the level kernel's window exceeds the default LDS limit
     if ((rc = hip_check(hipFuncSetAttribute((const void *)level3_kernel<true>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1633,6 +2014,7 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
 // 8, so the XCD remap stays exact; the threads walk the rest of the class),
 // and march_hist_peel, the distinct codes per wave counted with one atomic.
 static int64_t g_march_blocks = 4096;
+static int64_t g_slab = 0;  // "slab": 1 runs large levels as one level-fused launch
 static int64_t g_march_peel = 2;  // histogram: distinct codes per wave counted by ballot
 
 static int64_t g_tune_epoch = 0;  // bumped by every knob change
@@ -1646,6 +2028,14 @@ int tune_set(const char *name, int64_t v) {
       return LATEC_E_ARG;
     }
     g_march_blocks = (v + 7) / 8 * 8;
+    return LATEC_OK;
+  }
+  if (!strcmp(name, "slab")) {
+    if (v < 0 || v > 1) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_slab = v;
     return LATEC_OK;
   }
   if (!strcmp(name, "march_hist_peel")) {
@@ -1663,6 +2053,7 @@ int tune_set(const char *name, int64_t v) {
 int64_t tune_get(const char *name) {
   if (!strcmp(name, "march_blocks")) return g_march_blocks;
   if (!strcmp(name, "march_hist_peel")) return g_march_peel;
+  if (!strcmp(name, "slab")) return g_slab;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -1753,6 +2144,80 @@ static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool 
   return LATEC_OK;
 }
 
+// LDS of the slab kernel: ring of 4 even planes + the odd plane (+ the x of
+// both planes and the stage histogram windows when encoding).
+static size_t slab_smem(bool encode) {
+  return sizeof(float) * (size_t)SL_PLANE * (encode ? 7 : 5) +
+         (encode ? sizeof(uint32_t) * 5 * SL_HB : 0);
+}
+
+// Level-fused slab launch for a large 3-D level (one channel, u16 codes):
+// all five sub-stages of level k in one launch, coarse points read from the
+// compact grid of level k-1.  Returns 1 if not applicable.
+static bool slab_ok(const latec_plan *gp, const latec_plan *q, int k,
+                    const latec_interp_opts *o) {
+  if (!g_slab || k < 2 || gp->ndim != 3 || gp->channels != 1 || o->code_bytes != 2 ||
+      q->stages.size() != 6 || q->npoints >= 0x7fffffffll)
+    return false;
+  return q->shape[0] >= 2 * SL_SI && q->shape[1] >= 2 * SL_TJ && q->shape[2] >= 2 * SL_TK;
+}
+
+static int launch_slab(const latec_plan *gp, const latec_plan *q, const latec_plan *qc, int g0,
+                       bool encode, const float *x, const float *coarse, float *recon,
+                       void *codes, uint32_t *hist, uint32_t *out_rank, float *out_val,
+                       uint32_t *out_count, const double *ebs, const latec_interp_opts *o,
+                       hipStream_t stream) {
+  SlabArgs A;
+  memset(&A, 0, sizeof A);
+  for (int j = 0; j < 3; j++) {
+    A.n[j] = (int32_t)q->shape[j];
+    A.cn[j] = (int32_t)qc->shape[j];
+    if (qc->shape[j] != (q->shape[j] + 1) / 2) return 1;
+  }
+  A.tiles_j = (A.n[1] + SL_TJ - 1) / SL_TJ;
+  A.tiles_k = (A.n[2] + SL_TK - 1) / SL_TK;
+  A.segs = (A.n[0] + SL_SI - 1) / SL_SI;
+  const int64_t nb = (int64_t)A.tiles_j * A.tiles_k * A.segs;
+  if (nb > 0x7fffffffll) return 1;
+  A.nblocks = (int32_t)nb;
+  A.cubic = o->cubic;
+  A.center = o->center;
+  A.end_mode = o->end_mode;
+  A.round_mode = o->round_mode;
+  A.span = o->span;
+  A.offset = o->offset;
+  A.hist_words = hist_words(o->radius);
+  for (int js = 1; js <= 5; js++) {
+    const int gs = g0 + js - 1;
+    const GridArgs g = make_args(q, js, ebs[gs], o);
+    SlabStage &S = A.st[js - 1];
+    S.w = g.w;
+    S.inv_w = g.inv_w;
+    S.feb = g.feb;
+    S.R = g.R;
+    S.code_base = (int64_t)gp->channels * gp->stages[gs].offset;
+    S.gs = gs;
+    for (const ClassDesc &c : q->stages[js].classes) {
+      if (c.parity < 1 || c.parity > 7 || c.ndim != 3 || c.step != 2 || (c.flags & 1)) return 1;
+      for (int j = 0; j < 3; j++)
+        if (c.starts[j] != ((c.parity >> j) & 1)) return 1;
+      SlabClass &K = A.cl[c.parity];
+      K.offset = c.offset;
+      for (int j = 0; j < 3; j++) K.rstr[j] = c.rstr[j];
+    }
+  }
+  const unsigned grid = (unsigned)((nb + 7) / 8 * 8);
+  const size_t smem = slab_smem(encode);
+  if (encode)
+    hipLaunchKernelGGL(slab_kernel<true>, dim3(grid), dim3(SL_THREADS), smem, stream, A, x,
+                       coarse, recon, (uint16_t *)codes, hist, out_rank, out_val, out_count);
+  else
+    hipLaunchKernelGGL(slab_kernel<false>, dim3(grid), dim3(SL_THREADS), smem, stream, A,
+                       nullptr, coarse, recon, (uint16_t *)codes, hist, out_rank, out_val,
+                       out_count);
+  return hip_check(hipGetLastError(), "slab_kernel");
+}
+
 // Slab wavefront (opt-in, LATEC_SLAB_MB=<MiB per slab>): a level whose grid
 // exceeds four slabs runs its sub-stages slab by slab along axis 0 (weight-w
 // stages trail by w-1 slabs) so that each slab stays cache resident.
@@ -1779,8 +2244,14 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
     return rc;
   for (int k = 1; k <= K; k++) {
     const latec_plan *q = d->sub[k];
-    if (k > 1 && (rc = lattice(q, d->sub[k - 1], R(k), d->R[k - 1], false, stream))) return rc;
     const int g0 = 1 + (k - 1) * per;
+    if (slab_ok(p, q, k, o)) {  // reads level k-1 itself: no lattice scatter
+      rc = launch_slab(p, q, d->sub[k - 1], g0, encode, X(k), d->R[k - 1], R(k), codes, hist,
+                       out_rank, out_val, out_count, ebs, o, stream);
+      if (rc < 0) return rc;
+      if (rc == 0) continue;
+    }
+    if (k > 1 && (rc = lattice(q, d->sub[k - 1], R(k), d->R[k - 1], false, stream))) return rc;
     const int64_t n0 = q->shape[0];
     const int64_t plane = (int64_t)p->channels * (q->npoints / std::max<int64_t>(n0, 1)) * 8;
     int64_t T = n0;

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <list>
@@ -609,24 +610,6 @@ __global__ void __launch_bounds__(256) tile_count_kernel(const CodeT *codes, int
     if (lc[i]) atomicAdd(&counts[i], (unsigned long long)lc[i]);
 }
 
-// cost[shape_slot][level] for one stage: fixed-order float64 sums of
-// f64(w_f32) * f64(hist_f32), rounded to float32.
-__global__ void __launch_bounds__(64) fit_cost_kernel(const float *const *nl2, int nshapes,
-                                                      const float *hist32, int64_t A,
-                                                      float *cost) {
-  const int slot = blockIdx.x, lvl = threadIdx.x;
-  if (slot >= nshapes) return;
-  const float *row = nl2[slot] + lvl * A;
-  double tot = 0.0;
-  for (int64_t a = 0; a < A; a += 256) {
-    double seg = 0.0;
-    const int64_t b = a + 256 < A ? a + 256 : A;
-    for (int64_t k = a; k < b; k++) seg += (double)row[k] * (double)hist32[k];
-    tot += seg;
-  }
-  cost[slot * NLEV + lvl] = (float)tot;
-}
-
 static int choose_owx(int64_t n_out, int64_t n_codes) {  // rans.py:162-169
   if (n_codes <= 0 || n_out <= 0) return 0;
   const double rate = (double)n_out / (double)n_codes, lr = std::log(rate);
@@ -643,8 +626,35 @@ static int choose_owx(int64_t n_out, int64_t n_codes) {  // rans.py:162-169
   return best;
 }
 
+// cost[job][level] for a batch of (stage, shape) jobs: fixed-order float64
+// sums of f64(w_f32) * f64(hist_f32) (ascending symbols in segments of 256),
+// rounded to float32; one workgroup per job, one thread per level.
+struct CostJob {
+  const float *nl2;   // (64, A) -log2 weights of the job's shape
+  int64_t h_off, A;   // the stage's float32 histogram in the batch array
+};
+__global__ void __launch_bounds__(64) fit_cost_batch_kernel(const CostJob *jobs, int njobs,
+                                                            const float *hist32, float *cost) {
+  const int jb = blockIdx.x, lvl = threadIdx.x;
+  if (jb >= njobs) return;
+  const CostJob J = jobs[jb];
+  const float *row = J.nl2 + lvl * J.A;
+  const float *h = hist32 + J.h_off;
+  double tot = 0.0;
+  for (int64_t a = 0; a < J.A; a += 256) {
+    double seg = 0.0;
+    const int64_t b = a + 256 < J.A ? a + 256 : J.A;
+    for (int64_t k = a; k < b; k++) seg += (double)row[k] * (double)h[k];
+    tot += seg;
+  }
+  cost[(int64_t)jb * NLEV + lvl] = (float)tot;
+}
+
 // Fit stages given their full histograms (device), code arrays (device) and
-// per-stage code counts.
+// per-stage code counts.  All stages move together: one fold + scan launch,
+// one batch of window copies, one cost launch over every (stage, shape) job,
+// one copy of the costs back (plus the per-tile recount of the rare stages
+// whose bins reach 2^24).
 int fit_hists(int nst, const uint32_t *d_hist, const void *d_codes,
               const int64_t *code_off, const int64_t *n_codes, uint32_t R,
               int code_bytes, int precision, latec_fit *fits, hipStream_t st) {
@@ -665,39 +675,48 @@ int fit_hists(int nst, const uint32_t *d_hist, const void *d_codes,
   (void)hipFreeAsync(d_scan, st);
   if (rc) return rc;
 
-  std::lock_guard<std::mutex> lk(g_cache.mu);
+  // window copies of every stage, one synchronization
+  std::vector<std::vector<uint32_t>> win(nst);
   for (int s = 0; s < nst; s++) {
     latec_fit &f = fits[s];
     memset(&f, 0, sizeof f);
     f.n_codes = (uint32_t)n_codes[s];
-    f.kexp = scan[s].kexp;
-    if (!n_codes[s]) {
-      f.kexp = 1;
-      continue;
-    }
+    f.kexp = n_codes[s] ? scan[s].kexp : 1;
+    if (!n_codes[s]) continue;
+    const int64_t span = ((int64_t)1 << f.kexp) - 1;
+    win[s].assign((size_t)(2 * span + 1), 0u);
+    const int64_t c_lo = std::max<int64_t>((int64_t)R - span, 1);
+    const int64_t c_hi = std::min<int64_t>((int64_t)R + span, twoR - 1);
+    if (c_hi >= c_lo &&
+        (rc = hip_ok(hipMemcpyAsync(win[s].data() + (c_lo - ((int64_t)R - span)),
+                                    d_hist + s * HS + c_lo, sizeof(uint32_t) * (c_hi - c_lo + 1),
+                                    hipMemcpyDeviceToHost, st), "copy")))
+      return rc;
+  }
+  if ((rc = hip_ok(hipStreamSynchronize(st), "sync"))) return rc;
+
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  std::vector<float> h32all;
+  std::vector<int64_t> h_off(nst, 0);
+  std::vector<CostJob> jobs;
+  std::vector<int> job0(nst, 0), njob(nst, 0);
+  std::vector<std::array<int, 21>> sids(nst);
+  for (int s = 0; s < nst; s++) {
+    latec_fit &f = fits[s];
+    if (!n_codes[s]) continue;
     const int kexp = f.kexp;
     const int64_t A = (int64_t)2 << kexp, span = ((int64_t)1 << kexp) - 1;
     // float32 histogram over the reduced alphabet (rans.py:413-420)
-    // window of codes [R - span, R + span], clipped to the histogram [0, 2R)
-    std::vector<uint32_t> win((size_t)(2 * span + 1), 0u);
-    const int64_t c_lo = std::max<int64_t>((int64_t)R - span, 1);
-    const int64_t c_hi = std::min<int64_t>((int64_t)R + span, twoR - 1);
-    if (c_hi >= c_lo)
-      rc = hip_ok(hipMemcpyAsync(win.data() + (c_lo - ((int64_t)R - span)), d_hist + s * HS + c_lo,
-                                 sizeof(uint32_t) * (c_hi - c_lo + 1), hipMemcpyDeviceToHost, st),
-                  "copy");
-    if (rc) return rc;
-    if ((rc = hip_ok(hipStreamSynchronize(st), "sync"))) return rc;
     std::vector<float> h32(A);
     h32[0] = (float)scan[s].hist0;
-    for (int64_t k = 1; k < A; k++) h32[k] = (float)win[k - 1];
+    for (int64_t k = 1; k < A; k++) h32[k] = (float)win[s][k - 1];
     if (scan[s].maxbin >= (1u << 24) && n_codes[s] > (1 << 22)) {
       // bins >= 2^24 need the per-tile float32 accumulation
       std::vector<uint32_t> big;
       std::vector<int64_t> bigk;
       if (scan[s].hist0 >= (1u << 24)) { big.push_back(0); bigk.push_back(0); }
       for (int64_t k = 1; k < A && big.size() < 4; k++)
-        if (win[k - 1] >= (1u << 24)) {
+        if (win[s][k - 1] >= (1u << 24)) {
           big.push_back((uint32_t)(R - span + k - 1));
           bigk.push_back(k);
         }
@@ -736,44 +755,55 @@ int fit_hists(int nst, const uint32_t *d_hist, const void *d_codes,
     }
     f.n_out = scan[s].hist0;
     f.owx = choose_owx((int64_t)h32[0], n_codes[s]);
-    int ids[21];
-    const int ns = shape_ids(kexp, ids);
-    std::vector<const float *> tabs(ns);
+    const int ns = shape_ids(kexp, sids[s].data());
+    h_off[s] = (int64_t)h32all.size();
+    h32all.insert(h32all.end(), h32.begin(), h32.end());
+    job0[s] = (int)jobs.size();
+    njob[s] = ns;
     for (int i = 0; i < ns; i++) {
       float *t;
-      if ((rc = ensure_nl2(kexp, ids[i], f.owx, precision, st, &t))) return rc;
-      tabs[i] = t;
+      if ((rc = ensure_nl2(kexp, sids[s][i], f.owx, precision, st, &t))) return rc;
+      jobs.push_back({t, h_off[s], A});
     }
-    const float **d_tabs;
-    float *d_h32, *d_cost;
-    if ((rc = hip_ok(hipMallocAsync((void **)&d_tabs, sizeof(float *) * ns, st), "hipMallocAsync")))
-      return rc;
-    if ((rc = hip_ok(hipMallocAsync((void **)&d_h32, sizeof(float) * A, st), "hipMallocAsync")))
-      return rc;
-    if ((rc = hip_ok(hipMallocAsync((void **)&d_cost, sizeof(float) * ns * NLEV, st),
-                     "hipMallocAsync")))
-      return rc;
-    (void)hipMemcpyAsync(d_tabs, tabs.data(), sizeof(float *) * ns, hipMemcpyHostToDevice, st);
-    (void)hipMemcpyAsync(d_h32, h32.data(), sizeof(float) * A, hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(fit_cost_kernel, dim3(ns), dim3(NLEV), 0, st, (const float *const *)d_tabs,
-                       ns, d_h32, A, d_cost);
-    std::vector<float> cost(ns * NLEV);
-    rc = hip_ok(hipMemcpyAsync(cost.data(), d_cost, sizeof(float) * ns * NLEV,
-                               hipMemcpyDeviceToHost, st), "copy");
-    if (!rc) rc = hip_ok(hipStreamSynchronize(st), "sync");
-    (void)hipFreeAsync(d_tabs, st);
-    (void)hipFreeAsync(d_h32, st);
-    (void)hipFreeAsync(d_cost, st);
-    if (rc) return rc;
+  }
+  if (jobs.empty()) return LATEC_OK;
+  const int nj = (int)jobs.size();
+  CostJob *d_jobs;
+  float *d_h32, *d_cost;
+  if ((rc = hip_ok(hipMallocAsync((void **)&d_jobs, sizeof(CostJob) * nj, st), "hipMallocAsync")))
+    return rc;
+  if ((rc = hip_ok(hipMallocAsync((void **)&d_h32, sizeof(float) * h32all.size(), st),
+                   "hipMallocAsync")))
+    return rc;
+  if ((rc = hip_ok(hipMallocAsync((void **)&d_cost, sizeof(float) * (size_t)nj * NLEV, st),
+                   "hipMallocAsync")))
+    return rc;
+  (void)hipMemcpyAsync(d_jobs, jobs.data(), sizeof(CostJob) * nj, hipMemcpyHostToDevice, st);
+  (void)hipMemcpyAsync(d_h32, h32all.data(), sizeof(float) * h32all.size(), hipMemcpyHostToDevice,
+                       st);
+  hipLaunchKernelGGL(fit_cost_batch_kernel, dim3(nj), dim3(NLEV), 0, st, d_jobs, nj, d_h32,
+                     d_cost);
+  std::vector<float> cost((size_t)nj * NLEV);
+  rc = hip_ok(hipMemcpyAsync(cost.data(), d_cost, sizeof(float) * cost.size(),
+                             hipMemcpyDeviceToHost, st), "copy");
+  if (!rc) rc = hip_ok(hipStreamSynchronize(st), "sync");
+  (void)hipFreeAsync(d_jobs, st);
+  (void)hipFreeAsync(d_h32, st);
+  (void)hipFreeAsync(d_cost, st);
+  if (rc) return rc;
+  for (int s = 0; s < nst; s++) {
+    if (!n_codes[s]) continue;
+    latec_fit &f = fits[s];
     float best = INFINITY;
-    for (int i = 0; i < ns; i++) {
+    for (int i = 0; i < njob[s]; i++) {
+      const float *c = cost.data() + (size_t)(job0[s] + i) * NLEV;
       int bl = 0;
       for (int l = 1; l < NLEV; l++)
-        if (cost[i * NLEV + l] < cost[i * NLEV + bl]) bl = l;
-      if (cost[i * NLEV + bl] < best) {
-        best = cost[i * NLEV + bl];
+        if (c[l] < c[bl]) bl = l;
+      if (c[bl] < best) {
+        best = c[bl];
         f.level = bl;
-        f.shape = ids[i];
+        f.shape = sids[s][i];
       }
     }
     f.bits = best;

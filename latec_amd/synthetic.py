@@ -236,7 +236,12 @@ def batch_fields(n=256, seed=4, scale=1.0, device="cpu"):
             shape = tuple(max(8, int(rng.randint(32, 257) * scale)) for _ in range(3))
         kind = ("smooth", "turbulent", "discontinuous")[rng.randint(3)]
         f64 = rng.rand() < 0.3
-        x = mixed_field(shape, kind, seed * 100003 + i, device)
+        # generated on the next power-of-two box and cropped: a few FFT sizes
+        # serve every field (each new FFT length costs a kernel build)
+        box = tuple(1 << (n - 1).bit_length() for n in shape)
+        x = mixed_field(box, kind, seed * 100003 + i, device)
         x = x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
+        x = np.ascontiguousarray(x[tuple(slice(0, n) for n in shape)], np.float64)
+        x = (x - x.min()) / max(x.max() - x.min(), 1e-30)
         out.append(x.astype(np.float64 if f64 else np.float32))
     return out

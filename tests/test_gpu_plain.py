@@ -107,6 +107,9 @@ CASES = [
     ((40, 33, 17), 1, 3, 8, 1, "cubic", 1, 2, 1e-3, 0.0),
     ((40, 33, 17), 1, 3, 8, 1, "linear", 2, 1, 1e-3, 0.0),
     ((31, 29, 27), 1, 4, 16, 2, "cubic", 0, 1, 5e-4, 0.05),
+    # large enough for the level-fused slab kernel, with partial tiles and segments
+    ((70, 45, 133), 1, 3, 8, 1, "cubic", 0, 1, 1e-3, 0.05),
+    ((67, 33, 129), 1, 3, 8, 1, "linear", 2, 3, 1e-3, 0.02),
     ((12, 11, 10, 9), 1, 2, 4, 1, "cubic", 1, 1, 1e-3, 0.0),
     ((9, 8, 7, 6, 5), 1, 2, 4, 1, "cubic", 0, 3, 1e-2, 0.0),
     ((300,), 1, 5, 32, 1, "cubic", 0, 1, 1e-4, 0.001),
@@ -249,6 +252,31 @@ def test_gpu_interp_predict_standalone():
                     ref = O.predict(recon, mask, s, w, axis, order, center, 3)
                     assert np.array_equal(got, ref)
                 known |= mask
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] in ((70, 45, 133), (67, 33, 129))],
+                         ids=lambda c: f"{c[0]}-{c[5]}")
+def test_gpu_slab_kernel_bit_exact(case):
+    """The level-fused slab kernel (tuning knob "slab") on levels with partial
+    tiles and segments: codes, outliers, recon and stream equal the oracle's."""
+    from latec_amd import autotune
+
+    old = autotune.get()
+    autotune.set_knobs(slab=1)
+    try:
+        test_gpu_encode_matches_oracle_bit_exact(case)
+        shape, C, levels, stride, block, order, center, end, rel, rough = case
+        x = smooth(shape, seed=0, rough=rough)
+        eb = rel * float(x.max() - x.min())
+        pred = InterpPredictor(order, levels, stride, 1)
+        pred.center, pred.end_mode = center, end
+        stream, _ = latec_amd.compress(x, eb, pred, levels=levels, anchor_stride=stride,
+                                       anchor_block=1)
+        ref_stream, ref_recon = O.compress(x, eb, order, levels, stride, 1, center, end)
+        assert stream == ref_stream
+        assert np.array_equal(latec_amd.decompress(stream), ref_recon[0])
+    finally:
+        autotune.set_knobs(**old)
 
 
 def test_gpu_config2_full_size_roundtrip():
