@@ -147,9 +147,9 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
                             version=stream_version))
     lay = _Layout(plans, radius)
     buf = _BatchBuffers(lay)
-    host_x = np.concatenate([x.reshape(-1).astype(np.float32) for x, _, _ in prepped]) \
-        if prepped else np.zeros(1, np.float32)
-    buf.x[:lay.n_points].copy_(torch.from_numpy(host_x[:lay.n_points]))
+    for (x, _, _), off in zip(prepped, lay.x_off):  # each field straight into its slice
+        buf.x[off:off + x.size].copy_(torch.from_numpy(x.reshape(-1).astype(np.float32,
+                                                                            copy=False)))
     buf.recon.zero_()
     buf.hist.zero_()
     buf.cnt.zero_()
