@@ -839,6 +839,10 @@ struct MArgs {
   int32_t cubic, center, end_mode, round_mode, ncls, peel;
   double span, offset;
   int64_t hist_words;
+  const float *coarse;   // weight-1 launches: the compact grid of the level below, where
+                         // every weight-1 tap (a coarse point) is read; else nullptr
+  int32_t cn[3];         // its shape
+  int32_t write_coarse;  // class 100 stores each c-1 tap (a coarse point) into recon
   L3Stage st[3];
   MClass cl[3];
 };
@@ -942,6 +946,7 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
   constexpr int DA0 = (P & 1) && (!WIN || MA != 0) ? 0
                       : ((P >> 1) & 1) && (!WIN || MA != 1) ? 1 : 2;
   constexpr int DA1 = 2;
+  constexpr bool W1C = P == 1 || P == 2 || P == 4;  // weight-1 classes
   const L3Stage &S = A.st[K.stage];
   const uint32_t gstr[3] = {(uint32_t)A.n[1] * (uint32_t)A.n[2], (uint32_t)A.n[2], 1u};
   const uint32_t sMA = gstr[MA];
@@ -979,25 +984,46 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
     const uint32_t rk0 = (uint32_t)(K.offset + (int64_t)ci[0] * K.rstr[0] +
                                     (int64_t)ci[1] * K.rstr[1] + (int64_t)ci[2] * K.rstr[2]);
     const uint32_t cnt = tv ? min((uint32_t)M, K.m[MA] - ci[MA]) : 0u;
-    // direct axes: validity is constant along the march; invalid taps read
-    // the point itself (in bounds, value unused)
+    // tap source: the level's own recon, or, for the weight-1 classes when it
+    // is given, the compact grid of the level below (every weight-1 tap is a
+    // coarse point there: fine c -+ 1, -+ 3 is coarse (c >> 1) + {0, 1, -1, 2}).
+    // tb = the index of the point's c-1 tap along an odd axis; an invalid tap
+    // reads tb itself (in bounds, value unused)
+    const bool ct = W1C && A.coarse != nullptr;
+    const float *tsrc = ct ? A.coarse : (const float *)rec;
+    const uint32_t cs[3] = {(uint32_t)A.cn[1] * (uint32_t)A.cn[2], (uint32_t)A.cn[2], 1u};
+    const uint32_t tb0 = ct ? (uint32_t)(c[0] >> 1) * cs[0] + (uint32_t)(c[1] >> 1) * cs[1] +
+                                  (uint32_t)(c[2] >> 1)
+                            : f0;
+    const uint32_t tstep = ct ? cs[MA] : 2 * sMA;  // per point along the march
+    // direct axes: validity is constant along the march
     bool dv[2][3];       // vm3, vp1, vp3
-    int32_t doff[2][4];  // offsets of taps A (-3s), B (-s), C (+s), D (+3s)
+    int32_t doff[2][4];  // offsets of taps A (c-3), B (c-1), C (c+1), D (c+3) from tb
     bool dir_in = true;  // every direct axis has all four taps
 #pragma unroll
     for (int j = 0; j < NDIR; j++) {
       const int a = j == 0 ? DA0 : DA1;
       const int ca = c[a], na = A.n[a];
-      const int32_t sa = (int32_t)gstr[a];
+      const int32_t sa = ct ? (int32_t)cs[a] : (int32_t)gstr[a];
       dv[j][0] = ca >= 3;
       dv[j][1] = ca + 1 < na;
       dv[j][2] = ca + 3 < na;
       dir_in = dir_in && dv[j][0] && dv[j][2];
-      doff[j][0] = dv[j][0] ? -3 * sa : 0;
-      doff[j][1] = -sa;
-      doff[j][2] = dv[j][1] ? sa : 0;
-      doff[j][3] = dv[j][2] ? 3 * sa : 0;
+      if (ct) {
+        doff[j][0] = dv[j][0] ? -sa : 0;
+        doff[j][1] = 0;
+        doff[j][2] = dv[j][1] ? sa : 0;
+        doff[j][3] = dv[j][2] ? 2 * sa : 0;
+      } else {
+        doff[j][0] = dv[j][0] ? -3 * sa : 0;
+        doff[j][1] = -sa;
+        doff[j][2] = dv[j][1] ? sa : 0;
+        doff[j][3] = dv[j][2] ? 3 * sa : 0;
+      }
     }
+    // window offsets along MA from tb
+    const int32_t sw = ct ? (int32_t)cs[MA] : (int32_t)sMA;
+    const int32_t wo0 = ct ? -sw : -3 * sw, wo1 = ct ? 0 : -sw, wo2 = sw, wo3 = ct ? 2 * sw : 3 * sw;
     // the loads of point t: x (or its code), the window's new tap c+3, and
     // the direct taps -- all unconditional, issued one point ahead
     struct Loads {
@@ -1009,23 +1035,25 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
     auto load = [&](uint32_t t, Loads &L) {
       const bool has = t < cnt;
       const uint32_t f = has ? f0 + 2 * t * sMA : f0;
+      const uint32_t tb = has ? tb0 + t * tstep : tb0;
       if (ENC)
         L.xin = ldo(x, f);
       else
         L.cin = ldo(scodes, has ? rk0 + t * rMA : rk0);
-      if (WIN) L.w3 = ldo((const float *)rec, has && c[MA] + 2 * (int)t + 3 < nMA ? f + 3 * sMA : f);
+      if (WIN)
+        L.w3 = ldo(tsrc, has && c[MA] + 2 * (int)t + 3 < nMA ? (uint32_t)((int32_t)tb + wo3) : tb);
 #pragma unroll
       for (int j = 0; j < NDIR; j++)
 #pragma unroll
         for (int k = 0; k < 4; k++)
-          L.dt[j][k] = ldo((const float *)rec, (uint32_t)((int32_t)f + doff[j][k]));
+          L.dt[j][k] = ldo(tsrc, (uint32_t)((int32_t)tb + doff[j][k]));
     };
     double w0 = 0.0, w1 = 0.0, w2 = 0.0;  // window taps c-3, c-1, c+1
     if (WIN) {
       const int cm = c[MA];
-      w0 = (double)ldo((const float *)rec, cm >= 3 ? f0 - 3 * sMA : f0);
-      w1 = (double)ldo((const float *)rec, f0 - sMA);
-      w2 = (double)ldo((const float *)rec, cm + 1 < nMA ? f0 + sMA : f0);
+      w0 = (double)ldo(tsrc, cm >= 3 ? (uint32_t)((int32_t)tb0 + wo0) : tb0);
+      w1 = (double)ldo(tsrc, (uint32_t)((int32_t)tb0 + wo1));
+      w2 = (double)ldo(tsrc, cm + 1 < nMA ? (uint32_t)((int32_t)tb0 + wo2) : tb0);
     }
     Loads cur, nxt;
     load(0, cur);
@@ -1117,6 +1145,9 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
         }
       }
       if (active) sto(rec, f, r);
+      // the c-1 tap of a class-100 point is the coarse point below it: with
+      // write_coarse every coarse point of the level lands in recon here
+      if (P == 1 && ct && A.write_coarse && active) sto(rec, f - sMA, (float)w1);
       if (ENC)
         emit_wave(scodes, rank, code, xv, rank, orank, oval, ocnt, lhist, ghist, win_lo, active,
                   peel);
@@ -2071,7 +2102,9 @@ static bool march_ok(const latec_plan *gp, const latec_plan *q, const latec_inte
 static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool encode,
                         const float *x, float *recon, void *codes, uint32_t *hist,
                         uint32_t *out_rank, float *out_val, uint32_t *out_count,
-                        const double *ebs, const latec_interp_opts *o, hipStream_t stream) {
+                        const double *ebs, const latec_interp_opts *o, hipStream_t stream,
+                        const latec_plan *qc = nullptr, const float *coarse = nullptr,
+                        bool write_coarse = false) {
   static const int groups[3][3] = {{1, 2, 3}, {4, 0, 0}, {5, 0, 0}};  // sub-plan stages
   MArgs A[3];
   for (int gi = 0; gi < 3; gi++) {
@@ -2086,6 +2119,14 @@ static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool 
     a.offset = o->offset;
     a.hist_words = hist_words(o->radius);
     a.peel = (int32_t)g_march_peel;
+    if (gi == 0 && coarse) {  // the weight-1 stages read their taps from the level below
+      a.coarse = coarse;
+      for (int j = 0; j < 3; j++) {
+        a.cn[j] = (int32_t)qc->shape[j];
+        if (qc->shape[j] != (q->shape[j] + 1) / 2) return 1;
+      }
+      a.write_coarse = write_coarse ? 1 : 0;
+    }
     int ns = 0;
     for (int si = 0; si < 3 && groups[gi][si]; si++) {
       const int js = groups[gi][si], gs = g0 + js - 1;
@@ -2251,8 +2292,17 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
       if (rc < 0) return rc;
       if (rc == 0) continue;
     }
-    if (k > 1 && (rc = lattice(q, d->sub[k - 1], R(k), d->R[k - 1], false, stream))) return rc;
     const int64_t n0 = q->shape[0];
+    // march levels read the coarse points from the level below and, with an
+    // even number of planes, store them into this level themselves
+    bool march_direct = k > 1 && !level3_ok(p, q, o) && !getenv("LATEC_SLAB_MB") &&
+                        march_ok(p, q, o);
+    for (int j = 0; march_direct && j < 3; j++)
+      march_direct = d->sub[k - 1]->shape[j] == (q->shape[j] + 1) / 2;
+    const bool write_coarse = march_direct && n0 % 2 == 0;
+    if (k > 1 && !write_coarse &&
+        (rc = lattice(q, d->sub[k - 1], R(k), d->R[k - 1], false, stream)))
+      return rc;
     const int64_t plane = (int64_t)p->channels * (q->npoints / std::max<int64_t>(n0, 1)) * 8;
     int64_t T = n0;
     static const int64_t slab_bytes = getenv("LATEC_SLAB_MB") ? atoll(getenv("LATEC_SLAB_MB")) << 20 : 0;
@@ -2267,9 +2317,14 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
     }
     if (nslab == 1 && march_ok(p, q, o)) {
       rc = launch_march(p, q, g0, encode, X(k), R(k), codes, hist, out_rank, out_val, out_count,
-                        ebs, o, stream);
+                        ebs, o, stream, march_direct ? d->sub[k - 1] : nullptr,
+                        march_direct ? d->R[k - 1] : nullptr, write_coarse);
       if (rc < 0) return rc;
       if (rc == 0) continue;
+      if (write_coarse) {  // layout refused: the generic path needs the scattered coarse points
+        set_error("march layout mismatch");
+        return LATEC_E_ARG;
+      }
     }
     for (int64_t t = 0; t < nslab + nd - 1; t++) {
       for (int w = 1; w <= nd; w++) {
