@@ -1,6 +1,6 @@
 """Autotuner for the plain path's launch shapes.
 
-The 3-D level kernels have three launch-shape knobs, set process-wide with
+The 3-D level kernels have four launch-shape knobs, set process-wide with
 ``latec_tune_set``:
 
 * ``march_blocks`` -- workgroups per parity class (with 4 waves each, this is
@@ -9,7 +9,10 @@ The 3-D level kernels have three launch-shape knobs, set process-wide with
 * ``march_hist_peel`` -- how many distinct codes per wave the histogram
   counts with one atomic each before falling back to per-lane atomics;
 * ``slab`` -- whether large levels run as one level-fused launch (tiles of
-  the level marched plane by plane through LDS) or as three class launches.
+  the level marched plane by plane through LDS) or as three class launches;
+* ``pair`` -- whether the levels above the first run as line-pair launches
+  (one thread codes the two classes of a line along axis 2 together, so x is
+  read and recon written as whole lines) or as class launches.
 
 Neither changes a single code, outlier or reconstructed value (they only
 decide which thread does which point and how counts are summed).
@@ -29,10 +32,11 @@ import numpy as np
 
 from ._lib import check, lib, ptr
 
-KNOBS = ("march_blocks", "march_hist_peel", "slab")
+KNOBS = ("march_blocks", "march_hist_peel", "slab", "pair")
 CANDIDATES = {"march_blocks": (1024, 2048, 4096, 8192),
-              "march_hist_peel": (0, 1, 2, 4),
-              "slab": (0, 1)}
+              "march_hist_peel": (0, 2),
+              "slab": (0, 1),
+              "pair": (0, 1)}
 
 
 def get() -> dict:
@@ -107,7 +111,8 @@ def tune(d_x, shape, levels, stride, ebs, opts, reps=3, candidates=None, save=Tr
     ebs_arr = np.ascontiguousarray(ebs, np.float64)
     stream = torch.cuda.current_stream()
     try:
-        for values in itertools.product(*[cand[k] for k in KNOBS]):
+        # a knob the candidates leave out keeps its current value
+        for values in itertools.product(*[cand.get(k, (start[k],)) for k in KNOBS]):
             knobs = dict(zip(KNOBS, values))
             set_knobs(**knobs)
             codec.encode_device(d_x, plan, ebs, opts, graph=True, ws=ws)  # capture + warm

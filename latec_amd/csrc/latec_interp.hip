@@ -1197,6 +1197,381 @@ __global__ void __launch_bounds__(256) march_kernel(MArgs A, const float *__rest
 }
 
 // ---------------------------------------------------------------------------
+// Line-pair kernels (3-D levels above the first, one channel, 16-bit codes).
+//
+// A level's lines along axis 2 split by the parity (i & 1, j & 1) of their
+// position into four types, and each type holds two classes alternating
+// along k:
+//   (even, even): the coarse points (k even) and class 001 (k odd);
+//   (odd, even):  class 100 and class 101;   (even, odd): class 010 and 011;
+//   (odd, odd):   class 110 and class 111.
+// One thread codes the pair (k = 2m, 2m + 1) of a line and marches along
+// axis 0 or 1 through the lines of its type, so x is read and recon written
+// as whole lines.  Three launches follow the dependencies:
+//   phase 1, (e,e) lines: the coarse points (copied from the level below's
+//            compact grid) and 001 (taps: coarse points along k);
+//   phase 2, (o,e) lines marching along axis 0 and (e,o) lines marching
+//            along axis 1: 100 / 010 (taps: the (e,e) lines' even points
+//            along the march), then 101 / 011 (the (e,e) lines' odd points
+//            along the march, and the pair's own even points along k);
+//   phase 3, (o,o) lines marching along axis 0: 110 (taps: the (e,o) lines'
+//            even points along the march, the (o,e) lines' even points
+//            along axis 1), then 111 (the odd points of those lines, and
+//            110 along k).
+// The k-neighbours of an odd point are the even points of lanes l-1, l+1
+// and l+2 at the same step.  In phases 2 and 3 a wave therefore owns the 61
+// pairs of lanes 1..61 of a run of pairs numbered across the lines of the
+// type; its lanes 0, 62 and 63 recompute the even points of the
+// neighbouring runs (same inputs, same arithmetic, so the same bits) and
+// store and emit nothing.  Arithmetic is interp4 / the stage quantizer in
+// double, exactly as the other level kernels.
+// ---------------------------------------------------------------------------
+#define PAIR_M 8    // march steps per thread
+#define PAIR_OWN 61  // pairs a wave owns in phases 2 and 3
+
+struct PClass {
+  int32_t stage, pad;  // index into PArgs::st
+  int64_t offset;      // stage rank of the class's first point
+  int64_t rstr[3];     // stage-rank strides per class index
+};
+struct PSet {
+  int32_t ma, ra;       // marching axis and row axis (0 / 1)
+  int32_t pm, pr;       // parity of the set's lines along ma and ra
+  uint32_t nmarch;      // line positions along ma of that parity
+  uint32_t nrows;       // line positions along ra of that parity
+  uint32_t nwaves;      // waves per march chunk (runs of pairs)
+  uint32_t nthreads;    // nchunks * nwaves * 64
+  PClass ev, od;        // the classes at k = 2m and k = 2m + 1
+};
+struct PArgs {
+  int32_t n[3], cn[3];
+  const float *coarse;  // phase 1: the level below's compact grid
+  int32_t cubic, center, end_mode, round_mode, peel, pad;
+  double span, offset;
+  int64_t hist_words;
+  uint32_t npairs, own;  // pairs per line; pairs a wave owns
+  L3Stage st[3];
+  PSet set[2];
+};
+
+// the centre rule over the per-axis predictions v[0..nodd) (ascending axes)
+__device__ __forceinline__ double pair_combine(const double *v, int nodd, int center) {
+  if (nodd == 1) return v[0];
+  if (center == 0) {
+    double total = v[0] + v[1];
+    if (nodd == 3) total = total + v[2];
+    return total / (double)nodd;
+  }
+  return center == 1 ? v[0] : v[nodd - 1];
+}
+
+// One stage's quantizer constants.
+struct PQ {
+  double w, inv_w, Rd;
+  float feb;
+  uint32_t R;
+};
+
+// quantize_one (encode): code and reconstruction of x against prediction p.
+// Every lane of the wave calls it (the exact-division fallback is a vote).
+__device__ __forceinline__ void pair_quant(const PQ &Q, int round_mode, double span,
+                                           double offset, double pd, float xv, uint32_t &code,
+                                           float &r) {
+  const double d = (double)xv - pd;
+  const double tq = d * Q.inv_w;
+  double qd = rint(tq);
+  const bool fb = !(0.5 - fabs(tq - qd) > 1e-12 * fmax(fabs(tq), 1.0));
+  if (__any(fb)) {
+    if (fb) qd = rint(d / Q.w);
+  }
+  const bool inr = fabs(qd) < Q.Rd;
+  const int qi = inr ? (int)qd : 0;
+  const float cand = (float)(pd + Q.w * (double)qi);
+  float chk = cand;
+  if (round_mode == 1)
+    chk = rintf(cand);
+  else if (round_mode == 2)
+    chk = (float)((rint((double)cand * span + offset) - offset) / span);
+  const bool ok = inr && fabsf(xv - chk) <= Q.feb;
+  code = ok ? (uint32_t)(qi + (int)Q.R) : 0u;
+  r = ok ? cand : xv;
+}
+
+// Decode: reconstruction from a code; code 0 looks the value up among the
+// stage's outliers (sorted by rank), a missing one gives NaN.
+__device__ __forceinline__ float pair_dequant(const PQ &Q, double pd, uint32_t code, bool active,
+                                              uint32_t rank, const uint32_t *__restrict__ orank,
+                                              const float *__restrict__ oval, int64_t nout) {
+  float r = (float)(pd + Q.w * (double)((int)code - (int)Q.R));
+  if (__any(active && code == 0)) {
+    if (active && code == 0) {
+      int64_t lo = 0, hi = nout;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)orank[mid] < (int64_t)rank) lo = mid + 1; else hi = mid;
+      }
+      r = (lo < nout && (int64_t)orank[lo] == (int64_t)rank) ? oval[lo]
+                                                             : __int_as_float(0x7fc00000);
+    }
+  }
+  return r;
+}
+
+// a pair (k, k + 1) at element idx; when the pair has no odd point the odd
+// lane reads idx again (in bounds, value unused)
+__device__ __forceinline__ float2 pair_ld(const float *__restrict__ b, uint32_t idx, bool al,
+                                          bool odd_ok) {
+  if (al) return ldo((const float2 *)b, idx >> 1);
+  return make_float2(ldo(b, idx), ldo(b, odd_ok ? idx + 1 : idx));
+}
+
+template <int PH, bool ENC>
+__device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
+                                         const float *__restrict__ x, float *__restrict__ rec,
+                                         uint16_t *__restrict__ codes,
+                                         uint32_t *__restrict__ hist, uint32_t *lh,
+                                         uint32_t *__restrict__ out_rank,
+                                         float *__restrict__ out_val,
+                                         uint32_t *__restrict__ out_count) {
+  constexpr int M = PAIR_M;
+  constexpr int NE = PH == 1 ? 0 : PH == 2 ? 1 : 2;  // odd axes of the even class
+  constexpr int NO = NE + 1;                         // odd axes of the odd class
+  const int ma = T.ma, ra = T.ra;
+  const uint32_t gstr[3] = {(uint32_t)A.n[1] * (uint32_t)A.n[2], (uint32_t)A.n[2], 1u};
+  const uint32_t sma = gstr[ma], sra = gstr[ra];
+  const int n2 = A.n[2], nma = A.n[ma], nra = A.n[ra];
+  const bool al = (n2 & 1) == 0;  // every pair starts 8-byte aligned
+  const int cubic = A.cubic, end_mode = A.end_mode, center = A.center;
+  const int round_mode = A.round_mode, peel = A.peel;
+  const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
+  const PQ QE = {SE.w, SE.inv_w, (double)SE.R, SE.feb, SE.R};
+  const PQ QO = {SO.w, SO.inv_w, (double)SO.R, SO.feb, SO.R};
+  const uint32_t wlo_e = SE.R > HIST_WIN / 2 ? SE.R - HIST_WIN / 2 : 0;
+  const uint32_t wlo_o = SO.R > HIST_WIN / 2 ? SO.R - HIST_WIN / 2 : 0;
+  uint32_t *gh_e = ENC ? hist + (int64_t)SE.gs * A.hist_words : nullptr;
+  uint32_t *gh_o = ENC ? hist + (int64_t)SO.gs * A.hist_words : nullptr;
+  uint32_t *or_e = out_rank + SE.out_lo, *or_o = out_rank + SO.out_lo;
+  float *ov_e = out_val + SE.out_lo, *ov_o = out_val + SO.out_lo;
+  const int64_t nout_e = ENC ? 0 : (int64_t)out_count[SE.gs];
+  const int64_t nout_o = ENC ? 0 : (int64_t)out_count[SO.gs];
+  uint16_t *sc_e = codes + SE.code_base, *sc_o = codes + SO.code_base;
+  const uint32_t rma_e = (uint32_t)T.ev.rstr[ma], rma_o = (uint32_t)T.od.rstr[ma];
+  const int lane = __lane_id();
+  const uint32_t own = A.own, halo = PH == 1 ? 0u : 1u;
+  const uint32_t total = T.nrows * A.npairs;
+  const uint32_t cs0 = (uint32_t)A.cn[1] * (uint32_t)A.cn[2], cs1 = (uint32_t)A.cn[2];
+  const uint32_t G = gridDim.x;
+  const uint32_t lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  for (uint32_t base = lb * 256u; base < T.nthreads; base += G * 256u) {
+    const uint32_t gw = (base + threadIdx.x) >> 6;  // uniform in the wave
+    const bool wv = base + threadIdx.x < T.nthreads;
+    const uint32_t chunk = wv ? gw / T.nwaves : 0u;
+    const uint32_t run = wv ? gw - chunk * T.nwaves : 0u;
+    const int64_t P = (int64_t)run * own + lane - (int64_t)halo;
+    const bool owned = wv && lane >= (int)halo && lane < (int)(halo + own) && P < (int64_t)total;
+    const uint32_t Pc = (uint32_t)min(max(P, (int64_t)0), (int64_t)total - 1);
+    const uint32_t row = Pc / A.npairs, m = Pc - row * A.npairs;
+    int c[3];
+    c[ma] = T.pm + 2 * (int)(chunk * M);
+    c[ra] = T.pr + 2 * (int)row;
+    c[2] = 2 * (int)m;
+    const bool odd_ok = 2 * (int)m + 1 < n2;
+    const uint32_t f0 = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
+    uint32_t ci[3];
+    ci[ma] = chunk * M;
+    ci[ra] = row;
+    ci[2] = m;
+    const uint32_t rk_e0 = (uint32_t)(T.ev.offset + (int64_t)ci[0] * T.ev.rstr[0] +
+                                      (int64_t)ci[1] * T.ev.rstr[1] + (int64_t)m * T.ev.rstr[2]);
+    const uint32_t rk_o0 = (uint32_t)(T.od.offset + (int64_t)ci[0] * T.od.rstr[0] +
+                                      (int64_t)ci[1] * T.od.rstr[1] + (int64_t)m * T.od.rstr[2]);
+    const uint32_t cnt = wv ? min((uint32_t)M, T.nmarch - chunk * M) : 0u;
+    // taps along k of the odd point k = 2m + 1 (lanes l-1, l, l+1, l+2)
+    const bool kv0 = m >= 1, kv1 = 2 * (int)m + 2 < n2, kv2 = 2 * (int)m + 4 < n2;
+    // direct taps along ra (phase 3): validity is constant along the march
+    const int cr = c[ra];
+    const bool dv0 = cr >= 3, dv1 = cr + 1 < nra, dv2 = cr + 3 < nra;
+    int32_t doff[4] = {0, 0, 0, 0};
+    if (PH == 3) {
+      doff[0] = dv0 ? -3 * (int32_t)sra : -(int32_t)sra;
+      doff[1] = -(int32_t)sra;
+      doff[2] = dv1 ? (int32_t)sra : -(int32_t)sra;
+      doff[3] = dv2 ? 3 * (int32_t)sra : -(int32_t)sra;
+    }
+    // phase 1: the coarse row of the pair, m-1 .. m+2 (invalid taps read m)
+    const uint32_t cb0 = PH == 1 ? (uint32_t)(c[0] >> 1) * cs0 + (uint32_t)(c[1] >> 1) * cs1 + m
+                                 : 0u;
+    const int32_t co0 = kv0 ? -1 : 0, co2 = kv1 ? 1 : 0, co3 = kv2 ? 2 : 0;
+    struct Loads {
+      float2 xx;
+      uint32_t ce, co;
+      float2 w3;
+      float2 d[4];
+      float ct[4];
+    };
+    auto load = [&](uint32_t t, Loads &L) {
+      const bool has = t < cnt;
+      const uint32_t f = has ? f0 + 2 * t * sma : f0;
+      if (ENC) {
+        if (PH == 1)
+          L.xx.y = ldo(x, odd_ok ? f + 1 : f);
+        else
+          L.xx = pair_ld(x, f, al, odd_ok);
+      } else {
+        if (PH > 1) L.ce = ldo(sc_e, has ? rk_e0 + t * rma_e : rk_e0);
+        // a pair without an odd point has no odd rank: its formula can run
+        // past the stage (and the code array)
+        L.co = has && odd_ok ? ldo(sc_o, rk_o0 + t * rma_o) : 0u;
+      }
+      if (PH == 1) {
+        const uint32_t cb = has ? cb0 + t * cs0 : cb0;
+        L.ct[0] = ldo(A.coarse, (uint32_t)((int32_t)cb + co0));
+        L.ct[1] = ldo(A.coarse, cb);
+        L.ct[2] = ldo(A.coarse, (uint32_t)((int32_t)cb + co2));
+        L.ct[3] = ldo(A.coarse, (uint32_t)((int32_t)cb + co3));
+      } else {
+        const int cm = c[ma] + 2 * (int)t;
+        L.w3 = pair_ld(rec, has && cm + 3 < nma ? f + 3 * sma : f - sma, al, odd_ok);
+        if (PH == 3) {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            L.d[q] = pair_ld(rec, (uint32_t)((int32_t)f + doff[q]), al, odd_ok);
+        }
+      }
+    };
+    // the window along ma (phases 2, 3): taps c-3, c-1, c+1 of the first step
+    double we0 = 0.0, we1 = 0.0, we2 = 0.0, wo0 = 0.0, wo1 = 0.0, wo2 = 0.0;
+    if (PH > 1) {
+      const int cm = c[ma];
+      const float2 a = pair_ld(rec, cm >= 3 ? f0 - 3 * sma : f0 - sma, al, odd_ok);
+      const float2 b = pair_ld(rec, f0 - sma, al, odd_ok);
+      const float2 d = pair_ld(rec, cm + 1 < nma ? f0 + sma : f0 - sma, al, odd_ok);
+      we0 = a.x; wo0 = a.y; we1 = b.x; wo1 = b.y; we2 = d.x; wo2 = d.y;
+    }
+    Loads cur, nxt;
+    load(0, cur);
+#pragma unroll
+    for (uint32_t t = 0; t < (uint32_t)M; t++) {
+      if (t + 1 < (uint32_t)M) load(t + 1, nxt);
+      const bool active = owned && t < cnt;
+      const uint32_t f = f0 + 2 * t * sma;
+      const uint32_t rke = rk_e0 + t * rma_e, rko = rk_o0 + t * rma_o;
+      const int cm = c[ma] + 2 * (int)t;
+      const bool mv0 = cm >= 3, mv1 = cm + 1 < nma, mv2 = cm + 3 < nma;
+      const double we3 = PH > 1 ? (double)cur.w3.x : 0.0, wo3 = PH > 1 ? (double)cur.w3.y : 0.0;
+      // --- the even point
+      float re = 0.f, xe = 0.f;
+      uint32_t cde = 0;
+      if (PH == 1) {
+        re = cur.ct[1];  // the coarse point itself
+      } else {
+        double v[2];
+        if (cubic && __all(mv0 && mv2 && (PH == 2 || (dv0 && dv2)))) {
+          v[0] = cubic4(we0, we1, we2, we3);
+          if (PH == 3)
+            v[1] = cubic4((double)cur.d[0].x, (double)cur.d[1].x, (double)cur.d[2].x,
+                          (double)cur.d[3].x);
+        } else {
+          v[0] = interp4(we0, we1, we2, we3, mv0, mv1, mv2, cubic, end_mode);
+          if (PH == 3)
+            v[1] = interp4((double)cur.d[0].x, (double)cur.d[1].x, (double)cur.d[2].x,
+                           (double)cur.d[3].x, dv0, dv1, dv2, cubic, end_mode);
+        }
+        const double pd = (double)(float)pair_combine(v, NE, center);
+        if (ENC) {
+          xe = cur.xx.x;
+          pair_quant(QE, round_mode, A.span, A.offset, pd, xe, cde, re);
+        } else {
+          // every lane with a point decodes it: lanes 0, 62, 63 feed the k taps
+          cde = cur.ce;
+          re = pair_dequant(QE, pd, cde, wv && t < cnt, rke, or_e, ov_e, nout_e);
+        }
+      }
+      // --- the odd point: k taps from the even points of lanes l-1 .. l+2
+      double k0, k1, k2, k3;
+      if (PH == 1) {
+        k0 = cur.ct[0]; k1 = cur.ct[1]; k2 = cur.ct[2]; k3 = cur.ct[3];
+      } else {
+        k0 = (double)__shfl_up(re, 1);
+        k1 = (double)re;
+        k2 = (double)__shfl_down(re, 1);
+        k3 = (double)__shfl_down(re, 2);
+      }
+      double v[3];
+      if (cubic && __all(!odd_ok || (kv0 && kv2 && (PH == 1 || (mv0 && mv2)) &&
+                                     (PH < 3 || (dv0 && dv2))))) {
+        int i = 0;
+        if (PH > 1) v[i++] = cubic4(wo0, wo1, wo2, wo3);
+        if (PH == 3)
+          v[i++] = cubic4((double)cur.d[0].y, (double)cur.d[1].y, (double)cur.d[2].y,
+                          (double)cur.d[3].y);
+        v[i++] = cubic4(k0, k1, k2, k3);
+      } else {
+        int i = 0;
+        if (PH > 1) v[i++] = interp4(wo0, wo1, wo2, wo3, mv0, mv1, mv2, cubic, end_mode);
+        if (PH == 3)
+          v[i++] = interp4((double)cur.d[0].y, (double)cur.d[1].y, (double)cur.d[2].y,
+                           (double)cur.d[3].y, dv0, dv1, dv2, cubic, end_mode);
+        v[i++] = interp4(k0, k1, k2, k3, kv0, kv1, kv2, cubic, end_mode);
+      }
+      const double po = (double)(float)pair_combine(v, NO, center);
+      float ro, xo = 0.f;
+      uint32_t cdo;
+      if (ENC) {
+        xo = cur.xx.y;
+        pair_quant(QO, round_mode, A.span, A.offset, po, xo, cdo, ro);
+      } else {
+        cdo = cur.co;
+        ro = pair_dequant(QO, po, cdo, active && odd_ok, rko, or_o, ov_o, nout_o);
+      }
+      if (active) {
+        if (al) {
+          sto((float2 *)rec, f >> 1, make_float2(re, ro));
+        } else {
+          sto(rec, f, re);
+          if (odd_ok) sto(rec, f + 1, ro);
+        }
+      }
+      if (ENC) {
+        if (PH > 1)
+          emit_wave(sc_e, rke, cde, xe, rke, or_e, ov_e, out_count + SE.gs, lh, gh_e, wlo_e,
+                    active, peel);
+        emit_wave(sc_o, rko, cdo, xo, rko, or_o, ov_o, out_count + SO.gs, lh + HIST_WIN, gh_o,
+                  wlo_o, active && odd_ok, peel);
+      }
+      if (PH > 1) {
+        we0 = we1; we1 = we2; we2 = we3;
+        wo0 = wo1; wo1 = wo2; wo2 = wo3;
+      }
+      cur = nxt;
+    }
+  }
+}
+
+template <int PH, bool ENC>
+__global__ void __launch_bounds__(256) pair_kernel(PArgs A, const float *__restrict__ x,
+                                                   float *__restrict__ rec,
+                                                   uint16_t *__restrict__ codes,
+                                                   uint32_t *__restrict__ hist,
+                                                   uint32_t *__restrict__ out_rank,
+                                                   float *__restrict__ out_val,
+                                                   uint32_t *__restrict__ out_count) {
+  __shared__ uint32_t lh[ENC ? 2 * HIST_WIN : 1];
+  if (ENC) {
+    for (int i = threadIdx.x; i < 2 * HIST_WIN; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+  }
+  const PSet &T = A.set[blockIdx.y];
+  pair_set<PH, ENC>(A, T, x, rec, codes, hist, lh, out_rank, out_val, out_count);
+  if (ENC) {
+    __syncthreads();
+    const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
+    if (PH > 1) flush_hist(lh, hist + (int64_t)SE.gs * A.hist_words, SE.R);
+    flush_hist(lh + HIST_WIN, hist + (int64_t)SO.gs * A.hist_words, SO.R);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Level-fused slab kernel (3-D level grids, one channel, 16-bit codes).
 //
 // A workgroup owns a tile of SL_TJ x SL_TK columns (axes 1, 2) of the level
@@ -2046,6 +2421,7 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
 // and march_hist_peel, the distinct codes per wave counted with one atomic.
 static int64_t g_march_blocks = 4096;
 static int64_t g_slab = 0;  // "slab": 1 runs large levels as one level-fused launch
+static int64_t g_pair = 1;  // "pair": 1 runs 3-D levels above the first as line-pair launches
 static int64_t g_march_peel = 2;  // histogram: distinct codes per wave counted by ballot
 
 static int64_t g_tune_epoch = 0;  // bumped by every knob change
@@ -2069,6 +2445,14 @@ int tune_set(const char *name, int64_t v) {
     g_slab = v;
     return LATEC_OK;
   }
+  if (!strcmp(name, "pair")) {
+    if (v < 0 || v > 1) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_pair = v;
+    return LATEC_OK;
+  }
   if (!strcmp(name, "march_hist_peel")) {
     if (v < 0 || v > 8) {
       set_error(std::string("tuning value out of range for ") + name);
@@ -2085,6 +2469,7 @@ int64_t tune_get(const char *name) {
   if (!strcmp(name, "march_blocks")) return g_march_blocks;
   if (!strcmp(name, "march_hist_peel")) return g_march_peel;
   if (!strcmp(name, "slab")) return g_slab;
+  if (!strcmp(name, "pair")) return g_pair;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -2096,7 +2481,7 @@ static bool march_ok(const latec_plan *gp, const latec_plan *q, const latec_inte
   static const bool off = getenv("LATEC_NO_MARCH") != nullptr;
   if (off) return false;
   return gp->ndim == 3 && gp->channels == 1 && o->code_bytes == 2 && q->stages.size() == 6 &&
-         q->npoints < 0x7fffffffll;
+         q->npoints < (1ll << 30);  // byte offsets of floats fit 32 bits
 }
 
 static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool encode,
@@ -2263,6 +2648,140 @@ static int launch_slab(const latec_plan *gp, const latec_plan *q, const latec_pl
 // exceeds four slabs runs its sub-stages slab by slab along axis 0 (weight-w
 // stages trail by w-1 slabs) so that each slab stays cache resident.
 
+// Line-pair launches for a 3-D level above the first (one channel, u16
+// codes): phase 1 on the (e,e) lines, phase 2 on the (o,e) and (e,o) lines,
+// phase 3 on the (o,o) lines.  Reads the coarse points from the level
+// below's compact grid `coarse` (shape qc), so the level needs no lattice
+// scatter.  Returns 1, before launching anything, if the level's layout is
+// not the one the kernels assume.
+static bool pair_ok(const latec_plan *gp, const latec_plan *q, int k,
+                    const latec_interp_opts *o) {
+  return g_pair && k >= 2 && gp->ndim == 3 && gp->channels == 1 && o->code_bytes == 2 &&
+         q->stages.size() == 6 && q->npoints < (1ll << 30);
+}
+
+static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_plan *qc, int g0,
+                       bool encode, const float *x, const float *coarse, float *recon,
+                       void *codes, uint32_t *hist, uint32_t *out_rank, float *out_val,
+                       uint32_t *out_count, const double *ebs, const latec_interp_opts *o,
+                       hipStream_t stream) {
+  for (int j = 0; j < 3; j++)
+    if (qc->shape[j] != (q->shape[j] + 1) / 2) return 1;
+  // the sub-plan's classes by parity, in their stages: 100, 010, 001 alone,
+  // then the weight-2 stage, then 111
+  static const int want_stage[8] = {0, 1, 2, 4, 3, 4, 4, 5};
+  const ClassDesc *cls[8] = {};
+  for (int js = 1; js <= 5; js++)
+    for (const ClassDesc &c : q->stages[js].classes) {
+      if (c.parity < 1 || c.parity > 7 || c.ndim != 3 || c.step != 2 || (c.flags & 1) ||
+          want_stage[c.parity] != js || cls[c.parity])
+        return 1;
+      for (int j = 0; j < 3; j++)
+        if (c.starts[j] != ((c.parity >> j) & 1)) return 1;
+      cls[c.parity] = &c;
+    }
+  for (int pb = 1; pb < 8; pb++)
+    if (!cls[pb]) return 1;
+  const int64_t n[3] = {q->shape[0], q->shape[1], q->shape[2]};
+  const uint32_t npairs = (uint32_t)((n[2] + 1) / 2);
+  auto half = [&](int a, int par) { return (uint32_t)((n[a] - par + 1) / 2); };
+  PArgs A[3];
+  for (int ph = 0; ph < 3; ph++) {
+    PArgs &a = A[ph];
+    memset(&a, 0, sizeof a);
+    for (int j = 0; j < 3; j++) {
+      a.n[j] = (int32_t)n[j];
+      a.cn[j] = (int32_t)qc->shape[j];
+    }
+    a.coarse = coarse;
+    a.cubic = o->cubic;
+    a.center = o->center;
+    a.end_mode = o->end_mode;
+    a.round_mode = o->round_mode;
+    a.peel = (int32_t)g_march_peel;
+    a.span = o->span;
+    a.offset = o->offset;
+    a.hist_words = hist_words(o->radius);
+    a.npairs = npairs;
+    a.own = ph == 0 ? 64u : (uint32_t)PAIR_OWN;
+  }
+  int nst[3] = {0, 0, 0};
+  // the stage slot of sub-plan stage js in phase ph's argument block
+  auto stage_slot = [&](int ph, int js) -> int32_t {
+    PArgs &a = A[ph];
+    for (int i = 0; i < nst[ph]; i++)
+      if (a.st[i].gs == g0 + js - 1) return i;
+    const GridArgs g = make_args(q, js, ebs[g0 + js - 1], o);
+    L3Stage &S = a.st[nst[ph]];
+    S.w = g.w;
+    S.inv_w = g.inv_w;
+    S.feb = g.feb;
+    S.R = g.R;
+    S.code_base = (int64_t)gp->channels * gp->stages[g0 + js - 1].offset;
+    S.out_lo = S.code_base;
+    S.gs = g0 + js - 1;
+    return nst[ph]++;
+  };
+  auto set_class = [&](int ph, PClass &pc, int parity) {
+    const ClassDesc &c = *cls[parity];
+    pc.stage = stage_slot(ph, want_stage[parity]);
+    pc.offset = c.offset;
+    for (int j = 0; j < 3; j++) pc.rstr[j] = c.rstr[j];
+  };
+  // (ma, ra, pm, pr, even class, odd class) per set; phase 1's even class
+  // (the coarse points) codes nothing: its slot repeats the odd class
+  static const int sets[4][6] = {{0, 1, 0, 0, 4, 4}, {0, 1, 1, 0, 1, 5}, {1, 0, 1, 0, 2, 6},
+                                 {0, 1, 1, 1, 3, 7}};
+  static const int phase_of[4] = {0, 1, 1, 2};
+  int nset[3] = {0, 0, 0};
+  for (int si = 0; si < 4; si++) {
+    const int ph = phase_of[si];
+    const int *d = sets[si];
+    PSet &T = A[ph].set[nset[ph]++];
+    T.ma = d[0];
+    T.ra = d[1];
+    T.pm = d[2];
+    T.pr = d[3];
+    T.nmarch = half(T.ma, T.pm);
+    T.nrows = half(T.ra, T.pr);
+    set_class(ph, T.od, d[5]);
+    set_class(ph, T.ev, d[4]);
+    const int64_t pairs = (int64_t)T.nrows * npairs;
+    const int64_t own = A[ph].own;
+    const int64_t nw = (pairs + own - 1) / own;
+    const int64_t nchunks = ((int64_t)T.nmarch + PAIR_M - 1) / PAIR_M;
+    const int64_t nt = nchunks * nw * 64;
+    if (nt >= 0x7fffffffll) return 1;
+    T.nwaves = (uint32_t)nw;
+    T.nthreads = (uint32_t)nt;
+  }
+  for (int ph = 0; ph < 3; ph++) {
+    const PArgs &a = A[ph];
+    int64_t blocks = 1;
+    for (int si = 0; si < nset[ph]; si++)
+      blocks = std::max<int64_t>(blocks, ((int64_t)a.set[si].nthreads + 255) / 256);
+    blocks = std::min<int64_t>(blocks, g_march_blocks);
+    blocks = (blocks + 7) / 8 * 8;
+    const dim3 grid((unsigned)blocks, (unsigned)nset[ph]);
+#define LATEC_PAIR_LAUNCH(PH)                                                                 \
+  if (encode)                                                                                 \
+    hipLaunchKernelGGL((pair_kernel<PH, true>), grid, dim3(256), 0, stream, a, x, recon,     \
+                       (uint16_t *)codes, hist, out_rank, out_val, out_count);               \
+  else                                                                                        \
+    hipLaunchKernelGGL((pair_kernel<PH, false>), grid, dim3(256), 0, stream, a, nullptr,     \
+                       recon, (uint16_t *)codes, hist, out_rank, out_val, out_count);
+    if (ph == 0) {
+      LATEC_PAIR_LAUNCH(1)
+    } else if (ph == 1) {
+      LATEC_PAIR_LAUNCH(2)
+    } else {
+      LATEC_PAIR_LAUNCH(3)
+    }
+#undef LATEC_PAIR_LAUNCH
+  }
+  return hip_check(hipGetLastError(), "pair_kernel");
+}
+
 static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon, void *codes,
                        uint32_t *hist, uint32_t *out_rank, float *out_val,
                        uint32_t *out_count, const double *ebs, const latec_interp_opts *o,
@@ -2288,6 +2807,12 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
     const int g0 = 1 + (k - 1) * per;
     if (slab_ok(p, q, k, o)) {  // reads level k-1 itself: no lattice scatter
       rc = launch_slab(p, q, d->sub[k - 1], g0, encode, X(k), d->R[k - 1], R(k), codes, hist,
+                       out_rank, out_val, out_count, ebs, o, stream);
+      if (rc < 0) return rc;
+      if (rc == 0) continue;
+    }
+    if (pair_ok(p, q, k, o)) {  // reads level k-1 itself: no lattice scatter
+      rc = launch_pair(p, q, d->sub[k - 1], g0, encode, X(k), d->R[k - 1], R(k), codes, hist,
                        out_rank, out_val, out_count, ebs, o, stream);
       if (rc < 0) return rc;
       if (rc == 0) continue;
@@ -2321,11 +2846,10 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
                         march_direct ? d->R[k - 1] : nullptr, write_coarse);
       if (rc < 0) return rc;
       if (rc == 0) continue;
-      if (write_coarse) {  // layout refused: the generic path needs the scattered coarse points
-        set_error("march layout mismatch");
-        return LATEC_E_ARG;
-      }
     }
+    // the march refused the layout: the stage kernels need the scattered coarse points
+    if (write_coarse && (rc = lattice(q, d->sub[k - 1], R(k), d->R[k - 1], false, stream)))
+      return rc;
     for (int64_t t = 0; t < nslab + nd - 1; t++) {
       for (int w = 1; w <= nd; w++) {
         const int64_t sl = t - (w - 1);

@@ -279,6 +279,31 @@ def test_gpu_slab_kernel_bit_exact(case):
         autotune.set_knobs(**old)
 
 
+@pytest.mark.parametrize("case", [c for c in CASES if len(c[0]) == 3 and c[4] == 1],
+                         ids=lambda c: f"{c[0]}-{c[5]}-c{c[6]}e{c[7]}")
+def test_gpu_march_kernels_bit_exact(case):
+    """The per-class march kernels (tuning knob "pair" off) on every 3-D
+    case: the same codes, outliers, recon and stream as the oracle's."""
+    from latec_amd import autotune
+
+    old = autotune.get()
+    autotune.set_knobs(pair=0, slab=0)
+    try:
+        test_gpu_encode_matches_oracle_bit_exact(case)
+        shape, C, levels, stride, block, order, center, end, rel, rough = case
+        x = smooth(shape, seed=0, rough=rough)
+        eb = rel * float(x.max() - x.min())
+        pred = InterpPredictor(order, levels, stride, 1)
+        pred.center, pred.end_mode = center, end
+        stream, _ = latec_amd.compress(x, eb, pred, levels=levels, anchor_stride=stride,
+                                       anchor_block=1)
+        ref_stream, ref_recon = O.compress(x, eb, order, levels, stride, 1, center, end)
+        assert stream == ref_stream
+        assert np.array_equal(latec_amd.decompress(stream), ref_recon[0])
+    finally:
+        autotune.set_knobs(**old)
+
+
 def test_gpu_config2_full_size_roundtrip():
     """512^3, cubic, levels 5, stride 32, eb = 1e-3 * range: size-independent
     properties at full size (bound, decoder == encoder, stage record counts)."""
