@@ -823,12 +823,12 @@ __global__ void __launch_bounds__(256) code_hist_kernel(const CodeT *__restrict_
 // operation for operation, so codes, outliers and recon are bit-identical to
 // the stage kernels'.
 // ---------------------------------------------------------------------------
-#define MARCH_M 8
+#define MARCH_M 8  // most points per thread (MArgs::steps <= MARCH_M)
 
 struct MClass {
   int32_t parity, stage;  // stage: index into MArgs::st
   uint32_t m[3];          // class extents
-  uint32_t e[3];          // thread extents (the marching axis divided by MARCH_M)
+  uint32_t e[3];          // thread extents (the marching axis divided by MArgs::steps)
   uint32_t nthreads, pad;
   int64_t offset;         // stage rank of the class's first point
   int64_t rstr[3];        // stage-rank strides per class index
@@ -837,6 +837,7 @@ struct MClass {
 struct MArgs {
   int32_t n[3];
   int32_t cubic, center, end_mode, round_mode, ncls, peel;
+  int32_t steps, pad;  // points per thread along the march (1 .. MARCH_M)
   double span, offset;
   int64_t hist_words;
   const float *coarse;   // weight-1 launches: the compact grid of the level below, where
@@ -938,6 +939,7 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
                                             float *__restrict__ out_val,
                                             uint32_t *__restrict__ out_count) {
   constexpr int M = MARCH_M;
+  const uint32_t NS = (uint32_t)A.steps;
   constexpr int MA = (P & 1) ? 0 : (P & 2) ? 1 : 0;
   constexpr bool WIN = (P >> MA) & 1;
   constexpr int NODD = (P & 1) + ((P >> 1) & 1) + ((P >> 2) & 1);
@@ -976,14 +978,14 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
     u /= K.e[2];
     ci[1] = u % K.e[1];
     ci[0] = u / K.e[1];
-    ci[MA] *= M;
+    ci[MA] *= NS;
     int c[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) c[a] = ((P >> a) & 1) + 2 * (int)ci[a];
     const uint32_t f0 = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
     const uint32_t rk0 = (uint32_t)(K.offset + (int64_t)ci[0] * K.rstr[0] +
                                     (int64_t)ci[1] * K.rstr[1] + (int64_t)ci[2] * K.rstr[2]);
-    const uint32_t cnt = tv ? min((uint32_t)M, K.m[MA] - ci[MA]) : 0u;
+    const uint32_t cnt = tv ? min(NS, K.m[MA] - ci[MA]) : 0u;
     // tap source: the level's own recon, or, for the weight-1 classes when it
     // is given, the compact grid of the level below (every weight-1 tap is a
     // coarse point there: fine c -+ 1, -+ 3 is coarse (c >> 1) + {0, 1, -1, 2}).
@@ -1059,7 +1061,8 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
     load(0, cur);
 #pragma unroll
     for (uint32_t t = 0; t < (uint32_t)M; t++) {
-      if (t + 1 < (uint32_t)M) load(t + 1, nxt);
+      if (t >= NS) break;  // uniform
+      if (t + 1 < NS) load(t + 1, nxt);
       const bool active = t < cnt;
       const uint32_t f = f0 + 2 * t * sMA;
       const uint32_t rank = rk0 + t * rMA;
@@ -1226,7 +1229,7 @@ __global__ void __launch_bounds__(256) march_kernel(MArgs A, const float *__rest
 // store and emit nothing.  Arithmetic is interp4 / the stage quantizer in
 // double, exactly as the other level kernels.
 // ---------------------------------------------------------------------------
-#define PAIR_M 8    // march steps per thread
+#define PAIR_M 8    // most march steps per thread (PArgs::steps <= PAIR_M)
 #define PAIR_OWN 61  // pairs a wave owns in phases 2 and 3
 
 struct PClass {
@@ -1240,7 +1243,7 @@ struct PSet {
   uint32_t nmarch;      // line positions along ma of that parity
   uint32_t nrows;       // line positions along ra of that parity
   uint32_t nwaves;      // waves per march chunk (runs of pairs)
-  uint32_t nthreads;    // nchunks * nwaves * 64
+  uint32_t nthreads;    // nchunks * nwaves * 64, nchunks = ceil(nmarch / steps)
   PClass ev, od;        // the classes at k = 2m and k = 2m + 1
 };
 struct PArgs {
@@ -1250,6 +1253,7 @@ struct PArgs {
   double span, offset;
   int64_t hist_words;
   uint32_t npairs, own;  // pairs per line; pairs a wave owns
+  uint32_t steps, pad2;  // march steps per thread (1 .. PAIR_M): fewer on small levels
   L3Stage st[3];
   PSet set[2];
 };
@@ -1334,6 +1338,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
                                          float *__restrict__ out_val,
                                          uint32_t *__restrict__ out_count) {
   constexpr int M = PAIR_M;
+  const uint32_t S = A.steps;
   constexpr int NE = PH == 1 ? 0 : PH == 2 ? 1 : 2;  // odd axes of the even class
   constexpr int NO = NE + 1;                         // odd axes of the odd class
   const int ma = T.ma, ra = T.ra;
@@ -1371,21 +1376,24 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
     const bool owned = wv && lane >= (int)halo && lane < (int)(halo + own) && P < (int64_t)total;
     const uint32_t Pc = (uint32_t)min(max(P, (int64_t)0), (int64_t)total - 1);
     const uint32_t row = Pc / A.npairs, m = Pc - row * A.npairs;
+    // every lane of a wave has the same chunk, so the step count is uniform
+    const uint32_t cnt = wv ? min(S, T.nmarch - chunk * S) : 0u;
+    const uint32_t ucnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
+    if (ucnt == 0) continue;
     int c[3];
-    c[ma] = T.pm + 2 * (int)(chunk * M);
+    c[ma] = T.pm + 2 * (int)(chunk * S);
     c[ra] = T.pr + 2 * (int)row;
     c[2] = 2 * (int)m;
     const bool odd_ok = 2 * (int)m + 1 < n2;
     const uint32_t f0 = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
     uint32_t ci[3];
-    ci[ma] = chunk * M;
+    ci[ma] = chunk * S;
     ci[ra] = row;
     ci[2] = m;
     const uint32_t rk_e0 = (uint32_t)(T.ev.offset + (int64_t)ci[0] * T.ev.rstr[0] +
                                       (int64_t)ci[1] * T.ev.rstr[1] + (int64_t)m * T.ev.rstr[2]);
     const uint32_t rk_o0 = (uint32_t)(T.od.offset + (int64_t)ci[0] * T.od.rstr[0] +
                                       (int64_t)ci[1] * T.od.rstr[1] + (int64_t)m * T.od.rstr[2]);
-    const uint32_t cnt = wv ? min((uint32_t)M, T.nmarch - chunk * M) : 0u;
     // taps along k of the odd point k = 2m + 1 (lanes l-1, l, l+1, l+2)
     const bool kv0 = m >= 1, kv1 = 2 * (int)m + 2 < n2, kv2 = 2 * (int)m + 4 < n2;
     // direct taps along ra (phase 3): validity is constant along the march
@@ -1452,8 +1460,9 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
     load(0, cur);
 #pragma unroll
     for (uint32_t t = 0; t < (uint32_t)M; t++) {
-      if (t + 1 < (uint32_t)M) load(t + 1, nxt);
-      const bool active = owned && t < cnt;
+      if (t >= ucnt) break;
+      if (t + 1 < ucnt) load(t + 1, nxt);
+      const bool active = owned;
       const uint32_t f = f0 + 2 * t * sma;
       const uint32_t rke = rk_e0 + t * rma_e, rko = rk_o0 + t * rma_o;
       const int cm = c[ma] + 2 * (int)t;
@@ -1484,7 +1493,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
         } else {
           // every lane with a point decodes it: lanes 0, 62, 63 feed the k taps
           cde = cur.ce;
-          re = pair_dequant(QE, pd, cde, wv && t < cnt, rke, or_e, ov_e, nout_e);
+          re = pair_dequant(QE, pd, cde, wv, rke, or_e, ov_e, nout_e);
         }
       }
       // --- the odd point: k taps from the even points of lanes l-1 .. l+2
@@ -2422,6 +2431,9 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
 static int64_t g_march_blocks = 4096;
 static int64_t g_slab = 0;  // "slab": 1 runs large levels as one level-fused launch
 static int64_t g_pair = 1;  // "pair": 1 runs 3-D levels above the first as line-pair launches
+// "min_threads": a march or pair launch shortens its per-thread march (8, 4,
+// 2, 1 steps) until it has at least this many threads
+static int64_t g_min_threads = 1 << 19;
 static int64_t g_march_peel = 2;  // histogram: distinct codes per wave counted by ballot
 
 static int64_t g_tune_epoch = 0;  // bumped by every knob change
@@ -2453,6 +2465,14 @@ int tune_set(const char *name, int64_t v) {
     g_pair = v;
     return LATEC_OK;
   }
+  if (!strcmp(name, "min_threads")) {
+    if (v < 0 || v > (1 << 26)) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_min_threads = v;
+    return LATEC_OK;
+  }
   if (!strcmp(name, "march_hist_peel")) {
     if (v < 0 || v > 8) {
       set_error(std::string("tuning value out of range for ") + name);
@@ -2470,6 +2490,7 @@ int64_t tune_get(const char *name) {
   if (!strcmp(name, "march_hist_peel")) return g_march_peel;
   if (!strcmp(name, "slab")) return g_slab;
   if (!strcmp(name, "pair")) return g_pair;
+  if (!strcmp(name, "min_threads")) return g_min_threads;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -2504,6 +2525,7 @@ static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool 
     a.offset = o->offset;
     a.hist_words = hist_words(o->radius);
     a.peel = (int32_t)g_march_peel;
+    a.steps = MARCH_M;
     if (gi == 0 && coarse) {  // the weight-1 stages read their taps from the level below
       a.coarse = coarse;
       for (int j = 0; j < 3; j++) {
@@ -2538,15 +2560,40 @@ static int launch_march(const latec_plan *gp, const latec_plan *q, int g0, bool 
         const int ma = (c.parity & 1) ? 0 : (c.parity & 2) ? 1 : 0;
         for (int j = 0; j < 3; j++) {
           K.m[j] = c.sizes[j];
-          K.e[j] = j == ma ? (c.sizes[j] + MARCH_M - 1) / MARCH_M : c.sizes[j];
+          K.e[j] = c.sizes[j];
           K.rstr[j] = c.rstr[j];
         }
-        const int64_t nt = (int64_t)K.e[0] * K.e[1] * K.e[2];
-        if (nt >= 0x7fffffffll) return 1;
-        K.nthreads = (uint32_t)nt;
+
         K.offset = c.offset;
       }
       ns++;
+    }
+  }
+  // points per thread: MARCH_M, halved while the launch would have fewer
+  // than g_min_threads threads
+  for (int gi = 0; gi < 3; gi++) {
+    MArgs &a = A[gi];
+    auto threads = [&](int64_t st) {
+      int64_t t = 0;
+      for (int i = 0; i < a.ncls; i++) {
+        const MClass &K = a.cl[i];
+        const int ma = (K.parity & 1) ? 0 : (K.parity & 2) ? 1 : 0;
+        int64_t nt = 1;
+        for (int j = 0; j < 3; j++) nt *= j == ma ? ((int64_t)K.m[j] + st - 1) / st : K.m[j];
+        t += nt;
+      }
+      return t;
+    };
+    int64_t S = MARCH_M;
+    while (S > 1 && threads(S) < g_min_threads) S >>= 1;
+    a.steps = (int32_t)S;
+    for (int i = 0; i < a.ncls; i++) {
+      MClass &K = a.cl[i];
+      const int ma = (K.parity & 1) ? 0 : (K.parity & 2) ? 1 : 0;
+      K.e[ma] = (uint32_t)(((int64_t)K.m[ma] + S - 1) / S);
+      const int64_t nt = (int64_t)K.e[0] * K.e[1] * K.e[2];
+      if (nt >= 0x7fffffffll) return 1;
+      K.nthreads = (uint32_t)nt;
     }
   }
   for (int gi = 0; gi < 3; gi++) {
@@ -2749,11 +2796,28 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     const int64_t pairs = (int64_t)T.nrows * npairs;
     const int64_t own = A[ph].own;
     const int64_t nw = (pairs + own - 1) / own;
-    const int64_t nchunks = ((int64_t)T.nmarch + PAIR_M - 1) / PAIR_M;
-    const int64_t nt = nchunks * nw * 64;
-    if (nt >= 0x7fffffffll) return 1;
     T.nwaves = (uint32_t)nw;
-    T.nthreads = (uint32_t)nt;
+  }
+  // march steps per thread: PAIR_M, halved while a phase would have fewer
+  // than g_min_threads threads (a short level is latency-bound: more threads
+  // with shorter marches)
+  for (int ph = 0; ph < 3; ph++) {
+    PArgs &a = A[ph];
+    uint32_t S = PAIR_M;
+    auto threads = [&](uint32_t st) {
+      int64_t t = 0;
+      for (int si = 0; si < nset[ph]; si++)
+        t += ((int64_t)a.set[si].nmarch + st - 1) / st * a.set[si].nwaves * 64;
+      return t;
+    };
+    while (S > 1 && threads(S) < g_min_threads) S >>= 1;
+    a.steps = S;
+    for (int si = 0; si < nset[ph]; si++) {
+      PSet &T = a.set[si];
+      const int64_t nt = ((int64_t)T.nmarch + S - 1) / S * T.nwaves * 64;
+      if (nt >= 0x7fffffffll) return 1;
+      T.nthreads = (uint32_t)nt;
+    }
   }
   for (int ph = 0; ph < 3; ph++) {
     const PArgs &a = A[ph];

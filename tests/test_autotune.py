@@ -11,11 +11,14 @@ from latec_amd._lib import lib
 def test_knobs_round_trip_and_validation():
     start = autotune.get()
     try:
-        autotune.set_knobs(march_blocks=1000, march_hist_peel=3, slab=0, pair=0)
+        autotune.set_knobs(march_blocks=1000, march_hist_peel=3, slab=0, pair=0,
+                           min_threads=12345)
         assert autotune.get() == {"march_blocks": 1000, "march_hist_peel": 3, "slab": 0,
-                                  "pair": 0}
+                                  "pair": 0, "min_threads": 12345}
         with pytest.raises(ValueError):
             autotune.set_knobs(pair=2)
+        with pytest.raises(ValueError):
+            autotune.set_knobs(min_threads=-1)
         with pytest.raises(ValueError):
             autotune.set_knobs(march_blocks=4)
         with pytest.raises(ValueError):
@@ -53,8 +56,10 @@ def test_gpu_autotune_keeps_outputs_and_parity(tmp_path, monkeypatch):
                                         "slab": (0, 1), "pair": (0, 1)})
         assert len(got["table"]) == 16 and autotune.get() == got["knobs"]
         assert autotune.load(shape, 3, 8)["knobs"] == got["knobs"]
-        for blocks, slab, pair in ((8, 0, 0), (1024, 1, 0), (8, 0, 1), (1024, 0, 1)):
-            autotune.set_knobs(march_blocks=blocks, march_hist_peel=0, slab=slab, pair=pair)
+        for blocks, slab, pair, mt in ((8, 0, 0, 1 << 19), (1024, 1, 0, 1 << 19),
+                                       (8, 0, 1, 0), (1024, 0, 1, 1 << 24)):
+            autotune.set_knobs(march_blocks=blocks, march_hist_peel=0, slab=slab, pair=pair,
+                               min_threads=mt)
             s, _ = latec_amd.compress(x, eb, InterpPredictor("cubic", 3, 8, 1), levels=3,
                                       anchor_stride=8, anchor_block=1)
             ref, _ = O.compress(x, eb, "cubic", 3, 8, 1)
