@@ -247,6 +247,18 @@ __device__ __forceinline__ void flush_hist(const uint32_t *lhist, uint32_t *__re
     if (lhist[i]) atomicAdd(&gwin[i], lhist[i]);
 }
 
+// Histogram counts into a block's LDS window (workgroup scope) or straight
+// into the global bins (device scope).  The distinct scopes keep the two
+// atomics of a window-or-global branch apart: merged, they would become one
+// flat atomic on a selected pointer, which is slower and makes every later
+// wait on vector-memory loads wait for it too.
+__device__ __forceinline__ void lds_add(uint32_t *p, uint32_t k) {
+  __hip_atomic_fetch_add(p, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void gbl_add(uint32_t *p, uint32_t k) {
+  __hip_atomic_fetch_add(p, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename CodeT>
 __device__ __forceinline__ void emit(CodeT *__restrict__ codes, int64_t slot,
                                      uint32_t code, float x, uint32_t rank,
@@ -259,9 +271,9 @@ __device__ __forceinline__ void emit(CodeT *__restrict__ codes, int64_t slot,
     codes[slot] = (CodeT)code;
     const uint32_t b = code - win_lo;
     if (code != 0 && b < HIST_WIN)
-      atomicAdd(&lhist[b], 1u);
+      lds_add(&lhist[b], 1u);
     else
-      atomicAdd(&ghist[code], 1u);
+      gbl_add(&ghist[code], 1u);
   }
   // wave-aggregated outlier append
   const bool out = active && code == 0;
@@ -668,9 +680,9 @@ __global__ void __launch_bounds__(L3_THREADS) level3_kernel(
               codes[S.code_base + rank] = (uint16_t)code;
               const uint32_t b = code - win_lo;
               if (code != 0 && b < L3_HBINS)
-                atomicAdd(&lh[b], 1u);
+                lds_add(&lh[b], 1u);
               else
-                atomicAdd(&ghist[code], 1u);
+                gbl_add(&ghist[code], 1u);
             }
             const bool out = emit_ok && code == 0;
             const uint64_t m = __ballot(out);
@@ -797,9 +809,9 @@ __global__ void __launch_bounds__(256) code_hist_kernel(const CodeT *__restrict_
     const uint32_t code = (uint32_t)codes[i];
     const uint32_t b = code - win_lo;
     if (code != 0 && b < HIST_WIN)
-      atomicAdd(&lhist[b], 1u);
+      lds_add(&lhist[b], 1u);
     else if (code < 2 * R)
-      atomicAdd(&ghist[code], 1u);
+      gbl_add(&ghist[code], 1u);
   }
   __syncthreads();
   flush_hist(lhist, ghist, R);
@@ -869,9 +881,9 @@ __device__ __forceinline__ void hist_add(uint32_t *lhist, uint32_t *__restrict__
                                          uint32_t win_lo, uint32_t code, uint32_t k) {
   const uint32_t b = code - win_lo;
   if (code != 0 && b < HIST_WIN)
-    atomicAdd(&lhist[b], k);
+    lds_add(lhist + b, k);
   else
-    atomicAdd(&ghist[code], k);
+    gbl_add(ghist + code, k);
 }
 
 // emit() with the histogram counted per wave: the wave's most common codes
@@ -922,6 +934,24 @@ __device__ __forceinline__ T ldo(const T *__restrict__ base, uint32_t idx) {
 template <typename T>
 __device__ __forceinline__ void sto(T *__restrict__ base, uint32_t idx, T v) {
   *(T *)((char *)base + idx * (uint32_t)sizeof(T)) = v;
+}
+
+// RN(x / 3) without the division sequence: q0 = RN(x * RN(1/3)) is within an
+// ulp of x / 3, the remainder x - 3 q0 is exact under one FMA, and adding
+// remainder * RN(1/3) to q0 (Markstein's remainder correction) rounds to the
+// correctly rounded quotient for every |x| in [2^-968, 2^1020] (checked
+// bit for bit against the division on 4.6e8 operands across that range,
+// including multiples of 3 and their neighbours).  Other values (tiny, huge,
+// non-finite) take the division itself.  Every lane of the wave calls it.
+__device__ __forceinline__ double div3(double x) {
+  const double c = 1.0 / 3.0;
+  double q = x * c;
+  q = fma(fma(-q, 3.0, x), c, q);
+  const bool ok = fabs(x) >= 0x1p-968 && fabs(x) <= 0x1p1020;
+  if (__any(!ok)) {
+    if (!ok) q = x / 3.0;
+  }
+  return q;
 }
 
 // Cubic interior stencil of interp4 (all four taps valid).
@@ -1103,7 +1133,7 @@ __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
       } else if (center == 0) {
         double total = v[0] + v[1];
         if (NODD == 3) total = total + v[2];
-        res = total / (double)NODD;
+        res = NODD == 3 ? div3(total) : total * 0.5;
       } else {
         res = center == 1 ? v[0] : v[NODD - 1];
       }
@@ -1230,7 +1260,11 @@ __global__ void __launch_bounds__(256) march_kernel(MArgs A, const float *__rest
 // double, exactly as the other level kernels.
 // ---------------------------------------------------------------------------
 #define PAIR_M 8    // most march steps per thread (PArgs::steps <= PAIR_M)
+
 #define PAIR_OWN 61  // pairs a wave owns in phases 2 and 3
+#ifndef PAIR_DEPTH
+#define PAIR_DEPTH 1  // steps the loads run ahead of the arithmetic (1 or 2)
+#endif
 
 struct PClass {
   int32_t stage, pad;  // index into PArgs::st
@@ -1263,8 +1297,8 @@ __device__ __forceinline__ double pair_combine(const double *v, int nodd, int ce
   if (nodd == 1) return v[0];
   if (center == 0) {
     double total = v[0] + v[1];
-    if (nodd == 3) total = total + v[2];
-    return total / (double)nodd;
+    if (nodd == 3) return div3(total + v[2]);
+    return total * 0.5;
   }
   return center == 1 ? v[0] : v[nodd - 1];
 }
@@ -1456,12 +1490,18 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
       const float2 d = pair_ld(rec, cm + 1 < nma ? f0 + sma : f0 - sma, al, odd_ok);
       we0 = a.x; wo0 = a.y; we1 = b.x; wo1 = b.y; we2 = d.x; wo2 = d.y;
     }
-    Loads cur, nxt;
+    // loads run PAIR_DEPTH steps ahead of the arithmetic
+    Loads cur, nxt, nx2;
     load(0, cur);
-#pragma unroll
+    if (PAIR_DEPTH > 1 && 1 < ucnt) load(1, nxt);
+#pragma unroll 1
     for (uint32_t t = 0; t < (uint32_t)M; t++) {
       if (t >= ucnt) break;
-      if (t + 1 < ucnt) load(t + 1, nxt);
+      if (PAIR_DEPTH > 1) {
+        if (t + 2 < ucnt) load(t + 2, nx2);
+      } else {
+        if (t + 1 < ucnt) load(t + 1, nxt);
+      }
       const bool active = owned;
       const uint32_t f = f0 + 2 * t * sma;
       const uint32_t rke = rk_e0 + t * rma_e, rko = rk_o0 + t * rma_o;
@@ -1553,6 +1593,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
         wo0 = wo1; wo1 = wo2; wo2 = wo3;
       }
       cur = nxt;
+      if (PAIR_DEPTH > 1) nxt = nx2;
     }
   }
 }
@@ -1788,9 +1829,9 @@ __global__ void __launch_bounds__(SL_THREADS) slab_kernel(SlabArgs A, const floa
           scodes[rank] = (uint16_t)code;
           const uint32_t bb = code - win_lo;
           if (code != 0 && bb < SL_HB)
-            atomicAdd(&lh[si * SL_HB + bb], 1u);
+            lds_add(&lh[si * SL_HB + bb], 1u);
           else
-            atomicAdd(&hist[(int64_t)S.gs * A.hist_words + code], 1u);
+            gbl_add(&hist[(int64_t)S.gs * A.hist_words + code], 1u);
         }
         const bool out = owned && code == 0;
         const uint64_t m = __ballot(out);
@@ -2428,13 +2469,13 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
 // latec_tune_set): at most march_blocks workgroups per class (a multiple of
 // 8, so the XCD remap stays exact; the threads walk the rest of the class),
 // and march_hist_peel, the distinct codes per wave counted with one atomic.
-static int64_t g_march_blocks = 4096;
+static int64_t g_march_blocks = 2048;
 static int64_t g_slab = 0;  // "slab": 1 runs large levels as one level-fused launch
 static int64_t g_pair = 1;  // "pair": 1 runs 3-D levels above the first as line-pair launches
 // "min_threads": a march or pair launch shortens its per-thread march (8, 4,
 // 2, 1 steps) until it has at least this many threads
 static int64_t g_min_threads = 1 << 19;
-static int64_t g_march_peel = 2;  // histogram: distinct codes per wave counted by ballot
+static int64_t g_march_peel = 0;  // histogram: distinct codes per wave counted by ballot
 
 static int64_t g_tune_epoch = 0;  // bumped by every knob change
 static int64_t tune_epoch() { return g_tune_epoch; }
