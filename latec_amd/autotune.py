@@ -1,6 +1,6 @@
 """Autotuner for the plain path's launch shapes.
 
-The 3-D level kernels have five launch-shape knobs, set process-wide with
+The 3-D level kernels have six launch-shape knobs, set process-wide with
 ``latec_tune_set``:
 
 * ``march_blocks`` -- workgroups per parity class (with 4 waves each, this is
@@ -13,13 +13,16 @@ The 3-D level kernels have five launch-shape knobs, set process-wide with
 * ``pair`` -- whether the levels above the first run as line-pair launches
   (one thread codes the two classes of a line along axis 2 together, so x is
   read and recon written as whole lines) or as class launches;
+* ``pair_lines`` -- whether the pair launches give each workgroup whole
+  lines (every lane owns its pair, neighbours exchanged through LDS) or wave
+  runs of 61 owned pairs with three recomputing halo lanes;
 * ``min_threads`` -- a level launch shortens each thread's march (8, 4, 2, 1
   points) until it has at least this many threads: small levels are
   latency-bound, so they trade march length for parallelism.
 
-The slab kernel is measured slower than the class launches on every workload
-tried so far, so the default candidates leave it off (pass it explicitly to
-time it).
+The slab kernel and histogram peeling are measured slower than their
+alternatives on every workload tried so far, so the default candidates leave
+them at 0 (pass them explicitly to time them).
 
 Neither changes a single code, outlier or reconstructed value (they only
 decide which thread does which point and how counts are summed).
@@ -39,11 +42,12 @@ import numpy as np
 
 from ._lib import check, lib, ptr
 
-KNOBS = ("march_blocks", "march_hist_peel", "slab", "pair", "min_threads")
-CANDIDATES = {"march_blocks": (1024, 2048, 4096, 8192),
-              "march_hist_peel": (0, 2),
+KNOBS = ("march_blocks", "march_hist_peel", "slab", "pair", "pair_lines", "min_threads")
+CANDIDATES = {"march_blocks": (1024, 2048, 4096),
+              "march_hist_peel": (0,),
               "slab": (0,),
               "pair": (0, 1),
+              "pair_lines": (0, 1),
               "min_threads": (1 << 18, 1 << 19, 1 << 20)}
 
 

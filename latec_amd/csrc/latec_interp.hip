@@ -1252,12 +1252,16 @@ __global__ void __launch_bounds__(256) march_kernel(MArgs A, const float *__rest
 //            along axis 1), then 111 (the odd points of those lines, and
 //            110 along k).
 // The k-neighbours of an odd point are the even points of lanes l-1, l+1
-// and l+2 at the same step.  In phases 2 and 3 a wave therefore owns the 61
-// pairs of lanes 1..61 of a run of pairs numbered across the lines of the
-// type; its lanes 0, 62 and 63 recompute the even points of the
-// neighbouring runs (same inputs, same arithmetic, so the same bits) and
-// store and emit nothing.  Arithmetic is interp4 / the stage quantizer in
-// double, exactly as the other level kernels.
+// and l+2 at the same step.  When a line has at most 256 pairs, phases 2 and
+// 3 give each workgroup whole lines (thread = pair, 256 / npairs lines per
+// workgroup) and pass the even points through LDS, one barrier per step:
+// every lane owns its pair and every wave stores aligned runs.  Longer lines
+// use wave runs: a wave owns the 61 pairs of lanes 1..61 of a run of pairs
+// numbered across the lines of the type, and its lanes 0, 62 and 63
+// recompute the even points of the neighbouring runs (same inputs, same
+// arithmetic, so the same bits) and store and emit nothing.  Arithmetic is
+// interp4 / the stage quantizer in double, exactly as the other level
+// kernels.
 // ---------------------------------------------------------------------------
 #define PAIR_M 8    // most march steps per thread (PArgs::steps <= PAIR_M)
 
@@ -1276,8 +1280,9 @@ struct PSet {
   int32_t pm, pr;       // parity of the set's lines along ma and ra
   uint32_t nmarch;      // line positions along ma of that parity
   uint32_t nrows;       // line positions along ra of that parity
-  uint32_t nwaves;      // waves per march chunk (runs of pairs)
-  uint32_t nthreads;    // nchunks * nwaves * 64, nchunks = ceil(nmarch / steps)
+  uint32_t nwaves;      // wave runs: waves per march chunk (runs of pairs)
+  uint32_t nthreads;    // nchunks * (nwaves * 64 or bpc * 256), nchunks = ceil(nmarch / steps)
+  uint32_t lpb, bpc;    // line blocks (lpb > 0): whole lines per workgroup, workgroups per chunk
   PClass ev, od;        // the classes at k = 2m and k = 2m + 1
 };
 struct PArgs {
@@ -1367,7 +1372,7 @@ template <int PH, bool ENC>
 __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
                                          const float *__restrict__ x, float *__restrict__ rec,
                                          uint16_t *__restrict__ codes,
-                                         uint32_t *__restrict__ hist, uint32_t *lh,
+                                         uint32_t *__restrict__ hist, uint32_t *lh, float *xch,
                                          uint32_t *__restrict__ out_rank,
                                          float *__restrict__ out_val,
                                          uint32_t *__restrict__ out_count) {
@@ -1401,16 +1406,33 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   const uint32_t cs0 = (uint32_t)A.cn[1] * (uint32_t)A.cn[2], cs1 = (uint32_t)A.cn[2];
   const uint32_t G = gridDim.x;
   const uint32_t lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  const bool bm = PH > 1 && T.lpb != 0;  // line blocks (uniform)
+  uint32_t xstep = 0;  // steps taken by the workgroup: picks the exchange buffer
   for (uint32_t base = lb * 256u; base < T.nthreads; base += G * 256u) {
-    const uint32_t gw = (base + threadIdx.x) >> 6;  // uniform in the wave
-    const bool wv = base + threadIdx.x < T.nthreads;
-    const uint32_t chunk = wv ? gw / T.nwaves : 0u;
-    const uint32_t run = wv ? gw - chunk * T.nwaves : 0u;
-    const int64_t P = (int64_t)run * own + lane - (int64_t)halo;
-    const bool owned = wv && lane >= (int)halo && lane < (int)(halo + own) && P < (int64_t)total;
-    const uint32_t Pc = (uint32_t)min(max(P, (int64_t)0), (int64_t)total - 1);
-    const uint32_t row = Pc / A.npairs, m = Pc - row * A.npairs;
-    // every lane of a wave has the same chunk, so the step count is uniform
+    bool wv, owned;
+    uint32_t chunk, row, m;
+    if (bm) {  // nthreads is a multiple of 256: the whole workgroup is in range
+      const uint32_t blk = base >> 8;
+      chunk = blk / T.bpc;
+      const uint32_t line = threadIdx.x / A.npairs;
+      m = threadIdx.x - line * A.npairs;
+      row = (blk - chunk * T.bpc) * T.lpb + line;
+      wv = true;
+      owned = line < T.lpb && row < T.nrows;
+      row = min(row, T.nrows - 1);
+    } else {
+      const uint32_t gw = (base + threadIdx.x) >> 6;  // uniform in the wave
+      wv = base + threadIdx.x < T.nthreads;
+      chunk = wv ? gw / T.nwaves : 0u;
+      const uint32_t run = wv ? gw - chunk * T.nwaves : 0u;
+      const int64_t P = (int64_t)run * own + lane - (int64_t)halo;
+      owned = wv && lane >= (int)halo && lane < (int)(halo + own) && P < (int64_t)total;
+      const uint32_t Pc = (uint32_t)min(max(P, (int64_t)0), (int64_t)total - 1);
+      row = Pc / A.npairs;
+      m = Pc - row * A.npairs;
+    }
+    // every lane of a wave (of a workgroup, with line blocks) has the same
+    // chunk, so the step count is uniform
     const uint32_t cnt = wv ? min(S, T.nmarch - chunk * S) : 0u;
     const uint32_t ucnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)cnt);
     if (ucnt == 0) continue;
@@ -1540,6 +1562,18 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
       double k0, k1, k2, k3;
       if (PH == 1) {
         k0 = cur.ct[0]; k1 = cur.ct[1]; k2 = cur.ct[2]; k3 = cur.ct[3];
+      } else if (bm) {
+        // double-buffered across steps (and workgroup iterations): a buffer is
+        // written again only after the next step's barrier, which every
+        // thread reaches after its reads of this one
+        float *xb = xch + (xstep++ & 1) * 256;
+        const int tl = threadIdx.x;
+        xb[tl] = re;
+        __syncthreads();
+        k0 = (double)xb[tl >= 1 ? tl - 1 : tl];
+        k1 = (double)re;
+        k2 = (double)xb[tl + 1 < 256 ? tl + 1 : tl];
+        k3 = (double)xb[tl + 2 < 256 ? tl + 2 : tl];
       } else {
         k0 = (double)__shfl_up(re, 1);
         k1 = (double)re;
@@ -1607,12 +1641,13 @@ __global__ void __launch_bounds__(256) pair_kernel(PArgs A, const float *__restr
                                                    float *__restrict__ out_val,
                                                    uint32_t *__restrict__ out_count) {
   __shared__ uint32_t lh[ENC ? 2 * HIST_WIN : 1];
+  __shared__ float xch[PH > 1 ? 2 * 256 : 1];
   if (ENC) {
     for (int i = threadIdx.x; i < 2 * HIST_WIN; i += blockDim.x) lh[i] = 0;
     __syncthreads();
   }
   const PSet &T = A.set[blockIdx.y];
-  pair_set<PH, ENC>(A, T, x, rec, codes, hist, lh, out_rank, out_val, out_count);
+  pair_set<PH, ENC>(A, T, x, rec, codes, hist, lh, xch, out_rank, out_val, out_count);
   if (ENC) {
     __syncthreads();
     const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
@@ -2472,6 +2507,9 @@ static int launch_level3(const latec_plan *gp, const latec_plan *q, int g0, bool
 static int64_t g_march_blocks = 2048;
 static int64_t g_slab = 0;  // "slab": 1 runs large levels as one level-fused launch
 static int64_t g_pair = 1;  // "pair": 1 runs 3-D levels above the first as line-pair launches
+// "pair_lines": 1 gives phases 2 and 3 whole lines per workgroup when a line
+// has at most 256 pairs (LDS exchange), 0 always uses wave runs
+static int64_t g_pair_lines = 1;
 // "min_threads": a march or pair launch shortens its per-thread march (8, 4,
 // 2, 1 steps) until it has at least this many threads
 static int64_t g_min_threads = 1 << 19;
@@ -2506,6 +2544,14 @@ int tune_set(const char *name, int64_t v) {
     g_pair = v;
     return LATEC_OK;
   }
+  if (!strcmp(name, "pair_lines")) {
+    if (v < 0 || v > 1) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_pair_lines = v;
+    return LATEC_OK;
+  }
   if (!strcmp(name, "min_threads")) {
     if (v < 0 || v > (1 << 26)) {
       set_error(std::string("tuning value out of range for ") + name);
@@ -2532,6 +2578,7 @@ int64_t tune_get(const char *name) {
   if (!strcmp(name, "slab")) return g_slab;
   if (!strcmp(name, "pair")) return g_pair;
   if (!strcmp(name, "min_threads")) return g_min_threads;
+  if (!strcmp(name, "pair_lines")) return g_pair_lines;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -2838,7 +2885,16 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     const int64_t own = A[ph].own;
     const int64_t nw = (pairs + own - 1) / own;
     T.nwaves = (uint32_t)nw;
+    if (ph > 0 && g_pair_lines && npairs <= 256) {  // line blocks
+      T.lpb = 256 / npairs;
+      T.bpc = (T.nrows + T.lpb - 1) / T.lpb;
+    }
   }
+  // threads of a set for s steps per thread
+  auto set_threads = [&](const PSet &T, int64_t st) -> int64_t {
+    const int64_t nchunks = ((int64_t)T.nmarch + st - 1) / st;
+    return T.lpb ? nchunks * T.bpc * 256 : nchunks * T.nwaves * 64;
+  };
   // march steps per thread: PAIR_M, halved while a phase would have fewer
   // than g_min_threads threads (a short level is latency-bound: more threads
   // with shorter marches)
@@ -2847,15 +2903,14 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     uint32_t S = PAIR_M;
     auto threads = [&](uint32_t st) {
       int64_t t = 0;
-      for (int si = 0; si < nset[ph]; si++)
-        t += ((int64_t)a.set[si].nmarch + st - 1) / st * a.set[si].nwaves * 64;
+      for (int si = 0; si < nset[ph]; si++) t += set_threads(a.set[si], st);
       return t;
     };
     while (S > 1 && threads(S) < g_min_threads) S >>= 1;
     a.steps = S;
     for (int si = 0; si < nset[ph]; si++) {
       PSet &T = a.set[si];
-      const int64_t nt = ((int64_t)T.nmarch + S - 1) / S * T.nwaves * 64;
+      const int64_t nt = set_threads(T, S);
       if (nt >= 0x7fffffffll) return 1;
       T.nthreads = (uint32_t)nt;
     }
