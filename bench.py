@@ -446,9 +446,10 @@ def main():
                      "unit": "GB/s", "frac": ach_enc / HBM_PEAK,
                      "traffic": profiled_traffic(shape), "algorithmic_bytes": enc_bytes,
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
-                     "kernel": "one encode: anchors encode_stage_kernel x1 + march_kernel x15 "
-                               "(three per level) + lattice_kernel x7, one hipGraph "
-                               "(10 B/value + 4 B/outlier)",
+                     "kernel": "one encode: x pyramid lattice_kernel x4, anchors "
+                               "encode_stage_kernel x1, first level march_kernel x3, "
+                               "levels 2-5 pair_kernel x12 (three phases per level), one "
+                               "hipGraph (10 B/value + 4 B/outlier)",
                      "decode": {"achieved": ach_dec / 1e9, "frac": ach_dec / HBM_PEAK}},
         "cpu_baseline": cpu,
         "autotune": None if tuned is None else {"knobs": tuned["knobs"], "ms": tuned["ms"],

@@ -249,6 +249,7 @@ static inline uint32_t code_at(const void *codes, int cb, int64_t i) {
 
 static const int64_t LITERAL_MIN = 1 << 12;  // bitstream.py:172
 static const double SCOUT_SLACK = 0.85;      // bitstream.py:175
+static const int64_t PARALLEL_RANS_MIN = 1 << 20;  // stages coded beside their literal scout
 
 int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int64_t n_out,
                    uint32_t R, int zstd_level, const latec_fit *fit, const uint32_t *cdf_row,
@@ -277,6 +278,23 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
     }
     syms[i] = s;
   }
+  // A large stage codes its rANS words on a second thread while the literal
+  // scout (and maybe the full literal) runs: the record is the same, the
+  // stage's critical path is the longer of the two instead of their sum.
+  std::vector<uint32_t> words;
+  int64_t nw = -1;
+  std::thread rans_thread;
+  if (n >= LITERAL_MIN && n >= PARALLEL_RANS_MIN && fit->level >= 0 && fit->level < 64)
+    rans_thread = std::thread([&] {
+      words.resize((size_t)enc_cap(n));
+      nw = rans_encode(syms.data(), n, cdf_row, A, words.data(), enc_cap(n));
+    });
+  struct Join {
+    std::thread &t;
+    ~Join() {
+      if (t.joinable()) t.join();
+    }
+  } join{rans_thread};
   if (n >= LITERAL_MIN) {
     const bool wide = kexp > 7;
     std::vector<uint8_t> lit((size_t)n * (wide ? 2 : 1));
@@ -308,8 +326,11 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
       set_error("scale level exceeds table count");
       return LATEC_E_ARG;
     }
-    std::vector<uint32_t> words((size_t)enc_cap(n));
-    const int64_t nw = rans_encode(syms.data(), n, cdf_row, A, words.data(), enc_cap(n));
+    if (rans_thread.joinable()) rans_thread.join();
+    if (nw < 0) {  // not coded yet, or failed on the helper thread: code (or fail) here
+      words.resize((size_t)enc_cap(n));
+      nw = rans_encode(syms.data(), n, cdf_row, A, words.data(), enc_cap(n));
+    }
     if (nw < 0) return nw;
     if (nw * 4 > blob_cap) {
       set_error("record capacity");

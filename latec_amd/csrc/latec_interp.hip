@@ -1293,6 +1293,9 @@ struct PArgs {
   int64_t hist_words;
   uint32_t npairs, own;  // pairs per line; pairs a wave owns
   uint32_t steps, pad2;  // march steps per thread (1 .. PAIR_M): fewer on small levels
+  uint32_t xs[3];        // element strides of x along the level's axes (the input
+                         // grid's strides times the level's lattice step)
+  uint32_t pad3;
   L3Stage st[3];
   PSet set[2];
 };
@@ -1368,7 +1371,41 @@ __device__ __forceinline__ float2 pair_ld(const float *__restrict__ b, uint32_t 
   return make_float2(ldo(b, idx), ldo(b, odd_ok ? idx + 1 : idx));
 }
 
-template <int PH, bool ENC>
+// emit() of the pair kernels: the code, one histogram count and the outlier
+// append.  The LDS count is branch-free: a lane whose code is outside the
+// window (or that is inactive) adds 0 to the window's spare word HIST_WIN,
+// and the rare out-of-window codes are counted globally under a wave vote.
+__device__ __forceinline__ void emit_pair(uint16_t *__restrict__ codes, uint32_t slot,
+                                          uint32_t code, float x, uint32_t *__restrict__ out_rank,
+                                          float *__restrict__ out_val,
+                                          uint32_t *__restrict__ out_count, uint32_t *lhist,
+                                          uint32_t *__restrict__ ghist, uint32_t win_lo,
+                                          bool active) {
+  if (active) *(uint16_t *)((char *)codes + 2u * slot) = (uint16_t)code;
+  const uint32_t b = code - win_lo;
+  const bool inwin = code != 0 && b < HIST_WIN;
+  lds_add(lhist + (inwin ? b : (uint32_t)HIST_WIN), active && inwin ? 1u : 0u);
+  const bool glob = active && !inwin;
+  if (__any(glob)) {
+    if (glob) gbl_add(ghist + code, 1u);
+  }
+  const bool out = active && code == 0;
+  const uint64_t m = __ballot(out);
+  if (m) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(out_count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (out) {
+      const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      out_rank[k] = slot;
+      out_val[k] = x;
+    }
+  }
+}
+
+template <int PH, bool ENC, bool AL>
 __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
                                          const float *__restrict__ x, float *__restrict__ rec,
                                          uint16_t *__restrict__ codes,
@@ -1384,9 +1421,11 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   const uint32_t gstr[3] = {(uint32_t)A.n[1] * (uint32_t)A.n[2], (uint32_t)A.n[2], 1u};
   const uint32_t sma = gstr[ma], sra = gstr[ra];
   const int n2 = A.n[2], nma = A.n[ma], nra = A.n[ra];
-  const bool al = (n2 & 1) == 0;  // every pair starts 8-byte aligned
+  constexpr bool al = AL;  // every pair starts 8-byte aligned (an even line length)
+  const uint32_t xsma = A.xs[ma], xs2 = A.xs[2];
+  const bool xdirect = xs2 == 1 && A.xs[1] == gstr[1] && A.xs[0] == gstr[0];  // x is the level grid
   const int cubic = A.cubic, end_mode = A.end_mode, center = A.center;
-  const int round_mode = A.round_mode, peel = A.peel;
+  const int round_mode = A.round_mode;
   const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
   const PQ QE = {SE.w, SE.inv_w, (double)SE.R, SE.feb, SE.R};
   const PQ QO = {SO.w, SO.inv_w, (double)SO.R, SO.feb, SO.R};
@@ -1442,6 +1481,10 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
     c[2] = 2 * (int)m;
     const bool odd_ok = 2 * (int)m + 1 < n2;
     const uint32_t f0 = (uint32_t)c[0] * gstr[0] + (uint32_t)c[1] * gstr[1] + (uint32_t)c[2];
+    // x: the level's own grid (xs == gstr) or the input grid at the level's
+    // lattice step (pairs are then two strided loads)
+    const uint32_t fx0 = (uint32_t)c[0] * A.xs[0] + (uint32_t)c[1] * A.xs[1] +
+                         (uint32_t)c[2] * A.xs[2];
     uint32_t ci[3];
     ci[ma] = chunk * S;
     ci[ra] = row;
@@ -1477,10 +1520,16 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
       const bool has = t < cnt;
       const uint32_t f = has ? f0 + 2 * t * sma : f0;
       if (ENC) {
-        if (PH == 1)
-          L.xx.y = ldo(x, odd_ok ? f + 1 : f);
-        else
-          L.xx = pair_ld(x, f, al, odd_ok);
+        const uint32_t fx = has ? fx0 + 2 * t * xsma : fx0;
+        if (xdirect) {
+          if (PH == 1)
+            L.xx.y = ldo(x, odd_ok ? f + 1 : f);
+          else
+            L.xx = pair_ld(x, f, al, odd_ok);
+        } else {
+          if (PH > 1) L.xx.x = ldo(x, fx);
+          L.xx.y = ldo(x, odd_ok ? fx + xs2 : fx);
+        }
       } else {
         if (PH > 1) L.ce = ldo(sc_e, has ? rk_e0 + t * rma_e : rk_e0);
         // a pair without an odd point has no odd rank: its formula can run
@@ -1617,10 +1666,9 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
       }
       if (ENC) {
         if (PH > 1)
-          emit_wave(sc_e, rke, cde, xe, rke, or_e, ov_e, out_count + SE.gs, lh, gh_e, wlo_e,
-                    active, peel);
-        emit_wave(sc_o, rko, cdo, xo, rko, or_o, ov_o, out_count + SO.gs, lh + HIST_WIN, gh_o,
-                  wlo_o, active && odd_ok, peel);
+          emit_pair(sc_e, rke, cde, xe, or_e, ov_e, out_count + SE.gs, lh, gh_e, wlo_e, active);
+        emit_pair(sc_o, rko, cdo, xo, or_o, ov_o, out_count + SO.gs, lh + HIST_WIN + 1, gh_o,
+                  wlo_o, active && odd_ok);
       }
       if (PH > 1) {
         we0 = we1; we1 = we2; we2 = we3;
@@ -1632,27 +1680,31 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   }
 }
 
-template <int PH, bool ENC>
-__global__ void __launch_bounds__(256) pair_kernel(PArgs A, const float *__restrict__ x,
+#ifndef PAIR_WAVES_PER_SIMD
+#define PAIR_WAVES_PER_SIMD 1  // occupancy floor handed to the register allocator
+#endif
+template <int PH, bool ENC, bool AL>
+__global__ void __launch_bounds__(256, PAIR_WAVES_PER_SIMD) pair_kernel(PArgs A, const float *__restrict__ x,
                                                    float *__restrict__ rec,
                                                    uint16_t *__restrict__ codes,
                                                    uint32_t *__restrict__ hist,
                                                    uint32_t *__restrict__ out_rank,
                                                    float *__restrict__ out_val,
                                                    uint32_t *__restrict__ out_count) {
-  __shared__ uint32_t lh[ENC ? 2 * HIST_WIN : 1];
+  // two histogram windows of HIST_WIN bins and a spare word each
+  __shared__ uint32_t lh[ENC ? 2 * (HIST_WIN + 1) : 1];
   __shared__ float xch[PH > 1 ? 2 * 256 : 1];
   if (ENC) {
-    for (int i = threadIdx.x; i < 2 * HIST_WIN; i += blockDim.x) lh[i] = 0;
+    for (int i = threadIdx.x; i < 2 * (HIST_WIN + 1); i += blockDim.x) lh[i] = 0;
     __syncthreads();
   }
   const PSet &T = A.set[blockIdx.y];
-  pair_set<PH, ENC>(A, T, x, rec, codes, hist, lh, xch, out_rank, out_val, out_count);
+  pair_set<PH, ENC, AL>(A, T, x, rec, codes, hist, lh, xch, out_rank, out_val, out_count);
   if (ENC) {
     __syncthreads();
     const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
     if (PH > 1) flush_hist(lh, hist + (int64_t)SE.gs * A.hist_words, SE.R);
-    flush_hist(lh + HIST_WIN, hist + (int64_t)SO.gs * A.hist_words, SO.R);
+    flush_hist(lh + HIST_WIN + 1, hist + (int64_t)SO.gs * A.hist_words, SO.R);
   }
 }
 
@@ -2031,6 +2083,7 @@ __global__ void __launch_bounds__(SL_THREADS) slab_kernel(SlabArgs A, const floa
 // scatter fine[2m] = coarse[m].  One thread per coarse point, C channels.
 struct LatticeArgs {
   int32_t ndim, channels;
+  int32_t step, pad;  // coarse point m is fine point step * m on every axis
   int64_t fine[LATEC_MAX_DIM], coarse[LATEC_MAX_DIM];
   int64_t nfine, ncoarse;
 };
@@ -2054,13 +2107,13 @@ __global__ void __launch_bounds__(256) lattice_kernel(LatticeArgs a, float *__re
       const uint32_t cj = (uint32_t)a.coarse[j];
       const uint32_t m = rem % cj;
       rem /= cj;
-      fbase += 2 * m * fs;
+      fbase += (uint32_t)a.step * m * fs;
       fs *= (uint32_t)a.fine[j];
     }
     const uint32_t cbase = row * L;
     for (uint32_t col = lcol; col < L; col += tr)
       for (int ch = 0; ch < a.channels; ch++) {
-        const uint32_t fi = (uint32_t)(ch * a.nfine) + fbase + 2 * col;
+        const uint32_t fi = (uint32_t)(ch * a.nfine) + fbase + (uint32_t)a.step * col;
         const uint32_t ci = (uint32_t)(ch * a.ncoarse) + cbase + col;
         if (GATHER)
           coarse[ci] = fine[fi];
@@ -2355,11 +2408,12 @@ static int pyramid_setup(latec_plan *p) {
 }
 
 static int lattice(const latec_plan *fine_p, const latec_plan *coarse_p, float *fine,
-                   float *coarse, bool gather, hipStream_t stream) {
+                   float *coarse, bool gather, hipStream_t stream, int64_t step = 2) {
   LatticeArgs a;
   memset(&a, 0, sizeof a);
   a.ndim = fine_p->ndim;
   a.channels = fine_p->channels;
+  a.step = (int32_t)step;
   for (int j = 0; j < a.ndim; j++) {
     a.fine[j] = fine_p->shape[j];
     a.coarse[j] = coarse_p->shape[j];
@@ -2796,10 +2850,10 @@ static bool pair_ok(const latec_plan *gp, const latec_plan *q, int k,
 }
 
 static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_plan *qc, int g0,
-                       bool encode, const float *x, const float *coarse, float *recon,
-                       void *codes, uint32_t *hist, uint32_t *out_rank, float *out_val,
-                       uint32_t *out_count, const double *ebs, const latec_interp_opts *o,
-                       hipStream_t stream) {
+                       bool encode, const float *x, int64_t xstep, const float *coarse,
+                       float *recon, void *codes, uint32_t *hist, uint32_t *out_rank,
+                       float *out_val, uint32_t *out_count, const double *ebs,
+                       const latec_interp_opts *o, hipStream_t stream, bool launch = true) {
   for (int j = 0; j < 3; j++)
     if (qc->shape[j] != (q->shape[j] + 1) / 2) return 1;
   // the sub-plan's classes by parity, in their stages: 100, 010, 001 alone,
@@ -2839,6 +2893,13 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     a.hist_words = hist_words(o->radius);
     a.npairs = npairs;
     a.own = ph == 0 ? 64u : (uint32_t)PAIR_OWN;
+    // x is the input grid read at the level's lattice step (xstep 1: the level grid)
+    const int64_t xst[3] = {xstep * gp->shape[1] * gp->shape[2], xstep * gp->shape[2], xstep};
+    for (int j = 0; j < 3; j++) {
+      if (xstep == 1 && gp->shape[j] != n[j]) return 1;
+      if ((n[j] - 1) * xstep >= gp->shape[j] || xst[j] >= (1ll << 30)) return 1;
+      a.xs[j] = (uint32_t)xst[j];
+    }
   }
   int nst[3] = {0, 0, 0};
   // the stage slot of sub-plan stage js in phase ph's argument block
@@ -2915,6 +2976,7 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
       T.nthreads = (uint32_t)nt;
     }
   }
+  if (!launch) return 0;  // the layout check alone
   for (int ph = 0; ph < 3; ph++) {
     const PArgs &a = A[ph];
     int64_t blocks = 1;
@@ -2923,13 +2985,23 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     blocks = std::min<int64_t>(blocks, g_march_blocks);
     blocks = (blocks + 7) / 8 * 8;
     const dim3 grid((unsigned)blocks, (unsigned)nset[ph]);
-#define LATEC_PAIR_LAUNCH(PH)                                                                 \
-  if (encode)                                                                                 \
-    hipLaunchKernelGGL((pair_kernel<PH, true>), grid, dim3(256), 0, stream, a, x, recon,     \
-                       (uint16_t *)codes, hist, out_rank, out_val, out_count);               \
-  else                                                                                        \
-    hipLaunchKernelGGL((pair_kernel<PH, false>), grid, dim3(256), 0, stream, a, nullptr,     \
-                       recon, (uint16_t *)codes, hist, out_rank, out_val, out_count);
+    const bool al = n[2] % 2 == 0;
+#define LATEC_PAIR_GO(PH, E, AL_)                                                             \
+  hipLaunchKernelGGL((pair_kernel<PH, E, AL_>), grid, dim3(256), 0, stream, a,              \
+                     E ? x : nullptr, recon, (uint16_t *)codes, hist, out_rank, out_val,      \
+                     out_count);
+#define LATEC_PAIR_LAUNCH(PH)               \
+  if (encode) {                             \
+    if (al) {                               \
+      LATEC_PAIR_GO(PH, true, true)         \
+    } else {                                \
+      LATEC_PAIR_GO(PH, true, false)        \
+    }                                       \
+  } else if (al) {                          \
+    LATEC_PAIR_GO(PH, false, true)          \
+  } else {                                  \
+    LATEC_PAIR_GO(PH, false, false)         \
+  }
     if (ph == 0) {
       LATEC_PAIR_LAUNCH(1)
     } else if (ph == 1) {
@@ -2938,6 +3010,7 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
       LATEC_PAIR_LAUNCH(3)
     }
 #undef LATEC_PAIR_LAUNCH
+#undef LATEC_PAIR_GO
   }
   return hip_check(hipGetLastError(), "pair_kernel");
 }
@@ -2952,11 +3025,20 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
   const int K = d->K, nd = p->ndim, per = 2 * nd - 1;
   auto X = [&](int k) { return k == K ? x : (const float *)d->X[k]; };
   auto R = [&](int k) { return k == K ? recon : d->R[k]; };
-  if (encode) {  // lattice pyramid of the input, fine to coarse
-    for (int k = K - 1; k >= 1; k--) {
-      const latec_plan *fine = k + 1 == K ? p : d->sub[k + 1];
-      if ((rc = lattice(fine, d->sub[k], (float *)X(k + 1), d->X[k], true, stream))) return rc;
-    }
+  // levels whose pair launches read x straight from the input grid at their
+  // lattice step (no compact copy of x); the others get theirs gathered
+  uint64_t direct = 0;
+  auto xstep = [&](int k) { return (int64_t)1 << (K - k); };
+  for (int k = 2; k < K && k < 64; k++)
+    if (!slab_ok(p, d->sub[k], k, o) && pair_ok(p, d->sub[k], k, o) &&
+        launch_pair(p, d->sub[k], d->sub[k - 1], 1 + (k - 1) * per, true, x, xstep(k), nullptr, nullptr,
+                    nullptr, nullptr, nullptr, nullptr, nullptr, ebs, o, stream, false) == 0)
+      direct |= 1ull << k;
+  if (encode) {  // lattice pyramid of the input: each gathered level straight from x
+    for (int k = K - 1; k >= 1; k--)
+      if (!((direct >> k) & 1) &&
+          (rc = lattice(p, d->sub[k], (float *)x, d->X[k], true, stream, xstep(k))))
+        return rc;
   }
   // anchors: stage 0 of the first level's plan (its even lattice)
   if ((rc = launch_stage(p, d->sub[1], 0, 0, encode, X(1), R(1), codes, hist, out_rank, out_val,
@@ -2972,8 +3054,10 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
       if (rc == 0) continue;
     }
     if (pair_ok(p, q, k, o)) {  // reads level k-1 itself: no lattice scatter
-      rc = launch_pair(p, q, d->sub[k - 1], g0, encode, X(k), d->R[k - 1], R(k), codes, hist,
-                       out_rank, out_val, out_count, ebs, o, stream);
+      const bool dx = (direct >> k) & 1;
+      rc = launch_pair(p, q, d->sub[k - 1], g0, encode, dx ? x : X(k), dx ? xstep(k) : 1,
+                       d->R[k - 1], R(k), codes, hist, out_rank, out_val, out_count, ebs, o,
+                       stream);
       if (rc < 0) return rc;
       if (rc == 0) continue;
     }
