@@ -1266,9 +1266,6 @@ __global__ void __launch_bounds__(256) march_kernel(MArgs A, const float *__rest
 #define PAIR_M 8    // most march steps per thread (PArgs::steps <= PAIR_M)
 
 #define PAIR_OWN 61  // pairs a wave owns in phases 2 and 3
-#ifndef PAIR_DEPTH
-#define PAIR_DEPTH 1  // steps the loads run ahead of the arithmetic (1 or 2)
-#endif
 
 struct PClass {
   int32_t stage, pad;  // index into PArgs::st
@@ -1320,6 +1317,7 @@ struct PQ {
 
 // quantize_one (encode): code and reconstruction of x against prediction p.
 // Every lane of the wave calls it (the exact-division fallback is a vote).
+template <bool RND>  // RND false: round_mode is 0 (no output rounding check)
 __device__ __forceinline__ void pair_quant(const PQ &Q, int round_mode, double span,
                                            double offset, double pd, float xv, uint32_t &code,
                                            float &r) {
@@ -1334,10 +1332,12 @@ __device__ __forceinline__ void pair_quant(const PQ &Q, int round_mode, double s
   const int qi = inr ? (int)qd : 0;
   const float cand = (float)(pd + Q.w * (double)qi);
   float chk = cand;
-  if (round_mode == 1)
-    chk = rintf(cand);
-  else if (round_mode == 2)
-    chk = (float)((rint((double)cand * span + offset) - offset) / span);
+  if (RND) {
+    if (round_mode == 1)
+      chk = rintf(cand);
+    else if (round_mode == 2)
+      chk = (float)((rint((double)cand * span + offset) - offset) / span);
+  }
   const bool ok = inr && fabsf(xv - chk) <= Q.feb;
   code = ok ? (uint32_t)(qi + (int)Q.R) : 0u;
   r = ok ? cand : xv;
@@ -1371,17 +1371,45 @@ __device__ __forceinline__ float2 pair_ld(const float *__restrict__ b, uint32_t 
   return make_float2(ldo(b, idx), ldo(b, odd_ok ? idx + 1 : idx));
 }
 
+// Masked stores as one unconditional buffer instruction: a lane that must
+// not store writes to byte offset BUF_OFF_NONE, beyond the resource's range,
+// which the hardware drops.  With no branch around them the number of memory
+// operations a step issues is fixed, so the wait for the next step's loads
+// need not wait for this step's stores.  (gfx9 buffer resource: 32-bit data
+// format in the last word; 2^31 - 1 byte range, every array here is smaller.)
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+#define BUF_OFF_NONE 0x7fffffffu
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, 0x7fffffff,
+                                           0x00020000);
+}
+__device__ __forceinline__ void buf_st32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on,
+                                         float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, on ? byte_off : BUF_OFF_NONE, 0,
+                                        0);
+}
+__device__ __forceinline__ void buf_st64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on,
+                                         float a, float b) {
+  const u32x2_t v = {__float_as_uint(a), __float_as_uint(b)};
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, on ? byte_off : BUF_OFF_NONE, 0, 0);
+}
+__device__ __forceinline__ void buf_st16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on,
+                                         uint16_t v) {
+  __builtin_amdgcn_raw_buffer_store_b16(v, r, on ? byte_off : BUF_OFF_NONE, 0, 0);
+}
+
 // emit() of the pair kernels: the code, one histogram count and the outlier
 // append.  The LDS count is branch-free: a lane whose code is outside the
 // window (or that is inactive) adds 0 to the window's spare word HIST_WIN,
 // and the rare out-of-window codes are counted globally under a wave vote.
-__device__ __forceinline__ void emit_pair(uint16_t *__restrict__ codes, uint32_t slot,
-                                          uint32_t code, float x, uint32_t *__restrict__ out_rank,
+__device__ __forceinline__ void emit_pair(__amdgpu_buffer_rsrc_t codes, uint32_t cbase,
+                                          uint32_t slot, uint32_t code, float x,
+                                          uint32_t *__restrict__ out_rank,
                                           float *__restrict__ out_val,
                                           uint32_t *__restrict__ out_count, uint32_t *lhist,
                                           uint32_t *__restrict__ ghist, uint32_t win_lo,
                                           bool active) {
-  if (active) *(uint16_t *)((char *)codes + 2u * slot) = (uint16_t)code;
+  buf_st16(codes, cbase + 2u * slot, active, (uint16_t)code);
   const uint32_t b = code - win_lo;
   const bool inwin = code != 0 && b < HIST_WIN;
   lds_add(lhist + (inwin ? b : (uint32_t)HIST_WIN), active && inwin ? 1u : 0u);
@@ -1405,7 +1433,10 @@ __device__ __forceinline__ void emit_pair(uint16_t *__restrict__ codes, uint32_t
   }
 }
 
-template <int PH, bool ENC, bool AL>
+// Compile-time layout: AL pairs 8-byte aligned (even line length), RND output
+// rounding checked (round_mode != 0), BM line blocks (phases 2, 3), XD x is
+// the level's own grid (else the input grid at the level's lattice step).
+template <int PH, bool ENC, bool AL, bool RND, bool BM, bool XD>
 __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
                                          const float *__restrict__ x, float *__restrict__ rec,
                                          uint16_t *__restrict__ codes,
@@ -1423,7 +1454,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   const int n2 = A.n[2], nma = A.n[ma], nra = A.n[ra];
   constexpr bool al = AL;  // every pair starts 8-byte aligned (an even line length)
   const uint32_t xsma = A.xs[ma], xs2 = A.xs[2];
-  const bool xdirect = xs2 == 1 && A.xs[1] == gstr[1] && A.xs[0] == gstr[0];  // x is the level grid
+  constexpr bool xdirect = XD;
   const int cubic = A.cubic, end_mode = A.end_mode, center = A.center;
   const int round_mode = A.round_mode;
   const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
@@ -1438,6 +1469,10 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   const int64_t nout_e = ENC ? 0 : (int64_t)out_count[SE.gs];
   const int64_t nout_o = ENC ? 0 : (int64_t)out_count[SO.gs];
   uint16_t *sc_e = codes + SE.code_base, *sc_o = codes + SO.code_base;
+  // one code resource for both classes: each class's stage starts at a byte
+  // offset (the code array is far below 2^31 bytes)
+  const __amdgpu_buffer_rsrc_t rrec = buf_rsrc(rec), rcodes = buf_rsrc(codes);
+  const uint32_t cb_e = 2u * (uint32_t)SE.code_base, cb_o = 2u * (uint32_t)SO.code_base;
   const uint32_t rma_e = (uint32_t)T.ev.rstr[ma], rma_o = (uint32_t)T.od.rstr[ma];
   const int lane = __lane_id();
   const uint32_t own = A.own, halo = PH == 1 ? 0u : 1u;
@@ -1445,7 +1480,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   const uint32_t cs0 = (uint32_t)A.cn[1] * (uint32_t)A.cn[2], cs1 = (uint32_t)A.cn[2];
   const uint32_t G = gridDim.x;
   const uint32_t lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
-  const bool bm = PH > 1 && T.lpb != 0;  // line blocks (uniform)
+  constexpr bool bm = PH > 1 && BM;
   uint32_t xstep = 0;  // steps taken by the workgroup: picks the exchange buffer
   for (uint32_t base = lb * 256u; base < T.nthreads; base += G * 256u) {
     bool wv, owned;
@@ -1562,17 +1597,24 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
       we0 = a.x; wo0 = a.y; we1 = b.x; wo1 = b.y; we2 = d.x; wo2 = d.y;
     }
     // loads run PAIR_DEPTH steps ahead of the arithmetic
-    Loads cur, nxt, nx2;
-    load(0, cur);
-    if (PAIR_DEPTH > 1 && 1 < ucnt) load(1, nxt);
-#pragma unroll 1
+    // every step of the wave's chunk interior on every axis: the cubic
+    // stencil runs without per-step checks
+    const int cm0 = c[ma];
+    const bool chunk_in = PH == 1 || (cm0 >= 3 && cm0 + 2 * ((int)ucnt - 1) + 3 < nma);
+    const bool all_in = cubic && __all(chunk_in && (PH < 3 || (dv0 && dv2)) &&
+                                       (!odd_ok || (kv0 && kv2)));
+    // The loads run one step ahead of the arithmetic: step t's loads were
+    // issued at the start of step t-1 and are moved into `cur` at the start of
+    // step t, before step t+1's are issued.  (Issuing the next loads first and
+    // moving at the end of the step lets the compiler's wait at the loop head
+    // cover the just-issued loads too, which serialises every step on them.)
+    Loads cur, nxt;
+    load(0, nxt);
+#pragma unroll
     for (uint32_t t = 0; t < (uint32_t)M; t++) {
       if (t >= ucnt) break;
-      if (PAIR_DEPTH > 1) {
-        if (t + 2 < ucnt) load(t + 2, nx2);
-      } else {
-        if (t + 1 < ucnt) load(t + 1, nxt);
-      }
+      cur = nxt;
+      if (t + 1 < ucnt) load(t + 1, nxt);
       const bool active = owned;
       const uint32_t f = f0 + 2 * t * sma;
       const uint32_t rke = rk_e0 + t * rma_e, rko = rk_o0 + t * rma_o;
@@ -1586,7 +1628,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
         re = cur.ct[1];  // the coarse point itself
       } else {
         double v[2];
-        if (cubic && __all(mv0 && mv2 && (PH == 2 || (dv0 && dv2)))) {
+        if (all_in || (cubic && __all(mv0 && mv2 && (PH == 2 || (dv0 && dv2))))) {
           v[0] = cubic4(we0, we1, we2, we3);
           if (PH == 3)
             v[1] = cubic4((double)cur.d[0].x, (double)cur.d[1].x, (double)cur.d[2].x,
@@ -1600,7 +1642,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
         const double pd = (double)(float)pair_combine(v, NE, center);
         if (ENC) {
           xe = cur.xx.x;
-          pair_quant(QE, round_mode, A.span, A.offset, pd, xe, cde, re);
+          pair_quant<RND>(QE, round_mode, A.span, A.offset, pd, xe, cde, re);
         } else {
           // every lane with a point decodes it: lanes 0, 62, 63 feed the k taps
           cde = cur.ce;
@@ -1630,8 +1672,8 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
         k3 = (double)__shfl_down(re, 2);
       }
       double v[3];
-      if (cubic && __all(!odd_ok || (kv0 && kv2 && (PH == 1 || (mv0 && mv2)) &&
-                                     (PH < 3 || (dv0 && dv2))))) {
+      if (all_in || (cubic && __all(!odd_ok || (kv0 && kv2 && (PH == 1 || (mv0 && mv2)) &&
+                                                (PH < 3 || (dv0 && dv2)))))) {
         int i = 0;
         if (PH > 1) v[i++] = cubic4(wo0, wo1, wo2, wo3);
         if (PH == 3)
@@ -1651,31 +1693,28 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
       uint32_t cdo;
       if (ENC) {
         xo = cur.xx.y;
-        pair_quant(QO, round_mode, A.span, A.offset, po, xo, cdo, ro);
+        pair_quant<RND>(QO, round_mode, A.span, A.offset, po, xo, cdo, ro);
       } else {
         cdo = cur.co;
         ro = pair_dequant(QO, po, cdo, active && odd_ok, rko, or_o, ov_o, nout_o);
       }
-      if (active) {
-        if (al) {
-          sto((float2 *)rec, f >> 1, make_float2(re, ro));
-        } else {
-          sto(rec, f, re);
-          if (odd_ok) sto(rec, f + 1, ro);
-        }
+      if (al) {
+        buf_st64(rrec, 4u * f, active, re, ro);
+      } else {
+        buf_st32(rrec, 4u * f, active, re);
+        buf_st32(rrec, 4u * (f + 1), active && odd_ok, ro);
       }
       if (ENC) {
         if (PH > 1)
-          emit_pair(sc_e, rke, cde, xe, or_e, ov_e, out_count + SE.gs, lh, gh_e, wlo_e, active);
-        emit_pair(sc_o, rko, cdo, xo, or_o, ov_o, out_count + SO.gs, lh + HIST_WIN + 1, gh_o,
-                  wlo_o, active && odd_ok);
+          emit_pair(rcodes, cb_e, rke, cde, xe, or_e, ov_e, out_count + SE.gs, lh, gh_e, wlo_e,
+                    active);
+        emit_pair(rcodes, cb_o, rko, cdo, xo, or_o, ov_o, out_count + SO.gs, lh + HIST_WIN + 1,
+                  gh_o, wlo_o, active && odd_ok);
       }
       if (PH > 1) {
         we0 = we1; we1 = we2; we2 = we3;
         wo0 = wo1; wo1 = wo2; wo2 = wo3;
       }
-      cur = nxt;
-      if (PAIR_DEPTH > 1) nxt = nx2;
     }
   }
 }
@@ -1683,7 +1722,7 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
 #ifndef PAIR_WAVES_PER_SIMD
 #define PAIR_WAVES_PER_SIMD 1  // occupancy floor handed to the register allocator
 #endif
-template <int PH, bool ENC, bool AL>
+template <int PH, bool ENC, bool AL, bool RND, bool BM, bool XD>
 __global__ void __launch_bounds__(256, PAIR_WAVES_PER_SIMD) pair_kernel(PArgs A, const float *__restrict__ x,
                                                    float *__restrict__ rec,
                                                    uint16_t *__restrict__ codes,
@@ -1699,7 +1738,8 @@ __global__ void __launch_bounds__(256, PAIR_WAVES_PER_SIMD) pair_kernel(PArgs A,
     __syncthreads();
   }
   const PSet &T = A.set[blockIdx.y];
-  pair_set<PH, ENC, AL>(A, T, x, rec, codes, hist, lh, xch, out_rank, out_val, out_count);
+  pair_set<PH, ENC, AL, RND, BM, XD>(A, T, x, rec, codes, hist, lh, xch, out_rank, out_val,
+                                     out_count);
   if (ENC) {
     __syncthreads();
     const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
@@ -2849,6 +2889,42 @@ static bool pair_ok(const latec_plan *gp, const latec_plan *q, int k,
          q->stages.size() == 6 && q->npoints < (1ll << 30);
 }
 
+// Launches the pair kernel variant of a phase and layout bits (1 encode,
+// 2 aligned pairs, 4 rounding check, 8 line blocks, 16 x is the level grid);
+// only the combinations that occur are instantiated.
+template <int B>
+struct PairLaunch {
+  static constexpr bool E = B & 1, AL = B & 2, RND = B & 4, BM = B & 8, XD = B & 16;
+  static constexpr bool valid = E || (!RND && XD);
+  template <int PH>
+  static void one(dim3 grid, hipStream_t stream, const PArgs &a, const float *x, float *recon,
+                  uint16_t *codes, uint32_t *hist, uint32_t *out_rank, float *out_val,
+                  uint32_t *out_count) {
+    if constexpr (valid && (PH > 1 || !BM))
+      hipLaunchKernelGGL((pair_kernel<PH, E, AL, RND, BM, XD>), grid, dim3(256), 0, stream, a,
+                         x, recon, codes, hist, out_rank, out_val, out_count);
+  }
+  static void go(int ph, int bits, dim3 grid, hipStream_t stream, const PArgs &a,
+                 const float *x, float *recon, uint16_t *codes, uint32_t *hist,
+                 uint32_t *out_rank, float *out_val, uint32_t *out_count) {
+    if (bits != B) {
+      PairLaunch<B - 1>::go(ph, bits, grid, stream, a, x, recon, codes, hist, out_rank,
+                            out_val, out_count);
+    } else if (ph == 1) {
+      one<1>(grid, stream, a, x, recon, codes, hist, out_rank, out_val, out_count);
+    } else if (ph == 2) {
+      one<2>(grid, stream, a, x, recon, codes, hist, out_rank, out_val, out_count);
+    } else {
+      one<3>(grid, stream, a, x, recon, codes, hist, out_rank, out_val, out_count);
+    }
+  }
+};
+template <>
+struct PairLaunch<-1> {
+  static void go(int, int, dim3, hipStream_t, const PArgs &, const float *, float *, uint16_t *,
+                 uint32_t *, uint32_t *, float *, uint32_t *) {}
+};
+
 static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_plan *qc, int g0,
                        bool encode, const float *x, int64_t xstep, const float *coarse,
                        float *recon, void *codes, uint32_t *hist, uint32_t *out_rank,
@@ -2985,32 +3061,12 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     blocks = std::min<int64_t>(blocks, g_march_blocks);
     blocks = (blocks + 7) / 8 * 8;
     const dim3 grid((unsigned)blocks, (unsigned)nset[ph]);
-    const bool al = n[2] % 2 == 0;
-#define LATEC_PAIR_GO(PH, E, AL_)                                                             \
-  hipLaunchKernelGGL((pair_kernel<PH, E, AL_>), grid, dim3(256), 0, stream, a,              \
-                     E ? x : nullptr, recon, (uint16_t *)codes, hist, out_rank, out_val,      \
-                     out_count);
-#define LATEC_PAIR_LAUNCH(PH)               \
-  if (encode) {                             \
-    if (al) {                               \
-      LATEC_PAIR_GO(PH, true, true)         \
-    } else {                                \
-      LATEC_PAIR_GO(PH, true, false)        \
-    }                                       \
-  } else if (al) {                          \
-    LATEC_PAIR_GO(PH, false, true)          \
-  } else {                                  \
-    LATEC_PAIR_GO(PH, false, false)         \
-  }
-    if (ph == 0) {
-      LATEC_PAIR_LAUNCH(1)
-    } else if (ph == 1) {
-      LATEC_PAIR_LAUNCH(2)
-    } else {
-      LATEC_PAIR_LAUNCH(3)
-    }
-#undef LATEC_PAIR_LAUNCH
-#undef LATEC_PAIR_GO
+    // layout bits of the kernel variant (decode: no rounding check, x unused)
+    const int bits = (encode ? 1 : 0) | (n[2] % 2 == 0 ? 2 : 0) |
+                     (encode && o->round_mode != 0 ? 4 : 0) | (ph > 0 && a.set[0].lpb ? 8 : 0) |
+                     (!encode || xstep == 1 ? 16 : 0);
+    PairLaunch<31>::go(ph + 1, bits, grid, stream, a, encode ? x : nullptr, recon,
+                       (uint16_t *)codes, hist, out_rank, out_val, out_count);
   }
   return hip_check(hipGetLastError(), "pair_kernel");
 }
