@@ -1,6 +1,6 @@
 """Autotuner for the plain path's launch shapes.
 
-The 3-D level kernels have six launch-shape knobs, set process-wide with
+The 3-D level kernels have seven launch-shape knobs, set process-wide with
 ``latec_tune_set``:
 
 * ``march_blocks`` -- workgroups per parity class (with 4 waves each, this is
@@ -16,6 +16,10 @@ The 3-D level kernels have six launch-shape knobs, set process-wide with
 * ``pair_lines`` -- whether the pair launches give each workgroup whole
   lines (every lane owns its pair, neighbours exchanged through LDS) or wave
   runs of 61 owned pairs with three recomputing halo lanes;
+* ``pair_quads`` -- whether the line blocks of phases 2 and 3 give each
+  thread four points of a line (lines of a multiple of 4, at most 1024) or
+  two: bit 0 for the encoder (the one the tuner times), bit 1 for the
+  decoder;
 * ``min_threads`` -- a level launch shortens each thread's march (8, 4, 2, 1
   points) until it has at least this many threads: small levels are
   latency-bound, so they trade march length for parallelism.
@@ -42,13 +46,15 @@ import numpy as np
 
 from ._lib import check, lib, ptr
 
-KNOBS = ("march_blocks", "march_hist_peel", "slab", "pair", "pair_lines", "min_threads")
+KNOBS = ("march_blocks", "march_hist_peel", "slab", "pair", "pair_lines", "pair_quads",
+         "min_threads")
 CANDIDATES = {"march_blocks": (1024, 2048, 4096),
               "march_hist_peel": (0,),
               "slab": (0,),
               "pair": (0, 1),
               "pair_lines": (0, 1),
-              "min_threads": (1 << 18, 1 << 19, 1 << 20)}
+              "pair_quads": (2, 3),
+              "min_threads": (1 << 18, 1 << 19)}
 
 
 def get() -> dict:

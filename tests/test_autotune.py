@@ -12,9 +12,10 @@ def test_knobs_round_trip_and_validation():
     start = autotune.get()
     try:
         autotune.set_knobs(march_blocks=1000, march_hist_peel=3, slab=0, pair=0,
-                           min_threads=12345, pair_lines=0)
+                           min_threads=12345, pair_lines=0, pair_quads=0)
         assert autotune.get() == {"march_blocks": 1000, "march_hist_peel": 3, "slab": 0,
-                                  "pair": 0, "pair_lines": 0, "min_threads": 12345}
+                                  "pair": 0, "pair_lines": 0, "pair_quads": 0,
+                                  "min_threads": 12345}
         with pytest.raises(ValueError):
             autotune.set_knobs(pair=2)
         with pytest.raises(ValueError):
