@@ -1481,16 +1481,31 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   uint32_t xstep = 0;  // steps taken by the workgroup: picks the exchange buffer
   for (uint32_t base = lb * 256u; base < T.nthreads; base += G * 256u) {
     bool wv, owned;
-    uint32_t chunk, row, m;
+    uint32_t chunk, row, m, xpos = 0;
     if (bm) {  // nthreads is a multiple of 256: the whole workgroup is in range
       const uint32_t blk = base >> 8;
       chunk = blk / T.bpc;
-      const uint32_t line = threadIdx.x / A.npairs;
-      m = threadIdx.x - line * A.npairs;
+      // pairs whose odd point has all four k taps (1 .. npairs-3) first, the
+      // three end pairs of every line last: only the last wave meets the end
+      // rules along k
+      const uint32_t np = A.npairs, nin = np >= 3 ? np - 3 : 0, nb = np - nin;
+      const uint32_t tl = threadIdx.x;
+      uint32_t line;
+      if (tl < T.lpb * nin) {
+        line = tl / nin;
+        m = 1 + tl - line * nin;
+      } else {
+        const uint32_t b = tl - T.lpb * nin;
+        line = b / nb;
+        const uint32_t j = b - line * nb;
+        m = nin ? (j == 0 ? 0 : np - 3 + j) : j;
+      }
+      xpos = line < T.lpb ? line * np + m : tl;  // exchange slot (past the lines: tl)
       row = (blk - chunk * T.bpc) * T.lpb + line;
       wv = true;
       owned = line < T.lpb && row < T.nrows;
       row = min(row, T.nrows - 1);
+      m = min(m, np - 1);
     } else {
       const uint32_t gw = (base + threadIdx.x) >> 6;  // uniform in the wave
       wv = base + threadIdx.x < T.nthreads;
@@ -1655,13 +1670,13 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
         // written again only after the next step's barrier, which every
         // thread reaches after its reads of this one
         float *xb = xch + (xstep++ & 1) * 256;
-        const int tl = threadIdx.x;
-        xb[tl] = re;
+        xb[xpos] = re;
         __syncthreads();
-        k0 = (double)xb[tl >= 1 ? tl - 1 : tl];
+        // neighbours in the same line (invalid taps read the own slot, unused)
+        k0 = (double)xb[kv0 ? xpos - 1 : xpos];
         k1 = (double)re;
-        k2 = (double)xb[tl + 1 < 256 ? tl + 1 : tl];
-        k3 = (double)xb[tl + 2 < 256 ? tl + 2 : tl];
+        k2 = (double)xb[m + 1 < A.npairs ? xpos + 1 : xpos];
+        k3 = (double)xb[m + 2 < A.npairs ? xpos + 2 : xpos];
       } else {
         k0 = (double)__shfl_up(re, 1);
         k1 = (double)re;
