@@ -1343,6 +1343,40 @@ __device__ __forceinline__ void pair_quant(const PQ &Q, int round_mode, double s
   r = ok ? cand : xv;
 }
 
+// pair_quant of two points with one wave vote for the exact-division fallback.
+template <bool RND>
+__device__ __forceinline__ void pair_quant2(const PQ &Q, int round_mode, double span,
+                                            double offset, double pd0, float x0, double pd1,
+                                            float x1, uint32_t &code0, float &r0,
+                                            uint32_t &code1, float &r1) {
+  const double d0 = (double)x0 - pd0, d1 = (double)x1 - pd1;
+  const double t0 = d0 * Q.inv_w, t1 = d1 * Q.inv_w;
+  double q0 = rint(t0), q1 = rint(t1);
+  const bool fb0 = !(0.5 - fabs(t0 - q0) > 1e-12 * fmax(fabs(t0), 1.0));
+  const bool fb1 = !(0.5 - fabs(t1 - q1) > 1e-12 * fmax(fabs(t1), 1.0));
+  if (__any(fb0 || fb1)) {
+    if (fb0) q0 = rint(d0 / Q.w);
+    if (fb1) q1 = rint(d1 / Q.w);
+  }
+  auto finish = [&](double pd, float xv, double qd, uint32_t &code, float &r) {
+    const bool inr = fabs(qd) < Q.Rd;
+    const int qi = inr ? (int)qd : 0;
+    const float cand = (float)(pd + Q.w * (double)qi);
+    float chk = cand;
+    if (RND) {
+      if (round_mode == 1)
+        chk = rintf(cand);
+      else if (round_mode == 2)
+        chk = (float)((rint((double)cand * span + offset) - offset) / span);
+    }
+    const bool ok = inr && fabsf(xv - chk) <= Q.feb;
+    code = ok ? (uint32_t)(qi + (int)Q.R) : 0u;
+    r = ok ? cand : xv;
+  };
+  finish(pd0, x0, q0, code0, r0);
+  finish(pd1, x1, q1, code1, r1);
+}
+
 // Decode: reconstruction from a code; code 0 looks the value up among the
 // stage's outliers (sorted by rank), a missing one gives NaN.
 __device__ __forceinline__ float pair_dequant(const PQ &Q, double pd, uint32_t code, bool active,
@@ -1398,26 +1432,28 @@ __device__ __forceinline__ void buf_st16(__amdgpu_buffer_rsrc_t r, uint32_t byte
   __builtin_amdgcn_raw_buffer_store_b16(v, r, on ? byte_off : BUF_OFF_NONE, 0, 0);
 }
 
-// emit() of the pair kernels: the code, one histogram count and the outlier
-// append.  The LDS count is branch-free: a lane whose code is outside the
-// window (or that is inactive) adds 0 to the window's spare word HIST_WIN,
-// and the rare out-of-window codes are counted globally under a wave vote.
-__device__ __forceinline__ void emit_pair(__amdgpu_buffer_rsrc_t codes, uint32_t cbase,
-                                          uint32_t slot, uint32_t code, float x,
-                                          uint32_t *__restrict__ out_rank,
-                                          float *__restrict__ out_val,
-                                          uint32_t *__restrict__ out_count, uint32_t *lhist,
-                                          uint32_t *__restrict__ ghist, uint32_t win_lo,
-                                          bool active) {
+// emit() of the pair kernels in two halves, so that a step votes once for all
+// its points: emit_common stores the code and counts an in-window code in LDS
+// (branch-free: a lane whose code is outside the window, or that is inactive,
+// adds 0 to the window's spare word HIST_WIN) and says whether the point also
+// needs emit_rare (a code outside the window, counted globally, or an
+// outlier, appended).  emit_rare runs under a wave vote.
+__device__ __forceinline__ bool emit_common(__amdgpu_buffer_rsrc_t codes, uint32_t cbase,
+                                            uint32_t slot, uint32_t code, uint32_t *lhist,
+                                            uint32_t win_lo, bool active) {
   buf_st16(codes, cbase + 2u * slot, active, (uint16_t)code);
   const uint32_t b = code - win_lo;
   const bool inwin = code != 0 && b < HIST_WIN;
   lds_add(lhist + (inwin ? b : (uint32_t)HIST_WIN), active && inwin ? 1u : 0u);
-  const bool glob = active && !inwin;
-  if (__any(glob)) {
-    if (glob) gbl_add(ghist + code, 1u);
-  }
-  const bool out = active && code == 0;
+  return active && !inwin;
+}
+__device__ __forceinline__ void emit_rare(bool rare, uint32_t slot, uint32_t code, float x,
+                                          uint32_t *__restrict__ out_rank,
+                                          float *__restrict__ out_val,
+                                          uint32_t *__restrict__ out_count,
+                                          uint32_t *__restrict__ ghist) {
+  if (rare) gbl_add(ghist + code, 1u);
+  const bool out = rare && code == 0;
   const uint64_t m = __ballot(out);
   if (m) {
     const int lane = __lane_id();
@@ -1717,11 +1753,14 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
         buf_st32(rrec, 4u * (f + 1), active && odd_ok, ro);
       }
       if (ENC) {
-        if (PH > 1)
-          emit_pair(rcodes, cb_e, rke, cde, xe, or_e, ov_e, out_count + SE.gs, lh, gh_e, wlo_e,
-                    active);
-        emit_pair(rcodes, cb_o, rko, cdo, xo, or_o, ov_o, out_count + SO.gs, lh + HIST_WIN + 1,
-                  gh_o, wlo_o, active && odd_ok);
+        const bool rare_e =
+            PH > 1 && emit_common(rcodes, cb_e, rke, cde, lh, wlo_e, active);
+        const bool rare_o =
+            emit_common(rcodes, cb_o, rko, cdo, lh + HIST_WIN + 1, wlo_o, active && odd_ok);
+        if (__any(rare_e || rare_o)) {
+          if (PH > 1) emit_rare(rare_e, rke, cde, xe, or_e, ov_e, out_count + SE.gs, gh_e);
+          emit_rare(rare_o, rko, cdo, xo, or_o, ov_o, out_count + SO.gs, gh_o);
+        }
       }
       if (PH > 1) {
         we0 = we1; we1 = we2; we2 = we3;
@@ -1962,8 +2001,8 @@ __device__ __forceinline__ void quad_set(const PArgs &A, const PSet &T,
       if (ENC) {
         xe0 = cur.xx.x;
         xe1 = cur.xx.z;
-        pair_quant<RND>(QE, round_mode, A.span, A.offset, pe0, xe0, cde0, re0);
-        pair_quant<RND>(QE, round_mode, A.span, A.offset, pe1, xe1, cde1, re1);
+        pair_quant2<RND>(QE, round_mode, A.span, A.offset, pe0, xe0, pe1, xe1, cde0, re0, cde1,
+                         re1);
       } else {
         // every thread decodes its even points: its neighbours' odd points need them
         cde0 = cur.ce0;
@@ -2005,8 +2044,8 @@ __device__ __forceinline__ void quad_set(const PArgs &A, const PSet &T,
       if (ENC) {
         xo0 = cur.xx.y;
         xo1 = cur.xx.w;
-        pair_quant<RND>(QO, round_mode, A.span, A.offset, po0, xo0, cdo0, ro0);
-        pair_quant<RND>(QO, round_mode, A.span, A.offset, po1, xo1, cdo1, ro1);
+        pair_quant2<RND>(QO, round_mode, A.span, A.offset, po0, xo0, po1, xo1, cdo0, ro0, cdo1,
+                         ro1);
       } else {
         cdo0 = cur.co0;
         cdo1 = cur.co1;
@@ -2015,14 +2054,17 @@ __device__ __forceinline__ void quad_set(const PArgs &A, const PSet &T,
       }
       buf_st128(rrec, 4u * f, owned, re0, ro0, re1, ro1);
       if (ENC) {
-        emit_pair(rcodes, cb_e, rke, cde0, xe0, or_e, ov_e, out_count + SE.gs, lh, gh_e, wlo_e,
-                  owned);
-        emit_pair(rcodes, cb_e, rke + rs2_e, cde1, xe1, or_e, ov_e, out_count + SE.gs, lh, gh_e,
-                  wlo_e, owned);
-        emit_pair(rcodes, cb_o, rko, cdo0, xo0, or_o, ov_o, out_count + SO.gs,
-                  lh + HIST_WIN + 1, gh_o, wlo_o, owned);
-        emit_pair(rcodes, cb_o, rko + rs2_o, cdo1, xo1, or_o, ov_o, out_count + SO.gs,
-                  lh + HIST_WIN + 1, gh_o, wlo_o, owned);
+        const bool r0 = emit_common(rcodes, cb_e, rke, cde0, lh, wlo_e, owned);
+        const bool r1 = emit_common(rcodes, cb_e, rke + rs2_e, cde1, lh, wlo_e, owned);
+        const bool r2 = emit_common(rcodes, cb_o, rko, cdo0, lh + HIST_WIN + 1, wlo_o, owned);
+        const bool r3 =
+            emit_common(rcodes, cb_o, rko + rs2_o, cdo1, lh + HIST_WIN + 1, wlo_o, owned);
+        if (__any(r0 || r1 || r2 || r3)) {
+          emit_rare(r0, rke, cde0, xe0, or_e, ov_e, out_count + SE.gs, gh_e);
+          emit_rare(r1, rke + rs2_e, cde1, xe1, or_e, ov_e, out_count + SE.gs, gh_e);
+          emit_rare(r2, rko, cdo0, xo0, or_o, ov_o, out_count + SO.gs, gh_o);
+          emit_rare(r3, rko + rs2_o, cdo1, xo1, or_o, ov_o, out_count + SO.gs, gh_o);
+        }
       }
       wa[0] = wa[1]; wa[1] = wa[2]; wa[2] = wa3;
       wb[0] = wb[1]; wb[1] = wb[2]; wb[2] = wb3;
