@@ -121,6 +121,35 @@ def batch_leg(args):
                                   stream_version=5, **kw)
     t_one = time.perf_counter() - t0
     in_k = sum(f.astype(np.float32).nbytes for f in fields[:k])
+    # per-field GNN-vs-interpolation choice (the LATECGNN stream picks GNN or
+    # interpolation per stage) on up to 16 fields of at most 4 M values
+    sel = None
+    if not args.no_gnn:
+        from latec_amd.gnn_codec import GNNCodec, roundtrip_slack
+
+        idx = [i for i, f in enumerate(fields) if f.size <= (4 << 20)][:16]
+        sub = [fields[i] for i in idx]
+        codec = GNNCodec(args.gnn_ckpt)
+        ts = {}
+        t0 = time.perf_counter()
+        mixed = compress_batch(sub, 1e-3, gnn=codec, timings=ts, **kw)
+        t_sel = time.perf_counter() - t0
+        plain_bits = 8 * sum(len(streams[i]) for i in idx)
+        outs2 = decompress_batch(mixed, gnn=codec)
+        for x, y, m in zip(sub, outs2, mixed):
+            x32 = x.astype(np.float32)
+            eb = 1e-3 * (float(x32.max()) - float(x32.min()))
+            slack = (roundtrip_slack(float(x32.min()), float(x32.max()))
+                     if m[:8] == b"LATECGNN" else 0.0)
+            assert float(np.abs(np.asarray(y, np.float64) - x32).max()) <= eb + slack
+        n_sub = sum(f.size for f in sub)
+        sel = {"fields": len(sub), "values": int(n_sub), "gnn_streams": ts["gnn_streams"],
+               "bits_per_value_interp": plain_bits / n_sub,
+               "bits_per_value_selected": 8 * sum(len(m) for m in mixed) / n_sub,
+               "compress_s": t_sel, "gnn_s": ts["gnn_s"]}
+        note(f"GNN selection on {len(sub)} fields: {ts['gnn_streams']} LATECGNN, "
+             f"{sel['bits_per_value_interp']:.4f} -> {sel['bits_per_value_selected']:.4f} "
+             f"bits/value ({t_sel:.1f} s)")
     out = {"workload": f"{len(fields)} fields (half 2-D sides U[64,1024], half 3-D sides "
                        f"U[32,256], x{args.batch_scale:g}), 70% f32 / 30% f64, relative eb 1e-3, "
                        "cubic L4 stride 16, stream version 5",
@@ -131,7 +160,8 @@ def batch_leg(args):
            "decode_kernels_s": td["kernels_s"],
            "bits_per_value": 8 * sum(len(s) for s in streams) / n_values,
            "max_err_over_eb": worst,
-           "one_at_a_time_GBps_first32": in_k / t_one / 1e9}
+           "one_at_a_time_GBps_first32": in_k / t_one / 1e9,
+           "gnn_selection": sel}
     print(f"[bench] batch: {len(fields)} fields, {n_values / 1e6:.1f} M values, compress "
           f"{t_c:.2f} s ({out['compress_GBps']:.2f} GB/s, kernels {tm['kernels_s'] * 1e3:.1f} ms), "
           f"decompress {t_d:.2f} s; one at a time {out['one_at_a_time_GBps_first32']:.2f} GB/s",

@@ -17,6 +17,14 @@ stream_version=...)`` would, but runs the whole batch at once on the GPU:
 ``decompress_batch`` is the mirror: one upload of all payloads, one
 multi-lane decode of all stages, the outlier ranks on the GPU, one graph of
 persistent decode launches.
+
+With ``gnn=`` (a ``GNNCompressorCodec``) a field may also be coded by the
+learned predictor: its LATECGNN stream chooses, stage by stage, the GNN or
+interpolation prediction (whichever codes the stage in fewer bits, signalled
+in the stream's gate list), and the batch keeps, per field, the smaller of
+that stream and the batched interpolation stream.  ``decompress_batch``
+hands LATECGNN streams to the codec's ``uncompress`` and decodes the rest
+together.
 """
 
 from __future__ import annotations
@@ -113,11 +121,15 @@ def _prep(x):
 
 def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cubic",
                    radius=1 << 15, stream_version=VERSION_MULTILANE, zstd_level=9,
-                   use_graph=True, timings=None):
+                   use_graph=True, timings=None, gnn=None, gnn_fields=None):
     """Streams of ``fields`` (list of arrays, any mix of ranks 2..4 and
     shapes; float64 is cast to float32), each at absolute bound
     ``rel_eb * (max - min)`` of its own field, tune 'fast'.  Every stream
-    equals the single-field ``compress`` stream of that field."""
+    equals the single-field ``compress`` stream of that field -- unless
+    ``gnn`` (a ``GNNCompressorCodec``) is given: then each field listed in
+    ``gnn_fields`` (default: all) is also coded by ``gnn.compress(field,
+    rel_eb)`` (per-stage GNN-or-interpolation choice) and the smaller stream
+    is kept."""
     import time
 
     import torch
@@ -183,10 +195,20 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
         payloads = [pack_device(v, ft, zstd_level)[0] for v, ft in zip(views, fits)]
     streams = [write_stream(hd, pl, zstd_level) for hd, pl in zip(heads, payloads)]
     t4 = time.perf_counter()
+    n_gnn = 0
+    if gnn is not None:
+        for f in (range(len(fields)) if gnn_fields is None else gnn_fields):
+            g = gnn.compress(fields[f], float(rel_eb), radius=radius, zstd_level=zstd_level)
+            if len(g) < len(streams[f]):
+                streams[f] = g
+                n_gnn += 1
+    t5 = time.perf_counter()
     if timings is not None:
         timings.update({"setup_s": t1 - t0, "kernels_s": t2 - t1, "fit_s": t3 - t2,
                         "entropy_s": t4 - t3, "fields": len(plans),
                         "values": int(lay.n_points)})
+        if gnn is not None:
+            timings.update({"gnn_s": t5 - t4, "gnn_streams": n_gnn})
     return streams
 
 
@@ -276,9 +298,29 @@ def _pack_v5_all(views, fits, lay, buf, radius):
     return payloads
 
 
-def decompress_batch(streams, *, use_graph=True, timings=None):
+def decompress_batch(streams, *, use_graph=True, timings=None, gnn=None):
     """Arrays of version-5 streams written by ``compress_batch`` or
-    ``compress(..., stream_version=5)`` (float32; uint8 sources rounded)."""
+    ``compress(..., stream_version=5)`` (float32; uint8 sources rounded);
+    LATECGNN streams of a batch coded with ``gnn`` go to ``gnn.uncompress``."""
+    from .gnn_codec import _MAGIC as GNN_MAGIC
+
+    learned = [i for i, st in enumerate(streams) if bytes(st[:len(GNN_MAGIC)]) == GNN_MAGIC]
+    if learned:
+        if gnn is None:
+            raise ValueError("LATECGNN streams in the batch need the gnn codec")
+        rest = [i for i in range(len(streams)) if i not in set(learned)]
+        out = [None] * len(streams)
+        if rest:
+            for i, arr in zip(rest, _decompress_interp([streams[i] for i in rest],
+                                                       use_graph=use_graph, timings=timings)):
+                out[i] = arr
+        for i in learned:
+            out[i] = gnn.uncompress(streams[i])
+        return out
+    return _decompress_interp(streams, use_graph=use_graph, timings=timings)
+
+
+def _decompress_interp(streams, *, use_graph=True, timings=None):
     import time
 
     import torch

@@ -63,3 +63,44 @@ def test_gpu_batch_graph_and_direct_launches_agree():
     a = compress_batch(fields, 1e-3, levels=3, anchor_stride=8, use_graph=True)
     b = compress_batch(fields, 1e-3, levels=3, anchor_stride=8, use_graph=False)
     assert a == b
+
+
+@pytest.mark.gpu
+def test_gpu_batch_gnn_selection_keeps_bound_and_smaller_stream():
+    """Per-field GNN-vs-interpolation choice in the batch: every kept stream is
+    the smaller of the field's interpolation stream and its LATECGNN stream
+    (whose stages each pick GNN or interpolation), and the batch decodes
+    within the bound."""
+    import os
+
+    from latec_amd.batch import compress_batch, decompress_batch
+    from latec_amd.gnn_codec import GNNCodec, roundtrip_slack
+
+    ckpt = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "latec_amd", "checkpoints", "gnn_d64_agg2.pt")
+    codec = GNNCodec(ckpt)
+    rng = np.random.RandomState(12)
+    fields = []
+    for i, shape in enumerate([(96, 80), (24, 26, 28), (12, 12, 12, 12), (130, 64)]):
+        x = np.cumsum(np.cumsum(rng.randn(*shape), 0), -1)
+        if i == 2:  # a discontinuity: the learned predictor's kind of field
+            x[..., shape[-1] // 2:] += 5.0 * x.std()
+        x += 0.01 * rng.randn(*shape)
+        fields.append(((x - x.min()) / (x.max() - x.min())).astype(np.float32))
+    print("case batch gnn selection", flush=True)
+    plain = compress_batch(fields, 1e-3)
+    timings = {}
+    mixed = compress_batch(fields, 1e-3, gnn=codec, timings=timings)
+    assert timings["gnn_streams"] == sum(m != p for m, p in zip(mixed, plain))
+    for f, (m, p) in enumerate(zip(mixed, plain)):
+        if m != p:
+            assert m[:8] == b"LATECGNN" and len(m) < len(p)
+            assert m == codec.compress(fields[f], 1e-3)
+    out = decompress_batch(mixed, gnn=codec)
+    for x, y, m in zip(fields, out, mixed):
+        eb = 1e-3 * float(x.max() - x.min())
+        # a LATECGNN stream holds the bound up to its float32 normalization slack
+        slack = roundtrip_slack(float(x.min()), float(x.max())) if m[:8] == b"LATECGNN" else 0.0
+        assert np.abs(np.asarray(y, np.float64) - x).max() <= eb + slack
+    with pytest.raises(ValueError):
+        decompress_batch(mixed if timings["gnn_streams"] else [b"LATECGNN" + mixed[0][8:]])
