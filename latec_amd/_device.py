@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import threading
+
 import numpy as np
 
 _TORCH = None
@@ -40,12 +42,57 @@ def to_host(x, dtype=None) -> np.ndarray:
     return a.view(dtype) if dtype is not None else a
 
 
-def empty(n, dtype):
+_STAGING = threading.local()
+
+
+def staging(n, dtype):
+    """(pinned host tensor, numpy view) of ``n`` elements of numpy ``dtype``:
+    host memory that copies to and from HBM at the link's pinned rate.  One
+    buffer per thread and torch dtype, grown to the largest request; the
+    views stay valid until this thread's next request of the same dtype."""
+    t = torch()
+    bufs = getattr(_STAGING, "bufs", None)
+    if bufs is None:
+        bufs = _STAGING.bufs = {}
+    td = _tdtype(dtype)
+    buf = bufs.get(td)
+    if buf is None or buf.numel() < n:
+        buf = bufs[td] = None
+        buf = bufs[td] = t.empty(max(int(n), 1), dtype=td, pin_memory=True)
+    view = buf[:int(n)]
+    return view, view.numpy().view(dtype)
+
+
+def to_host_staged(x, dtype=None) -> np.ndarray:
+    """Device tensor -> numpy through the pinned staging buffer of x's dtype
+    (a synchronous copy); the array is valid as ``staging`` says."""
+    view, a = staging(x.numel(), _NPDTYPE[x.dtype])
+    view.copy_(x.detach().reshape(-1))
+    return a.view(dtype) if dtype is not None else a
+
+
+def _tdtype(dtype):
     t = torch()
     tmap = {np.float32: t.float32, np.float64: t.float64, np.uint16: t.int16,
             np.int16: t.int16, np.uint32: t.int32, np.int32: t.int32, np.uint8: t.uint8,
             np.int64: t.int64}
-    return t.empty(int(n), dtype=tmap[np.dtype(dtype).type], device=device())
+    return tmap[np.dtype(dtype).type]
+
+
+class _NpDtypes(dict):
+    def __missing__(self, td):
+        t = torch()
+        m = {t.float32: np.float32, t.float64: np.float64, t.int16: np.int16,
+             t.int32: np.int32, t.uint8: np.uint8, t.int64: np.int64}
+        self.update(m)
+        return m[td]
+
+
+_NPDTYPE = _NpDtypes()
+
+
+def empty(n, dtype):
+    return torch().empty(int(n), dtype=_tdtype(dtype), device=device())
 
 
 def zeros(n, dtype):

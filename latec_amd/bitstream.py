@@ -235,6 +235,33 @@ def unpack_record(buf: bytes, off: int, n_expect: int, radius: int):
     return codes[:n_codes], rank[:k].copy(), val[:k].copy(), off + used
 
 
+def unpack_record_into(arr: np.ndarray, off: int, n_expect: int, radius: int, codes: np.ndarray,
+                       rank: np.ndarray, val: np.ndarray) -> int:
+    """unpack_record writing into caller arrays: ``codes`` holds n_expect
+    codes, ``rank``/``val`` exactly the record's outlier count (``arr`` is the
+    payload as uint8).  Returns the next offset."""
+    n_codes, hlen, kexp, shape, level, owx, literal, length = parse_record(arr, off)
+    if not kexp:
+        raise ValueError("rANS stage record has no coder window")
+    if n_codes != n_expect:
+        raise ValueError("stage code count mismatch")
+    n_out = (length - 16 - hlen) // 4
+    if len(codes) < n_codes or len(rank) != n_out or len(val) != n_out:
+        raise ValueError("unpack buffers do not match the record")
+    cb = 2 if radius <= 32768 else 4
+    if codes.itemsize != cb:
+        raise ValueError("code buffer width does not match the radius")
+    row = R.cdf_row(kexp, shape, owx, level) if (n_codes and not literal) else \
+        np.zeros(5, np.uint32)
+    got = ctypes.c_int64()
+    used = check(lib().latec_unpack_stage(ptr(arr[off:]), len(arr) - off, int(radius), ptr(row),
+                                          n_expect, ptr(codes), cb, ptr(rank), ptr(val), n_out,
+                                          ctypes.byref(got)))
+    if got.value != n_out or used != length:
+        raise ValueError("stage record length mismatch")
+    return off + used
+
+
 def unpack_stage(buf, off, *, rans_levels=None, rans_tables=None, radius=DEFAULT_RADIUS):
     """bitstream.py:288-344: (codes u32, outliers f32, next offset)."""
     n_codes, hlen, kexp, shape, level, owx, literal, _ = parse_record(buf, off)

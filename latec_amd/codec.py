@@ -24,7 +24,7 @@ from . import rans as R
 from ._lib import Fit, check, lib, ptr
 from .bitstream import (DTYPE_CODES, DTYPE_IDS, END_MODE_SHIFT, FLAG_CUBIC, FLAG_INTERP,
                         FLAG_RANS, VERSION, VERSION_MULTILANE, Header, pack_record,
-                        pack_stage, parse_record, read_stream, unpack_record, unpack_stage,
+                        pack_stage, parse_record, read_stream, unpack_record_into, unpack_stage,
                         write_stream)
 from .levels import mask_tiles, stage_ebs, stage_masks
 from .predictor import (END_MODE_MAX, InterpPredictor, default_interp_center,
@@ -112,7 +112,7 @@ def pack_device(res: GpuEncodeResult, fits, zstd_level: int, threads: int = _THR
     plan, o = res.plan, res.opts
     C, ns = plan.channels, plan.n_stages
     cb = o.code_bytes
-    codes = D.to_host(res.d_codes, np.uint16 if cb == 2 else np.uint32)
+    codes = D.to_host_staged(res.d_codes, np.uint16 if cb == 2 else np.uint32)
     cnt = D.to_host(res.d_cnt, np.uint32)
     code_off = np.array([C * plan.offsets[s] for s in range(ns)], np.int64)
     n_codes = np.array([C * plan.counts[s] for s in range(ns)], np.int64)
@@ -153,35 +153,42 @@ def decode_device(payload: bytes, plan, ebs, o, threads: int = _THREADS):
     """Parse and entropy-decode every stage record (host threads), then run
     the fused decode kernels; returns the device reconstruction."""
     C, ns = plan.channels, plan.n_stages
-    offs, off = [], 0
+    arr = np.frombuffer(payload, np.uint8)
+    offs, n_out, off = [], [], 0
     for s in range(ns):
         offs.append(off)
-        off += parse_record(payload, off)[7]
+        _, hlen, _, _, _, _, _, length = parse_record(arr, off)
+        n_out.append((length - 16 - hlen) // 4)
+        off += length
     if off != len(payload):
         raise ValueError("trailing bytes in LATEC payload")
     cb = o.code_bytes
-    codes = np.empty(max(C * plan.npoints, 1), np.uint16 if cb == 2 else np.uint32)
-    rank = np.empty(max(C * plan.npoints, 1), np.uint32)
-    val = np.empty(max(C * plan.npoints, 1), np.float32)
-    cnt = np.zeros(ns, np.uint32)
+    n_all = C * plan.npoints
+    # codes land in pinned memory (one H2D copy at the pinned rate); the
+    # outlier ranks and values are packed per stage and copied stage by stage
+    t_codes, codes = D.staging(n_all, np.uint16 if cb == 2 else np.uint32)
+    out_at = np.concatenate([[0], np.cumsum(n_out)]).astype(np.int64)
+    rank = np.empty(max(int(out_at[-1]), 1), np.uint32)
+    val = np.empty(max(int(out_at[-1]), 1), np.float32)
 
     def one(s):
-        n = C * plan.counts[s]
-        c, r, v, _ = unpack_record(payload, offs[s], n, o.radius)
-        lo = C * plan.offsets[s]
-        codes[lo:lo + n] = c
-        rank[lo:lo + len(r)] = r
-        val[lo:lo + len(v)] = v
-        cnt[s] = len(r)
+        lo, a, b = C * plan.offsets[s], int(out_at[s]), int(out_at[s + 1])
+        unpack_record_into(arr, offs[s], C * plan.counts[s], o.radius, codes[lo:],
+                           rank[a:b], val[a:b])
 
     order = sorted(range(ns), key=lambda s: -plan.counts[s])
     with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
         list(ex.map(one, order))
     ws = workspace(plan, o.radius)
-    ws.codes.copy_(D.to_device(codes[:C * plan.npoints]))
-    ws.rank.copy_(D.to_device(rank[:C * plan.npoints]))
-    ws.val.copy_(D.to_device(val[:C * plan.npoints]))
-    ws.cnt.copy_(D.to_device(cnt))
+    ws.codes[:n_all].copy_(t_codes)
+    t_rank, t_val = D.torch().from_numpy(rank.view(np.int32)), D.torch().from_numpy(val)
+    for s in range(ns):
+        a, b = int(out_at[s]), int(out_at[s + 1])
+        if b > a:
+            lo = C * plan.offsets[s]
+            ws.rank[lo:lo + b - a].copy_(t_rank[a:b])
+            ws.val[lo:lo + b - a].copy_(t_val[a:b])
+    ws.cnt.copy_(D.to_device(np.asarray(n_out, np.uint32)))
     ws.recon.zero_()
     ebs_arr = np.ascontiguousarray(ebs, np.float64)
     check(lib().latec_interp_decode(plan.handle, 0, ns, ptr(ws.codes), ptr(ws.rank),

@@ -15,6 +15,7 @@
 // Stage record and literal back end: latec/bitstream.py:178-344.
 #include <stdint.h>
 #include <string.h>
+#define ZSTD_STATIC_LINKING_ONLY  // ZSTD_compressBegin_advanced / ZSTD_compressContinue
 #include <zstd.h>
 
 #include <algorithm>
@@ -33,11 +34,18 @@ struct SymInfo {
   uint64_t rcp;  // floor(2^64 / prob), 0 for prob == 1
 };
 
+// Decoder lookup: one 64-bit entry per 2^12-slot bucket of the 2^24 quantile
+// range holds the symbol at the bucket's first slot as left | prob << 24 |
+// k << 48 (left, prob < 2^24, k < 2^16), so a quantile inside that symbol
+// (every quantile of a bucket with no symbol boundary, >= 94% of buckets for
+// A <= 256) costs one 32 KiB-table load; the rest step forward through left[].
+static const int DLUT_SHIFT = 12;
+
 struct Model {
   int64_t A = 0;
   std::vector<uint32_t> left;  // A + 1 entries
   std::vector<SymInfo> sym;
-  std::vector<uint32_t> lut;   // quantile >> 8 -> first candidate symbol
+  std::vector<uint64_t> dlut;  // 2^(PREC - DLUT_SHIFT) entries
 };
 
 static void build_model(const uint32_t *cdf, int64_t A, Model &m, bool for_decode) {
@@ -60,12 +68,13 @@ static void build_model(const uint32_t *cdf, int64_t A, Model &m, bool for_decod
     s.rcp = s.prob > 1 ? (uint64_t)((((unsigned __int128)1) << 64) / s.prob) : 0;
   }
   if (for_decode) {
-    m.lut.assign((size_t)1 << (PREC - 8), 0);
+    m.dlut.assign((size_t)1 << (PREC - DLUT_SHIFT), 0);
     int64_t k = 0;
-    for (size_t b = 0; b < m.lut.size(); b++) {
-      const uint32_t q = (uint32_t)(b << 8);
+    for (size_t b = 0; b < m.dlut.size(); b++) {
+      const uint32_t q = (uint32_t)(b << DLUT_SHIFT);
       while (k + 1 < A && m.left[k + 1] <= q) k++;
-      m.lut[b] = (uint32_t)k;
+      m.dlut[b] = (uint64_t)m.sym[k].left | ((uint64_t)m.sym[k].prob << 24) |
+                  ((uint64_t)k << 48);
     }
   }
 }
@@ -105,10 +114,16 @@ static inline int64_t enc_cap(int64_t n) { return n + 2; }
 static inline uint32_t dec_get(uint64_t &state, const Model &m, const uint32_t *words,
                                int64_t &top) {
   const uint32_t qt = (uint32_t)(state & ((1u << PREC) - 1));
-  uint32_t k = m.lut[qt >> 8];
-  while (m.left[k + 1] <= qt) k++;
-  const SymInfo &s = m.sym[k];
-  state = (state >> PREC) * s.prob + (qt - s.left);
+  const uint64_t e = m.dlut[qt >> DLUT_SHIFT];
+  uint32_t left = (uint32_t)e & 0xffffffu, prob = (uint32_t)(e >> 24) & 0xffffffu;
+  uint32_t k = (uint32_t)(e >> 48);
+  if (qt - left >= prob) {  // past the bucket's first symbol
+    do k++;
+    while (m.left[k + 1] <= qt);
+    left = m.sym[k].left;
+    prob = m.sym[k].prob;
+  }
+  state = (state >> PREC) * prob + (qt - left);
   if (state < ((uint64_t)1 << 32) && top > 0) state = (state << 32) | words[--top];
   return k;
 }
@@ -251,6 +266,44 @@ static const int64_t LITERAL_MIN = 1 << 12;  // bitstream.py:172
 static const double SCOUT_SLACK = 0.85;      // bitstream.py:175
 static const int64_t PARALLEL_RANS_MIN = 1 << 20;  // stages coded beside their literal scout
 
+// One zstd frame of src at `level`, fed to the compressor in runs of whole
+// 128 KiB blocks over the one contiguous source buffer, with the parameters
+// ZSTD_compress derives from (level, n) and the content size pledged: the
+// frame is ZSTD_compress's one-shot frame byte for byte (same block
+// boundaries, same window), but the run can stop between runs of blocks.
+// Returns the frame size; ZB_OVER once `over(bytes written so far)` holds
+// (the final frame is at least that long, so whatever `over` rules out is
+// ruled out for the whole frame), ZB_STOPPED when *stop is raised, ZB_ERROR
+// on a zstd error.
+static const int64_t ZB_OVER = -1, ZB_STOPPED = -2, ZB_ERROR = -3;
+static const size_t ZB_RUN = (size_t)1 << 20;  // eight blocks per call
+
+template <class Over>
+static int64_t zstd_bounded(const uint8_t *src, size_t n, int level, uint8_t *dst, size_t cap,
+                            Over over, const std::atomic<bool> *stop) {
+  ZSTD_CCtx *c = ZSTD_createCCtx();
+  if (!c) return ZB_ERROR;
+  struct Free {
+    ZSTD_CCtx *c;
+    ~Free() { ZSTD_freeCCtx(c); }
+  } free_c{c};
+  if (ZSTD_isError(ZSTD_compressBegin_advanced(c, nullptr, 0, ZSTD_getParams(level, n, 0), n)))
+    return ZB_ERROR;
+  size_t at = 0, off = 0;
+  while (true) {
+    if (stop && stop->load(std::memory_order_relaxed)) return ZB_STOPPED;
+    const size_t k = std::min(n - off, ZB_RUN);
+    const bool last = off + k == n;
+    const size_t w = last ? ZSTD_compressEnd(c, dst + at, cap - at, src + off, k)
+                          : ZSTD_compressContinue(c, dst + at, cap - at, src + off, k);
+    if (ZSTD_isError(w)) return ZB_ERROR;
+    at += w;
+    off += k;
+    if (over(at)) return ZB_OVER;
+    if (last) return (int64_t)at;
+  }
+}
+
 int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int64_t n_out,
                    uint32_t R, int zstd_level, const latec_fit *fit, const uint32_t *cdf_row,
                    uint8_t *out, int64_t cap) {
@@ -269,7 +322,11 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
   const int64_t blob_cap = cap - head - tail;
   int64_t hlen = -1;
   int literal = 0;
+  const bool wide = kexp > 7;
+  const bool big = n >= LITERAL_MIN && n >= PARALLEL_RANS_MIN;
   std::vector<uint32_t> syms((size_t)n);
+  std::vector<uint8_t> lit;
+  if (n >= LITERAL_MIN) lit.resize((size_t)n * (wide ? 2 : 1));
   for (int64_t i = 0; i < n; i++) {
     const uint32_t s = narrow_sym(code_at(codes, cb, i), R, kexp);
     if (s >= (uint64_t)A) {
@@ -278,32 +335,65 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
     }
     syms[i] = s;
   }
+  if (n >= LITERAL_MIN) {
+    if (wide)
+      for (int64_t i = 0; i < n; i++) ((uint16_t *)lit.data())[i] = (uint16_t)syms[i];
+    else
+      for (int64_t i = 0; i < n; i++) lit[i] = (uint8_t)syms[i];
+  }
   // A large stage codes its rANS words on a second thread while the literal
   // scout (and maybe the full literal) runs: the record is the same, the
   // stage's critical path is the longer of the two instead of their sum.
   std::vector<uint32_t> words;
   int64_t nw = -1;
   std::thread rans_thread;
-  if (n >= LITERAL_MIN && n >= PARALLEL_RANS_MIN && fit->level >= 0 && fit->level < 64)
+  if (big && fit->level >= 0 && fit->level < 64)
     rans_thread = std::thread([&] {
       words.resize((size_t)enc_cap(n));
       nw = rans_encode(syms.data(), n, cdf_row, A, words.data(), enc_cap(n));
     });
+  // A large stage also runs the full-level literal frame beside its scout
+  // from the start: the scout stops as soon as its output rules the literal
+  // out (scout * slack >= bits / 8) and then stops the full run; the full run
+  // stops as soon as its own output does (full * 8 >= bits).  Both are the
+  // one-shot frames when they finish, so the record is the one the serial
+  // scout-then-full order writes.
+  std::vector<uint8_t> full_buf;
+  int64_t full_len = ZB_STOPPED;
+  std::atomic<bool> stop_full(false);
+  std::thread full_thread;
+  const double bits = (double)fit->bits;
+  auto full_over = [bits](size_t at) { return (double)at * 8.0 >= bits; };
+  if (big) {
+    full_buf.resize(ZSTD_compressBound(lit.size()));
+    full_thread = std::thread([&] {
+      full_len = zstd_bounded(lit.data(), lit.size(), zstd_level, full_buf.data(),
+                              full_buf.size(), full_over, &stop_full);
+    });
+  }
   struct Join {
     std::thread &t;
     ~Join() {
       if (t.joinable()) t.join();
     }
-  } join{rans_thread};
-  if (n >= LITERAL_MIN) {
-    const bool wide = kexp > 7;
-    std::vector<uint8_t> lit((size_t)n * (wide ? 2 : 1));
-    for (int64_t i = 0; i < n; i++) {
-      if (wide)
-        ((uint16_t *)lit.data())[i] = (uint16_t)syms[i];
-      else
-        lit[i] = (uint8_t)syms[i];
+  } join{rans_thread}, join_full{full_thread};
+  if (big) {
+    std::vector<uint8_t> tmp(ZSTD_compressBound(lit.size()));
+    const int64_t scout = zstd_bounded(
+        lit.data(), lit.size(), 1, tmp.data(), tmp.size(),
+        [bits](size_t at) { return (double)at * SCOUT_SLACK >= bits / 8.0; }, nullptr);
+    if (scout < 0) stop_full.store(true, std::memory_order_relaxed);
+    full_thread.join();
+    if (scout >= 0 && full_len >= 0) {
+      if (full_len > blob_cap) {
+        set_error("record capacity");
+        return LATEC_E_CAPACITY;
+      }
+      memcpy(blob, full_buf.data(), (size_t)full_len);
+      hlen = full_len;
+      literal = 1;
     }
+  } else if (n >= LITERAL_MIN) {
     std::vector<uint8_t> tmp(ZSTD_compressBound(lit.size()));
     const size_t scout = ZSTD_compress(tmp.data(), tmp.size(), lit.data(), lit.size(), 1);
     if (!ZSTD_isError(scout) && (double)scout * SCOUT_SLACK < (double)fit->bits / 8.0) {

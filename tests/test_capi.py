@@ -90,3 +90,42 @@ def test_unpack_stage_of_oracle_records():
         assert np.array_equal(c.astype(np.uint32), codes)
         assert np.array_equal(rk[:no.value], np.flatnonzero(codes == 0))
         assert np.array_equal(vl[:no.value], outl)
+
+
+@pytest.mark.parametrize("kind", ["runs", "iid", "wide"])
+def test_pack_stages_large_stage_equals_oracle_record(kind):
+    """A stage above a million codes takes the concurrent path (rANS, the
+    literal scout and the full-level literal frame side by side, each zstd
+    frame fed in runs of blocks so it can stop early); its record must be
+    the oracle's serial scout-then-full record byte for byte."""
+    import ctypes
+
+    R = 1 << 15
+    n = 1_200_000
+    rng = np.random.RandomState(5)
+    if kind == "runs":  # literal back end wins
+        codes = np.repeat(np.arange(n // 400) % 7 + R - 3, 400).astype(np.uint32)
+    elif kind == "iid":  # scout passes, the full frame loses to rANS
+        codes = (np.rint(rng.laplace(0, 3.0, n)) + R).astype(np.uint32)
+    else:  # 16-bit literal symbols
+        codes = (np.rint(rng.laplace(0, 60.0, n)) + R).astype(np.uint32)
+    codes[::997] = 0
+    outl = rng.randn(int((codes == 0).sum())).astype(np.float32)
+    want = O.pack_stage(codes, outl)
+    kexp = O.choose_kexp(codes, R)
+    level, shape, owx, bits = O.choose_const_level(codes, R, kexp)
+    fit = (_lib.Fit * 1)()
+    fit[0].kexp, fit[0].level, fit[0].shape, fit[0].owx, fit[0].bits = kexp, level, shape, owx, bits
+    row = np.ascontiguousarray(O.build_cdfs(kexp, shape, 24, owx)[level], np.uint32)
+    c16 = codes.astype(np.uint16)
+    cap = 64 + 8 * n + 4 * len(outl) + 1024
+    out = np.empty(cap, np.uint8)
+    rec_len = np.zeros(1, np.int64)
+    z = lambda v: np.array([v], np.int64)
+    size = _lib.check(_lib.lib().latec_pack_stages(
+        1, _lib.ptr(c16), 2, _lib.ptr(z(0)), _lib.ptr(z(n)), _lib.ptr(outl), _lib.ptr(z(0)),
+        _lib.ptr(z(len(outl))), R, 9, ctypes.cast(fit, ctypes.c_void_p), _lib.ptr(row),
+        _lib.ptr(z(0)), _lib.ptr(out), cap, _lib.ptr(rec_len), 4))
+    assert out[:size].tobytes() == want
+    if kind == "runs":
+        assert struct.unpack_from("<IQ", want, 0)[1] >> 63 == 1
