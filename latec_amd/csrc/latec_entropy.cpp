@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -324,53 +325,74 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
   int literal = 0;
   const bool wide = kexp > 7;
   const bool big = n >= LITERAL_MIN && n >= PARALLEL_RANS_MIN;
-  std::vector<uint32_t> syms((size_t)n);
-  std::vector<uint8_t> lit;
-  if (n >= LITERAL_MIN) lit.resize((size_t)n * (wide ? 2 : 1));
-  for (int64_t i = 0; i < n; i++) {
-    const uint32_t s = narrow_sym(code_at(codes, cb, i), R, kexp);
-    if (s >= (uint64_t)A) {
-      set_error("code outside the narrowed rANS alphabet");
-      return LATEC_E_ARG;
+  const size_t lit_bytes = n >= LITERAL_MIN ? (size_t)n * (wide ? 2 : 1) : 0;
+  // buffers without value-initialisation: every byte used is written first
+  std::unique_ptr<uint32_t[]> syms(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
+  std::unique_ptr<uint8_t[]> lit(new uint8_t[std::max<size_t>(lit_bytes, 1)]);
+  // narrow [lo, hi) into syms (and the literal bytes); false on a code outside
+  // the window
+  auto narrow = [&](int64_t lo, int64_t hi) {
+    bool ok = true;
+    for (int64_t i = lo; i < hi; i++) {
+      const uint32_t s = narrow_sym(code_at(codes, cb, i), R, kexp);
+      ok &= s < (uint64_t)A;
+      syms[i] = s;
+      if (lit_bytes) {
+        if (wide)
+          ((uint16_t *)lit.get())[i] = (uint16_t)s;
+        else
+          lit[i] = (uint8_t)s;
+      }
     }
-    syms[i] = s;
+    return ok;
+  };
+  bool in_window = true;
+  if (big) {  // four threads over four runs
+    const int T = 4;
+    bool oks[T];
+    std::thread nt[T - 1];
+    for (int t = 1; t < T; t++)
+      nt[t - 1] = std::thread([&, t] { oks[t] = narrow(n * t / T, n * (t + 1) / T); });
+    oks[0] = narrow(0, n / T);
+    for (auto &t : nt) t.join();
+    for (int t = 0; t < T; t++) in_window &= oks[t];
+  } else {
+    in_window = narrow(0, n);
   }
-  if (n >= LITERAL_MIN) {
-    if (wide)
-      for (int64_t i = 0; i < n; i++) ((uint16_t *)lit.data())[i] = (uint16_t)syms[i];
-    else
-      for (int64_t i = 0; i < n; i++) lit[i] = (uint8_t)syms[i];
+  if (!in_window) {
+    set_error("code outside the narrowed rANS alphabet");
+    return LATEC_E_ARG;
   }
-  // A large stage codes its rANS words on a second thread while the literal
-  // scout (and maybe the full literal) runs: the record is the same, the
-  // stage's critical path is the longer of the two instead of their sum.
-  std::vector<uint32_t> words;
-  int64_t nw = -1;
-  std::thread rans_thread;
-  if (big && fit->level >= 0 && fit->level < 64)
-    rans_thread = std::thread([&] {
-      words.resize((size_t)enc_cap(n));
-      nw = rans_encode(syms.data(), n, cdf_row, A, words.data(), enc_cap(n));
-    });
   // A large stage also runs the full-level literal frame beside its scout
   // from the start: the scout stops as soon as its output rules the literal
   // out (scout * slack >= bits / 8) and then stops the full run; the full run
   // stops as soon as its own output does (full * 8 >= bits).  Both are the
   // one-shot frames when they finish, so the record is the one the serial
   // scout-then-full order writes.
-  std::vector<uint8_t> full_buf;
+  std::unique_ptr<uint8_t[]> full_buf;
   int64_t full_len = ZB_STOPPED;
   std::atomic<bool> stop_full(false);
   std::thread full_thread;
   const double bits = (double)fit->bits;
   auto full_over = [bits](size_t at) { return (double)at * 8.0 >= bits; };
-  if (big) {
-    full_buf.resize(ZSTD_compressBound(lit.size()));
+  if (big)
     full_thread = std::thread([&] {
-      full_len = zstd_bounded(lit.data(), lit.size(), zstd_level, full_buf.data(),
-                              full_buf.size(), full_over, &stop_full);
+      const size_t bound = ZSTD_compressBound(lit_bytes);
+      full_buf.reset(new uint8_t[bound]);
+      full_len = zstd_bounded(lit.get(), lit_bytes, zstd_level, full_buf.get(), bound, full_over,
+                              &stop_full);
     });
-  }
+  // A large stage codes its rANS words on a further thread while the literal
+  // frames run: the record is the same, the stage's critical path is the
+  // longest of the three instead of their sum.
+  std::unique_ptr<uint32_t[]> words;
+  int64_t nw = -1;
+  std::thread rans_thread;
+  if (big && fit->level >= 0 && fit->level < 64)
+    rans_thread = std::thread([&] {
+      words.reset(new uint32_t[(size_t)enc_cap(n)]);
+      nw = rans_encode(syms.get(), n, cdf_row, A, words.get(), enc_cap(n));
+    });
   struct Join {
     std::thread &t;
     ~Join() {
@@ -378,9 +400,10 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
     }
   } join{rans_thread}, join_full{full_thread};
   if (big) {
-    std::vector<uint8_t> tmp(ZSTD_compressBound(lit.size()));
+    const size_t bound = ZSTD_compressBound(lit_bytes);
+    std::unique_ptr<uint8_t[]> tmp(new uint8_t[bound]);
     const int64_t scout = zstd_bounded(
-        lit.data(), lit.size(), 1, tmp.data(), tmp.size(),
+        lit.get(), lit_bytes, 1, tmp.get(), bound,
         [bits](size_t at) { return (double)at * SCOUT_SLACK >= bits / 8.0; }, nullptr);
     if (scout < 0) stop_full.store(true, std::memory_order_relaxed);
     full_thread.join();
@@ -389,16 +412,16 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
         set_error("record capacity");
         return LATEC_E_CAPACITY;
       }
-      memcpy(blob, full_buf.data(), (size_t)full_len);
+      memcpy(blob, full_buf.get(), (size_t)full_len);
       hlen = full_len;
       literal = 1;
     }
   } else if (n >= LITERAL_MIN) {
-    std::vector<uint8_t> tmp(ZSTD_compressBound(lit.size()));
-    const size_t scout = ZSTD_compress(tmp.data(), tmp.size(), lit.data(), lit.size(), 1);
+    std::vector<uint8_t> tmp(ZSTD_compressBound(lit_bytes));
+    const size_t scout = ZSTD_compress(tmp.data(), tmp.size(), lit.get(), lit_bytes, 1);
     if (!ZSTD_isError(scout) && (double)scout * SCOUT_SLACK < (double)fit->bits / 8.0) {
       const size_t full =
-          ZSTD_compress(tmp.data(), tmp.size(), lit.data(), lit.size(), zstd_level);
+          ZSTD_compress(tmp.data(), tmp.size(), lit.get(), lit_bytes, zstd_level);
       if (!ZSTD_isError(full) && (double)full * 8.0 < (double)fit->bits) {
         if ((int64_t)full > blob_cap) {
           set_error("record capacity");
@@ -418,15 +441,15 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
     }
     if (rans_thread.joinable()) rans_thread.join();
     if (nw < 0) {  // not coded yet, or failed on the helper thread: code (or fail) here
-      words.resize((size_t)enc_cap(n));
-      nw = rans_encode(syms.data(), n, cdf_row, A, words.data(), enc_cap(n));
+      words.reset(new uint32_t[(size_t)enc_cap(n)]);
+      nw = rans_encode(syms.get(), n, cdf_row, A, words.get(), enc_cap(n));
     }
     if (nw < 0) return nw;
     if (nw * 4 > blob_cap) {
       set_error("record capacity");
       return LATEC_E_CAPACITY;
     }
-    memcpy(blob, words.data(), (size_t)nw * 4);
+    memcpy(blob, words.get(), (size_t)nw * 4);
     hlen = nw * 4;
   }
   const uint64_t framed = (uint64_t)hlen | ((uint64_t)kexp << 45) |
@@ -482,18 +505,18 @@ int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t
   }
   const uint8_t *blob = buf + 12;
   const int64_t A = (int64_t)2 << kexp;
-  std::vector<uint32_t> syms((size_t)n);
+  std::unique_ptr<uint32_t[]> syms(new uint32_t[std::max<uint32_t>(n, 1)]);
   if (literal) {
     const bool wide = kexp > 7;
     const size_t sb = (size_t)n * (wide ? 2 : 1);
-    std::vector<uint8_t> lit(sb ? sb : 1);
-    const size_t got = ZSTD_decompress(lit.data(), sb, blob, (size_t)hlen);
+    std::unique_ptr<uint8_t[]> lit(new uint8_t[sb ? sb : 1]);
+    const size_t got = ZSTD_decompress(lit.get(), sb, blob, (size_t)hlen);
     if (ZSTD_isError(got) || got < sb) {
       set_error("bad literal stage frame");
       return LATEC_E_FORMAT;
     }
     for (uint32_t i = 0; i < n; i++)
-      syms[i] = wide ? ((const uint16_t *)lit.data())[i] : lit[i];
+      syms[i] = wide ? ((const uint16_t *)lit.get())[i] : lit[i];
   } else if (n == 0) {
     if (hlen) {
       set_error("non-empty rANS blob for an empty stage");
@@ -504,9 +527,9 @@ int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t
       set_error("rANS payload is not aligned to 32-bit words");
       return LATEC_E_FORMAT;
     }
-    std::vector<uint32_t> words((size_t)hlen / 4);
-    memcpy(words.data(), blob, (size_t)hlen);
-    rc = rans_decode(words.data(), hlen / 4, cdf_row, A, syms.data(), n);
+    std::unique_ptr<uint32_t[]> words(new uint32_t[(size_t)hlen / 4]);
+    memcpy(words.get(), blob, (size_t)hlen);
+    rc = rans_decode(words.get(), hlen / 4, cdf_row, A, syms.get(), n);
     if (rc) return rc;
   }
   uint32_t no;
