@@ -317,8 +317,8 @@ def main():
     def compress_once():
         res = codec.encode_device(xd, plan, ebs, o, graph=True, ws=ws)
         fits = codec.fit_device(res)
-        payload, _, n_out = codec.pack_device(res, fits, 9)
-        return write_stream(header, payload, 9), int(n_out.sum())
+        payload, _, n_out, frame = codec.pack_device_framed(res, fits, 9)
+        return write_stream(header, payload, 9, frame=frame), int(n_out.sum())
 
     def decompress_once(stream):
         h, payload = read_stream(stream)

@@ -257,6 +257,23 @@ int64_t latec_pack_stages(int n_stages, const void *codes, int code_bytes,
                           uint8_t *out, int64_t cap, int64_t *rec_len,
                           int threads);
 
+/* latec_pack_stages plus the outer zstd frame of the payload at frame_level
+ * (1..22), i.e. the body write_stream adds after the header
+ * (latec/bitstream.py:155-157: ZstdCompressor(level).compress(payload)).
+ * frame_cap >= latec_zstd_bound(cap); *frame_len receives the frame size.
+ * The frame is started once every stage has its final record or its rANS
+ * record, and redone if a large stage's literal frame wins: the outputs
+ * equal latec_pack_stages followed by latec_zstd_compress. */
+int64_t latec_pack_stages_framed(int n_stages, const void *codes, int code_bytes,
+                                 const int64_t *code_off, const int64_t *n_codes,
+                                 const float *outl, const int64_t *out_off,
+                                 const int64_t *n_out, uint32_t radius,
+                                 int zstd_level, const latec_fit *fits,
+                                 const uint32_t *cdf_rows, const int64_t *row_off,
+                                 uint8_t *out, int64_t cap, int64_t *rec_len,
+                                 int threads, int frame_level, uint8_t *frame,
+                                 int64_t frame_cap, int64_t *frame_len);
+
 /* zstd one-shot frames (the reference's zstandard.ZstdCompressor(level)
  * .compress / ZstdDecompressor().decompress). */
 int64_t latec_zstd_bound(int64_t n);

@@ -98,6 +98,8 @@ SIGNATURES = {
     "latec_stage_header": (i32, [p, i64, p, p, p, p, p, p, p]),
     "latec_unpack_stage": (i64, [p, i64, u32, p, i64, p, i32, p, p, i64, p]),
     "latec_pack_stages": (i64, [i32, p, i32, p, p, p, p, p, u32, i32, p, p, p, p, i64, p, i32]),
+    "latec_pack_stages_framed": (i64, [i32, p, i32, p, p, p, p, p, u32, i32, p, p, p, p, i64, p,
+                                       i32, i32, p, i64, p]),
     "latec_zstd_bound": (i64, [i64]),
     "latec_zstd_compress": (i64, [p, i64, i32, p, i64]),
     "latec_zstd_content_size": (i64, [p, i64]),

@@ -120,10 +120,13 @@ def zstd_decompress(data: bytes) -> bytes:
     return dst[:n].tobytes()
 
 
-def write_stream(header: Header, payload: bytes, zstd_level: int = 9) -> bytes:
+def write_stream(header: Header, payload: bytes, zstd_level: int = 9, *,
+                 frame: bytes | None = None) -> bytes:
+    """bitstream.py:155-157.  ``frame``: the payload's zstd frame at
+    zstd_level when the caller already has it (latec_pack_stages_framed)."""
     if header.version == VERSION_MULTILANE:
         return header.pack() + payload
-    return header.pack() + zstd_compress(payload, zstd_level)
+    return header.pack() + (zstd_compress(payload, zstd_level) if frame is None else frame)
 
 
 def read_stream(data: bytes):

@@ -108,7 +108,19 @@ def fit_device(res: GpuEncodeResult, precision: int = 24):
 
 
 def pack_device(res: GpuEncodeResult, fits, zstd_level: int, threads: int = _THREADS):
-    """Copy codes/outliers to the host and pack every stage record."""
+    """Copy codes/outliers to the host and pack every stage record:
+    (payload, record lengths, outliers per stage)."""
+    return _pack_device(res, fits, zstd_level, threads, None)[:3]
+
+
+def pack_device_framed(res: GpuEncodeResult, fits, zstd_level: int, threads: int = _THREADS):
+    """pack_device plus the outer zstd frame of the payload at zstd_level
+    (write_stream's body), started while the last large stages finish:
+    (payload, record lengths, outliers per stage, frame)."""
+    return _pack_device(res, fits, zstd_level, threads, zstd_level)
+
+
+def _pack_device(res, fits, zstd_level, threads, frame_level):
     plan, o = res.plan, res.opts
     C, ns = plan.channels, plan.n_stages
     cb = o.code_bytes
@@ -141,12 +153,18 @@ def pack_device(res: GpuEncodeResult, fits, zstd_level: int, threads: int = _THR
     cap = int(64 * ns + 4 * (n_codes.sum() + 2 * ns) + 4 * n_out.sum() + (1 << 16))
     payload = np.empty(cap, np.uint8)
     rec_len = np.zeros(ns, np.int64)
-    size = check(lib().latec_pack_stages(ns, ptr(codes), cb, ptr(code_off), ptr(n_codes),
-                                         ptr(outl), ptr(out_off), ptr(n_out), o.radius,
-                                         int(zstd_level), ctypes.cast(fits, ctypes.c_void_p),
-                                         ptr(cdf_rows), ptr(row_off), ptr(payload), cap,
-                                         ptr(rec_len), int(threads)))
-    return payload[:size].tobytes(), rec_len, n_out
+    args = (ns, ptr(codes), cb, ptr(code_off), ptr(n_codes), ptr(outl), ptr(out_off),
+            ptr(n_out), o.radius, int(zstd_level), ctypes.cast(fits, ctypes.c_void_p),
+            ptr(cdf_rows), ptr(row_off), ptr(payload), cap, ptr(rec_len), int(threads))
+    if frame_level is None:
+        size = check(lib().latec_pack_stages(*args))
+        return payload[:size].tobytes(), rec_len, n_out, None
+    fcap = int(lib().latec_zstd_bound(cap))
+    frame = np.empty(fcap, np.uint8)
+    flen = ctypes.c_int64()
+    size = check(lib().latec_pack_stages_framed(*args, int(frame_level), ptr(frame), fcap,
+                                                ctypes.byref(flen)))
+    return payload[:size].tobytes(), rec_len, n_out, frame[:flen.value].tobytes()
 
 
 def decode_device(payload: bytes, plan, ebs, o, threads: int = _THREADS):
@@ -424,14 +442,15 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
 
     def encode(ebs):
         st = new_stats()
+        frame = None
         if fast:
-            payload, recon = _compress_region_gpu_impl(field, d_field, ebs, predictor, radius,
-                                                       round_output, st, zstd_level,
-                                                       stream_version)
+            payload, recon, frame = _compress_region_gpu_impl(field, d_field, ebs, predictor,
+                                                              radius, round_output, st,
+                                                              zstd_level, stream_version)
         else:
             payload, recon = _compress_region(field, masks, ebs, predictor, radius,
                                               round_output, st, zstd_level)
-        return payload, recon, len(payload), st
+        return payload, recon, len(payload), st, frame
 
     def ebs_for(ratio):
         return stage_ebs(region_shape, levels, anchor_stride, anchor_block, eb, ratio)
@@ -475,10 +494,10 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
             predictor.center = center
         if has_end_mode:
             predictor.end_mode = end_mode
-        payload, rc, raw_bytes, st = encode(ebs_for(ratio))
+        payload, rc, raw_bytes, st, frame = encode(ebs_for(ratio))
         header = make_header(center, ratio, end_mode)
         t0 = time.time()
-        stream = write_stream(header, payload, zstd_level)
+        stream = write_stream(header, payload, zstd_level, frame=frame)
         st["entropy_s"] += time.time() - t0
         st["raw_payload_bytes"] = raw_bytes
         st["compressed_bytes"] = len(stream)
@@ -548,12 +567,13 @@ def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_outp
     fits = fit_device(res)
     stats["quantize_s"] += time.time() - t0
     t0 = time.time()
+    frame = None
     if stream_version == VERSION_MULTILANE:
         from . import mlrans
 
         payload, rec_len, n_out = mlrans.pack_payload(res, fits)
     else:
-        payload, rec_len, n_out = pack_device(res, fits, zstd_level)
+        payload, rec_len, n_out, frame = pack_device_framed(res, fits, zstd_level)
     stats["entropy_s"] += time.time() - t0
     recon = D.to_host(res.d_recon).reshape(C, *region)
     for s in range(plan.n_stages):
@@ -563,7 +583,7 @@ def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_outp
         stats["stage_model_bits"][s] += float(fits[s].bits)
     stats["outliers"] += int(n_out.sum())
     _region_error_stats(stats, plan, field, recon)
-    return payload, recon
+    return payload, recon, frame
 
 
 def decode_device_v5(payload: bytes, plan, ebs, o):

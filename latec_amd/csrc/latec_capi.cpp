@@ -52,14 +52,15 @@ int rans_decode(const uint32_t *, int64_t, const uint32_t *, int64_t, uint32_t *
 int rans_decode_levels(const uint32_t *, int64_t, const uint8_t *, const uint32_t *, int64_t,
                        uint32_t *, int64_t);
 int64_t pack_stage(const void *, int, int64_t, const float *, int64_t, uint32_t, int,
-                   const latec_fit *, const uint32_t *, uint8_t *, int64_t);
+                   const latec_fit *, const uint32_t *, uint8_t *, int64_t, const RecordHook *);
 int stage_header(const uint8_t *, int64_t, uint32_t *, int64_t *, int *, int *, int *, int *,
                  int *);
 int64_t unpack_stage(const uint8_t *, int64_t, uint32_t, const uint32_t *, int64_t, void *,
                      int, uint32_t *, float *, int64_t, int64_t *);
 int64_t pack_stages(int, const void *, int, const int64_t *, const int64_t *, const float *,
                     const int64_t *, const int64_t *, uint32_t, int, const latec_fit *,
-                    const uint32_t *, const int64_t *, uint8_t *, int64_t, int64_t *, int);
+                    const uint32_t *, const int64_t *, uint8_t *, int64_t, int64_t *, int, int,
+                    uint8_t *, int64_t, int64_t *);
 int64_t zstd_compress(const uint8_t *, int64_t, int, uint8_t *, int64_t);
 int64_t zstd_content_size(const uint8_t *, int64_t);
 int64_t zstd_decompress(const uint8_t *, int64_t, uint8_t *, int64_t);
@@ -289,7 +290,7 @@ int64_t latec_pack_stage(const void *codes, int code_bytes, int64_t n, const flo
                          int64_t n_out, uint32_t radius, int zstd_level, const latec_fit *fit,
                          const uint32_t *cdf_row, uint8_t *out, int64_t cap) {
   return pack_stage(codes, code_bytes, n, outl, n_out, radius, zstd_level, fit, cdf_row, out,
-                    cap);
+                    cap, nullptr);
 }
 int latec_stage_header(const uint8_t *buf, int64_t len, uint32_t *n_codes, int64_t *hlen,
                        int *kexp, int *shape, int *level, int *owx, int *literal) {
@@ -309,7 +310,23 @@ int64_t latec_pack_stages(int n_stages, const void *codes, int code_bytes,
                           const int64_t *row_off, uint8_t *out, int64_t cap, int64_t *rec_len,
                           int threads) {
   return pack_stages(n_stages, codes, code_bytes, code_off, n_codes, outl, out_off, n_out,
-                     radius, zstd_level, fits, cdf_rows, row_off, out, cap, rec_len, threads);
+                     radius, zstd_level, fits, cdf_rows, row_off, out, cap, rec_len, threads, -1,
+                     nullptr, 0, nullptr);
+}
+int64_t latec_pack_stages_framed(int n_stages, const void *codes, int code_bytes,
+                                 const int64_t *code_off, const int64_t *n_codes,
+                                 const float *outl, const int64_t *out_off, const int64_t *n_out,
+                                 uint32_t radius, int zstd_level, const latec_fit *fits,
+                                 const uint32_t *cdf_rows, const int64_t *row_off, uint8_t *out,
+                                 int64_t cap, int64_t *rec_len, int threads, int frame_level,
+                                 uint8_t *frame, int64_t frame_cap, int64_t *frame_len) {
+  if (frame_level < 1 || frame_level > 22 || !frame || !frame_len) {
+    set_error("pack_stages_framed: frame level 1..22 and a frame buffer are required");
+    return LATEC_E_ARG;
+  }
+  return pack_stages(n_stages, codes, code_bytes, code_off, n_codes, outl, out_off, n_out,
+                     radius, zstd_level, fits, cdf_rows, row_off, out, cap, rec_len, threads,
+                     frame_level, frame, frame_cap, frame_len);
 }
 int64_t latec_zstd_bound(int64_t n) { return (int64_t)ZSTD_compressBound((size_t)n); }
 int64_t latec_zstd_compress(const uint8_t *src, int64_t n, int level, uint8_t *dst,

@@ -1,6 +1,7 @@
 // latec_common.h -- shared declarations of the LATEC MI355X library.
 #pragma once
 #include <stdint.h>
+#include <functional>
 #include <string>
 
 #include "../../include/latec.h"
@@ -8,6 +9,9 @@
 namespace latec {
 
 void set_error(const std::string &msg);
+
+// Receives a stage record (bytes, length) on the thread that wrote it.
+using RecordHook = std::function<void(const uint8_t *, int64_t)>;
 void set_hip_error(const char *what, int code);
 
 // Narrowed-symbol helpers (latec/rans.py:83-97).

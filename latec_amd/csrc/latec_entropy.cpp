@@ -20,7 +20,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -305,9 +308,27 @@ static int64_t zstd_bounded(const uint8_t *src, size_t n, int level, uint8_t *ds
   }
 }
 
+// Header and tail of a stage record around a blob already at out + 12:
+// [n u32][hlen u45 | kexp u4 | shape u5 | level u6 | owx u3 | literal u1]
+// [blob][n_out u32][outliers f32].  Returns the record length.
+static int64_t put_record(uint8_t *out, int64_t n, int64_t hlen, int kexp, const latec_fit *fit,
+                          int literal, const float *outl, int64_t n_out) {
+  const uint64_t framed = (uint64_t)hlen | ((uint64_t)kexp << 45) |
+                          ((uint64_t)fit->shape << 49) | ((uint64_t)fit->level << 54) |
+                          ((uint64_t)fit->owx << 60) | ((uint64_t)literal << 63);
+  const uint32_t nc = (uint32_t)n;
+  memcpy(out, &nc, 4);
+  memcpy(out + 4, &framed, 8);
+  uint8_t *p = out + 12 + hlen;
+  const uint32_t no = (uint32_t)n_out;
+  memcpy(p, &no, 4);
+  if (n_out) memcpy(p + 4, outl, 4 * (size_t)n_out);
+  return 12 + hlen + 4 + 4 * n_out;
+}
+
 int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int64_t n_out,
                    uint32_t R, int zstd_level, const latec_fit *fit, const uint32_t *cdf_row,
-                   uint8_t *out, int64_t cap) {
+                   uint8_t *out, int64_t cap, const RecordHook *rans_record) {
   const int kexp = fit->kexp;
   if (kexp < 1 || kexp > 15) {
     set_error("rANS coder window does not fit the stage frame");
@@ -392,6 +413,13 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
     rans_thread = std::thread([&] {
       words.reset(new uint32_t[(size_t)enc_cap(n)]);
       nw = rans_encode(syms.get(), n, cdf_row, A, words.get(), enc_cap(n));
+      // the record this stage writes unless its literal frame wins, handed
+      // out before the literal frames end
+      if (rans_record && nw >= 0 && nw * 4 <= blob_cap) {
+        std::unique_ptr<uint8_t[]> rec(new uint8_t[(size_t)(head + nw * 4 + tail)]);
+        memcpy(rec.get() + 12, words.get(), (size_t)nw * 4);
+        (*rans_record)(rec.get(), put_record(rec.get(), n, nw * 4, kexp, fit, 0, outl, n_out));
+      }
     });
   struct Join {
     std::thread &t;
@@ -452,17 +480,7 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
     memcpy(blob, words.get(), (size_t)nw * 4);
     hlen = nw * 4;
   }
-  const uint64_t framed = (uint64_t)hlen | ((uint64_t)kexp << 45) |
-                          ((uint64_t)fit->shape << 49) | ((uint64_t)fit->level << 54) |
-                          ((uint64_t)fit->owx << 60) | ((uint64_t)literal << 63);
-  const uint32_t nc = (uint32_t)n;
-  memcpy(out, &nc, 4);
-  memcpy(out + 4, &framed, 8);
-  uint8_t *p = blob + hlen;
-  const uint32_t no = (uint32_t)n_out;
-  memcpy(p, &no, 4);
-  if (n_out) memcpy(p + 4, outl, 4 * (size_t)n_out);
-  return head + hlen + tail;
+  return put_record(out, n, hlen, kexp, fit, literal, outl, n_out);
 }
 
 int stage_header(const uint8_t *buf, int64_t len, uint32_t *n_codes, int64_t *hlen,
@@ -560,16 +578,31 @@ int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t
   return 12 + hlen + 4 + 4 * (int64_t)no;
 }
 
+// Every stage record of a payload, stages on a pool of `threads` workers
+// (largest first), concatenated in stage order.  With frame_level >= 0 the
+// calling thread also writes the zstd frame of the payload at that level
+// (the one-shot frame, see zstd_bounded) and starts it early: once every
+// stage has either its final record or its rANS record (a large stage whose
+// literal frames are still running), it frames that payload; if a large
+// stage's final record then differs (its literal frame won), the payload is
+// rebuilt from the final records and framed again.  The outputs are those of
+// packing everything first and framing after.
 int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
                     const int64_t *n_codes, const float *outl, const int64_t *out_off,
                     const int64_t *n_out, uint32_t R, int zstd_level, const latec_fit *fits,
                     const uint32_t *cdf_rows, const int64_t *row_off, uint8_t *out,
-                    int64_t cap, int64_t *rec_len, int threads) {
-  std::vector<std::vector<uint8_t>> recs(ns);
-  std::vector<int64_t> rcs(ns, 0);
+                    int64_t cap, int64_t *rec_len, int threads, int frame_level,
+                    uint8_t *frame, int64_t frame_cap, int64_t *frame_len) {
+  const bool framing = frame_level >= 0;
+  std::vector<std::unique_ptr<uint8_t[]>> recs(ns), spec(ns);
+  std::vector<int64_t> rcs(ns, 0), spec_len(ns, -1);
+  std::vector<char> done(ns, 0);
   std::vector<int> order(ns);
   for (int i = 0; i < ns; i++) order[i] = i;
   std::sort(order.begin(), order.end(), [&](int a, int b) { return n_codes[a] > n_codes[b]; });
+  std::mutex mu;
+  std::condition_variable cv;
+  std::atomic<bool> spec_broken(false);
   std::atomic<int> next(0);
   auto work = [&]() {
     while (true) {
@@ -577,33 +610,87 @@ int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
       if (j >= ns) break;
       const int s = order[j];
       const int64_t rc_cap = 16 + 4 * n_out[s] + 4 * (n_codes[s] + 2) + 64;
-      recs[s].resize((size_t)rc_cap);
+      recs[s].reset(new uint8_t[(size_t)rc_cap]);
       const void *cs = (const uint8_t *)codes + (size_t)cb * code_off[s];
-      rcs[s] = pack_stage(cs, cb, n_codes[s], outl + out_off[s], n_out[s], R, zstd_level,
-                          &fits[s], cdf_rows + row_off[s], recs[s].data(), rc_cap);
+      RecordHook hook = [&, s](const uint8_t *rec, int64_t len) {
+        std::unique_ptr<uint8_t[]> copy(new uint8_t[(size_t)len]);
+        memcpy(copy.get(), rec, (size_t)len);
+        std::lock_guard<std::mutex> g(mu);
+        spec[s] = std::move(copy);
+        spec_len[s] = len;
+        cv.notify_all();
+      };
+      const int64_t rc = pack_stage(cs, cb, n_codes[s], outl + out_off[s], n_out[s], R,
+                                    zstd_level, &fits[s], cdf_rows + row_off[s], recs[s].get(),
+                                    rc_cap, framing ? &hook : nullptr);
+      // spec[s] was set (if at all) on a thread pack_stage joined
+      const bool differs = spec_len[s] >= 0 &&
+                           (rc != spec_len[s] || memcmp(recs[s].get(), spec[s].get(), rc));
+      std::lock_guard<std::mutex> g(mu);
+      rcs[s] = rc;
+      done[s] = 1;
+      if (differs) spec_broken.store(true);
+      cv.notify_all();
     }
   };
   if (threads < 1) threads = 1;
   threads = std::min(threads, ns);
   std::vector<std::thread> pool;
-  for (int t = 1; t < threads; t++) pool.emplace_back(work);
-  work();
-  for (auto &t : pool) t.join();
-  int64_t off = 0;
-  for (int s = 0; s < ns; s++) {
-    if (rcs[s] < 0) {
-      set_error("stage " + std::to_string(s) + ": pack failed");
-      return rcs[s];
+  for (int t = framing ? 0 : 1; t < threads; t++) pool.emplace_back(work);
+  if (!framing) work();
+  // payload of stage records in stage order: the final record of every
+  // stage in `fin`, else its rANS record; false on a stage error or capacity
+  int64_t size = 0;
+  auto assemble = [&](const std::vector<char> &fin) -> int64_t {
+    int64_t off = 0;
+    for (int s = 0; s < ns; s++) {
+      const int64_t len = fin[s] ? rcs[s] : spec_len[s];
+      if (len < 0) {
+        set_error("stage " + std::to_string(s) + ": pack failed");
+        return len;
+      }
+      if (off + len > cap) {
+        set_error("payload capacity");
+        return LATEC_E_CAPACITY;
+      }
+      memcpy(out + off, fin[s] ? recs[s].get() : spec[s].get(), (size_t)len);
+      rec_len[s] = len;
+      off += len;
     }
-    if (off + rcs[s] > cap) {
-      set_error("payload capacity");
+    return off;
+  };
+  int64_t flen = ZB_STOPPED;
+  if (framing) {
+    std::vector<char> fin(ns);
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] {
+        for (int s = 0; s < ns; s++)
+          if (!done[s] && spec_len[s] < 0) return false;
+        return true;
+      });
+      fin = done;  // records fixed now; the others are rANS records
+    }
+    size = spec_broken.load() ? -1 : assemble(fin);
+    if (size >= 0)
+      flen = zstd_bounded(out, (size_t)size, frame_level, frame, (size_t)frame_cap,
+                          [](size_t) { return false; }, &spec_broken);
+  }
+  for (auto &t : pool) t.join();
+  if (framing && (flen < 0 || spec_broken.load())) {  // frame again from the final records
+    size = assemble(std::vector<char>(ns, 1));
+    if (size < 0) return size;
+    flen = zstd_bounded(out, (size_t)size, frame_level, frame, (size_t)frame_cap,
+                        [](size_t) { return false; }, nullptr);
+    if (flen < 0) {
+      set_error("payload frame capacity");
       return LATEC_E_CAPACITY;
     }
-    memcpy(out + off, recs[s].data(), (size_t)rcs[s]);
-    rec_len[s] = rcs[s];
-    off += rcs[s];
+  } else if (!framing) {
+    size = assemble(std::vector<char>(ns, 1));
   }
-  return off;
+  if (framing) *frame_len = flen;
+  return size;
 }
 
 int64_t zstd_compress(const uint8_t *src, int64_t n, int level, uint8_t *dst, int64_t cap) {

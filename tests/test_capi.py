@@ -129,3 +129,64 @@ def test_pack_stages_large_stage_equals_oracle_record(kind):
     assert out[:size].tobytes() == want
     if kind == "runs":
         assert struct.unpack_from("<IQ", want, 0)[1] >> 63 == 1
+
+
+@pytest.mark.parametrize("kinds,threads", [(("small", "iid", "small2"), 4),
+                                           (("runs", "small", "iid"), 4),
+                                           (("iid", "runs"), 1)])
+def test_pack_stages_framed_equals_pack_then_zstd(kinds, threads):
+    """The framed pack (outer zstd frame started on the large stages' rANS
+    records, redone when a literal frame wins) writes the payload and frame
+    that packing first and framing the payload after would."""
+    import ctypes
+
+    R = 1 << 15
+    rng = np.random.RandomState(7)
+    stages = []
+    for kind in kinds:
+        if kind == "runs":
+            n = 1_200_000
+            c = np.repeat(np.arange(n // 400) % 7 + R - 3, 400).astype(np.uint32)
+        elif kind == "iid":
+            c = (np.rint(rng.laplace(0, 3.0, 1_100_000)) + R).astype(np.uint32)
+        else:
+            c = (np.rint(rng.laplace(0, 1.5 if kind == "small" else 9.0, 9000)) + R).astype(
+                np.uint32)
+        c[::991] = 0
+        stages.append(c)
+    ns = len(stages)
+    n_codes = np.array([len(c) for c in stages], np.int64)
+    code_off = np.concatenate([[0], np.cumsum(n_codes)[:-1]]).astype(np.int64)
+    codes = np.concatenate(stages).astype(np.uint16)
+    outs = [rng.randn(int((c == 0).sum())).astype(np.float32) for c in stages]
+    n_out = np.array([len(v) for v in outs], np.int64)
+    out_off = np.concatenate([[0], np.cumsum(n_out)[:-1]]).astype(np.int64)
+    outl = np.concatenate(outs)
+    fits = (_lib.Fit * ns)()
+    rows = []
+    for i, c in enumerate(stages):
+        kexp = O.choose_kexp(c, R)
+        level, shape, owx, bits = O.choose_const_level(c, R, kexp)
+        f = fits[i]
+        f.kexp, f.level, f.shape, f.owx, f.bits = kexp, level, shape, owx, bits
+        rows.append(np.asarray(O.build_cdfs(kexp, shape, 24, owx)[level], np.uint32))
+    row_off = np.concatenate([[0], np.cumsum([len(r) for r in rows])[:-1]]).astype(np.int64)
+    cdf = np.ascontiguousarray(np.concatenate(rows), np.uint32)
+    cap = int(64 * ns + 4 * (n_codes.sum() + 2 * ns) + 4 * n_out.sum() + (1 << 16))
+    P = _lib.ptr
+    common = (ns, P(codes), 2, P(code_off), P(n_codes), P(outl), P(out_off), P(n_out), R, 9,
+              ctypes.cast(fits, ctypes.c_void_p), P(cdf), P(row_off))
+    want = np.empty(cap, np.uint8)
+    want_len = np.zeros(ns, np.int64)
+    size = _lib.check(_lib.lib().latec_pack_stages(*common, P(want), cap, P(want_len), threads))
+    want_frame = O._zstd(want[:size].tobytes(), 9)
+    got = np.empty(cap, np.uint8)
+    got_len = np.zeros(ns, np.int64)
+    fcap = int(_lib.lib().latec_zstd_bound(cap))
+    frame = np.empty(fcap, np.uint8)
+    flen = ctypes.c_int64()
+    gsize = _lib.check(_lib.lib().latec_pack_stages_framed(
+        *common, P(got), cap, P(got_len), threads, 9, P(frame), fcap, ctypes.byref(flen)))
+    assert gsize == size and got[:size].tobytes() == want[:size].tobytes()
+    assert np.array_equal(got_len, want_len)
+    assert frame[:flen.value].tobytes() == want_frame

@@ -332,6 +332,13 @@ def test_gpu_config2_full_size_roundtrip():
     res = codec.encode_device(xd, plan, ebs, o)
     fits = codec.fit_device(res)
     payload, rec_len, n_out = codec.pack_device(res, fits, 9)
+    # the framed pack (outer frame started on the large stages' rANS records)
+    # writes the same payload and the one-shot zstd-9 frame of it
+    from latec_amd.bitstream import zstd_compress
+
+    payload_f, rec_len_f, _, frame = codec.pack_device_framed(res, fits, 9)
+    assert payload_f == payload and np.array_equal(rec_len_f, rec_len)
+    assert frame == zstd_compress(payload, 9)
     enc_recon = res.d_recon.clone()
     err = (enc_recon - xd).abs().max().item()
     assert err <= eb
