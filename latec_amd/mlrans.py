@@ -94,21 +94,25 @@ def pack_payload(res, fits):
                                     ptr(scratch), int(sw), ptr(out), cap, ptr(blob_off),
                                     ptr(blob_len), ptr(None)))
     total = int((blob_off + blob_len).max(initial=0))
-    blobs = D.to_host(out[:total], np.uint8) if total else np.zeros(0, np.uint8)
+    # blobs through the pinned staging buffer, then each record written once
+    # into the payload buffer
+    blobs = D.to_host_staged(out[:total], np.uint8) if total else np.zeros(0, np.uint8)
     n_out = D.to_host(res.d_cnt, np.uint32).astype(np.int64)
     outl, out_off = sorted_outliers(res, plan, n_out)
-    parts, rec_len = [], np.zeros(ns, np.int64)
+    rec_len = 16 + blob_len + 4 * n_out
+    payload = np.empty(int(rec_len.sum()), np.uint8)
+    at = 0
     for s in range(ns):
         kx, shape, owx, level = params[s]
-        blob = blobs[blob_off[s]:blob_off[s] + blob_len[s]].tobytes()
-        k = int(n_out[s])
-        rec = b"".join((struct.pack("<IQ", int(n[s]), _pack_hlen(len(blob), kx, shape, level,
-                                                                 owx)),
-                        blob, struct.pack("<I", k),
-                        outl[out_off[s]:out_off[s] + k].tobytes()))
-        rec_len[s] = len(rec)
-        parts.append(rec)
-    return b"".join(parts), rec_len, n_out
+        hl, k = int(blob_len[s]), int(n_out[s])
+        payload[at:at + 12] = np.frombuffer(
+            struct.pack("<IQ", int(n[s]), _pack_hlen(hl, kx, shape, level, owx)), np.uint8)
+        payload[at + 12:at + 12 + hl] = blobs[blob_off[s]:blob_off[s] + hl]
+        payload[at + 12 + hl:at + 16 + hl] = np.frombuffer(struct.pack("<I", k), np.uint8)
+        payload[at + 16 + hl:at + 16 + hl + 4 * k] = \
+            outl[out_off[s]:out_off[s] + k].view(np.uint8)
+        at += int(rec_len[s])
+    return payload.tobytes(), rec_len, n_out
 
 
 def decode_payload(payload: bytes, plan, ws):
