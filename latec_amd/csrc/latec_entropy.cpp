@@ -211,8 +211,10 @@ int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t 
   return nw;
 }
 
-int rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t A,
-                uint32_t *syms, int64_t n) {
+// n symbols of one CDF row, each handed to sink(i, symbol) as it is decoded.
+template <class Sink>
+static int rans_decode_to(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t A,
+                          int64_t n, Sink sink) {
   Model m;
   build_model(cdf, A, m, true);
   uint64_t state;
@@ -221,12 +223,17 @@ int rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t 
     set_error("malformed rANS payload");
     return LATEC_E_FORMAT;
   }
-  for (int64_t i = 0; i < n; i++) syms[i] = dec_get(state, m, words, top);
+  for (int64_t i = 0; i < n; i++) sink(i, dec_get(state, m, words, top));
   if (top != 0 || state != 0) {
     set_error("trailing data in rANS payload");
     return LATEC_E_FORMAT;
   }
   return LATEC_OK;
+}
+
+int rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t A,
+                uint32_t *syms, int64_t n) {
+  return rans_decode_to(words, nw, cdf, A, n, [syms](int64_t i, uint32_t k) { syms[i] = k; });
 }
 
 int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
@@ -523,7 +530,20 @@ int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t
   }
   const uint8_t *blob = buf + 12;
   const int64_t A = (int64_t)2 << kexp;
-  std::unique_ptr<uint32_t[]> syms(new uint32_t[std::max<uint32_t>(n, 1)]);
+  // each decoded symbol is widened into `codes` at once; the ranks of the
+  // outlier codes (0) are kept up to out_cap and counted
+  int64_t k = 0;
+  auto emit = [&](int64_t i, uint32_t sym) {
+    const uint32_t c = widen_sym(sym, R, kexp);
+    if (cb == 2)
+      ((uint16_t *)codes)[i] = (uint16_t)c;
+    else
+      ((uint32_t *)codes)[i] = c;
+    if (c == 0) {
+      if (k < out_cap) out_rank[k] = (uint32_t)i;
+      k++;
+    }
+  };
   if (literal) {
     const bool wide = kexp > 7;
     const size_t sb = (size_t)n * (wide ? 2 : 1);
@@ -533,8 +553,7 @@ int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t
       set_error("bad literal stage frame");
       return LATEC_E_FORMAT;
     }
-    for (uint32_t i = 0; i < n; i++)
-      syms[i] = wide ? ((const uint16_t *)lit.get())[i] : lit[i];
+    for (uint32_t i = 0; i < n; i++) emit(i, wide ? ((const uint16_t *)lit.get())[i] : lit[i]);
   } else if (n == 0) {
     if (hlen) {
       set_error("non-empty rANS blob for an empty stage");
@@ -547,7 +566,7 @@ int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t
     }
     std::unique_ptr<uint32_t[]> words(new uint32_t[(size_t)hlen / 4]);
     memcpy(words.get(), blob, (size_t)hlen);
-    rc = rans_decode(words.get(), hlen / 4, cdf_row, A, syms.get(), n);
+    rc = rans_decode_to(words.get(), hlen / 4, cdf_row, A, n, emit);
     if (rc) return rc;
   }
   uint32_t no;
@@ -555,18 +574,6 @@ int64_t unpack_stage(const uint8_t *buf, int64_t len, uint32_t R, const uint32_t
   if (12 + hlen + 4 + 4 * (int64_t)no > len || (int64_t)no > out_cap) {
     set_error("truncated stage record");
     return LATEC_E_FORMAT;
-  }
-  int64_t k = 0;
-  for (uint32_t i = 0; i < n; i++) {
-    const uint32_t c = widen_sym(syms[i], R, kexp);
-    if (cb == 2)
-      ((uint16_t *)codes)[i] = (uint16_t)c;
-    else
-      ((uint32_t *)codes)[i] = c;
-    if (c == 0) {
-      if (k < (int64_t)no) out_rank[k] = i;
-      k++;
-    }
   }
   if (k != (int64_t)no) {
     set_error("outlier count mismatch: " + std::to_string(k) + " zeros vs " +
