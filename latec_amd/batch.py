@@ -138,17 +138,28 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
         raise ValueError(f"unsupported version {stream_version}")
     t0 = time.perf_counter()
     prepped = [_prep(x) for x in fields]
-    plans, ebs_list, heads, opts = [], [], [], []
+    plans, preds, opts = [], [], []
     for x, shape, src in prepped:
         pred = InterpPredictor(order, levels, anchor_stride, 1)
         center, end_mode = pred.resolved(len(shape))
+        preds.append((pred, center, end_mode))
         opts.append(opts_for(radius, src == 0, cubic=order == "cubic", center=center,
                              end_mode=end_mode))
-        vmin, vmax = float(x.min()), float(x.max())
+        plans.append(get_plan(shape, 1, levels, anchor_stride, 1))
+    lay = _Layout(plans, radius)
+    buf = _BatchBuffers(lay)
+    for (x, _, _), off in zip(prepped, lay.x_off):  # each field straight into its slice
+        buf.x[off:off + x.size].copy_(torch.from_numpy(x.reshape(-1).astype(np.float32,
+                                                                            copy=False)))
+    # each field's range from its float32 values in HBM (exact, as the host min/max)
+    rng = torch.stack([torch.stack(torch.aminmax(buf.x[off:off + x.size]))
+                       for (x, _, _), off in zip(prepped, lay.x_off)]).cpu().numpy()
+    ebs_list, heads = [], []
+    for (x, shape, src), (pred, center, end_mode), (lo, hi) in zip(prepped, preds, rng):
+        vmin, vmax = float(lo), float(hi)
         if vmax <= vmin:
             vmax = vmin + 1.0
         eb = float(rel_eb) * (vmax - vmin)
-        plans.append(get_plan(shape, 1, levels, anchor_stride, 1))
         ebs_list.append(np.ascontiguousarray(
             stage_ebs(shape, levels, anchor_stride, 1, eb, pred.fast_eb_ratio), np.float64))
         heads.append(Header(channels=1, src_dtype=src, spatial=shape, eb=eb, levels=levels,
@@ -157,11 +168,6 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
                             flags=pred.stream_flag | (end_mode << END_MODE_SHIFT),
                             interp_center=center, eb_ratio=pred.fast_eb_ratio,
                             version=stream_version))
-    lay = _Layout(plans, radius)
-    buf = _BatchBuffers(lay)
-    for (x, _, _), off in zip(prepped, lay.x_off):  # each field straight into its slice
-        buf.x[off:off + x.size].copy_(torch.from_numpy(x.reshape(-1).astype(np.float32,
-                                                                            copy=False)))
     buf.recon.zero_()
     buf.hist.zero_()
     buf.cnt.zero_()
@@ -238,8 +244,6 @@ def _fit_all(plans, lay, buf, radius):
 def _pack_v5_all(views, fits, lay, buf, radius):
     """Version-5 payloads of every field, all stages coded in one
     multi-lane call."""
-    from .mlrans import sorted_outliers
-
     n, code_off, params = [], [], []
     for f, (v, ft) in enumerate(zip(views, fits)):
         p = v.plan
@@ -274,14 +278,33 @@ def _pack_v5_all(views, fits, lay, buf, radius):
     total = int((blob_off + blob_len).max(initial=0))
     blobs = D.to_host(out[:total], np.uint8) if total else np.zeros(0, np.uint8)
     all_out = D.to_host(buf.cnt[:max(lay.n_cnt, 1)], np.uint32).astype(np.int64)
+    # every stage's outliers (rank, value) gathered on the device and copied
+    # in one go, then put in rank order per stage on the host
+    pos, seg = [], []
+    for f, v in enumerate(views):
+        p = v.plan
+        for s in range(p.n_stages):
+            k = int(all_out[lay.cnt_base[f] + s])
+            if k:
+                lo = lay.code_base[f] + p.offsets[s]
+                pos.append(np.arange(lo, lo + k, dtype=np.int64))
+            seg.append(k)
+    seg_at = np.concatenate([[0], np.cumsum(seg)]).astype(np.int64)
+    if pos:
+        d_pos = D.torch().from_numpy(np.concatenate(pos)).to(buf.rank.device)
+        g_rank = D.to_host(buf.rank[d_pos], np.uint32)
+        g_val = D.to_host(buf.val[d_pos])
     payloads, s0 = [], 0
     for f, v in enumerate(views):
         p = v.plan
         n_out = all_out[lay.cnt_base[f]:lay.cnt_base[f] + p.n_stages]
-        if n_out.any():
-            outl, out_off = sorted_outliers(v, p, n_out)
-        else:
-            outl, out_off = np.zeros(1, np.float32), np.zeros(p.n_stages, np.int64)
+        outl = np.zeros(max(int(n_out.sum()), 1), np.float32)
+        out_off = np.concatenate([[0], np.cumsum(n_out)[:-1]]).astype(np.int64)
+        for s in range(p.n_stages):
+            a, b = int(seg_at[s0 + s]), int(seg_at[s0 + s + 1])
+            if b > a:
+                outl[out_off[s]:out_off[s] + b - a] = \
+                    g_val[a:b][np.argsort(g_rank[a:b], kind="stable")]
         parts = []
         for s in range(p.n_stages):
             g = s0 + s
@@ -404,13 +427,18 @@ def _decompress_interp(streams, *, use_graph=True, timings=None):
     check(lib().latec_zero_ranks(ns, ptr(buf.codes), 2, ptr(code_off), ptr(n), ptr(buf.rank),
                                  ptr(buf.cnt), ptr(scratch), sw, ptr(None)))
     cnt = D.to_host(buf.cnt[:ns], np.uint32)
-    host_val = np.zeros(max(lay.n_codes, 1), np.float32)
+    # outlier values go to their stages' slots: one scatter of the values
+    # there are, not a copy of the whole value array
+    pos, vals = [], []
     for s in range(ns):
         if int(cnt[s]) != len(outs[s]):
             raise ValueError("outlier count mismatch")
         if len(outs[s]):
-            host_val[code_off[s]:code_off[s] + len(outs[s])] = outs[s]
-    buf.val.copy_(torch.from_numpy(host_val))
+            pos.append(np.arange(code_off[s], code_off[s] + len(outs[s]), dtype=np.int64))
+            vals.append(outs[s])
+    if pos:
+        d_pos = torch.from_numpy(np.concatenate(pos)).to(buf.val.device)
+        buf.val[d_pos] = torch.from_numpy(np.concatenate(vals)).to(buf.val.device)
     buf.recon.zero_()
     base_r = buf.recon.data_ptr()
     oa = _opts_array(opts)
