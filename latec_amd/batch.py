@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import ctypes
 import struct
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -39,7 +40,7 @@ from . import rans as R
 from ._lib import check, lib, ptr
 from .bitstream import (DTYPE_IDS, END_MODE_SHIFT, VERSION, VERSION_MULTILANE, Header,
                         _pack_hlen, parse_record, read_stream, write_stream)
-from .codec import pack_device
+from .codec import _THREADS, pack_device
 from .levels import stage_ebs
 from .predictor import InterpPredictor, get_plan
 from .quantizer import opts_for
@@ -137,7 +138,8 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
     if stream_version not in (VERSION, VERSION_MULTILANE):
         raise ValueError(f"unsupported version {stream_version}")
     t0 = time.perf_counter()
-    prepped = [_prep(x) for x in fields]
+    with ThreadPoolExecutor(max_workers=_THREADS) as ex:  # float64 casts run in parallel
+        prepped = list(ex.map(_prep, fields))
     plans, preds, opts = [], [], []
     for x, shape, src in prepped:
         pred = InterpPredictor(order, levels, anchor_stride, 1)
@@ -458,10 +460,14 @@ def _decompress_interp(streams, *, use_graph=True, timings=None):
         lib().latec_batch_destroy(h)
     t2 = time.perf_counter()
     host = D.to_host(buf.recon[:max(lay.n_points, 1)])
-    out = []
-    for f, (hd, p) in enumerate(zip(heads, plans)):
-        canvas = host[lay.x_off[f]:lay.x_off[f] + p.npoints].reshape(1, *hd.spatial)
-        out.append(_finalize(canvas, hd))
+
+    def finish(f):  # source-dtype casts (float64, uint8) run in parallel
+        hd, p = heads[f], plans[f]
+        return _finalize(host[lay.x_off[f]:lay.x_off[f] + p.npoints].reshape(1, *hd.spatial),
+                         hd)
+
+    with ThreadPoolExecutor(max_workers=_THREADS) as ex:
+        out = list(ex.map(finish, range(len(plans))))
     if timings is not None:
         timings.update({"setup_s": t1 - t0, "kernels_s": t2 - t1,
                         "finish_s": time.perf_counter() - t2})
