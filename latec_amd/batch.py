@@ -38,7 +38,8 @@ import numpy as np
 from . import _device as D
 from . import rans as R
 from ._lib import check, lib, ptr
-from .bitstream import (DTYPE_IDS, END_MODE_SHIFT, VERSION, VERSION_MULTILANE, Header,
+from .bitstream import (DTYPE_CODES, DTYPE_IDS, END_MODE_SHIFT, VERSION, VERSION_MULTILANE,
+                        Header,
                         _pack_hlen, parse_record, read_stream, write_stream)
 from .codec import _THREADS, pack_device
 from .levels import stage_ebs
@@ -150,9 +151,13 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
         plans.append(get_plan(shape, 1, levels, anchor_stride, 1))
     lay = _Layout(plans, radius)
     buf = _BatchBuffers(lay)
-    for (x, _, _), off in zip(prepped, lay.x_off):  # each field straight into its slice
+    def upload(f):  # each field straight into its slice, fields by parallel threads
+        x, off = prepped[f][0], lay.x_off[f]
         buf.x[off:off + x.size].copy_(torch.from_numpy(x.reshape(-1).astype(np.float32,
                                                                             copy=False)))
+
+    with ThreadPoolExecutor(max_workers=_THREADS) as ex:
+        list(ex.map(upload, range(len(prepped))))
     # each field's range from its float32 values in HBM (exact, as the host min/max)
     rng = torch.stack([torch.stack(torch.aminmax(buf.x[off:off + x.size]))
                        for (x, _, _), off in zip(prepped, lay.x_off)]).cpu().numpy()
@@ -278,7 +283,7 @@ def _pack_v5_all(views, fits, lay, buf, radius):
                                     radius, ptr(d_rows), ptr(row_off), ptr(scratch), sw,
                                     ptr(out), cap, ptr(blob_off), ptr(blob_len), ptr(None)))
     total = int((blob_off + blob_len).max(initial=0))
-    blobs = D.to_host(out[:total], np.uint8) if total else np.zeros(0, np.uint8)
+    blobs = D.to_host_staged(out[:total], np.uint8) if total else np.zeros(0, np.uint8)
     all_out = D.to_host(buf.cnt[:max(lay.n_cnt, 1)], np.uint32).astype(np.int64)
     # every stage's outliers (rank, value) gathered on the device and copied
     # in one go, then put in rank order per stage on the host
@@ -307,18 +312,21 @@ def _pack_v5_all(views, fits, lay, buf, radius):
             if b > a:
                 outl[out_off[s]:out_off[s] + b - a] = \
                     g_val[a:b][np.argsort(g_rank[a:b], kind="stable")]
-        parts = []
+        # the field's records written once into one buffer
+        hl = blob_len[s0:s0 + p.n_stages]
+        pay = np.empty(int((16 + hl + 4 * n_out).sum()), np.uint8)
+        at = 0
         for s in range(p.n_stages):
             g = s0 + s
             kx, shape, owx, level = params[g]
-            blob = blobs[blob_off[g]:blob_off[g] + blob_len[g]].tobytes()
-            k = int(n_out[s])
-            parts.append(struct.pack("<IQ", int(n[g]), _pack_hlen(len(blob), kx, shape, level,
-                                                                  owx)))
-            parts.append(blob)
-            parts.append(struct.pack("<I", k))
-            parts.append(outl[out_off[s]:out_off[s] + k].tobytes())
-        payloads.append(b"".join(parts))
+            h, k = int(hl[s]), int(n_out[s])
+            pay[at:at + 12] = np.frombuffer(
+                struct.pack("<IQ", int(n[g]), _pack_hlen(h, kx, shape, level, owx)), np.uint8)
+            pay[at + 12:at + 12 + h] = blobs[blob_off[g]:blob_off[g] + h]
+            pay[at + 12 + h:at + 16 + h] = np.frombuffer(struct.pack("<I", k), np.uint8)
+            pay[at + 16 + h:at + 16 + h + 4 * k] = outl[out_off[s]:out_off[s] + k].view(np.uint8)
+            at += 16 + h + 4 * k
+        payloads.append(pay.tobytes())
         s0 += p.n_stages
     return payloads
 
@@ -459,12 +467,19 @@ def _decompress_interp(streams, *, use_graph=True, timings=None):
     finally:
         lib().latec_batch_destroy(h)
     t2 = time.perf_counter()
-    host = D.to_host(buf.recon[:max(lay.n_points, 1)])
-
-    def finish(f):  # source-dtype casts (float64, uint8) run in parallel
+    def finish(f):
+        # each field straight into its own output array (float64 sources
+        # widened on the device first), fields copied by parallel threads;
+        # other source types through the codec's finalize
         hd, p = heads[f], plans[f]
-        return _finalize(host[lay.x_off[f]:lay.x_off[f] + p.npoints].reshape(1, *hd.spatial),
-                         hd)
+        rec = buf.recon[lay.x_off[f]:lay.x_off[f] + p.npoints]
+        dt = np.dtype(DTYPE_CODES[hd.src_dtype])
+        if hd.channels == 1 and dt in (np.float32, np.float64):
+            out = np.empty(hd.spatial, dt)
+            torch.from_numpy(out.reshape(-1)).copy_(
+                rec if dt == np.float32 else rec.to(torch.float64))
+            return out
+        return _finalize(D.to_host(rec).reshape(1, *hd.spatial), hd)
 
     with ThreadPoolExecutor(max_workers=_THREADS) as ex:
         out = list(ex.map(finish, range(len(plans))))
