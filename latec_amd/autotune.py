@@ -19,7 +19,7 @@ The 3-D level kernels have seven launch-shape knobs, set process-wide with
 * ``pair_quads`` -- whether the line blocks of phases 2 and 3 give each
   thread four points of a line (lines of a multiple of 4, at most 1024) or
   two: bit 0 for the encoder (the one the tuner times), bit 1 for the
-  decoder;
+  decoder, bits 2 and 3 the same for phase 2 alone;
 * ``min_threads`` -- a level launch shortens each thread's march (8, 4, 2, 1
   points) until it has at least this many threads: small levels are
   latency-bound, so they trade march length for parallelism.

@@ -2956,8 +2956,10 @@ static int64_t g_pair = 1;  // "pair": 1 runs 3-D levels above the first as line
 static int64_t g_pair_lines = 1;
 // "pair_quads": bit 0 (encode) and bit 1 (decode) run phases 2 and 3 with
 // four points per thread when the lines allow it (length a multiple of 4, at
-// most 1024).  Measured at 512^3: decode 15 % faster, encode 4 % slower (its
-// larger register set costs a wave per SIMD), so by default decode only.
+// most 1024); bits 2 (encode) and 3 (decode) do so for phase 2 alone.
+// Measured at 512^3: decode 15 % faster, encode 4 % slower (its larger
+// register set costs a wave per SIMD), so by default decode only; quads in
+// phase 2 alone: encode 1 % slower, decode 3 % slower than quads in both.
 static int64_t g_pair_quads = 2;
 // "min_threads": a march or pair launch shortens its per-thread march (8, 4,
 // 2, 1 steps) until it has at least this many threads
@@ -2994,7 +2996,7 @@ int tune_set(const char *name, int64_t v) {
     return LATEC_OK;
   }
   if (!strcmp(name, "pair_quads")) {
-    if (v < 0 || v > 3) {
+    if (v < 0 || v > 15) {
       set_error(std::string("tuning value out of range for ") + name);
       return LATEC_E_ARG;
     }
@@ -3347,8 +3349,12 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
   const int64_t n[3] = {q->shape[0], q->shape[1], q->shape[2]};
   const uint32_t npairs = (uint32_t)((n[2] + 1) / 2);
   const uint32_t nq = (uint32_t)(n[2] / 4);
-  const bool quads = g_pair_lines && ((g_pair_quads >> (encode ? 0 : 1)) & 1) &&
-                     n[2] % 4 == 0 && nq >= 1 && nq <= 256;
+  const bool quad_lines = g_pair_lines && n[2] % 4 == 0 && nq >= 1 && nq <= 256;
+  const int qb = encode ? 0 : 1;
+  // phases 2 (ph 1) and 3 (ph 2) as line quads
+  const bool quads_ph[3] = {false,
+                            quad_lines && ((g_pair_quads >> qb) & 1 || (g_pair_quads >> (qb + 2)) & 1),
+                            quad_lines && ((g_pair_quads >> qb) & 1)};
   auto half = [&](int a, int par) { return (uint32_t)((n[a] - par + 1) / 2); };
   PArgs A[3];
   for (int ph = 0; ph < 3; ph++) {
@@ -3425,7 +3431,7 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     const int64_t own = A[ph].own;
     const int64_t nw = (pairs + own - 1) / own;
     T.nwaves = (uint32_t)nw;
-    if (ph > 0 && quads) {  // line quads: whole lines of quads per workgroup
+    if (quads_ph[ph]) {  // line quads: whole lines of quads per workgroup
       T.lpb = 256 / nq;
       T.bpc = (T.nrows + T.lpb - 1) / T.lpb;
     } else if (ph > 0 && g_pair_lines && npairs <= 256) {  // line blocks
@@ -3471,7 +3477,7 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     const int bits = (encode ? 1 : 0) | (n[2] % 2 == 0 ? 2 : 0) |
                      (encode && o->round_mode != 0 ? 4 : 0) | (ph > 0 && a.set[0].lpb ? 8 : 0) |
                      (!encode || xstep == 1 ? 16 : 0);
-    if (ph > 0 && quads)
+    if (quads_ph[ph])
       QuadLaunch<7>::go(ph + 1, bits & 1 | (bits & 4) >> 1 | (bits & 16) >> 2, grid, stream,
                         a, encode ? x : nullptr, recon, (uint16_t *)codes, hist, out_rank,
                         out_val, out_count);

@@ -281,15 +281,17 @@ def test_gpu_slab_kernel_bit_exact(case):
 
 @pytest.mark.parametrize("knobs", [(0, 0, 1, 3), (0, 1 << 24, 1, 3), (1, 1 << 24, 1, 3),
                                    (1, 0, 1, 3), (1, 0, 0, 3), (1, 1 << 24, 0, 3), (1, 0, 1, 0),
-                                   (1, 1 << 24, 1, 0)],
+                                   (1, 1 << 24, 1, 0), (1, 0, 1, 12), (1, 1 << 24, 1, 12)],
                          ids=["march", "march-short", "quads-short", "quads", "pair-waves",
-                              "pair-waves-short", "pair-lines", "pair-lines-short"])
+                              "pair-waves-short", "pair-lines", "pair-lines-short",
+                              "quads-phase2", "quads-phase2-short"])
 @pytest.mark.parametrize("case", [c for c in CASES if len(c[0]) == 3 and c[4] == 1],
                          ids=lambda c: f"{c[0]}-{c[5]}-c{c[6]}e{c[7]}")
 def test_gpu_level_kernel_launch_shapes_bit_exact(case, knobs):
     """The per-class march kernels (tuning knob "pair" off), the pair kernels
     with wave runs instead of line blocks ("pair_lines" off) or with pairs
-    instead of quads ("pair_quads" off), and each with one point per thread
+    instead of quads ("pair_quads" off, or quads in phase 2 only), and each
+    with one point per thread
     ("min_threads" above every level's size) on every 3-D case: the same
     codes, outliers, recon and stream as the oracle's."""
     from latec_amd import autotune
