@@ -81,9 +81,14 @@ def _narrow(codes, radius: int, kexp: int) -> np.ndarray:
 
 
 def _widen(syms, radius: int, kexp: int) -> np.ndarray:
-    span = (1 << int(kexp)) - 1
-    codes = np.asarray(syms).astype(np.int64) - span - 1 + int(radius)
-    return np.where(np.asarray(syms) == 0, 0, codes).astype(np.uint32)
+    """Narrowed symbols -> codes (0 stays 0): sym - span - 1 + radius in
+    uint32 arithmetic (every nonzero symbol of a valid stream widens to a
+    code >= 1)."""
+    syms = np.asarray(syms)
+    off = np.uint32((int(radius) - (1 << int(kexp))) % (1 << 32))
+    codes = np.add(syms, off, dtype=np.uint32, casting="unsafe")
+    codes[syms == 0] = 0
+    return codes
 
 
 def choose_outlier_weight(n_outliers: int, n_codes: int) -> int:
