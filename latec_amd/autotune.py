@@ -124,7 +124,7 @@ def tune(d_x, shape, levels, stride, ebs, opts, reps=3, candidates=None, save=Tr
     plan = get_plan(shape, 1, levels, stride, 1)
     ws = codec.workspace(plan, opts.radius)
     start = get()
-    table, ref_sum, best = [], None, None
+    table, ref, best = [], None, None
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ebs_arr = np.ascontiguousarray(ebs, np.float64)
     stream = torch.cuda.current_stream()
@@ -147,13 +147,12 @@ def tune(d_x, shape, levels, stride, ebs, opts, reps=3, candidates=None, save=Tr
                 ev1.record(stream)
                 torch.cuda.synchronize()
                 ms.append(ev0.elapsed_time(ev1))
-            # identical outputs for every launch shape
-            csum = (int(ws.codes.to(torch.int64).sum()),
-                    int(ws.recon.view(torch.int32).to(torch.int64).sum()),
-                    int(ws.cnt.to(torch.int64).sum()))
-            if ref_sum is None:
-                ref_sum = csum
-            elif csum != ref_sum:
+            # identical outputs for every launch shape: codes, reconstruction
+            # bits and outlier counts equal the first candidate's exactly
+            got = (ws.codes, ws.recon.view(torch.int32), ws.cnt)
+            if ref is None:
+                ref = tuple(t.clone() for t in got)
+            elif not all(torch.equal(a, b) for a, b in zip(got, ref)):
                 raise RuntimeError(f"launch shape {knobs} changed the encode output")
             t = float(np.median(ms))
             table.append({"knobs": knobs, "ms": t})

@@ -43,6 +43,7 @@ from .bitstream import (DTYPE_CODES, DTYPE_IDS, END_MODE_SHIFT, VERSION, VERSION
                         _pack_hlen, parse_record, read_stream, write_stream)
 from .codec import _THREADS, pack_device
 from .levels import stage_ebs
+from .mlrans import check_blob
 from .predictor import InterpPredictor, get_plan
 from .quantizer import opts_for
 
@@ -398,6 +399,7 @@ def _decompress_interp(streams, *, use_graph=True, timings=None):
             if literal:
                 raise ValueError("version-5 stage record has the literal bit set")
             params.append((int(kexp) if n_codes else 1, int(shape), int(owx), int(level)))
+            check_blob(pl, off + 12, hlen, int(n_codes))
             blob_off.append(at + off + 12)
             blob_len.append(hlen)
             n.append(n_codes)

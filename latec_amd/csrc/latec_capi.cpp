@@ -12,6 +12,7 @@ namespace latec {
 static thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
+const std::string &thread_error() { return g_err; }
 void set_hip_error(const char *what, int code) {
   g_err = std::string(what) + ": " + hipGetErrorString((hipError_t)code);
 }

@@ -9,6 +9,8 @@
 namespace latec {
 
 void set_error(const std::string &msg);
+// the calling thread's last error message (what latec_last_error returns there)
+const std::string &thread_error();
 
 // Receives a stage record (bytes, length) on the thread that wrote it.
 using RecordHook = std::function<void(const uint8_t *, int64_t)>;

@@ -24,6 +24,8 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -32,6 +34,29 @@
 namespace latec {
 
 static const int PREC = 24;
+
+// Start fn on a new thread, or run it here when the system cannot start one
+// (std::system_error): the result is the same, only later.  The caller joins
+// `t` (through a guard built before the first start, so an exception never
+// destroys a joinable thread).
+template <typename F>
+static void start_or_run(std::thread &t, F &&fn) {
+  try {
+    t = std::thread(fn);
+  } catch (const std::system_error &) {
+    fn();
+  }
+}
+
+// Joins every joinable thread it holds when it goes out of scope.
+struct JoinAll {
+  std::thread *t;
+  int n;
+  ~JoinAll() {
+    for (int i = 0; i < n; i++)
+      if (t[i].joinable()) t[i].join();
+  }
+};
 
 struct SymInfo {
   uint32_t left, prob;
@@ -379,10 +404,12 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
     const int T = 4;
     bool oks[T];
     std::thread nt[T - 1];
-    for (int t = 1; t < T; t++)
-      nt[t - 1] = std::thread([&, t] { oks[t] = narrow(n * t / T, n * (t + 1) / T); });
-    oks[0] = narrow(0, n / T);
-    for (auto &t : nt) t.join();
+    {
+      JoinAll guard{nt, T - 1};
+      for (int t = 1; t < T; t++)
+        start_or_run(nt[t - 1], [&, t] { oks[t] = narrow(n * t / T, n * (t + 1) / T); });
+      oks[0] = narrow(0, n / T);
+    }
     for (int t = 0; t < T; t++) in_window &= oks[t];
   } else {
     in_window = narrow(0, n);
@@ -400,11 +427,13 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
   std::unique_ptr<uint8_t[]> full_buf;
   int64_t full_len = ZB_STOPPED;
   std::atomic<bool> stop_full(false);
-  std::thread full_thread;
+  std::thread full_thread, rans_thread;
+  // joined on every return path, and built before either thread starts
+  JoinAll join_full{&full_thread, 1}, join_rans{&rans_thread, 1};
   const double bits = (double)fit->bits;
   auto full_over = [bits](size_t at) { return (double)at * 8.0 >= bits; };
   if (big)
-    full_thread = std::thread([&] {
+    start_or_run(full_thread, [&] {
       const size_t bound = ZSTD_compressBound(lit_bytes);
       full_buf.reset(new uint8_t[bound]);
       full_len = zstd_bounded(lit.get(), lit_bytes, zstd_level, full_buf.get(), bound, full_over,
@@ -415,9 +444,8 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
   // longest of the three instead of their sum.
   std::unique_ptr<uint32_t[]> words;
   int64_t nw = -1;
-  std::thread rans_thread;
   if (big && fit->level >= 0 && fit->level < 64)
-    rans_thread = std::thread([&] {
+    start_or_run(rans_thread, [&] {
       words.reset(new uint32_t[(size_t)enc_cap(n)]);
       nw = rans_encode(syms.get(), n, cdf_row, A, words.get(), enc_cap(n));
       // the record this stage writes unless its literal frame wins, handed
@@ -428,12 +456,6 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
         (*rans_record)(rec.get(), put_record(rec.get(), n, nw * 4, kexp, fit, 0, outl, n_out));
       }
     });
-  struct Join {
-    std::thread &t;
-    ~Join() {
-      if (t.joinable()) t.join();
-    }
-  } join{rans_thread}, join_full{full_thread};
   if (big) {
     const size_t bound = ZSTD_compressBound(lit_bytes);
     std::unique_ptr<uint8_t[]> tmp(new uint8_t[bound]);
@@ -441,7 +463,7 @@ int64_t pack_stage(const void *codes, int cb, int64_t n, const float *outl, int6
         lit.get(), lit_bytes, 1, tmp.get(), bound,
         [bits](size_t at) { return (double)at * SCOUT_SLACK >= bits / 8.0; }, nullptr);
     if (scout < 0) stop_full.store(true, std::memory_order_relaxed);
-    full_thread.join();
+    if (full_thread.joinable()) full_thread.join();
     if (scout >= 0 && full_len >= 0) {
       if (full_len > blob_cap) {
         set_error("record capacity");
@@ -604,6 +626,7 @@ int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
   std::vector<std::unique_ptr<uint8_t[]>> recs(ns), spec(ns);
   std::vector<int64_t> rcs(ns, 0), spec_len(ns, -1);
   std::vector<char> done(ns, 0);
+  std::vector<std::string> errs(ns);
   std::vector<int> order(ns);
   for (int i = 0; i < ns; i++) order[i] = i;
   std::sort(order.begin(), order.end(), [&](int a, int b) { return n_codes[a] > n_codes[b]; });
@@ -630,6 +653,8 @@ int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
       const int64_t rc = pack_stage(cs, cb, n_codes[s], outl + out_off[s], n_out[s], R,
                                     zstd_level, &fits[s], cdf_rows + row_off[s], recs[s].get(),
                                     rc_cap, framing ? &hook : nullptr);
+      // the message was set on this worker's thread: keep it for the caller
+      if (rc < 0) errs[s] = thread_error();
       // spec[s] was set (if at all) on a thread pack_stage joined
       const bool differs = spec_len[s] >= 0 &&
                            (rc != spec_len[s] || memcmp(recs[s].get(), spec[s].get(), rc));
@@ -642,9 +667,16 @@ int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
   };
   if (threads < 1) threads = 1;
   threads = std::min(threads, ns);
-  std::vector<std::thread> pool;
-  for (int t = framing ? 0 : 1; t < threads; t++) pool.emplace_back(work);
-  if (!framing) work();
+  std::vector<std::thread> pool(threads);
+  int started = 0;
+  JoinAll join_pool{pool.data(), threads};
+  try {
+    for (int t = framing ? 0 : 1; t < threads; t++, started++) pool[t] = std::thread(work);
+  } catch (const std::system_error &) {
+    // fewer workers: they take stages from the shared counter, so the
+    // ones that started (or this thread, below) finish the rest
+  }
+  if (!framing || !started) work();
   // payload of stage records in stage order: the final record of every
   // stage in `fin`, else its rANS record; false on a stage error or capacity
   int64_t size = 0;
@@ -653,7 +685,8 @@ int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
     for (int s = 0; s < ns; s++) {
       const int64_t len = fin[s] ? rcs[s] : spec_len[s];
       if (len < 0) {
-        set_error("stage " + std::to_string(s) + ": pack failed");
+        set_error("stage " + std::to_string(s) + ": " +
+                  (errs[s].empty() ? std::string("pack failed") : errs[s]));
         return len;
       }
       if (off + len > cap) {
@@ -683,7 +716,8 @@ int64_t pack_stages(int ns, const void *codes, int cb, const int64_t *code_off,
       flen = zstd_bounded(out, (size_t)size, frame_level, frame, (size_t)frame_cap,
                           [](size_t) { return false; }, &spec_broken);
   }
-  for (auto &t : pool) t.join();
+  for (auto &t : pool)
+    if (t.joinable()) t.join();
   if (framing && (flen < 0 || spec_broken.load())) {  // frame again from the final records
     size = assemble(std::vector<char>(ns, 1));
     if (size < 0) return size;

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -936,13 +937,23 @@ __device__ __forceinline__ void sto(T *__restrict__ base, uint32_t idx, T v) {
   *(T *)((char *)base + idx * (uint32_t)sizeof(T)) = v;
 }
 
-// RN(x / 3) without the division sequence: q0 = RN(x * RN(1/3)) is within an
-// ulp of x / 3, the remainder x - 3 q0 is exact under one FMA, and adding
-// remainder * RN(1/3) to q0 (Markstein's remainder correction) rounds to the
-// correctly rounded quotient for every |x| in [2^-968, 2^1020] (checked
-// bit for bit against the division on 4.6e8 operands across that range,
-// including multiples of 3 and their neighbours).  Other values (tiny, huge,
-// non-finite) take the division itself.  Every lane of the wave calls it.
+// RN(x / 3) without the division sequence.  Let c = RN(1/3) = (1/3)(1 - 2^-54)
+// (1/3 = 0.0101...b, and the double below it is the nearest), q0 = RN(x c) and
+// q = RN(q0 + r c) with r = x - 3 q0 from one FMA.  Then
+//   |q0 - x/3| <= |q0 - x c| + |x c - x/3| <= ulp(x/3)/2 + 2^-54 |x/3| < ulp(x/3),
+// because ulp(t) > 2^-53 |t| for every normal t: q0 is within one ulp of x/3.
+// A quotient within one ulp leaves a remainder x - 3 q0 that is a double, so
+// the FMA computes r exactly, and with c within half an ulp of 1/3 Markstein's
+// theorem (P. Markstein, IBM J. Res. Dev. 34(1), 1990, Theorem 1; restated in
+// Muller et al., Handbook of Floating-Point Arithmetic, the division chapter)
+// gives q = RN(x / 3) exactly.  Both steps assume that q0 and r are normal
+// and that nothing overflows: for |x| >= 2^-968, q0 >= 2^-970 and r (a
+// multiple of ulp(q0) >= 2^-1022) is zero or normal; |x| <= 2^1020 keeps 3 q0
+// finite; the correction is one FMA, so r c is never rounded on its own.
+// Other values (tiny, huge, non-finite) take the division itself.  (Also
+// checked bit for bit against the division on 4.6e8 operands across the
+// range, including multiples of 3 and their neighbours.)  Every lane of the
+// wave calls it.
 __device__ __forceinline__ double div3(double x) {
   const double c = 1.0 / 3.0;
   double q = x * c;
@@ -2968,8 +2979,16 @@ static int64_t g_march_peel = 0;  // histogram: distinct codes per wave counted 
 
 static int64_t g_tune_epoch = 0;  // bumped by every knob change
 static int64_t tune_epoch() { return g_tune_epoch; }
+// "quad_launches" (read, or set to reset): line-quad launches made so far, so a
+// caller can tell whether a level ran as quads (launches recorded into a graph
+// count once, when recorded)
+static std::atomic<int64_t> g_quad_launches{0};
 
 int tune_set(const char *name, int64_t v) {
+  if (!strcmp(name, "quad_launches")) {
+    g_quad_launches = v;
+    return LATEC_OK;
+  }
   g_tune_epoch++;
   if (!strcmp(name, "march_blocks")) {
     if (v < 8 || v > (1 << 20)) {
@@ -3039,6 +3058,7 @@ int64_t tune_get(const char *name) {
   if (!strcmp(name, "min_threads")) return g_min_threads;
   if (!strcmp(name, "pair_lines")) return g_pair_lines;
   if (!strcmp(name, "pair_quads")) return g_pair_quads;
+  if (!strcmp(name, "quad_launches")) return g_quad_launches;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -3477,6 +3497,7 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
     const int bits = (encode ? 1 : 0) | (n[2] % 2 == 0 ? 2 : 0) |
                      (encode && o->round_mode != 0 ? 4 : 0) | (ph > 0 && a.set[0].lpb ? 8 : 0) |
                      (!encode || xstep == 1 ? 16 : 0);
+    if (quads_ph[ph]) g_quad_launches++;
     if (quads_ph[ph])
       QuadLaunch<7>::go(ph + 1, bits & 1 | (bits & 4) >> 1 | (bits & 16) >> 2, grid, stream,
                         a, encode ? x : nullptr, recon, (uint16_t *)codes, hist, out_rank,

@@ -242,8 +242,9 @@ __global__ void __launch_bounds__(256) dec_kernel(const Stage *__restrict__ st,
   const uint32_t *blob = (const uint32_t *)(payload + S.scratch);
   if (l == 0 && blob[0] != (uint32_t)L) atomicOr(status, 8);
   const uint32_t w0 = blob[1 + l], w1 = blob[2 + l];
-  // S.word_base holds the blob's length in words: the lane's words must lie in it
-  if (w1 < w0 + 2 || 2 + L + (int64_t)w1 > S.word_base) {
+  // S.word_base holds the blob's length in words: the lane's words must lie in
+  // it (compared in 64 bits, so an offset near 2^32 cannot wrap the test)
+  if ((uint64_t)w1 < (uint64_t)w0 + 2 || 2 + L + (int64_t)w1 > S.word_base) {
     atomicOr(status, 1);
     return;
   }
@@ -571,8 +572,9 @@ int latec_mlrans_decode(int ns, const uint8_t *d_payload, int64_t payload_len,
       (rc = hip_ok(hipMemcpyAsync(d_status, &zero, sizeof zero, hipMemcpyHostToDevice, s),
                    "hipMemcpyAsync")))
     return rc;
-  // each blob's header (L and its lane offsets) is checked on the host side
-  // before launch by the caller; the kernel flags malformed lanes in status
+  // the kernel checks each lane's offsets against its blob before reading
+  // words (latec_amd/mlrans.py also validates every blob header on the host
+  // before this call) and flags malformed lanes in status
   if (code_bytes == 2)
     hipLaunchKernelGGL(dec_kernel<uint16_t>, dim3((unsigned)nbk), dim3(256), 0, s, d_st, d_bk,
                        d_payload, radius, d_rows, (uint16_t *)d_codes, d_status);

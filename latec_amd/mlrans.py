@@ -115,6 +115,26 @@ def pack_payload(res, fits):
     return payload.tobytes(), rec_len, n_out
 
 
+def check_blob(payload, at: int, nbytes: int, n_codes: int) -> None:
+    """Validate a version-5 blob header on the host before any device read:
+    ``[L][L + 1 lane word offsets][words]`` with L the lane count of an
+    ``n_codes``-symbol stage, offsets starting at 0, at least two words (the
+    coder state) per lane, and the last offset ending the blob exactly.
+    Raises ValueError otherwise."""
+    if not n_codes:
+        if nbytes:
+            raise ValueError("multi-lane blob of an empty stage is not empty")
+        return
+    L = int(lib().latec_mlrans_lanes(int(n_codes)))
+    if at % 4 or nbytes % 4 or nbytes < 4 * (2 + 3 * L) or at + nbytes > len(payload):
+        raise ValueError("multi-lane blob is truncated or misaligned")
+    head = np.frombuffer(payload, np.uint32, count=L + 2, offset=at).astype(np.int64)
+    offs = head[1:]
+    if (head[0] != L or offs[0] != 0 or np.any(np.diff(offs) < 2)
+            or 2 + L + offs[-1] != nbytes // 4):
+        raise ValueError("corrupt multi-lane rANS blob header")
+
+
 def decode_payload(payload: bytes, plan, ws):
     """Entropy-decode a version-5 payload into the workspace's device codes,
     outlier ranks, values and counts (``codec.Workspace``)."""
@@ -133,6 +153,7 @@ def decode_payload(payload: bytes, plan, ws):
         params.append((int(kexp) if n_codes else 1, int(shape), int(owx), int(level)))
         blob_off[s] = off + 12
         blob_len[s] = hlen
+        check_blob(payload, off + 12, hlen, int(n_codes))
         (k,) = struct.unpack_from("<I", payload, off + 12 + hlen)
         outs.append(np.frombuffer(payload, np.float32, count=k, offset=off + 16 + hlen))
         off += rlen

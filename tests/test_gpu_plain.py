@@ -110,6 +110,15 @@ CASES = [
     # large enough for the level-fused slab kernel, with partial tiles and segments
     ((70, 45, 133), 1, 3, 8, 1, "cubic", 0, 1, 1e-3, 0.05),
     ((67, 33, 129), 1, 3, 8, 1, "linear", 2, 3, 1e-3, 0.02),
+    # the line lengths of the 512^3 schedule (cubic L5 stride 32): level grids
+    # of 64, 128, 256 and 512 points along the last axis, so the finest level
+    # has 256 pairs (one line per workgroup) and 128 quads per line
+    ((33, 40, 512), 1, 5, 32, 1, "cubic", 0, 1, 1e-3, 0.02),
+    # 257 pairs at the finest level: lines longer than a workgroup (wave runs)
+    ((20, 18, 513), 1, 5, 32, 1, "cubic", 0, 1, 1e-3, 0.02),
+    # line quads with 8 and 16 quads per line, and with 1 and 2
+    ((36, 30, 64), 1, 3, 8, 1, "cubic", 0, 3, 1e-3, 0.05),
+    ((40, 36, 8), 1, 3, 8, 1, "linear", 2, 1, 1e-3, 0.02),
     ((12, 11, 10, 9), 1, 2, 4, 1, "cubic", 1, 1, 1e-3, 0.0),
     ((9, 8, 7, 6, 5), 1, 2, 4, 1, "cubic", 0, 3, 1e-2, 0.0),
     ((300,), 1, 5, 32, 1, "cubic", 0, 1, 1e-4, 0.001),
@@ -299,9 +308,14 @@ def test_gpu_level_kernel_launch_shapes_bit_exact(case, knobs):
     old = autotune.get()
     autotune.set_knobs(pair=knobs[0], min_threads=knobs[1], pair_lines=knobs[2],
                        pair_quads=knobs[3], slab=0)
+    shape, C, levels, stride, block, order, center, end, rel, rough = case
+    quad_ok = knobs[0] == 1 and knobs[2] == 1 and _has_quad_level(shape, levels, stride)
     try:
+        check(lib().latec_tune_set(b"quad_launches", 0))
         test_gpu_encode_matches_oracle_bit_exact(case)
-        shape, C, levels, stride, block, order, center, end, rel, rough = case
+        # the quad kernels really ran where the knobs and the line lengths ask for them
+        enc_quads = int(lib().latec_tune_get(b"quad_launches"))
+        assert (enc_quads > 0) == (quad_ok and bool(knobs[3] & 5)), enc_quads
         x = smooth(shape, seed=0, rough=rough)
         eb = rel * float(x.max() - x.min())
         pred = InterpPredictor(order, levels, stride, 1)
@@ -310,40 +324,99 @@ def test_gpu_level_kernel_launch_shapes_bit_exact(case, knobs):
                                        anchor_block=1)
         ref_stream, ref_recon = O.compress(x, eb, order, levels, stride, 1, center, end)
         assert stream == ref_stream
+        check(lib().latec_tune_set(b"quad_launches", 0))
         assert np.array_equal(latec_amd.decompress(stream), ref_recon[0])
+        dec_quads = int(lib().latec_tune_get(b"quad_launches"))
+        assert (dec_quads > 0) == (quad_ok and bool(knobs[3] & 10)), dec_quads
     finally:
         autotune.set_knobs(**old)
 
 
-def test_gpu_config2_full_size_roundtrip():
-    """512^3, cubic, levels 5, stride 32, eb = 1e-3 * range: size-independent
-    properties at full size (bound, decoder == encoder, stage record counts)."""
-    from latec_amd.synthetic import config2_field
+def _has_quad_level(shape, levels, stride):
+    """Whether a level above the first (the ones the pair and quad kernels
+    run) has lines along the last axis that the quad kernels take: a length
+    that is a multiple of 4, at most 1024.  A level at lattice step s codes
+    the grid x[::s], whose last axis has ceil(n / s) points."""
+    steps = [max(stride >> k, 1) for k in range(2, levels + 1)]
+    lens = [-(-shape[-1] // s) for s in steps]
+    return any(n % 4 == 0 and n <= 1024 for n in lens)
 
-    dev = D.device()
-    xd = config2_field(device=dev).reshape(-1).contiguous()
-    xmin, xmax = float(xd.min()), float(xd.max())
-    eb = 1e-3 * (xmax - xmin)
-    shape = (512, 512, 512)
-    plan = get_plan(shape, 1, 5, 32, 1)
-    pred = InterpPredictor("cubic", 5, 32, 1)
+
+def _config2_against_oracle(xd, shape):
+    """Encode the device field xd (cubic L5 stride 32, block 1, eb = 1e-3 *
+    range, fast mode) with the stage kernels and with the oracle on its host
+    copy: reconstruction bits, every stage's codes, the outliers (through the
+    payload bytes) and the payload itself must be equal.  Returns what the
+    callers check further."""
     from latec_amd.levels import stage_ebs
 
-    ebs = stage_ebs(shape, 5, 32, 1, eb, 0.9)
+    x = D.to_host(xd).reshape(1, *shape)
+    eb = 1e-3 * (float(x.max()) - float(x.min()))
+    plan = get_plan(shape, 1, 5, 32, 1)
+    pred = InterpPredictor("cubic", 5, 32, 1)
     o = codec._interp_opts(pred, 3, 1 << 15, False)
+    ebs = stage_ebs(shape, 5, 32, 1, eb, 0.9)
+    ref_ebs = O.stage_ebs_ref(3, 5, 32, eb, 0.9)
+    assert list(ebs) == list(ref_ebs)
+    print("oracle encode", shape, flush=True)
+    ref_payload, ref_recon, _ = O.compress_payload(x, ref_ebs, 5, 32, 1, "cubic", o.center,
+                                                   o.end_mode)
+    print("GPU encode", shape, flush=True)
     res = codec.encode_device(xd, plan, ebs, o)
+    torch.cuda.synchronize()
+    recon = D.to_host(res.d_recon).reshape(x.shape)
+    assert np.array_equal(recon.view(np.uint32), ref_recon.view(np.uint32))
+    del recon, ref_recon
     fits = codec.fit_device(res)
-    payload, rec_len, n_out = codec.pack_device(res, fits, 9)
-    # the framed pack (outer frame started on the large stages' rANS records)
-    # writes the same payload and the one-shot zstd-9 frame of it
-    from latec_amd.bitstream import zstd_compress
+    payload, rec_len, n_out, frame = codec.pack_device_framed(res, fits, 9)
+    assert payload == ref_payload
+    return x, eb, plan, ebs, o, res, payload, rec_len, n_out, frame
 
-    payload_f, rec_len_f, _, frame = codec.pack_device_framed(res, fits, 9)
-    assert payload_f == payload and np.array_equal(rec_len_f, rec_len)
+
+def test_gpu_config2_slab_matches_oracle():
+    """The first 64 planes of the 512^3 field (64 x 512 x 512, cubic L5
+    stride 32): codes of every stage, reconstruction, payload and the whole
+    stream through the public API equal the oracle's."""
+    from latec_amd.synthetic import config2_field
+
+    shape = (64, 512, 512)
+    xd = config2_field(device=D.device())[:64].contiguous().reshape(-1)
+    x, eb, plan, ebs, o, res, payload, _, _, _ = _config2_against_oracle(xd, shape)
+    ref = _oracle_stage_codes(payload, plan.counts, 1)
+    codes = D.to_host(res.d_codes, np.uint16)
+    for s in range(plan.n_stages):
+        lo, n = plan.offsets[s], plan.counts[s]
+        assert np.array_equal(codes[lo:lo + n].astype(np.uint32), ref[s][0]), f"stage {s}"
+    pred = InterpPredictor("cubic", 5, 32, 1)
+    stream, stats = latec_amd.compress(x[0], eb, pred, levels=5, anchor_stride=32,
+                                       anchor_block=1)
+    ref_stream, ref_recon = O.compress(x[0], eb, "cubic", 5, 32, 1, None, 1)
+    assert stream == ref_stream
+    dec = latec_amd.decompress(stream)
+    assert np.array_equal(dec.view(np.uint32), ref_recon[0].view(np.uint32))
+    assert float(np.abs(dec - x[0]).max()) <= eb
+
+
+@pytest.mark.timeout(900)
+def test_gpu_config2_full_size_matches_oracle():
+    """512^3, cubic, levels 5, stride 32, eb = 1e-3 * range (the benchmarked
+    field): reconstruction and payload bytes equal the oracle's (about a
+    minute of oracle time), the framed pack writes the same payload and the
+    one-shot zstd-9 frame of it, every value is within the bound and the
+    decoder reproduces the encoder's reconstruction."""
+    from latec_amd.bitstream import zstd_compress
+    from latec_amd.synthetic import config2_field
+
+    shape = (512, 512, 512)
+    xd = config2_field(device=D.device()).reshape(-1).contiguous()
+    x, eb, plan, ebs, o, res, payload, rec_len, n_out, frame = \
+        _config2_against_oracle(xd, shape)
+    del x
     assert frame == zstd_compress(payload, 9)
+    p2, rec_len2, _ = codec.pack_device(res, codec.fit_device(res), 9)
+    assert p2 == payload and np.array_equal(rec_len2, rec_len)
     enc_recon = res.d_recon.clone()
-    err = (enc_recon - xd).abs().max().item()
-    assert err <= eb
+    assert (enc_recon - xd).abs().max().item() <= eb
     d_recon = codec.decode_device(payload, plan, ebs, o)
     assert torch.equal(d_recon, enc_recon)
     print("C2 payload bytes", len(payload), "bits/value", 8 * len(payload) / xd.numel(),

@@ -182,3 +182,48 @@ def test_gpu_v5_rejects_corrupt_blob():
     print("case v5 corrupt", flush=True)
     with pytest.raises(ValueError):
         latec_amd.decompress(bytes(bad))
+
+
+def test_host_blob_check_rejects_wrapping_lane_offsets():
+    """A crafted version-5 blob whose lane offsets wrap a 32-bit sum (a lane
+    starting at word 0xFFFFFFFE) is rejected on the host, before any device
+    read: by check_blob directly, and by decode_payload while it parses the
+    records (it raises before it touches the workspace)."""
+    from latec_amd import mlrans
+    from latec_amd.predictor import get_plan
+
+    cdf = _cdf()
+    n = 3 * 4096 + 5  # four lanes
+    syms = _laplace_syms(n, 6, 3.0, 11)
+    blob = O.mlrans_encode_syms(syms, cdf)
+    L = O.mlrans_lanes(n)
+    assert L == 4 and blob[0] == L
+    raw = blob.astype("<u4").tobytes()
+    mlrans.check_blob(raw, 0, len(raw), n)  # the genuine blob passes
+    for lane, value in ((1, 0xFFFFFFFE), (2, 0xFFFFFFFF), (0, 3), (L, int(blob[L]) + 1)):
+        bad = blob.copy()
+        bad[1 + lane] = value
+        with pytest.raises(ValueError, match="multi-lane"):
+            mlrans.check_blob(bad.astype("<u4").tobytes(), 0, len(raw), n)
+    # wrong lane count, truncation, misalignment
+    bad = blob.copy()
+    bad[0] = L + 1
+    with pytest.raises(ValueError):
+        mlrans.check_blob(bad.astype("<u4").tobytes(), 0, len(raw), n)
+    with pytest.raises(ValueError):
+        mlrans.check_blob(raw[:-4], 0, len(raw) - 4, n)
+    with pytest.raises(ValueError):
+        mlrans.check_blob(b"\0\0" + raw, 2, len(raw), n)
+    # through decode_payload: a 1-D plan whose stages are all rANS records
+    plan = get_plan((64,), 1, 2, 4, 1)
+    recs = []
+    for s in range(plan.n_stages):
+        k = plan.counts[s]
+        codes = np.full(k, 1 << 15, np.uint32)
+        recs.append(O.pack_stage_v5(codes, np.zeros(0, np.float32)))
+    s_last = max(range(plan.n_stages), key=lambda s: plan.counts[s])
+    rec = bytearray(recs[s_last])
+    struct.pack_into("<I", rec, 12 + 4, 0xFFFFFFFE)  # lane 0's end offset
+    recs[s_last] = bytes(rec)
+    with pytest.raises(ValueError, match="multi-lane"):
+        mlrans.decode_payload(b"".join(recs), plan, None)
