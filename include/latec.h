@@ -460,6 +460,23 @@ int latec_mlrans_encode(int ns, const void *d_codes, int code_bytes,
                         uint32_t *d_scratch, int64_t scratch_words,
                         uint8_t *d_out, int64_t out_cap, int64_t *blob_off,
                         int64_t *blob_len, void *stream);
+/* The same encode with no workspace argument and no device allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its workspace is a fixed device array of
+ * latec_mlrans_noalloc_words() = 2^23 u32 words (32 MiB) per GPU, held
+ * until the process ends; stages whose latec_mlrans_scratch_words exceeds
+ * it are refused with LATEC_E_CAPACITY before anything runs.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialized by the caller.  Output and
+ * errors as latec_mlrans_encode; synchronizes the stream. */
+int64_t latec_mlrans_noalloc_words(void);
+int latec_mlrans_encode_noalloc(int ns, const void *d_codes, int code_bytes,
+                                const int64_t *code_off, const int64_t *n,
+                                const int32_t *kexp, uint32_t radius,
+                                const uint32_t *d_rows, const int64_t *row_off,
+                                uint8_t *d_out, int64_t out_cap, int64_t *blob_off,
+                                int64_t *blob_len, void *stream);
 /* Decode the blobs at blob_off / blob_len (bytes, in the device payload)
  * into d_codes at code_off; LATEC_E_FORMAT for a malformed blob.
  * Synchronizes the stream. */

@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "latec_common.h"
@@ -383,6 +384,38 @@ static int64_t enc_scratch_words(int ns, int64_t need, int64_t lanes) {
          al4((int64_t)(sizeof(Stage) / 4) * ns) + al4(lanes) + 4;
 }
 
+
+// Workspace of latec_mlrans_encode_noalloc: a fixed device array, one copy per
+// GPU for the life of the process (2^23 words = 32 MiB).  Host code never
+// passes the array's name to a kernel or a copy (in host code it is only a
+// stand-in for the symbol): it uses the device address hipGetSymbolAddress
+// returns, looked up once per device with that device current.
+static constexpr int64_t NOALLOC_WORDS = (int64_t)1 << 23;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static uint32_t *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch` (looked up on its first use).
+static int noalloc_scratch(uint32_t **out) {
+  int dev = 0;
+  int rc = hip_ok(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("latec_mlrans_encode_noalloc: device index out of range");
+    return LATEC_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    if ((rc = hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")))
+      return rc;
+    g_scratch_ptr[dev] = (uint32_t *)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return LATEC_OK;
+}
+
 extern "C" {
 
 int64_t latec_mlrans_lanes(int64_t n) { return lanes_for(n); }
@@ -509,6 +542,36 @@ int latec_mlrans_encode(int ns, const void *d_codes, int code_bytes, const int64
     if ((rc = hip_ok(hipGetLastError(), "mlrans assemble_kernel"))) return rc;
   }
   return hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors end here
+}
+
+int64_t latec_mlrans_noalloc_words(void) { return NOALLOC_WORDS; }
+
+int latec_mlrans_encode_noalloc(int ns, const void *d_codes, int code_bytes,
+                                const int64_t *code_off, const int64_t *n, const int32_t *kexp,
+                                uint32_t radius, const uint32_t *d_rows, const int64_t *row_off,
+                                uint8_t *d_out, int64_t out_cap, int64_t *blob_off,
+                                int64_t *blob_len, void *stream) {
+  if (ns < 0 || !n) {
+    set_error("latec_mlrans_encode_noalloc: bad arguments");
+    return LATEC_E_ARG;
+  }
+  for (int i = 0; i < ns; i++)
+    if (n[i] < 0 || n[i] > NOALLOC_WORDS) {
+      set_error("latec_mlrans_encode_noalloc: bad stage size");
+      return LATEC_E_ARG;
+    }
+  const int64_t need = latec_mlrans_scratch_words(ns, n);
+  if (need > NOALLOC_WORDS) {
+    set_error("latec_mlrans_encode_noalloc: these stages need " + std::to_string(need) +
+              " workspace words, more than its fixed " + std::to_string(NOALLOC_WORDS) +
+              "; use latec_mlrans_encode with a workspace");
+    return LATEC_E_CAPACITY;
+  }
+  uint32_t *ws = nullptr;
+  const int rc = noalloc_scratch(&ws);
+  if (rc) return rc;
+  return latec_mlrans_encode(ns, d_codes, code_bytes, code_off, n, kexp, radius, d_rows, row_off,
+                             ws, NOALLOC_WORDS, d_out, out_cap, blob_off, blob_len, stream);
 }
 
 int latec_mlrans_decode(int ns, const uint8_t *d_payload, int64_t payload_len,
