@@ -196,8 +196,13 @@ int latec_fit_codes(const uint32_t *d_hist, const void *d_codes, int64_t n,
                     uint32_t radius, int code_bytes, int precision,
                     latec_fit *fit, void *stream);
 
-/* scale_to_level (rans.py:270-278) on device: levels u8. */
-int latec_scale_to_level(const float *d_scale, int64_t n, double eb,
+/* scale_to_level (rans.py:270-278) on device: levels u8, by comparison with
+ * 63 ascending positive float32 thresholds (host array), the smallest float32
+ * at each level 1..63 under the host's NumPy scale_to_level for this eb, found
+ * by bisection over float32 bit patterns as latec/gnn_codec.py:827-861 does
+ * (latec_amd.rans.scale_thresholds).  Level = thresholds at or below the
+ * scale, so the device levels equal the host's NumPy levels exactly. */
+int latec_scale_to_level(const float *d_scale, int64_t n, const float *thresholds,
                          uint8_t *d_levels, void *stream);
 
 /* ------------------------------------------------------------------ */

@@ -89,7 +89,7 @@ SIGNATURES = {
     "latec_fit_const": (i32, [p, i32, i32, p, p, u32, i32, i32, p, p]),
     "latec_fit_codes": (i32, [p, p, i64, u32, i32, i32, p, p]),
     "latec_fit_stages": (i32, [i32, p, p, p, p, u32, i32, i32, p, p]),
-    "latec_scale_to_level": (i32, [p, i64, f64, p, p]),
+    "latec_scale_to_level": (i32, [p, i64, p, p, p]),
     "latec_rans_encode": (i64, [p, i64, p, i64, p, i64]),
     "latec_rans_encode_levels": (i64, [p, p, i64, p, i64, p, i64]),
     "latec_rans_decode": (i32, [p, i64, p, i64, p, i64]),

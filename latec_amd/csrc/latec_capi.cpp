@@ -43,7 +43,7 @@ int neg_log2_weights(int, int, int, int, float *, hipStream_t);
 int build_cdfs(int, int, int, int, int, int, uint32_t *, hipStream_t);
 int fit_hists(int, const uint32_t *, const void *, const int64_t *, const int64_t *, uint32_t,
               int, int, latec_fit *, hipStream_t);
-int scale_to_level(const float *, int64_t, double, uint8_t *, hipStream_t);
+int scale_to_level(const float *, int64_t, const float *, uint8_t *, hipStream_t);
 int64_t hist_words(uint32_t R);
 // latec_entropy.cpp
 int64_t rans_encode(const uint32_t *, int64_t, const uint32_t *, int64_t, uint32_t *, int64_t);
@@ -264,9 +264,9 @@ int latec_fit_codes(const uint32_t *d_hist, const void *d_codes, int64_t n, uint
 
 int64_t latec_hist_words(uint32_t radius) { return hist_words(radius); }
 
-int latec_scale_to_level(const float *d_scale, int64_t n, double eb, uint8_t *d_levels,
-                         void *stream) {
-  return scale_to_level(d_scale, n, eb, d_levels, S(stream));
+int latec_scale_to_level(const float *d_scale, int64_t n, const float *thresholds,
+                         uint8_t *d_levels, void *stream) {
+  return scale_to_level(d_scale, n, thresholds, d_levels, S(stream));
 }
 
 int64_t latec_rans_encode(const uint32_t *syms, int64_t n, const uint32_t *cdf, int64_t A,

@@ -266,24 +266,27 @@ __device__ __forceinline__ int stage_of(const int64_t *c, int ndim, int S) {
 }
 
 // Usable known sample at chunk-local coordinates c before stage g.stage.
-__device__ __forceinline__ bool usable(const GeomArgs &g, const int64_t *c) {
+// A cell of the frame is known before the stage when its stage in the
+// schedule's periodic pattern is earlier (stage_of works on residues mod the
+// stride, so halo cells take the pattern too).  A known cell is usable when
+// it lies in the chunk, or, in a halo frame, inside the tensor and the frame
+// and in a coded chunk or on the anchor lattice (gnn_predictor.py:1241-1275,
+// 1338-1356).  Returns 0 (not known), 1 (known, unusable), 2 (usable).
+__device__ __forceinline__ int cell_state(const GeomArgs &g, const int64_t *c) {
+  const int st = stage_of(c, g.ndim, g.levels_log2);
+  if (st >= g.stage) return 0;
   bool inside = true;
   for (int j = 0; j < g.ndim; j++) inside &= (c[j] >= 0 && c[j] < g.cshape[j]);
-  if (!inside) {
-    if (!g.has_halo) return false;
-    // halo: inside the tensor, in a coded chunk or an anchor
-    int64_t cid = 0;
-    for (int j = 0; j < g.ndim; j++) {
-      const int64_t gc = g.origin[j] + c[j];
-      if (gc < 0 || gc >= g.tshape[j]) return false;
-      if (c[j] < -g.pad_lo || c[j] >= g.cshape[j] + g.pad_lo) return false;
-      cid = cid * g.grid[j] + gc / g.edges[j];
-    }
-    const int st = stage_of(c, g.ndim, g.levels_log2);
-    if (st >= g.stage) return false;
-    return st == 0 || g.coded[cid];
+  if (inside) return 2;
+  if (!g.has_halo) return 1;
+  int64_t cid = 0;
+  for (int j = 0; j < g.ndim; j++) {
+    const int64_t gc = g.origin[j] + c[j];
+    if (gc < 0 || gc >= g.tshape[j]) return 1;
+    if (c[j] < -g.pad_lo || c[j] >= g.cshape[j] + g.pad_lo) return 1;
+    cid = cid * g.grid[j] + gc / g.edges[j];
   }
-  return stage_of(c, g.ndim, g.levels_log2) < g.stage;
+  return (st == 0 || g.coded[cid]) ? 2 : 1;
 }
 
 // One thread per query point of the stage (enumerated class by class).
@@ -320,10 +323,16 @@ __global__ void geometry_kernel(const ClassDesc *__restrict__ cls, GeomArgs g,
     int res[2];
     for (int side = 0; side < 2; side++) {
       const int sg = side ? -1 : 1;
+      // the nearest known cell along the line (gnn_predictor.py:139-176); the
+      // line is live only when that cell is usable
       int hit = 0;
-      for (int t = 1; t <= g.limit && !hit; t++) {
+      for (int t = 1; t <= g.limit; t++) {
         for (int j = 0; j < g.ndim; j++) nb[j] = coord[j] + (int64_t)sg * t * dirs[l * g.ndim + j];
-        if (usable(g, nb)) hit = t;
+        const int cs = cell_state(g, nb);
+        if (cs) {
+          hit = cs == 2 ? t : 0;
+          break;
+        }
       }
       res[side] = hit;
     }

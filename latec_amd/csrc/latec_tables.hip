@@ -811,33 +811,47 @@ int fit_hists(int nst, const uint32_t *d_hist, const void *d_codes,
   return LATEC_OK;
 }
 
-// scale_to_level (rans.py:270-278) on device.
-__global__ void scale_to_level_kernel(const float *s, int64_t n, float flo, float fhi,
-                                      double llo, double den, uint8_t *out) {
+// scale_to_level (rans.py:270-278) on device by comparisons only: the level of
+// v is the number of the 63 float32 thresholds at or below it (thr[k] = the
+// smallest float32 the host's NumPy scale_to_level puts at level k + 1), so the
+// device reproduces the host's float32 log rounding without evaluating a log
+// (the bucketization of latec/gnn_codec.py:827-861).  A NaN scale compares
+// below every threshold: level 0.
+struct ScaleThr {
+  float t[64];  // t[63] = +inf (a sentinel above every finite scale)
+};
+__global__ void scale_to_level_kernel(const float *s, int64_t n, ScaleThr thr, uint8_t *out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    float v = s[i];
-    v = v < flo ? flo : (v > fhi ? fhi : v);
-    // float32 log of the clipped scale, promoted to float64 (NEP 50)
-    const double t = ((double)(float)log((double)v) - llo) / den;
-    double r = rint(t * 63.0);
-    r = r < 0 ? 0 : (r > 63 ? 63 : r);
-    out[i] = (uint8_t)r;
+    const float v = s[i];
+    // branch-free search: largest c in [0, 63] with thr[c - 1] <= v
+    int c = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+      c += (v >= thr.t[c + step - 1]) ? step : 0;
+    out[i] = (uint8_t)c;
   }
 }
 
-int scale_to_level(const float *d_scale, int64_t n, double eb, uint8_t *d_levels,
+int scale_to_level(const float *d_scale, int64_t n, const float *thresholds, uint8_t *d_levels,
                    hipStream_t st) {
-  if (!(eb > 0)) {
-    set_error("eb must be > 0");
+  if (!thresholds) {
+    set_error("scale_to_level: thresholds required");
     return LATEC_E_ARG;
   }
+  ScaleThr thr;
+  for (int k = 0; k < 63; k++) {
+    thr.t[k] = thresholds[k];
+    if (!(thr.t[k] > 0.0f) || (k && thr.t[k] < thr.t[k - 1])) {
+      set_error("scale level thresholds must be positive and ascending");
+      return LATEC_E_ARG;
+    }
+  }
+  thr.t[63] = INFINITY;
   if (!n) return LATEC_OK;
-  const double lo = eb / 16.0, hi = eb * 4096.0;
-  const double llo = std::log(lo), den = std::log(hi) - llo;
   unsigned blocks = (unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
-  hipLaunchKernelGGL(scale_to_level_kernel, dim3(blocks), dim3(256), 0, st, d_scale, n,
-                     (float)lo, (float)hi, llo, den, d_levels);
+  hipLaunchKernelGGL(scale_to_level_kernel, dim3(blocks), dim3(256), 0, st, d_scale, n, thr,
+                     d_levels);
   return hip_ok(hipGetLastError(), "scale_to_level_kernel");
 }
 

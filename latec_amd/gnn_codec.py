@@ -205,6 +205,22 @@ def _gate_scale_thresholds(eb: float):
     return np.ascontiguousarray(lo_bits.view(np.float32)), np.ascontiguousarray(grid)
 
 
+def gate_costs_device(model, x, pred, scale, cands, n: int, eb: float, radius: int, tabs):
+    """The gate's rate proxy on the device (gnn_codec.py:777-797, 920-971): a
+    (1 + 4 * len(cands), 13) float64 array of bit sums, row 0 the model's
+    prediction, then each interpolation candidate at scale shifts 0, 2, 4, 6,
+    in 13 buckets of the scale against eb * 2^T.  ``tabs`` is
+    _gate_scale_thresholds(eb)."""
+    thr, grid = tabs
+    W = np.zeros((1 + 4 * len(cands)) * 13, np.float64)
+    ipa = (ctypes.c_void_p * len(cands))(*[ptr(c).value for c in cands])
+    check(lib().latec_gnn_gate_costs(model.handle, ptr(x), ptr(pred), ptr(scale),
+                                     ctypes.cast(ipa, ctypes.c_void_p), len(cands), int(n),
+                                     float(eb), int(radius), ptr(thr), ptr(grid), ptr(W),
+                                     ptr(None)))
+    return W.reshape(-1, 13)
+
+
 def _gate_select(W: np.ndarray):
     """Branch-free choice of (kind, dir, T, shift) from the per-bucket rate
     sums W (1 + 12 candidates, 13 buckets), gnn_codec.py:920-971."""
@@ -482,16 +498,10 @@ class GNNCompressorCodec:
                     else:
                         if ebs[s] not in gate_tabs:
                             gate_tabs[ebs[s]] = _gate_scale_thresholds(ebs[s])
-                        thr, grid = gate_tabs[ebs[s]]
-                        W = np.zeros(13 * 13, np.float64)
-                        ips = (ptr(cands[1]).value, ptr(cands[2]).value, ptr(cands[3]).value)
-                        ipa = (ctypes.c_void_p * 3)(*ips)
-                        check(lib().latec_gnn_gate_costs(pred_obj.model.handle, ptr(x_st),
-                                                         ptr(p), ptr(sc),
-                                                         ctypes.cast(ipa, ctypes.c_void_p), 3, n,
-                                                         float(ebs[s]), radius, ptr(thr),
-                                                         ptr(grid), ptr(W), ptr(None)))
-                        gk, gd, gt, gs = _gate_select(W.reshape(13, 13))
+                        W = gate_costs_device(pred_obj.model, x_st, p, sc,
+                                              (cands[1], cands[2], cands[3]), n,
+                                              float(ebs[s]), radius, gate_tabs[ebs[s]])
+                        gk, gd, gt, gs = _gate_select(W)
                         word = (gd << _GATE_DIR_SHIFT) + gk * 256 + gt * 16 + gs
                         if gate_fine and is_finest[s]:
                             fine_desc = (gk, gd, gt, gs)
@@ -510,7 +520,7 @@ class GNNCompressorCodec:
                                              ptr(None)))
                 recon.view(-1)[gidx] = rec
                 lv = T.empty(n, dtype=T.uint8, device=D.device())
-                check(lib().latec_scale_to_level(ptr(sc), n, float(ebs[s]), ptr(lv), ptr(None)))
+                R.scale_to_level_device(sc, n, float(ebs[s]), lv)
                 h_codes = D.to_host(codes, np.uint16 if o.code_bytes == 2 else np.uint32)
                 h_val = D.to_host(val[:k]) if k else np.zeros(0, np.float32)
                 h_lv = D.to_host(lv, np.uint8)
@@ -623,7 +633,7 @@ class GNNCompressorCodec:
                                                              float(ebs[s]) * 2.0 ** gt,
                                                              float(2.0 ** -gs), gd, ptr(None)))
                 lv = T.empty(n, dtype=T.uint8, device=D.device())
-                check(lib().latec_scale_to_level(ptr(sc), n, float(ebs[s]), ptr(lv), ptr(None)))
+                R.scale_to_level_device(sc, n, float(ebs[s]), lv)
                 h_lv = D.to_host(lv, np.uint8)
                 codes, outl, off = unpack_stage(payload, off, rans_levels=h_lv,
                                                 rans_tables=tables[s], radius=radius)

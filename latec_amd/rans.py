@@ -188,14 +188,63 @@ def const_tables(radius: int) -> RansTables:
     return build_laplace_tables(1.0, int(radius))
 
 
+def scale_to_level_host(scale, eb: float) -> np.ndarray:
+    """rans.py:270-278 on the host with NumPy: the definition the device levels
+    reproduce (its float32 log is NumPy's, whose rounding depends on the
+    machine's SIMD code path)."""
+    if eb <= 0:
+        raise ValueError("eb must be > 0")
+    scale = np.asarray(scale, np.float32)
+    lo, hi = float(eb) / SCALE_LO_DIV, float(eb) * SCALE_HI_MULT
+    t = (np.log(np.clip(scale, lo, hi)) - np.log(lo)) / (np.log(hi) - np.log(lo))
+    return np.rint(t * (N_LEVELS - 1)).clip(0, N_LEVELS - 1).astype(np.uint8)
+
+
+@lru_cache(maxsize=1024)
+def _thresholds(eb: float) -> bytes:
+    # Non-negative float32 values order like their bit patterns as unsigned
+    # integers, so the first pattern at or above each level is found by
+    # bisection over [0, bits(f32(eb * 4096))] (gnn_codec.py:827-861 does the
+    # same search).  All 63 levels move together; 32 halvings close a 2^31 span.
+    want = np.arange(1, N_LEVELS, dtype=np.int64)
+    lo = np.zeros(N_LEVELS - 1, np.int64)
+    hi = np.full(N_LEVELS - 1, int(np.float32(float(eb) * SCALE_HI_MULT).view(np.uint32)),
+                 np.int64)
+    for _ in range(33):
+        mid = (lo + hi) // 2
+        at = scale_to_level_host(mid.astype(np.uint32).view(np.float32), eb) >= want
+        hi = np.where(at, mid, hi)
+        lo = np.where(at, lo, mid + 1)
+    assert np.all(lo == hi)
+    return lo.astype(np.uint32).view(np.float32).tobytes()
+
+
+def scale_thresholds(eb: float) -> np.ndarray:
+    """The 63 float32 thresholds of ``latec_scale_to_level`` for this eb: entry k
+    is the smallest float32 that scale_to_level_host puts at level k + 1."""
+    if eb <= 0:
+        raise ValueError("eb must be > 0")
+    return np.frombuffer(_thresholds(float(eb)), np.float32)
+
+
+def scale_to_level_device(d_scale, n: int, eb: float, d_levels, stream=None) -> None:
+    """Device levels (u8) of n device float32 scales: equal to
+    scale_to_level_host on every value (asynchronous on the stream)."""
+    thr = np.ascontiguousarray(scale_thresholds(eb))
+    check(lib().latec_scale_to_level(ptr(d_scale), int(n), ptr(thr), ptr(d_levels),
+                                     ptr(stream)))
+
+
 def scale_to_level(scale, eb: float, n_levels: int = 64) -> np.ndarray:
     """rans.py:270-278 evaluated on the GPU."""
     if eb <= 0:
         raise ValueError("eb must be > 0")
+    if n_levels != N_LEVELS:
+        raise ValueError("only 64 scale levels are supported")
     s = np.ascontiguousarray(scale, np.float32).ravel()
     ds = D.to_device(s)
     out = D.empty(len(s), np.uint8)
-    check(lib().latec_scale_to_level(ptr(ds), len(s), float(eb), ptr(out), ptr(None)))
+    scale_to_level_device(ds, len(s), eb, out)
     D.torch().cuda.synchronize()
     return D.to_host(out).reshape(np.shape(scale))
 

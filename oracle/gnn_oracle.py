@@ -84,6 +84,30 @@ def scan_steps(known: np.ndarray, coords: np.ndarray, dirs, max_radius: int,
     return sp, sn
 
 
+def frame_scan_steps(known, usable, coords, dirs, limit):
+    """Chunk-frame geometry (gnn_predictor.py:139-176 with the validity of
+    :1241-1275): along +d and -d the first cell within ``limit`` steps that
+    is ``known`` (the schedule's periodic pattern before the stage, halo
+    included); the step counts only when that cell is ``usable`` (interior,
+    or a decoded halo cell), else the line is dead (0).  The frame must hold
+    every cell within ``limit`` steps of a query.  Returns (sp, sn) (L, M)."""
+    M = coords.shape[0]
+    sp = np.zeros((len(dirs), M), np.int64)
+    sn = np.zeros((len(dirs), M), np.int64)
+    for li, d in enumerate(dirs):
+        for out, sgn in ((sp, 1), (sn, -1)):
+            dv = sgn * np.asarray(d)
+            done = np.zeros(M, bool)
+            for t in range(1, int(limit) + 1):
+                nb = tuple((coords + t * dv).T)
+                hit = known[nb] & ~done
+                out[li, hit & usable[nb]] = t
+                done |= hit
+                if done.all():
+                    break
+    return sp, sn
+
+
 def stage_geometry(known, coords, shape, dirs, max_radius):
     """Per-stage geometry dict: query flat index, steps, line offsets."""
     sp, sn = scan_steps(known, coords, dirs, max_radius)

@@ -110,3 +110,31 @@ def test_laplace_nll_matches_cdf():
     mass = laplace.cdf(t + eb, mu, b) - laplace.cdf(t - eb, mu, b)
     ok = mass > 1e-9  # inside the uncompressed range (< ~30 bits)
     np.testing.assert_allclose(bits[ok], -np.log2(mass[ok]), rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("shape,levels", [((13, 11), 2), ((9, 10, 7), 2), ((5, 6, 5, 4), 1)])
+def test_frame_scan_equals_whole_field_scan_on_one_chunk(shape, levels):
+    """The chunk-frame rule (first pattern-known cell, live only if usable)
+    reduces to the whole-field scan when the frame is the tensor plus a halo
+    that nothing may use: a line that leaves the tensor never re-enters it."""
+    from oracle import oracle as O
+
+    stride = 1 << levels
+    geoms, dirs = G.schedule_geoms(shape, levels, stride, 1, stride, 2)
+    tile = np.full((2 * stride,) * len(shape), -1)
+    for s, (m, _, _, _) in enumerate(O.stage_plan(tile.shape, levels, stride, 1)):
+        tile[m] = s
+    tile = tile[(slice(0, stride),) * len(shape)]
+    fshape = tuple(n + 2 * stride for n in shape)
+    idx = np.indices(fshape).reshape(len(shape), -1).T - stride
+    inside = np.all((idx >= 0) & (idx < np.asarray(shape)), 1).reshape(fshape)
+    gi = 0
+    for s, (m, _, _, _) in enumerate(O.stage_plan(shape, levels, stride, 1)):
+        if not m.any():
+            continue
+        known = (tile[tuple((idx % stride).T)] < s).reshape(fshape)
+        coords = np.stack(np.nonzero(m), 1)
+        sp, sn = G.frame_scan_steps(known, inside, coords + stride, dirs, stride)
+        assert np.array_equal(sp, geoms[gi]["sp"]) and np.array_equal(sn, geoms[gi]["sn"])
+        gi += 1
+    assert gi == len(geoms)

@@ -83,11 +83,25 @@ def test_gpu_fit_matches_reference_golden():
         assert abs(fit.bits - f["bits"]) <= 3e-7 * max(1.0, f["bits"]), f
 
 
-def test_gpu_scale_to_level_matches_golden():
+def test_gpu_scale_to_level_exact():
+    """Device levels equal the host NumPy scale_to_level on every value, with
+    no tolerance: on the golden scales (and the golden levels themselves where
+    this machine's NumPy float32 log rounds as the machine that wrote them
+    did), and on every threshold and its float32 neighbours."""
     fv = np.load(os.path.join(G, "fit_vectors.npz"))
+    rng = np.random.RandomState(5)
     for i, eb in enumerate(fv["scale_ebs"]):
-        got = rans.scale_to_level(fv["scales"], eb)
-        assert (got != fv["scale_levels"][i]).sum() <= 2  # float32 log ulps
+        eb = float(eb)
+        host = rans.scale_to_level_host(fv["scales"], eb)
+        assert np.array_equal(rans.scale_to_level(fv["scales"], eb), host)
+        if np.array_equal(host, fv["scale_levels"][i]):
+            assert np.array_equal(rans.scale_to_level(fv["scales"], eb), fv["scale_levels"][i])
+        t = rans.scale_thresholds(eb)
+        v = np.concatenate([t, np.nextafter(t, 0, dtype=np.float32),
+                            np.nextafter(t, np.inf, dtype=np.float32),
+                            (eb * np.exp(rng.uniform(-4, 9.5, 100000))).astype(np.float32),
+                            np.array([0, -1, np.inf, eb / 16, eb * 4096], np.float32)])
+        assert np.array_equal(rans.scale_to_level(v, eb), rans.scale_to_level_host(v, eb))
 
 
 def _oracle_stage_codes(payload, counts, C):

@@ -112,3 +112,24 @@ def test_tile_histogram_accumulation_is_float32_per_tile():
     for a in range(0, n, 1 << 22):
         ref = np.float32(np.float64(ref) + min(1 << 22, n - a))
     assert h[2] == ref
+
+
+def test_scale_thresholds_reproduce_host_levels():
+    """The bisected thresholds the device compares against: the number of
+    thresholds at or below a scale equals the host NumPy level of that scale
+    (on the golden scales, on random scales over the whole clipped range and
+    past it, and on every threshold and its float32 neighbours); on this
+    machine the host levels also equal the golden ones."""
+    from latec_amd import rans as R
+
+    rng = np.random.RandomState(2)
+    for i, eb in enumerate(FIT["scale_ebs"]):
+        eb = float(eb)
+        t = R.scale_thresholds(eb)
+        assert t.dtype == np.float32 and len(t) == 63 and np.all(np.diff(t) >= 0)
+        v = np.concatenate([FIT["scales"].astype(np.float32), t,
+                            np.nextafter(t, 0, dtype=np.float32),
+                            np.nextafter(t, np.inf, dtype=np.float32),
+                            (eb * np.exp(rng.uniform(-4, 9.5, 200000))).astype(np.float32)])
+        assert np.array_equal(np.searchsorted(t, v, side="right"), R.scale_to_level_host(v, eb))
+        assert np.array_equal(R.scale_to_level_host(FIT["scales"], eb), FIT["scale_levels"][i])
