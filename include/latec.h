@@ -134,6 +134,18 @@ int latec_interp_predict(latec_plan *p, int stage, const float *d_recon,
                          void *stream);
 
 /* u32 words of one stage histogram (2R bins + striped window copies). */
+/* Per-stage error moments of a finished encode (_stage_error_stats,
+ * codec.py:353-381): d_out (device, n_stages x 5 float64) gets for each
+ * stage pred_sae, pred_sse, recon_sae, recon_sse, recon_max over its C * n
+ * points, the prediction recomputed from d_recon (the encoder's and the
+ * decoder's prediction, bit for bit), differences in float32, sums in
+ * float64 in a fixed order (identical on every run).  d_scratch holds
+ * latec_interp_stats_words(p) float64 words.  Synchronizes the stream. */
+int64_t latec_interp_stats_words(latec_plan *p);
+int latec_interp_stats(latec_plan *p, const float *d_x, const float *d_recon,
+                       const latec_interp_opts *o, double *d_out, double *d_scratch,
+                       int64_t scratch_words, void *stream);
+
 int64_t latec_hist_words(uint32_t radius);
 
 /* Add the histogram of n codes (u16/u32) into d_hist (latec_hist_words). */

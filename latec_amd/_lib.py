@@ -133,6 +133,8 @@ SIGNATURES = {
     "latec_mlrans_encode": (i32, [i32, p, i32, p, p, p, u32, p, p, p, i64, p, i64, p, p, p]),
     "latec_mlrans_decode": (i32, [i32, p, i64, p, p, p, p, u32, p, p, p, i32, p, p, i64, p]),
     "latec_mlrans_noalloc_words": (i64, []),
+    "latec_interp_stats_words": (i64, [p]),
+    "latec_interp_stats": (i32, [p, p, p, p, p, p, i64, p]),
     "latec_mlrans_encode_noalloc": (i32, [i32, p, i32, p, p, p, u32, p, p, p, i64, p, p, p]),
     "latec_zero_ranks": (i32, [i32, p, i32, p, p, p, p, p, i64, p]),
     "latec_batch_create": (i32, [i32, p, p, p, p, p, p, p, p, i32, p, p, p, p, p,

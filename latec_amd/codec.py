@@ -16,6 +16,7 @@ import ctypes
 import os
 import time
 from concurrent.futures import ThreadPoolExecutor
+from typing import NamedTuple
 
 import numpy as np
 
@@ -221,20 +222,6 @@ def _interp_opts(predictor, ndim, radius, round_output):
                     end_mode=end)
 
 
-def _region_error_stats(stats, plan, x, recon):
-    """Per-stage reconstruction error moments (codec.py:353-381), from the
-    final reconstruction gathered by stage mask on the host."""
-    err = np.abs(x.astype(np.float32) - recon)
-    C = x.shape[0]
-    for s, mask in enumerate(stage_masks(plan.shape, plan.levels, plan.stride, plan.block)):
-        if not plan.counts[s]:
-            continue
-        e = err.reshape(C, -1)[:, mask.reshape(-1)]
-        stats["stage_recon_sae"][s] += float(e.sum(dtype=np.float64))
-        stats["stage_recon_sse"][s] += float(np.square(e, dtype=np.float64).sum())
-        stats["stage_recon_max"][s] = max(stats["stage_recon_max"][s], float(e.max(initial=0)))
-
-
 # ---------------------------------------------------------------------------
 # generic predictor loop (codec.py:384-510)
 # ---------------------------------------------------------------------------
@@ -388,171 +375,207 @@ def _fast_path(predictor) -> bool:
                                                                  False)
 
 
+class _Input:
+    """The array handed to ``compress`` as a (C, *S) float32 field.
+
+    An image is 2-D, or 3-D with a last axis of 1 or 3 channels (channel
+    last, SZ's convention); any other array is a one-channel field of its
+    own rank.  vmin / vmax are the float32 field's extremes, vmax pushed to
+    vmin + 1 for a constant field (codec.py:75-97)."""
+
+    def __init__(self, img):
+        img = np.asarray(img)
+        image = img.ndim <= 2 or (img.ndim == 3 and img.shape[-1] in (1, 3))
+        chan_last = img if image and img.ndim == 3 else img[..., None]
+        self.spatial = tuple(chan_last.shape[:-1])
+        self.channels = int(chan_last.shape[-1])
+        self.src_dtype = DTYPE_IDS[np.dtype(img.dtype)]
+        f32 = np.asarray(chan_last, np.float32)
+        self.vmin, self.vmax = float(f32.min()), float(f32.max())
+        if self.vmax <= self.vmin:
+            self.vmax = self.vmin + 1.0
+        self.field = np.ascontiguousarray(np.moveaxis(f32, -1, 0))
+        self.round_output = bool(np.issubdtype(img.dtype, np.integer))
+        self.nbytes = img.nbytes
+
+
+def _new_stats(n_stages: int) -> dict:
+    """The per-encode stats dict (keys of codec.py:120-135)."""
+    st = {"predict_s": 0.0, "quantize_s": 0.0, "entropy_s": 0.0, "outliers": 0}
+    for key, zero in (("stage_codes", 0), ("stage_outliers", 0), ("stage_payload_bytes", 0),
+                      ("stage_model_bits", 0.0), ("stage_pred_sae", 0.0),
+                      ("stage_pred_sse", 0.0), ("stage_recon_sae", 0.0),
+                      ("stage_recon_sse", 0.0), ("stage_recon_max", 0.0)):
+        st[key] = [zero] * n_stages
+    return st
+
+
+class _Candidate(NamedTuple):
+    size: int
+    sse: float
+    ratio: float
+    center: int
+    end: int
+    stream: bytes
+    recon: np.ndarray
+    stats: dict
+
+
+def _tune_space(predictor, tune: str, eb_ratio):
+    """What the encoder-side search tries (codec.py:171-199): (ratios, centres,
+    sweep).  A fixed eb_ratio, or fast mode, or a predictor without knobs gives
+    one ratio and one centre; a sweep tries ratios 1.0 .. 0.7 and the three
+    centres, base centre first."""
+    tunable = getattr(predictor, "tunable", False)
+    sweep = bool(tunable and tune != "fast" and eb_ratio is None)
+    if eb_ratio is not None:
+        ratios = [eb_ratio]
+    elif sweep:
+        ratios = [1.0, 0.9, 0.8, 0.7]
+    else:
+        ratios = [getattr(predictor, "fast_eb_ratio", 1.0)]
+    base = getattr(predictor, "center", 0)
+    centres = [base] + ([c for c in (0, 1, 2) if c != base]
+                        if sweep and hasattr(predictor, "center") else [])
+    return ratios, centres, sweep
+
+
+def _pick(cands, slack: float, rd: bool) -> _Candidate:
+    """Smallest stream, ties by reconstruction SSE then (ratio, centre, end);
+    in rate-distortion mode the lowest SSE among the streams within ``slack``
+    of the smallest (codec.py:244-251)."""
+    if rd:
+        limit = min(c.size for c in cands) * slack
+        return min((c for c in cands if c.size <= limit),
+                   key=lambda c: (c.sse, c.size, c.ratio, c.center, c.end))
+    return min(cands, key=lambda c: (c.size, c.sse, c.ratio, c.center, c.end))
+
+
 def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, radius=1 << 15,
              zstd_level=9, eb_ratio=None, tune="fast", tune_size_slack=1.05, verbose=False,
              *, stream_version=VERSION):
-    """codec.py:45-285: returns (stream, stats).  ``stream_version=5`` (an
-    addition) writes the multi-lane rANS stream version (latec_amd.mlrans)
-    with the same codes, outliers and model fit; the default 4 is the
-    reference's layout."""
+    """latec.compress (codec.py:45-285): (stream, stats).
+
+    The search: encode the base candidate; when sweeping, encode every end
+    mode at the first ratio and base centre and keep the end mode of the
+    smallest (size, SSE) stream so far; then every ratio x centre at that end
+    mode.  A (ratio, centre, end) triple is encoded at most once.  The chosen
+    candidate's centre and end mode are left on the predictor.
+    ``stream_version=5`` (an addition) writes the multi-lane rANS version with
+    the same codes, outliers and model fit; 4 is the reference's layout."""
     if stream_version not in (VERSION, VERSION_MULTILANE):
         raise ValueError(f"unsupported version {stream_version}")
-    is_image = img.ndim <= 2 or (img.ndim == 3 and img.shape[-1] in (1, 3))
-    if is_image:
-        if img.ndim == 2:
-            img = img[..., None]
-        spatial = img.shape[:-1]
-        c = img.shape[-1]
-        fimg = np.asarray(img, np.float32)
-    else:
-        spatial = img.shape
-        c = 1
-        fimg = np.asarray(img, np.float32)[..., None]
-    src_dtype = DTYPE_IDS[np.dtype(img.dtype)]
-    vmin = float(fimg.min())
-    vmax = float(fimg.max())
-    if vmax <= vmin:
-        vmax = vmin + 1.0
-    for name, val in (("levels", levels), ("anchor_stride", anchor_stride),
-                      ("anchor_block", anchor_block)):
-        got = getattr(predictor, name, val)
-        if got != val:
-            raise ValueError(f"predictor {name}={got} != compress {name}={val}")
-    field = np.ascontiguousarray(np.moveaxis(fimg, -1, 0))
-    region_shape = field.shape[1:]
-    make_masks = getattr(predictor, "stage_masks", stage_masks)
-    masks = make_masks(region_shape, levels, anchor_stride, anchor_block)
-    flags = getattr(predictor, "stream_flag", 0)
+    inp = _Input(img)
+    for name, want in (("levels", levels), ("anchor_stride", anchor_stride),
+                       ("anchor_block", anchor_block)):
+        got = getattr(predictor, name, want)
+        if got != want:
+            raise ValueError(f"predictor {name}={got} != compress {name}={want}")
+    shape = inp.field.shape[1:]
+    masks = getattr(predictor, "stage_masks", stage_masks)(shape, levels, anchor_stride,
+                                                           anchor_block)
     use_rans = getattr(predictor, "provides_scale", False)
-    flags |= FLAG_RANS if use_rans else 0
-    round_output = np.issubdtype(np.dtype(img.dtype), np.integer)
+    flags = getattr(predictor, "stream_flag", 0) | (FLAG_RANS if use_rans else 0)
     fast = _fast_path(predictor)
     if stream_version == VERSION_MULTILANE and not fast:
         raise ValueError("stream version 5 needs the package's InterpPredictor")
-    d_field = D.to_device(field.reshape(-1)) if fast else None
-
-    def new_stats():
-        k = len(masks)
-        return {"predict_s": 0.0, "quantize_s": 0.0, "entropy_s": 0.0, "outliers": 0,
-                "stage_codes": [0] * k, "stage_outliers": [0] * k,
-                "stage_payload_bytes": [0] * k, "stage_model_bits": [0.0] * k,
-                "stage_pred_sae": [0.0] * k, "stage_pred_sse": [0.0] * k,
-                "stage_recon_sae": [0.0] * k, "stage_recon_sse": [0.0] * k,
-                "stage_recon_max": [0.0] * k}
-
-    def encode(ebs):
-        st = new_stats()
-        frame = None
-        if fast:
-            payload, recon, frame = _compress_region_gpu_impl(field, d_field, ebs, predictor,
-                                                              radius, round_output, st,
-                                                              zstd_level, stream_version)
-        else:
-            payload, recon = _compress_region(field, masks, ebs, predictor, radius,
-                                              round_output, st, zstd_level)
-        return payload, recon, len(payload), st, frame
-
-    def ebs_for(ratio):
-        return stage_ebs(region_shape, levels, anchor_stride, anchor_block, eb, ratio)
-
-    def make_header(center, ratio, end_mode):
-        return Header(channels=c, src_dtype=src_dtype, spatial=tuple(spatial), eb=float(eb),
-                      levels=levels, anchor_stride=anchor_stride, anchor_block=anchor_block,
-                      radius=radius, max_radius=getattr(predictor, "max_radius", 0),
-                      agg_level=(-1 if getattr(predictor, "agg_level", None) is None
-                                 else int(predictor.agg_level)),
-                      vmin=vmin, vmax=vmax,
-                      ckpt_hash=getattr(predictor, "checkpoint_hash", b"\0" * 16),
-                      flags=flags | (int(end_mode) << END_MODE_SHIFT), interp_center=center,
-                      eb_ratio=ratio, version=stream_version)
-
     if tune not in ("fast", "size", "rd"):
         raise ValueError("tune must be 'fast', 'size', or 'rd'")
-    tunable = getattr(predictor, "tunable", False)
-    has_center = hasattr(predictor, "center")
-    has_end_mode = hasattr(predictor, "end_mode")
-    if has_center and getattr(predictor, "center") is None:
-        predictor.center = default_interp_center(len(region_shape))
-    if has_end_mode and getattr(predictor, "end_mode") is None:
-        predictor.end_mode = default_interp_end_mode(len(region_shape))
-    if eb_ratio is not None:
-        ratio_cands = [eb_ratio]
-    elif tunable and tune != "fast":
-        ratio_cands = [1.0, 0.9, 0.8, 0.7]
-    else:
-        ratio_cands = [getattr(predictor, "fast_eb_ratio", 1.0)]
-    sweep = tunable and tune != "fast" and eb_ratio is None
-    base_center = getattr(predictor, "center", 0)
-    center_cands = [base_center]
-    if sweep and has_center:
-        center_cands += [x for x in (0, 1, 2) if x != base_center]
-    base_end = getattr(predictor, "end_mode", 0)
-    candidates = []
+    # rank-aware defaults resolved once the rank is known (codec.py:183-187)
+    if hasattr(predictor, "center") and predictor.center is None:
+        predictor.center = default_interp_center(len(shape))
+    if hasattr(predictor, "end_mode") and predictor.end_mode is None:
+        predictor.end_mode = default_interp_end_mode(len(shape))
+    ratios, centres, sweep = _tune_space(predictor, tune, eb_ratio)
+    d_field = D.to_device(inp.field.reshape(-1)) if fast else None
+    agg = getattr(predictor, "agg_level", None)
 
-    def run(ratio, center, end_mode):
-        if has_center:
+    def header(center, ratio, end):
+        return Header(channels=inp.channels, src_dtype=inp.src_dtype, spatial=inp.spatial,
+                      eb=float(eb), levels=levels, anchor_stride=anchor_stride,
+                      anchor_block=anchor_block, radius=radius,
+                      max_radius=getattr(predictor, "max_radius", 0),
+                      agg_level=-1 if agg is None else int(agg), vmin=inp.vmin, vmax=inp.vmax,
+                      ckpt_hash=getattr(predictor, "checkpoint_hash", b"\0" * 16),
+                      flags=flags | (int(end) << END_MODE_SHIFT), interp_center=center,
+                      eb_ratio=ratio, version=stream_version)
+
+    tried: dict = {}
+
+    def encode(ratio, center, end):
+        key = (ratio, center, end)
+        if key in tried:
+            return
+        if hasattr(predictor, "center"):
             predictor.center = center
-        if has_end_mode:
-            predictor.end_mode = end_mode
-        payload, rc, raw_bytes, st, frame = encode(ebs_for(ratio))
-        header = make_header(center, ratio, end_mode)
+        if hasattr(predictor, "end_mode"):
+            predictor.end_mode = end
+        st = _new_stats(len(masks))
+        ebs = stage_ebs(shape, levels, anchor_stride, anchor_block, eb, ratio)
+        if fast:
+            payload, recon, frame = _compress_region_gpu_impl(
+                inp.field, d_field, ebs, predictor, radius, inp.round_output, st, zstd_level,
+                stream_version)
+        else:
+            payload, recon = _compress_region(inp.field, masks, ebs, predictor, radius,
+                                              inp.round_output, st, zstd_level)
+            frame = None
         t0 = time.time()
-        stream = write_stream(header, payload, zstd_level, frame=frame)
+        stream = write_stream(header(center, ratio, end), payload, zstd_level, frame=frame)
         st["entropy_s"] += time.time() - t0
-        st["raw_payload_bytes"] = raw_bytes
+        st["raw_payload_bytes"] = len(payload)
         st["compressed_bytes"] = len(stream)
         st["recon_sse"] = float(sum(st["stage_recon_sse"]))
-        cand = (len(stream), st["recon_sse"], ratio, center, end_mode, stream, rc, st)
-        candidates.append(cand)
-        return cand
+        tried[key] = _Candidate(len(stream), st["recon_sse"], ratio, center, end, stream,
+                                recon, st)
 
-    seen = set()
+    base_end = getattr(predictor, "end_mode", 0)
+    encode(ratios[0], centres[0], base_end)
+    end = base_end
+    if sweep and hasattr(predictor, "end_mode"):
+        for e in range(END_MODE_MAX + 1):
+            encode(ratios[0], centres[0], e)
+        end = min(tried.values(), key=lambda c: (c.size, c.sse)).end
+    for ratio in ratios:
+        for center in centres:
+            encode(ratio, center, end)
 
-    def run_once(ratio, center, end_mode):
-        if (ratio, center, end_mode) not in seen:
-            seen.add((ratio, center, end_mode))
-            run(ratio, center, end_mode)
-
-    run_once(ratio_cands[0], base_center, base_end)
-    chosen_end = base_end
-    if sweep and has_end_mode:
-        for end_mode in range(END_MODE_MAX + 1):
-            run_once(ratio_cands[0], base_center, end_mode)
-        chosen_end = min(candidates, key=lambda x: (x[0], x[1]))[4]
-    for ratio in ratio_cands:
-        for center in center_cands:
-            run_once(ratio, center, chosen_end)
-
-    if tune == "rd" and eb_ratio is None and tunable and len(candidates) > 1:
-        min_size = min(x[0] for x in candidates)
-        feasible = [x for x in candidates if x[0] <= min_size * tune_size_slack]
-        best = min(feasible, key=lambda x: (x[1], x[0], x[2], x[3], x[4]))
-    else:
-        best = min(candidates, key=lambda x: (x[0], x[1], x[2], x[3], x[4]))
-    _, _, chosen_ratio, chosen_center, chosen_end, stream, recon_canvas, stats = best
-    stats["tune_candidates"] = len(candidates)
-    stats["search_predict_s"] = sum(x[7]["predict_s"] for x in candidates)
-    stats["search_quantize_s"] = sum(x[7]["quantize_s"] for x in candidates)
-    stats["search_entropy_s"] = sum(x[7]["entropy_s"] for x in candidates)
-    if has_center:
-        predictor.center = chosen_center
-    if has_end_mode:
-        predictor.end_mode = chosen_end
-    stats["eb_ratio"] = chosen_ratio
-    stats["interp_center"] = chosen_center
-    stats["interp_end_mode"] = chosen_end
-    stats["tune"] = tune
-    stats["tune_size_slack"] = (tune_size_slack if tune == "rd" and eb_ratio is None and tunable
-                                else 1.0)
+    cands = list(tried.values())
+    tunable = getattr(predictor, "tunable", False)
+    rd = tune == "rd" and eb_ratio is None and tunable and len(cands) > 1
+    best = _pick(cands, tune_size_slack, rd)
+    stats = best.stats
+    stats["tune_candidates"] = len(cands)
+    for k in ("predict_s", "quantize_s", "entropy_s"):
+        stats["search_" + k] = sum(c.stats[k] for c in cands)
+    if hasattr(predictor, "center"):
+        predictor.center = best.center
+    if hasattr(predictor, "end_mode"):
+        predictor.end_mode = best.end
+    stats.update(eb_ratio=best.ratio, interp_center=best.center, interp_end_mode=best.end,
+                 tune=tune,
+                 tune_size_slack=tune_size_slack if tune == "rd" and eb_ratio is None
+                 and tunable else 1.0)
     if verbose:
-        print(f"tuned: eb_ratio={chosen_ratio} center={chosen_center} end_mode={chosen_end} "
-              f"compressed={len(stream)} bytes raw={stats['raw_payload_bytes']} bytes "
+        print(f"tuned: eb_ratio={best.ratio} center={best.center} end_mode={best.end} "
+              f"compressed={best.size} bytes raw={stats['raw_payload_bytes']} bytes "
               f"sse={stats['recon_sse']:.6g}")
-    stats["recon"] = _finalize(recon_canvas, make_header(chosen_center, chosen_ratio,
-                                                         chosen_end))
-    stats["original_bytes"] = img.nbytes
-    stats["ratio"] = img.nbytes / len(stream)
-    return stream, stats
+    stats["recon"] = _finalize(best.recon, header(best.center, best.ratio, best.end))
+    stats["original_bytes"] = inp.nbytes
+    stats["ratio"] = inp.nbytes / best.size
+    return best.stream, stats
+
+
+def stage_stats_device(res: "GpuEncodeResult", d_x) -> np.ndarray:
+    """(n_stages, 5) float64 error moments of a device encode: pred_sae,
+    pred_sse, recon_sae, recon_sse, recon_max (latec_interp_stats)."""
+    plan = res.plan
+    words = int(lib().latec_interp_stats_words(plan.handle))
+    scratch = D.empty(words, np.float64)
+    out = D.empty(plan.n_stages * 5, np.float64)
+    check(lib().latec_interp_stats(plan.handle, ptr(d_x), ptr(res.d_recon), _optr(res.opts),
+                                   ptr(out), ptr(scratch), words, ptr(None)))
+    return D.to_host(out, np.float64).reshape(plan.n_stages, 5)
 
 
 def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_output, stats,
@@ -565,6 +588,7 @@ def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_outp
     t0 = time.time()
     res = encode_device(d_field, plan, ebs, o)
     fits = fit_device(res)
+    mom = stage_stats_device(res, d_field)
     stats["quantize_s"] += time.time() - t0
     t0 = time.time()
     frame = None
@@ -581,8 +605,13 @@ def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_outp
         stats["stage_outliers"][s] += int(n_out[s])
         stats["stage_payload_bytes"][s] += int(rec_len[s])
         stats["stage_model_bits"][s] += float(fits[s].bits)
+        if plan.counts[s]:
+            stats["stage_pred_sae"][s] += float(mom[s, 0])
+            stats["stage_pred_sse"][s] += float(mom[s, 1])
+            stats["stage_recon_sae"][s] += float(mom[s, 2])
+            stats["stage_recon_sse"][s] += float(mom[s, 3])
+            stats["stage_recon_max"][s] = max(stats["stage_recon_max"][s], float(mom[s, 4]))
     stats["outliers"] += int(n_out.sum())
-    _region_error_stats(stats, plan, field, recon)
     return payload, recon, frame
 
 

@@ -29,6 +29,9 @@ int interp_decode(latec_plan *, int, int, const void *, const uint32_t *, const 
 int interp_encode_graph(latec_plan *, const float *, float *, void *, uint32_t *, uint32_t *,
                         float *, uint32_t *, const double *, const latec_interp_opts *,
                         hipStream_t);
+int64_t interp_stats_words(latec_plan *);
+int interp_stats(latec_plan *, const float *, const float *, const latec_interp_opts *, double *,
+                 double *, int64_t, hipStream_t);
 int interp_predict(latec_plan *, int, const float *, float *, const latec_interp_opts *,
                    hipStream_t);
 int code_hist(const void *, int64_t, int, uint32_t, uint32_t *, hipStream_t);
@@ -194,6 +197,14 @@ int latec_interp_encode_graph(latec_plan *p, const float *x, float *recon, void 
 int latec_interp_predict(latec_plan *p, int stage, const float *d_recon, float *d_pred,
                          const latec_interp_opts *o, void *stream) {
   return interp_predict(p, stage, d_recon, d_pred, o, S(stream));
+}
+
+int64_t latec_interp_stats_words(latec_plan *p) { return interp_stats_words(p); }
+
+int latec_interp_stats(latec_plan *p, const float *d_x, const float *d_recon,
+                       const latec_interp_opts *o, double *d_out, double *d_scratch,
+                       int64_t scratch_words, void *stream) {
+  return interp_stats(p, d_x, d_recon, o, d_out, d_scratch, scratch_words, S(stream));
 }
 
 int latec_code_hist(const void *d_codes, int64_t n, int code_bytes, uint32_t radius,

@@ -797,6 +797,73 @@ __global__ void __launch_bounds__(256) predict_stage_kernel(
   });
 }
 
+// Per-stage error moments of a finished encode (codec.py:353-381): every
+// point's prediction is recomputed from the final reconstruction (each point
+// is written once, at its own stage, so the earlier stages' values the
+// encoder predicted from are still there: the decoder's prediction, bit for
+// bit) and the stage's sums are taken in float64 over float32 differences:
+//   pred_sae = sum |x - p|, pred_sse = sum (x - p)^2,
+//   recon_sae = sum |x - r|, recon_sse = sum (x - r)^2, recon_max = max |x - r|
+// (NaN-sticky, as NumPy's max).  Each workgroup writes its five partials
+// (a fixed-shape LDS tree); stats_finish_kernel adds them in block order.
+constexpr int NSTAT = 5;
+__device__ __forceinline__ double nan_max(double a, double b) {
+  return (b > a || b != b) && a == a ? b : a;
+}
+
+template <int ND, typename I, bool BLK1>
+__global__ void __launch_bounds__(256) stats_stage_kernel(
+    GridArgs g, const ClassDesc *__restrict__ classes, const uint32_t *__restrict__ excl,
+    const float *__restrict__ x, const float *__restrict__ recon, int anchors,
+    double *__restrict__ partial) {
+  __shared__ double red[NSTAT][256];
+  const ClassDesc &c = classes[blockIdx.y];
+  const int parity = c.parity;
+  double acc[NSTAT] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  walk_rows<ND, I, BLK1>(c, g, excl, [&](bool active, I flat, int64_t rank,
+                                          const int64_t *coord) {
+    if (!active) return;
+    for (int ch = 0; ch < g.channels; ch++) {
+      const I cofs = (I)ch * (I)g.N;
+      const float p = anchors ? 0.0f
+                              : predict_point<ND, I>(recon + cofs, g, parity, flat, coord);
+      const float xv = x[cofs + flat], rv = recon[cofs + flat];
+      const float pe = xv - p, re = fabsf(xv - rv);
+      acc[0] += (double)fabsf(pe);
+      acc[1] += (double)pe * (double)pe;
+      acc[2] += (double)re;
+      acc[3] += (double)re * (double)re;
+      acc[4] = nan_max(acc[4], (double)re);
+    }
+  });
+  const int t = threadIdx.x;
+  for (int k = 0; k < NSTAT; k++) red[k][t] = acc[k];
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if (t < h) {
+      for (int k = 0; k < 4; k++) red[k][t] += red[k][t + h];
+      red[4][t] = nan_max(red[4][t], red[4][t + h]);
+    }
+    __syncthreads();
+  }
+  if (t < NSTAT)
+    partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NSTAT + t] = red[t][0];
+}
+
+// out[s * 5 + k] = the stage's partials [off[s], off[s + 1]) summed in order
+// (one workgroup per stage; thread k < 5 walks moment k).
+__global__ void stats_finish_kernel(const double *__restrict__ partial,
+                                    const int64_t *__restrict__ off, double *__restrict__ out) {
+  const int st = blockIdx.x, k = threadIdx.x;
+  if (k >= NSTAT) return;
+  double a = 0.0;
+  for (int64_t b = off[st]; b < off[st + 1]; b++) {
+    const double v = partial[b * NSTAT + k];
+    a = k == 4 ? nan_max(a, v) : a + v;
+  }
+  out[st * NSTAT + k] = a;
+}
+
 template <typename CodeT>
 __global__ void __launch_bounds__(256) code_hist_kernel(const CodeT *__restrict__ codes,
                                                         int64_t n, uint32_t R,
@@ -3672,6 +3739,66 @@ int interp_predict(latec_plan *p, int stage, const float *recon, float *pred,
   hipLaunchKernelGGL((predict_stage_kernel<0, int64_t, false>), grid, dim3(256), 0, stream, g,
                      p->dev->classes + p->dev->class_off[stage], ex, recon, pred);
   return hip_check(hipGetLastError(), "predict_stage_kernel");
+}
+
+// Stage error moments of an encode (stats_stage_kernel): out[s * 5 + k] for
+// every stage of the plan (device, float64); scratch holds the partials
+// (interp_stats_words(p) float64 words, the first n_stages + 1 of them the
+// partial offsets).
+int64_t interp_stats_words(latec_plan *p) {
+  int64_t nb = 0;
+  for (const StageDesc &st : p->stages)
+    if (st.count) nb += (int64_t)row_blocks(st) * (int64_t)st.classes.size();
+  return (int64_t)p->stages.size() + 1 + NSTAT * nb;
+}
+
+int interp_stats(latec_plan *p, const float *x, const float *recon,
+                 const latec_interp_opts *o, double *out, double *scratch,
+                 int64_t scratch_words, hipStream_t stream) {
+  int rc = check_opts(o);
+  if (rc) return rc;
+  if ((rc = plan_upload(p))) return rc;
+  const int ns = (int)p->stages.size();
+  if (scratch_words < interp_stats_words(p)) {
+    set_error("latec_interp_stats: scratch too small");
+    return LATEC_E_CAPACITY;
+  }
+  std::vector<int64_t> off(ns + 1, 0);
+  for (int s = 0; s < ns; s++) {
+    const StageDesc &st = p->stages[s];
+    off[s + 1] = off[s] + (st.count ? (int64_t)row_blocks(st) * (int64_t)st.classes.size() : 0);
+  }
+  int64_t *d_off = (int64_t *)scratch;
+  double *partial = scratch + ns + 1;
+  if ((rc = hip_check(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * (ns + 1),
+                                     hipMemcpyHostToDevice, stream), "hipMemcpyAsync")))
+    return rc;
+  const KSel sel = pick(p);
+  for (int s = 0; s < ns; s++) {
+    const StageDesc &st = p->stages[s];
+    if (!st.count) continue;
+    GridArgs g = make_args(p, s, 1.0, o);
+    dim3 grid(row_blocks(st), (unsigned)st.classes.size());
+    const ClassDesc *cls = p->dev->classes + p->dev->class_off[s];
+    const uint32_t *ex = p->dev->excluded ? p->dev->excluded + p->dev->excl_off[s] : nullptr;
+    double *ps = partial + NSTAT * off[s];
+    if (sel == KSel::Fast && p->block == 1) {
+      switch (p->ndim) {
+        case 1: hipLaunchKernelGGL((stats_stage_kernel<1, uint32_t, true>), grid, dim3(256), 0, stream, g, cls, ex, x, recon, s == 0, ps); break;
+        case 2: hipLaunchKernelGGL((stats_stage_kernel<2, uint32_t, true>), grid, dim3(256), 0, stream, g, cls, ex, x, recon, s == 0, ps); break;
+        case 3: hipLaunchKernelGGL((stats_stage_kernel<3, uint32_t, true>), grid, dim3(256), 0, stream, g, cls, ex, x, recon, s == 0, ps); break;
+        default: hipLaunchKernelGGL((stats_stage_kernel<4, uint32_t, true>), grid, dim3(256), 0, stream, g, cls, ex, x, recon, s == 0, ps); break;
+      }
+    } else {
+      hipLaunchKernelGGL((stats_stage_kernel<0, int64_t, false>), grid, dim3(256), 0, stream, g,
+                         cls, ex, x, recon, s == 0, ps);
+    }
+    if ((rc = hip_check(hipGetLastError(), "stats_stage_kernel"))) return rc;
+  }
+  hipLaunchKernelGGL(stats_finish_kernel, dim3(ns), dim3(64), 0, stream, partial, d_off, out);
+  if ((rc = hip_check(hipGetLastError(), "stats_finish_kernel"))) return rc;
+  // the host offsets are read by the copy above: finish before they go
+  return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
 int code_hist(const void *codes, int64_t n, int cb, uint32_t R, uint32_t *hist,

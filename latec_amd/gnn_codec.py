@@ -292,6 +292,7 @@ class GNNCompressorCodec:
         self.checkpoint_hash = load_model(self.checkpoint_path)[3]
         self._predictors: dict = {}
         self.last_stats: dict = {}
+        self.last_decoded = None  # the last uncompress's normalized reconstruction (device)
         self.count_flops = False  # count the model's GEMM flops into last_stats (syncs)
 
     @staticmethod
@@ -535,6 +536,8 @@ class GNNCompressorCodec:
         stats["model_s"] = sum(a.elapsed_time(b) for a, b in events) * 1e-3
         stats["gate_words"] = len(gates or []) + len(fine_gates or [])
         stats["gates_fired"] = sum(1 for g in (gates or []) + (fine_gates or []) if g)
+        # the normalized reconstruction the decoder must reproduce bit for bit
+        stats["recon"] = recon
         self.last_stats = stats
         return b"".join(parts), gates, fine_gates
 
@@ -656,6 +659,7 @@ class GNNCompressorCodec:
             raise ValueError("trailing bytes in LATEC GNN payload")
         if gates is not None and gi != len(gates):
             raise ValueError("gate list length does not match the stream")
+        self.last_decoded = recon  # normalized, as the encoder's stats["recon"]
         values = recon[0].cpu().numpy()
         values *= vmax - vmin
         values += vmin

@@ -450,3 +450,62 @@ def test_gpu_3d_outliers_and_nonfinite_match_oracle():
     assert stream == ref_stream
     dec = latec_amd.decompress(stream)
     assert np.array_equal(dec.view(np.uint32), ref_recon[0].view(np.uint32))
+
+
+def _reference_moments(x, recon, shape, levels, stride, block, order, center, end,
+                       tile=1 << 20):
+    """Per-stage moments as codec.py:353-381 computes them (float32
+    differences a 2^20 tile at a time, float64 sums), on the oracle's
+    reconstruction with the oracle's per-stage predictions from it."""
+    rows = []
+    C = x.shape[0]
+    for s, (mask, st, w, ax) in enumerate(O.stage_plan(shape, levels, stride, block)):
+        n = int(mask.sum())
+        if not n:
+            rows.append((0.0, 0.0, 0.0, 0.0, 0.0))
+            continue
+        vals = x.reshape(C, -1)[:, mask.reshape(-1)]
+        rec = recon.reshape(C, -1)[:, mask.reshape(-1)]
+        pred = (np.zeros((C, n), np.float32) if s == 0
+                else O.predict(recon, mask, st, w, ax, order, center, end))
+        m = [0.0, 0.0, 0.0, 0.0, 0.0]
+        for a in range(0, n, tile):
+            v = vals[:, a:a + tile]
+            pe = v - pred[:, a:a + tile]
+            re = np.abs(v - rec[:, a:a + tile])
+            m[0] += float(np.abs(pe).sum(dtype=np.float64))
+            m[1] += float(np.square(pe, dtype=np.float64).sum())
+            m[2] += float(re.sum(dtype=np.float64))
+            m[3] += float(np.square(re, dtype=np.float64).sum())
+            m[4] = max(m[4], float(re.max(initial=0.0)))
+        rows.append(tuple(m))
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("case", [
+    ((96, 80, 40), 1, 3, 8, 1, "cubic", 0, 1, 1e-3, 0.02),
+    ((64, 48), 3, 4, 16, 4, "cubic", 0, 1, 1e-2, 0.01),
+    ((33, 40, 512), 1, 5, 32, 1, "cubic", 0, 1, 1e-3, 0.02),
+    ((12, 11, 10, 9), 1, 2, 4, 1, "linear", 2, 3, 1e-3, 0.0),
+], ids=lambda c: f"{c[0]}-C{c[1]}-b{c[4]}")
+def test_gpu_stage_stats_match_reference_moments(case):
+    """stats['stage_pred_*'] / ['stage_recon_*'] of the GPU path equal the
+    reference's per-stage moments on the oracle's encode to float64 rounding
+    of the summation order (the max exactly)."""
+    shape, C, levels, stride, block, order, center, end, rel, rough = case
+    x = np.stack([smooth(shape, seed=c, rough=rough) for c in range(C)])
+    eb = rel * float(x.max() - x.min())
+    img = x[0] if C == 1 else np.moveaxis(x, 0, -1)
+    pred = InterpPredictor(order, levels, stride, block, center=center, end_mode=end)
+    stream, stats = latec_amd.compress(img, eb, pred, levels=levels, anchor_stride=stride,
+                                       anchor_block=block)
+    ref_stream, ref_recon = O.compress(img, eb, order, levels, stride, block, center, end)
+    assert stream == ref_stream
+    ref = _reference_moments(x, ref_recon, shape, levels, stride, block, order, center, end)
+    got = np.array([stats[k] for k in ("stage_pred_sae", "stage_pred_sse", "stage_recon_sae",
+                                       "stage_recon_sse", "stage_recon_max")]).T
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got[:, :4], ref[:, :4], rtol=1e-12, atol=0)
+    assert np.array_equal(got[:, 4], ref[:, 4])
+    assert stats["recon_sse"] == float(sum(stats["stage_recon_sse"]))
+    assert all(v > 0 for v, n in zip(stats["stage_pred_sae"], stats["stage_codes"]) if n)

@@ -262,3 +262,37 @@ def test_oracle_stream_roundtrip(shape, block, order, center, end):
     dec = O.decompress(stream)
     assert np.array_equal(dec, recon)
     assert np.abs(dec[0] - x).max() <= eb
+
+
+def test_compress_input_and_search_contract():
+    """compress's input convention and tune search (codec.py:75-97, 171-251):
+    image vs field shapes, the candidate space per mode and the selection
+    keys, host-side (no encode)."""
+    from latec_amd.codec import _Candidate, _Input, _pick, _tune_space
+    from latec_amd.predictor import InterpPredictor
+
+    a = _Input(np.zeros((5, 7), np.uint8))
+    assert (a.spatial, a.channels, a.field.shape, a.round_output) == ((5, 7), 1, (1, 5, 7), True)
+    assert a.vmax == a.vmin + 1.0
+    b = _Input(np.arange(5 * 7 * 3, dtype=np.float32).reshape(5, 7, 3))
+    assert (b.spatial, b.channels, b.field.shape) == ((5, 7), 3, (3, 5, 7))
+    assert np.array_equal(b.field[2], np.arange(105, dtype=np.float32).reshape(5, 7, 3)[..., 2])
+    c = _Input(np.ones((4, 5, 6), np.float32))
+    assert (c.spatial, c.channels) == ((4, 5, 6), 1)
+    d = _Input(np.ones((4, 5, 6, 2), np.float32))
+    assert (d.spatial, d.channels) == ((4, 5, 6, 2), 1)
+    p = InterpPredictor("cubic", 3, 8, 1)
+    p.center = 1
+    assert _tune_space(p, "fast", None) == ([0.9], [1], False)
+    assert _tune_space(p, "size", None) == ([1.0, 0.9, 0.8, 0.7], [1, 0, 2], True)
+    assert _tune_space(p, "rd", 0.8) == ([0.8], [1], False)
+
+    class Plain:
+        pass
+
+    assert _tune_space(Plain(), "size", None) == ([1.0], [0], False)
+    mk = lambda size, sse, r, c, e: _Candidate(size, sse, r, c, e, b"", None, {})
+    cands = [mk(100, 5.0, 0.9, 0, 1), mk(100, 4.0, 1.0, 0, 1), mk(104, 1.0, 0.8, 2, 1),
+             mk(106, 0.5, 0.7, 0, 3)]
+    assert _pick(cands, 1.05, False) == cands[1]  # size, then sse
+    assert _pick(cands, 1.05, True) == cands[2]  # lowest sse within 5 % of the smallest
