@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--shape", type=str, default="512,512,512")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-planes", type=int, default=64)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the all-cores CPU baseline (the box's CPU share is 16)")
+    ap.add_argument("--no-api", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--skip-decode", action="store_true")
@@ -170,12 +173,97 @@ def batch_leg(args):
     return out
 
 
+def host_cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_all_cores(xd, shape, levels, stride, threads):
+    """The C oracle on every slab of the field at once: the field cut into
+    ``threads`` slabs along axis 0, each compressed (payload, no outer frame)
+    on its own thread (the oracle releases the GIL), so the rate is what the
+    host's cores give for the same algorithm on independent slabs."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from latec_amd import _device as D
+    from oracle import oracle as O
+
+    nproc = os.cpu_count() or 1
+    t = max(1, min(threads, nproc, shape[0]))
+    bounds = [shape[0] * i // t for i in range(t + 1)]
+    host = D.to_host(xd.reshape(shape))
+    slabs = [np.ascontiguousarray(host[a:b])[None] for a, b in zip(bounds, bounds[1:])]
+    del host
+
+    def one(sl):
+        eb = 1e-3 * float(sl.max() - sl.min())
+        O.compress_payload(sl, O.stage_ebs_ref(len(shape), levels, stride, eb, 0.9), levels,
+                           stride, 1, "cubic", 0, 1)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=t) as ex:
+        list(ex.map(one, slabs))
+    dt = time.perf_counter() - t0
+    nbytes = sum(sl.nbytes for sl in slabs)
+    print(f"[bench] cpu oracle, {t} threads: {dt:.1f} s on {nbytes // 4} values",
+          file=sys.stderr, flush=True)
+    return {"value": nbytes / dt / 1e9, "unit": "GB/s", "cores": t, "nproc": nproc,
+            "cpu_model": host_cpu_model(),
+            "sample": f"the whole field as {t} slabs of ~{shape[0] // t} planes, one C-oracle "
+                      "compress (payload, without the outer zstd frame) per slab, all slabs at "
+                      "once on separate threads"}
+
+
+def api_leg(xd, shape, levels, stride, eb, steps):
+    """latec_amd.compress / decompress on a host array (the reference's call):
+    input upload, encode, stage moments, fit, pack, zstd frame, reconstruction
+    back to the host, and the decode from stream bytes to a host array."""
+    import numpy as np
+
+    import latec_amd
+    from latec_amd import InterpPredictor
+    from latec_amd import _device as D
+
+    x = D.to_host(xd.reshape(shape))
+    pred = InterpPredictor("cubic", levels, stride, 1)
+    stream, st = latec_amd.compress(x, eb, pred, levels=levels, anchor_stride=stride,
+                                    anchor_block=1)
+    latec_amd.decompress(stream)
+    tc, td = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        stream, st = latec_amd.compress(x, eb, pred, levels=levels, anchor_stride=stride,
+                                        anchor_block=1)
+        tc.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        y = latec_amd.decompress(stream)
+        td.append(time.perf_counter() - t0)
+    assert float(np.abs(y - x).max()) <= eb
+    out = {"compress_s": float(np.median(tc)), "decompress_s": float(np.median(td)),
+           "compress_GBps": x.nbytes / float(np.median(tc)) / 1e9,
+           "decompress_GBps": x.nbytes / float(np.median(td)) / 1e9,
+           "bits_per_value": 8 * len(stream) / x.size,
+           "stats_keys": sorted(k for k in st if k.startswith("stage_"))}
+    print(f"[bench] api: compress {out['compress_s']:.3f} s, decompress "
+          f"{out['decompress_s']:.3f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def gnn_leg(args):
     """GNN codec on the 4-D anisotropic field (per-axis correlation U[1,8],
     50% turbulence, 3 hyperplane jumps), levels 3, eb = 1e-3 * range: BF16 and
     FP32 kernels against plain cubic interpolation on the same GPU, the BF16 /
     FP32 prediction RMS against an FP64 run of the same network (oracle, on a
-    10^4 crop), and the model GEMM throughput."""
+    16^4 crop), the reference's 16-chunk partition of the same tensor, and the
+    model GEMM throughput."""
     import numpy as np
     import torch
 
@@ -219,6 +307,26 @@ def gnn_leg(args):
         print(f"[bench] gnn {numerics}: {res[numerics]['bits_per_value']:.4f} bits/value, "
               f"compress {tc:.2f} s, decompress {td:.2f} s, model {stats['model_s']:.3f} s",
               file=sys.stderr, flush=True)
+    # the reference's default partition of this tensor: 16 chunks of 32^4 with
+    # halo context (its point budget 2^21; here the automatic budget keeps the
+    # tensor as one chunk)
+    codec = GNNCodec(args.gnn_ckpt, numerics="bf16")
+    codec.compress(x, rel, levels=lv, chunk_size=32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s_ch = codec.compress(x, rel, levels=lv, chunk_size=32)
+    torch.cuda.synchronize()
+    t_ch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    y = codec.uncompress(s_ch).numpy()
+    td_ch = time.perf_counter() - t0
+    assert float(np.abs(y - x).max()) <= rel * S + roundtrip_slack(x.min(), x.max())
+    out["bf16_16_chunks"] = {"chunk_edge": 32, "bits_per_value": 8 * len(s_ch) / x.size,
+                             "compress_s": t_ch, "decompress_s": td_ch,
+                             "compress_GBps": x.nbytes / t_ch / 1e9,
+                             "decompress_GBps": x.nbytes / td_ch / 1e9}
+    print(f"[bench] gnn bf16, 16 chunks: {8 * len(s_ch) / x.size:.4f} bits/value, compress "
+          f"{t_ch:.2f} s, decompress {td_ch:.2f} s", file=sys.stderr, flush=True)
     stride = 1 << lv
     t0 = time.perf_counter()
     s_int, _ = compress(x, rel * S, InterpPredictor("cubic", lv, stride, 1), levels=lv,
@@ -232,7 +340,7 @@ def gnn_leg(args):
     from latec_amd.gnn_predictor import ChunkedGNNPredictor
     from oracle import gnn_oracle as G
 
-    crop = tuple(min(10, n) for n in shape)
+    crop = tuple(min(16, n) for n in shape)
     xc = x[tuple(slice(0, c) for c in crop)]
     xc = ((xc - xc.min()) / (xc.max() - xc.min())).astype(np.float32)
     from latec_amd import gnn_model as GM
@@ -445,11 +553,19 @@ def main():
         t0 = time.perf_counter()
         O.compress_payload(sample[None], s_ebs, levels, stride, 1, "cubic", 0, 1)
         dt = time.perf_counter() - t0
-        cpu = {"value": sample.nbytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+        one = {"value": sample.nbytes / dt / 1e9, "unit": "GB/s", "cores": 1,
                "sample": f"first {planes} planes of the same field ({sample.size} values), "
                          "one compress (payload, without the outer zstd frame), C oracle"}
         print(f"[bench] cpu oracle {dt:.1f} s on {sample.size} values", file=sys.stderr,
               flush=True)
+        cpu = {**cpu_all_cores(xd, shape, levels, stride, args.cpu_threads), "kind": "port",
+               "one_core": one}
+
+    api = None
+    if not args.no_api:
+        api = api_leg(xd, shape, levels, stride, eb, max(2, min(args.steps, 3)))
+        api["pipeline_compress_s"] = t_comp
+        api["pipeline_decompress_s"] = t_dec
 
     line = {
         "metric": "compress/decompress GB/s + bits/value at fixed eb: GNN-BF16 vs plain interp",
@@ -487,6 +603,7 @@ def main():
                                                 "candidates": len(tuned["table"]),
                                                 "table": tuned["table"]},
         "stream_v5": v5,
+        "api": api,
     }
     if not args.no_gnn and os.path.exists(args.gnn_ckpt):
         line["gnn"] = gnn_leg(args)

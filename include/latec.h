@@ -49,9 +49,15 @@ int latec_abi_version(void);
  * reference has no GPU kernels to shape.  Knobs of the 3-D level kernels:
  * "march_blocks" (workgroups per parity class, 8..2^20, rounded up to a
  * multiple of 8), "march_hist_peel" (0..8 distinct codes per wave counted
- * with one histogram atomic each) and "slab" (1: large levels run as one
- * level-fused launch; 0: three class launches per level).  Process-wide;
- * no knob changes a code or a reconstructed value. */
+ * with one histogram atomic each), "slab" (1: large levels run as one
+ * level-fused launch; 0: three class launches per level), "pair" (1: levels
+ * above the first as line-pair launches), "pair_lines" (1: whole lines per
+ * workgroup), "pair_quads" (bits: line quads for encode / decode, and for
+ * phase 2 alone), "min_threads" (march shortening threshold) and
+ * "pair_waves" (1: the finest level's encode phases 2 and 3 compiled for
+ * at least 8 waves per SIMD).  "quad_launches" reads (or, set, resets) a
+ * count of line-quad launches.  Process-wide; no knob changes a code or a
+ * reconstructed value. */
 int latec_tune_set(const char *name, int64_t value);
 int64_t latec_tune_get(const char *name);
 
@@ -145,6 +151,15 @@ int64_t latec_interp_stats_words(latec_plan *p);
 int latec_interp_stats(latec_plan *p, const float *d_x, const float *d_recon,
                        const latec_interp_opts *o, double *d_out, double *d_scratch,
                        int64_t scratch_words, void *stream);
+
+/* float32 min and max of nseg segments [off[g], off[g] + len[g]) of device
+ * d_x (host off / len, len > 0) into d_out[2 g], d_out[2 g + 1] (device), as
+ * NumPy's min() / max() (a NaN makes both NaN): the value range compress
+ * takes from its input (codec.py:93-97).  d_scratch holds
+ * latec_minmax_words(nseg) float32 words.  Synchronizes the stream. */
+int64_t latec_minmax_words(int nseg);
+int latec_minmax(int nseg, const float *d_x, const int64_t *off, const int64_t *len,
+                 float *d_out, float *d_scratch, int64_t scratch_words, void *stream);
 
 int64_t latec_hist_words(uint32_t radius);
 
@@ -510,6 +525,17 @@ int latec_zero_ranks(int ns, const void *d_codes, int code_bytes,
                      const int64_t *code_off, const int64_t *n,
                      uint32_t *d_out_rank, uint32_t *d_out_count,
                      uint32_t *d_scratch, int64_t scratch_words, void *stream);
+
+/* latec_zero_ranks plus each outlier's value: d_out_val[code_off + k] =
+ * d_x[code_off + d_out_rank[code_off + k]] (d_x in stage order like the
+ * codes), so the outliers come out sorted by stage rank, the order
+ * latec/quantizer.py:28-87 lists them in. */
+int latec_order_outliers(int ns, const void *d_codes, int code_bytes,
+                         const int64_t *code_off, const int64_t *n, const float *d_x,
+                         uint32_t *d_out_rank, float *d_out_val, uint32_t *d_out_count,
+                         uint32_t *d_scratch, int64_t scratch_words, void *stream);
+/* Scratch (u32 words) latec_zero_ranks / latec_order_outliers need. */
+int64_t latec_zero_ranks_words(int ns, const int64_t *n);
 
 /* ------------------------------------------------------------------ */
 /* Batched uneven workload: many independent fields (anchor_block 1,   */

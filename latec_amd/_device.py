@@ -71,6 +71,18 @@ def to_host_staged(x, dtype=None) -> np.ndarray:
     return a.view(dtype) if dtype is not None else a
 
 
+def to_host_pinned(x, dtype=None) -> np.ndarray:
+    """Device tensor -> a new numpy array in pinned host memory (one copy at
+    the link's pinned rate; torch caches freed pinned blocks, so a repeated
+    call of the same size does not pin pages again).  The array keeps its
+    buffer alive."""
+    t = torch()
+    host = t.empty(x.shape, dtype=x.dtype, pin_memory=True)
+    host.copy_(x.detach())
+    a = host.numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
 def _tdtype(dtype):
     t = torch()
     tmap = {np.float32: t.float32, np.float64: t.float64, np.uint16: t.int16,
@@ -99,3 +111,18 @@ def zeros(n, dtype):
     z = empty(n, dtype)
     z.zero_()
     return z
+
+
+def minmax(d_x, segments) -> np.ndarray:
+    """(nseg, 2) float32 [min, max] of device float32 segments (offset, length)
+    of d_x, NaN-propagating as NumPy (latec_minmax)."""
+    from ._lib import check, lib, ptr
+
+    off = np.ascontiguousarray([int(a) for a, _ in segments], np.int64)
+    ln = np.ascontiguousarray([int(b) for _, b in segments], np.int64)
+    words = int(lib().latec_minmax_words(len(off)))
+    scratch = empty(words, np.float32)
+    out = empty(2 * len(off), np.float32)
+    check(lib().latec_minmax(len(off), ptr(d_x), ptr(off), ptr(ln), ptr(out), ptr(scratch),
+                             words, ptr(None)))
+    return to_host(out, np.float32).reshape(-1, 2)

@@ -134,6 +134,10 @@ SIGNATURES = {
     "latec_mlrans_decode": (i32, [i32, p, i64, p, p, p, p, u32, p, p, p, i32, p, p, i64, p]),
     "latec_mlrans_noalloc_words": (i64, []),
     "latec_interp_stats_words": (i64, [p]),
+    "latec_minmax_words": (i64, [i32]),
+    "latec_minmax": (i32, [i32, p, p, p, p, p, i64, p]),
+    "latec_order_outliers": (i32, [i32, p, i32, p, p, p, p, p, p, p, i64, p]),
+    "latec_zero_ranks_words": (i64, [i32, p]),
     "latec_interp_stats": (i32, [p, p, p, p, p, p, i64, p]),
     "latec_mlrans_encode_noalloc": (i32, [i32, p, i32, p, p, p, u32, p, p, p, i64, p, p, p]),
     "latec_zero_ranks": (i32, [i32, p, i32, p, p, p, p, p, i64, p]),

@@ -1,6 +1,6 @@
 """Autotuner for the plain path's launch shapes.
 
-The 3-D level kernels have seven launch-shape knobs, set process-wide with
+The 3-D level kernels have eight launch-shape knobs, set process-wide with
 ``latec_tune_set``:
 
 * ``march_blocks`` -- workgroups per parity class (with 4 waves each, this is
@@ -22,7 +22,11 @@ The 3-D level kernels have seven launch-shape knobs, set process-wide with
   decoder, bits 2 and 3 the same for phase 2 alone;
 * ``min_threads`` -- a level launch shortens each thread's march (8, 4, 2, 1
   points) until it has at least this many threads: small levels are
-  latency-bound, so they trade march length for parallelism.
+  latency-bound, so they trade march length for parallelism;
+* ``pair_waves`` -- the waves-per-SIMD dial of the finest level's encode
+  phases 2 and 3: 0 the compiler's own register budget (77 / 103 VGPRs, six
+  and four waves per SIMD), 1 the variant compiled for at least eight waves
+  per SIMD (64 VGPRs, with spills).
 
 The slab kernel and histogram peeling are measured slower than their
 alternatives on every workload tried so far, so the default candidates leave
@@ -47,14 +51,15 @@ import numpy as np
 from ._lib import check, lib, ptr
 
 KNOBS = ("march_blocks", "march_hist_peel", "slab", "pair", "pair_lines", "pair_quads",
-         "min_threads")
+         "min_threads", "pair_waves")
 CANDIDATES = {"march_blocks": (1024, 2048, 4096),
               "march_hist_peel": (0,),
               "slab": (0,),
               "pair": (0, 1),
               "pair_lines": (0, 1),
               "pair_quads": (2, 3),
-              "min_threads": (1 << 18, 1 << 19)}
+              "min_threads": (1 << 18, 1 << 19),
+              "pair_waves": (0, 1)}
 
 
 def get() -> dict:

@@ -160,8 +160,7 @@ def compress_batch(fields, rel_eb=1e-3, *, levels=4, anchor_stride=16, order="cu
     with ThreadPoolExecutor(max_workers=_THREADS) as ex:
         list(ex.map(upload, range(len(prepped))))
     # each field's range from its float32 values in HBM (exact, as the host min/max)
-    rng = torch.stack([torch.stack(torch.aminmax(buf.x[off:off + x.size]))
-                       for (x, _, _), off in zip(prepped, lay.x_off)]).cpu().numpy()
+    rng = D.minmax(buf.x, [(off, x.size) for (x, _, _), off in zip(prepped, lay.x_off)])
     ebs_list, heads = [], []
     for (x, shape, src), (pred, center, end_mode), (lo, hi) in zip(prepped, preds, rng):
         vmin, vmax = float(lo), float(hi)

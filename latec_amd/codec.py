@@ -383,7 +383,7 @@ class _Input:
     own rank.  vmin / vmax are the float32 field's extremes, vmax pushed to
     vmin + 1 for a constant field (codec.py:75-97)."""
 
-    def __init__(self, img):
+    def __init__(self, img, upload: bool = False):
         img = np.asarray(img)
         image = img.ndim <= 2 or (img.ndim == 3 and img.shape[-1] in (1, 3))
         chan_last = img if image and img.ndim == 3 else img[..., None]
@@ -391,10 +391,15 @@ class _Input:
         self.channels = int(chan_last.shape[-1])
         self.src_dtype = DTYPE_IDS[np.dtype(img.dtype)]
         f32 = np.asarray(chan_last, np.float32)
-        self.vmin, self.vmax = float(f32.min()), float(f32.max())
+        self.field = np.ascontiguousarray(np.moveaxis(f32, -1, 0))
+        self.d_field = None
+        if upload:  # the range from the float32 values in HBM (latec_minmax)
+            self.d_field = D.to_device(self.field.reshape(-1))
+            self.vmin, self.vmax = map(float, D.minmax(self.d_field, [(0, self.field.size)])[0])
+        else:
+            self.vmin, self.vmax = float(f32.min()), float(f32.max())
         if self.vmax <= self.vmin:
             self.vmax = self.vmin + 1.0
-        self.field = np.ascontiguousarray(np.moveaxis(f32, -1, 0))
         self.round_output = bool(np.issubdtype(img.dtype, np.integer))
         self.nbytes = img.nbytes
 
@@ -465,7 +470,8 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
     the same codes, outliers and model fit; 4 is the reference's layout."""
     if stream_version not in (VERSION, VERSION_MULTILANE):
         raise ValueError(f"unsupported version {stream_version}")
-    inp = _Input(img)
+    fast = _fast_path(predictor)
+    inp = _Input(img, upload=fast)
     for name, want in (("levels", levels), ("anchor_stride", anchor_stride),
                        ("anchor_block", anchor_block)):
         got = getattr(predictor, name, want)
@@ -476,7 +482,6 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
                                                            anchor_block)
     use_rans = getattr(predictor, "provides_scale", False)
     flags = getattr(predictor, "stream_flag", 0) | (FLAG_RANS if use_rans else 0)
-    fast = _fast_path(predictor)
     if stream_version == VERSION_MULTILANE and not fast:
         raise ValueError("stream version 5 needs the package's InterpPredictor")
     if tune not in ("fast", "size", "rd"):
@@ -487,7 +492,7 @@ def compress(img, eb, predictor, levels=4, anchor_stride=16, anchor_block=1, rad
     if hasattr(predictor, "end_mode") and predictor.end_mode is None:
         predictor.end_mode = default_interp_end_mode(len(shape))
     ratios, centres, sweep = _tune_space(predictor, tune, eb_ratio)
-    d_field = D.to_device(inp.field.reshape(-1)) if fast else None
+    d_field = inp.d_field
     agg = getattr(predictor, "agg_level", None)
 
     def header(center, ratio, end):
@@ -599,7 +604,7 @@ def _compress_region_gpu_impl(field, d_field, ebs, predictor, radius, round_outp
     else:
         payload, rec_len, n_out, frame = pack_device_framed(res, fits, zstd_level)
     stats["entropy_s"] += time.time() - t0
-    recon = D.to_host(res.d_recon).reshape(C, *region)
+    recon = D.to_host_pinned(res.d_recon).reshape(C, *region)
     for s in range(plan.n_stages):
         stats["stage_codes"][s] += C * plan.counts[s]
         stats["stage_outliers"][s] += int(n_out[s])
@@ -656,7 +661,7 @@ def decompress(stream: bytes, predictor_factory=None) -> np.ndarray:
             d_recon = decode_device_v5(payload, plan, ebs, o)
         else:
             d_recon = decode_device(payload, plan, ebs, o)
-        canvas = D.to_host(d_recon).reshape(header.channels, *spatial)
+        canvas = D.to_host_pinned(d_recon).reshape(header.channels, *spatial)
         return _finalize(canvas, header)
     if header.version == VERSION_MULTILANE:
         raise ValueError("stream version 5 needs the package's InterpPredictor")

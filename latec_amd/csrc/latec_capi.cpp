@@ -30,6 +30,9 @@ int interp_encode_graph(latec_plan *, const float *, float *, void *, uint32_t *
                         float *, uint32_t *, const double *, const latec_interp_opts *,
                         hipStream_t);
 int64_t interp_stats_words(latec_plan *);
+int64_t minmax_words(int);
+int minmax(int, const float *, const int64_t *, const int64_t *, float *, float *, int64_t,
+           hipStream_t);
 int interp_stats(latec_plan *, const float *, const float *, const latec_interp_opts *, double *,
                  double *, int64_t, hipStream_t);
 int interp_predict(latec_plan *, int, const float *, float *, const latec_interp_opts *,
@@ -200,6 +203,13 @@ int latec_interp_predict(latec_plan *p, int stage, const float *d_recon, float *
 }
 
 int64_t latec_interp_stats_words(latec_plan *p) { return interp_stats_words(p); }
+
+int64_t latec_minmax_words(int nseg) { return minmax_words(nseg); }
+
+int latec_minmax(int nseg, const float *d_x, const int64_t *off, const int64_t *len,
+                 float *d_out, float *d_scratch, int64_t scratch_words, void *stream) {
+  return minmax(nseg, d_x, off, len, d_out, d_scratch, scratch_words, S(stream));
+}
 
 int latec_interp_stats(latec_plan *p, const float *d_x, const float *d_recon,
                        const latec_interp_opts *o, double *d_out, double *d_scratch,

@@ -864,6 +864,56 @@ __global__ void stats_finish_kernel(const double *__restrict__ partial,
   out[st * NSTAT + k] = a;
 }
 
+// Segment min / max of float32 values as NumPy's min() / max() give them: a
+// NaN anywhere makes both NaN.  Pass 1: block b of segment g reduces its
+// slice to a partial pair; pass 2 folds a segment's partials in order.
+__device__ __forceinline__ float nan_min_f(float a, float b) {
+  return (b < a || b != b) && a == a ? b : a;
+}
+__device__ __forceinline__ float nan_max_f(float a, float b) {
+  return (b > a || b != b) && a == a ? b : a;
+}
+constexpr int MM_BLOCKS = 256;  // partial pairs per segment
+__global__ void __launch_bounds__(256) minmax_kernel(const float *__restrict__ x,
+                                                     const int64_t *__restrict__ seg,
+                                                     float *__restrict__ part) {
+  const int g = blockIdx.y;
+  const int64_t lo = seg[2 * g], n = seg[2 * g + 1];
+  float mn = INFINITY, mx = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)MM_BLOCKS * 256) {
+    const float v = x[lo + i];
+    mn = nan_min_f(mn, v);
+    mx = nan_max_f(mx, v);
+  }
+  __shared__ float smn[256], smx[256];
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      smn[threadIdx.x] = nan_min_f(smn[threadIdx.x], smn[threadIdx.x + h]);
+      smx[threadIdx.x] = nan_max_f(smx[threadIdx.x], smx[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * ((int64_t)g * MM_BLOCKS + blockIdx.x)] = smn[0];
+    part[2 * ((int64_t)g * MM_BLOCKS + blockIdx.x) + 1] = smx[0];
+  }
+}
+__global__ void minmax_finish_kernel(const float *__restrict__ part, int nseg,
+                                     float *__restrict__ out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nseg) return;
+  float mn = INFINITY, mx = -INFINITY;
+  for (int b = 0; b < MM_BLOCKS; b++) {
+    mn = nan_min_f(mn, part[2 * ((int64_t)g * MM_BLOCKS + b)]);
+    mx = nan_max_f(mx, part[2 * ((int64_t)g * MM_BLOCKS + b) + 1]);
+  }
+  out[2 * g] = mn;
+  out[2 * g + 1] = mx;
+}
+
 template <typename CodeT>
 __global__ void __launch_bounds__(256) code_hist_kernel(const CodeT *__restrict__ codes,
                                                         int64_t n, uint32_t R,
@@ -1851,8 +1901,14 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
 #ifndef PAIR_WAVES_PER_SIMD
 #define PAIR_WAVES_PER_SIMD 1  // occupancy floor handed to the register allocator
 #endif
-template <int PH, bool ENC, bool AL, bool RND, bool BM, bool XD>
-__global__ void __launch_bounds__(256, PAIR_WAVES_PER_SIMD) pair_kernel(PArgs A, const float *__restrict__ x,
+// WPE: the occupancy floor in waves per SIMD (the register budget the
+// allocator works to); the "pair_waves" knob picks PAIR_WAVES_ALT for the
+// finest level's encode phases 2 and 3.
+#ifndef PAIR_WAVES_ALT
+#define PAIR_WAVES_ALT 8
+#endif
+template <int PH, bool ENC, bool AL, bool RND, bool BM, bool XD, int WPE = PAIR_WAVES_PER_SIMD>
+__global__ void __launch_bounds__(256, WPE) pair_kernel(PArgs A, const float *__restrict__ x,
                                                    float *__restrict__ rec,
                                                    uint16_t *__restrict__ codes,
                                                    uint32_t *__restrict__ hist,
@@ -3042,6 +3098,11 @@ static int64_t g_pair_quads = 2;
 // "min_threads": a march or pair launch shortens its per-thread march (8, 4,
 // 2, 1 steps) until it has at least this many threads
 static int64_t g_min_threads = 1 << 19;
+// "pair_waves": 1 runs the finest level's encode phases 2 and 3 (the launches
+// that read x at its own grid) as the variant compiled for at least
+// PAIR_WAVES_ALT waves per SIMD (a smaller register budget: more waves in
+// flight, more spills); 0 the compiler's own occupancy
+static int64_t g_pair_waves = 0;
 static int64_t g_march_peel = 0;  // histogram: distinct codes per wave counted by ballot
 
 static int64_t g_tune_epoch = 0;  // bumped by every knob change
@@ -3089,6 +3150,14 @@ int tune_set(const char *name, int64_t v) {
     g_pair_quads = v;
     return LATEC_OK;
   }
+  if (!strcmp(name, "pair_waves")) {
+    if (v < 0 || v > 1) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_pair_waves = v;
+    return LATEC_OK;
+  }
   if (!strcmp(name, "pair_lines")) {
     if (v < 0 || v > 1) {
       set_error(std::string("tuning value out of range for ") + name);
@@ -3126,6 +3195,7 @@ int64_t tune_get(const char *name) {
   if (!strcmp(name, "pair_lines")) return g_pair_lines;
   if (!strcmp(name, "pair_quads")) return g_pair_quads;
   if (!strcmp(name, "quad_launches")) return g_quad_launches;
+  if (!strcmp(name, "pair_waves")) return g_pair_waves;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -3354,6 +3424,13 @@ struct PairLaunch {
   static void one(dim3 grid, hipStream_t stream, const PArgs &a, const float *x, float *recon,
                   uint16_t *codes, uint32_t *hist, uint32_t *out_rank, float *out_val,
                   uint32_t *out_count) {
+    if constexpr (valid && E && XD && PH > 1) {
+      if (g_pair_waves) {
+        hipLaunchKernelGGL((pair_kernel<PH, E, AL, RND, BM, XD, PAIR_WAVES_ALT>), grid, dim3(256),
+                           0, stream, a, x, recon, codes, hist, out_rank, out_val, out_count);
+        return;
+      }
+    }
     if constexpr (valid && (PH > 1 || !BM))
       hipLaunchKernelGGL((pair_kernel<PH, E, AL, RND, BM, XD>), grid, dim3(256), 0, stream, a,
                          x, recon, codes, hist, out_rank, out_val, out_count);
@@ -3799,6 +3876,40 @@ int interp_stats(latec_plan *p, const float *x, const float *recon,
   if ((rc = hip_check(hipGetLastError(), "stats_finish_kernel"))) return rc;
   // the host offsets are read by the copy above: finish before they go
   return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+// (min, max) of nseg segments [off, off + len) of device float32 x into
+// out[2 g], out[2 g + 1] (device); scratch: minmax_words(nseg) float32 words.
+int64_t minmax_words(int nseg) { return 2 * (int64_t)nseg * MM_BLOCKS + 4 * (int64_t)nseg + 4; }
+
+int minmax(int nseg, const float *x, const int64_t *off, const int64_t *len, float *out,
+           float *scratch, int64_t scratch_words, hipStream_t stream) {
+  if (nseg <= 0) return LATEC_OK;
+  if (scratch_words < minmax_words(nseg)) {
+    set_error("latec_minmax: scratch too small");
+    return LATEC_E_CAPACITY;
+  }
+  std::vector<int64_t> seg(2 * (size_t)nseg);
+  for (int g = 0; g < nseg; g++) {
+    if (len[g] <= 0) {
+      set_error("latec_minmax: empty segment");
+      return LATEC_E_ARG;
+    }
+    seg[2 * g] = off[g];
+    seg[2 * g + 1] = len[g];
+  }
+  float *part = scratch;
+  int64_t *d_seg = (int64_t *)(scratch + 2 * (int64_t)nseg * MM_BLOCKS);
+  int rc;
+  if ((rc = hip_check(hipMemcpyAsync(d_seg, seg.data(), sizeof(int64_t) * seg.size(),
+                                     hipMemcpyHostToDevice, stream), "hipMemcpyAsync")))
+    return rc;
+  hipLaunchKernelGGL(minmax_kernel, dim3(MM_BLOCKS, (unsigned)nseg), dim3(256), 0, stream, x,
+                     d_seg, part);
+  hipLaunchKernelGGL(minmax_finish_kernel, dim3((nseg + 255) / 256), dim3(256), 0, stream,
+                     part, nseg, out);
+  if ((rc = hip_check(hipGetLastError(), "minmax_kernel"))) return rc;
+  return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // seg ends here
 }
 
 int code_hist(const void *codes, int64_t n, int cb, uint32_t R, uint32_t *hist,

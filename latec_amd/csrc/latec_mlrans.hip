@@ -336,12 +336,16 @@ __global__ void __launch_bounds__(1024) zscan_kernel(const ZStage *__restrict__ 
   if (threadIdx.x == 0) out_count[blockIdx.x] = carry;
 }
 
+// With x / out_val: each outlier's value x[rank] (in stage order) goes to
+// out_val beside its rank.
 template <typename CodeT>
 __global__ void __launch_bounds__(256) zwrite_kernel(const ZStage *__restrict__ zs,
                                                      const int32_t *__restrict__ chunk_stage,
                                                      const CodeT *__restrict__ codes,
                                                      const uint32_t *__restrict__ chunk_cnt,
-                                                     uint32_t *__restrict__ out_rank) {
+                                                     uint32_t *__restrict__ out_rank,
+                                                     const float *__restrict__ x = nullptr,
+                                                     float *__restrict__ out_val = nullptr) {
   const int gc = blockIdx.x;
   const int si = chunk_stage[gc];
   const ZStage Z = zs[si];
@@ -360,7 +364,11 @@ __global__ void __launch_bounds__(256) zwrite_kernel(const ZStage *__restrict__ 
     __syncthreads();
     uint32_t pre = base_sh;
     for (int v = 0; v < wv; v++) pre += wave_cnt[v];
-    if (z) dst[pre + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = (uint32_t)j;
+    if (z) {
+      const uint32_t k = pre + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      dst[k] = (uint32_t)j;
+      if (out_val) out_val[Z.code_off + k] = x[Z.code_off + j];
+    }
     __syncthreads();
     if (threadIdx.x == 0) base_sh += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
     __syncthreads();
@@ -657,9 +665,10 @@ int latec_mlrans_decode(int ns, const uint8_t *d_payload, int64_t payload_len,
   return LATEC_OK;
 }
 
-int latec_zero_ranks(int ns, const void *d_codes, int code_bytes, const int64_t *code_off,
-                     const int64_t *n, uint32_t *d_out_rank, uint32_t *d_out_count,
-                     uint32_t *d_scratch, int64_t scratch_words, void *stream) {
+static int zero_ranks(int ns, const void *d_codes, int code_bytes, const int64_t *code_off,
+                      const int64_t *n, const float *d_x, uint32_t *d_out_rank,
+                      float *d_out_val, uint32_t *d_out_count, uint32_t *d_scratch,
+                      int64_t scratch_words, void *stream) {
   if (ns < 0 || (code_bytes != 2 && code_bytes != 4)) {
     set_error("latec_zero_ranks: bad arguments");
     return LATEC_E_ARG;
@@ -705,13 +714,38 @@ int latec_zero_ranks(int ns, const void *d_codes, int code_bytes, const int64_t 
   if (nc) {
     if (code_bytes == 2)
       hipLaunchKernelGGL(zwrite_kernel<uint16_t>, dim3((unsigned)nc), dim3(256), 0, s, d_zs, d_cs,
-                         (const uint16_t *)d_codes, d_cnt, d_out_rank);
+                         (const uint16_t *)d_codes, d_cnt, d_out_rank, d_x, d_out_val);
     else
       hipLaunchKernelGGL(zwrite_kernel<uint32_t>, dim3((unsigned)nc), dim3(256), 0, s, d_zs, d_cs,
-                         (const uint32_t *)d_codes, d_cnt, d_out_rank);
+                         (const uint32_t *)d_codes, d_cnt, d_out_rank, d_x, d_out_val);
   }
   if ((rc = hip_ok(hipGetLastError(), "zero-rank kernels"))) return rc;
   return hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors end here
+}
+
+int latec_zero_ranks(int ns, const void *d_codes, int code_bytes, const int64_t *code_off,
+                     const int64_t *n, uint32_t *d_out_rank, uint32_t *d_out_count,
+                     uint32_t *d_scratch, int64_t scratch_words, void *stream) {
+  return zero_ranks(ns, d_codes, code_bytes, code_off, n, nullptr, d_out_rank, nullptr,
+                    d_out_count, d_scratch, scratch_words, stream);
+}
+
+int latec_order_outliers(int ns, const void *d_codes, int code_bytes, const int64_t *code_off,
+                         const int64_t *n, const float *d_x, uint32_t *d_out_rank,
+                         float *d_out_val, uint32_t *d_out_count, uint32_t *d_scratch,
+                         int64_t scratch_words, void *stream) {
+  if (!d_x || !d_out_val) {
+    set_error("latec_order_outliers: values required");
+    return LATEC_E_ARG;
+  }
+  return zero_ranks(ns, d_codes, code_bytes, code_off, n, d_x, d_out_rank, d_out_val,
+                    d_out_count, d_scratch, scratch_words, stream);
+}
+
+int64_t latec_zero_ranks_words(int ns, const int64_t *n) {
+  int64_t nc = 0;
+  for (int i = 0; i < ns; i++) nc += (n[i] + ZCHUNK - 1) / ZCHUNK;
+  return (((int64_t)(sizeof(ZStage) / 4) * ns + 3) & ~3ll) + 2 * nc + 8;
 }
 
 }  // extern "C"

@@ -45,11 +45,13 @@ def quantize_device(dx, dpred, n: int, eb: float, o: InterpOpts, d_hist=None):
     check(lib().latec_quantize(ptr(dx), ptr(dpred), int(n), float(eb), _opts_ptr(o),
                                ptr(codes), ptr(rank), ptr(val), ptr(cnt), ptr(hist), ptr(None)))
     k = int(cnt.item())
-    if k > 1:
-        r = rank[:k].to(D.torch().int64)
-        order = D.torch().argsort(r)
-        rank[:k] = rank[:k][order]
-        val[:k] = val[:k][order]
+    if k > 1:  # the kernel appends outliers in any order: list them by position
+        nn = np.array([n], np.int64)
+        words = int(lib().latec_zero_ranks_words(1, ptr(nn)))
+        scratch = D.empty(words, np.uint32)
+        check(lib().latec_order_outliers(1, ptr(codes), o.code_bytes, ptr(np.zeros(1, np.int64)),
+                                         ptr(nn), ptr(dx), ptr(rank), ptr(val), ptr(cnt),
+                                         ptr(scratch), words, ptr(None)))
     return codes, rank, val, k, hist
 
 

@@ -630,8 +630,21 @@ void orc_scale_to_level(const float *scale, int64_t n, double eb,
 }
 
 /* ------------------------------------------------------------------------ */
-/* Stack rANS (design of constriction 0.4 AnsCoder<u32,u64> with a 24-bit   */
-/* Categorical(perfect=False) model) -- rans.py:466-535.  PARITY UNPINNED.   */
+/* Stack rANS -- rans.py:466-535, 629-657.  The reference calls the Rust    */
+/* crate constriction ~= 0.4.2 (pyproject.toml:16), absent here.  Restated  */
+/* from its published design:                                                */
+/*  - the model: Python Categorical(probabilities, perfect=False) builds    */
+/*    ContiguousCategoricalEntropyModel::from_floating_point_probabilities_ */
+/*    fast (module stream::model): scale = f32((2^24 - A) / sum), left[k] = */
+/*    floor(f32(cum_k * scale)) + k, every symbol keeping >= 1;             */
+/*  - the coder: AnsCoder<u32, u64> (module stream::stack), a stack: the    */
+/*    reference encodes symbols in reverse so they decode in order; a push  */
+/*    renormalizes (state >> 40 >= p: the low u32 word out, state >>= 32)   */
+/*    before state = (state / p) << 24 | (left + state % p);                */
+/*  - the seal (into_compressed): the final state's low word, then its      */
+/*    high word when nonzero.                                               */
+/* Pinned by hand-derived word arrays (tests/test_capi.py RANS_KATS) and     */
+/* round trips; the bytes against constriction itself stay UNPINNED.         */
 /* ------------------------------------------------------------------------ */
 
 /* Float probabilities (float32 of the table's pmf, rans.py:646-649) to

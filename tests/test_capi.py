@@ -190,3 +190,45 @@ def test_pack_stages_framed_equals_pack_then_zstd(kinds, threads):
     assert gsize == size and got[:size].tobytes() == want[:size].tobytes()
     assert np.array_equal(got_len, want_len)
     assert frame[:flen.value].tobytes() == want_frame
+
+
+# Hand-derived known answers for the stack rANS coder (the design of
+# constriction 0.4's AnsCoder<u32, u64> over Categorical(probs, perfect=False)
+# at 24-bit precision, latec/rans.py:466-535, 629-657).  Model: the table row
+# cdf = [0, 3, 8, 2^24] (pmf 3, 5, 2^24 - 8).  Leaky quantization: scale =
+# f32((2^24 - 3) / 2^24) = 1 - 3 * 2^-24; left[k] = floor(f32(cum_k * scale)) + k
+# -> left[1] = floor(f32(3 - 9 * 2^-24) = 3 - 2^-21) + 1 = 3,
+#    left[2] = floor(8 - 3 * 2^-21) + 2 = 9, left[3] = 2^24
+# (probabilities 3, 6, 2^24 - 9: not the table's 3, 5).  Symbols are pushed
+# last to first; a push of s: if state >> 40 >= p(s), emit the low word and
+# state >>= 32; then state = (state // p) << 24 | (left[s] + state % p).  The
+# seal writes the final state's low word, then its high word if nonzero.
+#   [1, 0, 2]: 0 -push 2-> 9 -push 0-> 0x3000000 -push 1-> 2^47 + 3
+#              -> words [0x3, 0x8000]
+#   [0, 0, 0, 2]: 9 -> 0x3000000 -> 2^48 -push 0: 2^48 >> 40 = 256 >= 3, emit
+#              0x0, state 2^16 -> (21845 << 24) | 1 = 0x5555000001
+#              -> words [0x0, 0x55000001, 0x55]
+# A wrong quantizer (round instead of floor, no +k), push order, word order,
+# renormalization test or seal changes at least one of these words.
+RANS_KAT_CDF = np.array([0, 3, 8, 1 << 24], np.uint32)
+RANS_KATS = [([1, 0, 2], [0x3, 0x8000]),
+             ([0, 0, 0, 2], [0x0, 0x55000001, 0x55]),
+             # p(2) > state: each push only adds left[2] = 9 -> 9, 18, 27
+             ([2, 2, 2], [0x1B])]
+
+
+@pytest.mark.parametrize("syms,words", RANS_KATS)
+def test_rans_known_answers(syms, words):
+    syms = np.array(syms, np.uint32)
+    cdf = RANS_KAT_CDF
+    assert list(O.rans_encode_syms(syms, cdf)) == words
+    out = np.empty(len(syms) + 4, np.uint32)
+    nw = _lib.check(_lib.lib().latec_rans_encode(_lib.ptr(syms), len(syms), _lib.ptr(cdf),
+                                                 len(cdf) - 1, _lib.ptr(out), len(out)))
+    assert list(out[:nw]) == words
+    assert np.array_equal(O.rans_decode_syms(np.array(words, np.uint32), len(syms), cdf), syms)
+    dec = np.empty(len(syms), np.uint32)
+    w = np.array(words, np.uint32)
+    assert _lib.lib().latec_rans_decode(_lib.ptr(w), len(w), _lib.ptr(cdf), len(cdf) - 1,
+                                        _lib.ptr(dec), len(syms)) == 0
+    assert np.array_equal(dec, syms)

@@ -509,3 +509,77 @@ def test_gpu_stage_stats_match_reference_moments(case):
     assert np.array_equal(got[:, 4], ref[:, 4])
     assert stats["recon_sse"] == float(sum(stats["stage_recon_sse"]))
     assert all(v > 0 for v, n in zip(stats["stage_pred_sae"], stats["stage_codes"]) if n)
+
+
+@pytest.mark.parametrize("tune", ["size", "rd"])
+def test_gpu_tune_sweep_picks_the_oracle_candidate(tune):
+    """tune='size' / 'rd' (codec.py:171-254): the candidates (every end mode
+    at the first ratio, then ratio x centre at the kept end mode) are encoded
+    by the oracle, the winner chosen by the reference's keys, and the GPU
+    path's stream must be that candidate's oracle stream."""
+    from latec_amd.predictor import END_MODE_MAX, default_interp_center, default_interp_end_mode
+
+    x = smooth((48, 40, 24), seed=8, rough=0.01)
+    eb = 1e-3 * float(x.max() - x.min())
+    stream, st = latec_amd.compress(x, eb, InterpPredictor("cubic", 3, 8, 1), levels=3,
+                                    anchor_stride=8, anchor_block=1, tune=tune)
+    c0, e0 = default_interp_center(3), default_interp_end_mode(3)
+    ratios = [1.0, 0.9, 0.8, 0.7]
+    centres = [c0] + [c for c in (0, 1, 2) if c != c0]
+    cands = {}
+
+    def run(r, c, e):
+        if (r, c, e) not in cands:
+            s, rec = O.compress(x, eb, "cubic", 3, 8, 1, c, e, eb_ratio=r)
+            cands[(r, c, e)] = (len(s), float(np.square(x - rec[0], dtype=np.float64).sum()), s)
+
+    run(ratios[0], c0, e0)
+    for e in range(END_MODE_MAX + 1):
+        run(ratios[0], c0, e)
+    end = min(cands.items(), key=lambda kv: (kv[1][0], kv[1][1]))[0][2]
+    for r in ratios:
+        for c in centres:
+            run(r, c, end)
+    keyed = [(size, sse, r, c, e, s) for (r, c, e), (size, sse, s) in cands.items()]
+    if tune == "rd":
+        lim = min(k[0] for k in keyed) * 1.05
+        best = min((k for k in keyed if k[0] <= lim), key=lambda k: (k[1], k[0], k[2], k[3], k[4]))
+    else:
+        best = min(keyed, key=lambda k: k[:5])
+    assert st["tune_candidates"] == len(cands)
+    assert (st["eb_ratio"], st["interp_center"], st["interp_end_mode"]) == best[2:5]
+    assert stream == best[5]
+    assert np.abs(latec_amd.decompress(stream) - x).max() <= eb
+
+
+def test_gpu_choose_shape_matches_reference_golden():
+    """choose_shape (rans.py:204-243, per-point levels) on the GPU-built level
+    weights picks the reference's shape on every recorded stage
+    (tests/golden/make_shape_golden.py)."""
+    fv = np.load(os.path.join(G, "shape_vectors.npz"))
+    meta = fv["meta"]
+    assert len(meta) >= 50 and len({int(m[4]) for m in meta}) >= 5
+    for i, n, eb, kexp, shape in meta:
+        i = int(i)
+        codes = fv[f"codes{i}"].astype(np.uint32)
+        levels = fv[f"levels{i}"]
+        assert len(codes) == int(n)
+        tables = rans.build_laplace_tables(float(eb), 1 << 15)
+        assert rans.choose_kexp(codes, 1 << 15) == int(kexp)
+        assert rans.choose_shape(codes, levels, tables, int(kexp)) == int(shape), i
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if len(c[0]) == 3 and c[4] == 1],
+                         ids=lambda c: f"{c[0]}-{c[5]}-c{c[6]}e{c[7]}")
+def test_gpu_pair_waves_variant_bit_exact(case):
+    """The finest level's encode phases compiled for eight waves per SIMD
+    (tuning knob "pair_waves"): the same codes, outliers, recon and payload as
+    the oracle's."""
+    from latec_amd import autotune
+
+    old = autotune.get()
+    autotune.set_knobs(pair=1, pair_lines=1, pair_quads=2, slab=0, pair_waves=1)
+    try:
+        test_gpu_encode_matches_oracle_bit_exact(case)
+    finally:
+        autotune.set_knobs(**old)
