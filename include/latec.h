@@ -425,13 +425,17 @@ int latec_scatter_f32(const float *d_src, const int64_t *d_idx, int64_t n,
 /* forward/backward kernels composed by torch.autograd.Functions.      */
 /* ------------------------------------------------------------------ */
 
-/* out[r, :] = src[idx[r], :] (rows of width w) / dsrc[idx[r], :] += dout[r, :]
- * (atomic) / E[idx[r], :] = vals[r, :] (vals NULL: zeros). */
+/* out[r, :] = src[idx[r], :] (rows of width w) / E[idx[r], :] = vals[r, :]
+ * (vals NULL: zeros). */
 int latec_train_gather(const float *d_src, int64_t w, const int64_t *d_idx,
                        int64_t n, float *d_out, void *stream);
+/* dsrc (n_src rows of width w) = sum over r of dout[r, :] into row idx[r],
+ * deterministic: 64-bit fixed-point sums scaled by max |dout| (exact in any
+ * order), one rounding to float32.  d_acc: n_src * w + 1 words of 64 bits
+ * (scratch, cleared here). */
 int latec_train_scatter_add(const float *d_dout, int64_t w,
                             const int64_t *d_idx, int64_t n, float *d_dsrc,
-                            void *stream);
+                            int64_t n_src, uint64_t *d_acc, void *stream);
 int latec_train_write_rows(float *d_E, int64_t w, const int64_t *d_idx,
                            int64_t n, const float *d_vals, void *stream);
 /* C (M x N, row stride ldc) [+]= A * B (+ bias[N]) with generic strides
@@ -442,6 +446,13 @@ int latec_train_gemm(int64_t M, int64_t N, int64_t K, const float *d_A,
                      int64_t sbn, float *d_C, int64_t ldc, const float *d_bias,
                      int accumulate, float *d_work, int64_t work_floats,
                      void *stream);
+/* The same product with both operands rounded to bf16 and FP32 accumulation
+ * on v_mfma_f32_16x16x32_bf16 tiles (BF16 training; FP32 master weights). */
+int latec_train_gemm_bf16(int64_t M, int64_t N, int64_t K, const float *d_A,
+                          int64_t sam, int64_t sak, const float *d_B, int64_t sbk,
+                          int64_t sbn, float *d_C, int64_t ldc, const float *d_bias,
+                          int accumulate, float *d_work, int64_t work_floats,
+                          void *stream);
 /* out[n] [+]= sum_m X[m, n] in a fixed order. */
 int latec_train_colsum(int64_t M, int64_t N, const float *d_X, float *d_out,
                        int accumulate, float *d_work, int64_t work_floats,
@@ -455,11 +466,15 @@ int latec_train_attn(int64_t n_items, const int64_t *d_off, const float *d_s,
                      const float *d_v, int d, float null_s,
                      const float *d_null_v, float *d_out, float *d_w_rows,
                      float *d_w_null, void *stream);
+/* Backward: d_dnull (d + 1 floats) gets the null score's gradient then the
+ * null value's, summed over items in a fixed order through d_work (at least
+ * n_items * (d + 1) + 1024 * (d + 1) floats). */
 int latec_train_attn_bwd(int64_t n_items, const int64_t *d_off,
                          const float *d_v, int d, const float *d_null_v,
                          const float *d_w_rows, const float *d_w_null,
                          const float *d_dout, float *d_ds, float *d_dv,
-                         float *d_dnull_s, float *d_dnull_v, void *stream);
+                         float *d_dnull, float *d_work, int64_t work_floats,
+                         void *stream);
 /* Head loss: mu = sigmoid(o0), log_b = log2(eb) + clamp(o1, -4, 12),
  * bits = discretized-Laplace code length (+ mse_w (mu - t)^2); writes bits,
  * d bits / d o (M x 2) and mu (may be NULL). */

@@ -118,15 +118,17 @@ SIGNATURES = {
     "latec_gather_f32": (i32, [p, p, i64, p, p]),
     "latec_scatter_f32": (i32, [p, p, i64, p, p]),
     "latec_train_gather": (i32, [p, i64, p, i64, p, p]),
-    "latec_train_scatter_add": (i32, [p, i64, p, i64, p, p]),
+    "latec_train_scatter_add": (i32, [p, i64, p, i64, p, i64, p, p]),
     "latec_train_write_rows": (i32, [p, i64, p, i64, p, p]),
     "latec_train_gemm": (i32, [i64, i64, i64, p, i64, i64, p, i64, i64, p, i64, p, i32, p, i64,
+                               p]),
+    "latec_train_gemm_bf16": (i32, [i64, i64, i64, p, i64, i64, p, i64, i64, p, i64, p, i32, p, i64,
                                p]),
     "latec_train_colsum": (i32, [i64, i64, p, p, i32, p, i64, p]),
     "latec_train_gelu": (i32, [p, p, i64, p]),
     "latec_train_gelu_bwd": (i32, [p, p, p, i64, p]),
     "latec_train_attn": (i32, [i64, p, p, p, i32, f32, p, p, p, p, p]),
-    "latec_train_attn_bwd": (i32, [i64, p, p, i32, p, p, p, p, p, p, p, p, p]),
+    "latec_train_attn_bwd": (i32, [i64, p, p, i32, p, p, p, p, p, p, p, p, i64, p]),
     "latec_train_nll": (i32, [i64, p, p, p, f32, p, p, p, p]),
     "latec_mlrans_lanes": (i64, [i64]),
     "latec_mlrans_scratch_words": (i64, [i32, p]),

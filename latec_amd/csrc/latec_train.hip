@@ -11,9 +11,13 @@
 //   segmented attention pool ....... AttnPool over a CSR row segment + null
 //   Laplace head loss .............. sigmoid / clamp / discretized-Laplace
 //                                    bits and their gradients
-// Parameter-gradient reductions that cross rows use fixed-order partial sums
-// (split-K GEMM, column sums); the scatter-add of neighbour gradients uses
-// atomics (training only; inference never calls these kernels).
+// Every reduction is deterministic: parameter-gradient reductions that cross
+// rows use fixed-order partial sums (split-K GEMM, column sums, the null
+// token's gradients), and the scatter-add of neighbour gradients accumulates
+// in 64-bit fixed point, whose integer sums do not depend on the order of
+// the adds.  GEMMs run in FP32 or, for BF16 training, as v_mfma_f32_16x16x32_bf16
+// tiles (operands rounded to bf16, FP32 accumulation; the FP32 master weights
+// stay in the optimizer).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -49,15 +53,63 @@ __global__ void gather_rows_kernel(const float *__restrict__ src, int64_t w,
   }
 }
 
-__global__ void scatter_add_rows_kernel(const float *__restrict__ dout, int64_t w,
-                                        const int64_t *__restrict__ idx, int64_t n,
-                                        float *__restrict__ dsrc) {
+// Deterministic scatter-add: amax = max |dout| (float bits, atomicMax, which
+// is order-independent); each term becomes llrint(dout * 2^sh) with
+// 2^sh * amax < 2^(62 - h) and n < 2^h, so the 64-bit integer sum of up to n
+// terms cannot overflow and is exact whatever order the adds land in; the
+// float result is that sum times 2^-sh (one rounding).
+__global__ void fill_kernel(uint32_t *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+__global__ void absmax_kernel(const float *__restrict__ x, int64_t n,
+                              uint32_t *__restrict__ amax_bits) {
+  uint32_t m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t b = __float_as_uint(x[i]) & 0x7FFFFFFFu;  // |x| (NaN stays above inf)
+    m = b > m ? b : m;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint32_t t = __shfl_xor(m, o);
+    m = t > m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(amax_bits, m);
+}
+
+__device__ __forceinline__ int fixed_shift(uint32_t amax_bits, int headroom) {
+  int e;
+  (void)frexpf(__uint_as_float(amax_bits), &e);  // amax < 2^e
+  return 62 - headroom - e;
+}
+
+__global__ void scatter_fixed_kernel(const float *__restrict__ dout, int64_t w,
+                                     const int64_t *__restrict__ idx, int64_t n,
+                                     const uint32_t *__restrict__ amax_bits, int headroom,
+                                     unsigned long long *__restrict__ acc) {
+  const uint32_t ab = *amax_bits;
+  if (ab == 0u || ab >= 0x7F800000u) return;  // all zero, or non-finite (see finish)
+  const double sc = ldexp(1.0, fixed_shift(ab, headroom));
   const int64_t total = n * w;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = i / w, c = i - r * w;
-    atomicAdd(dsrc + idx[r] * w + c, dout[i]);
+    const long long q = llrint((double)dout[i] * sc);
+    if (q) atomicAdd(acc + idx[r] * w + c, (unsigned long long)q);
   }
+}
+
+__global__ void fixed_finish_kernel(const unsigned long long *__restrict__ acc, int64_t n,
+                                    const uint32_t *__restrict__ amax_bits, int headroom,
+                                    float *__restrict__ dsrc) {
+  const uint32_t ab = *amax_bits;
+  const double inv = ab && ab < 0x7F800000u ? ldexp(1.0, -fixed_shift(ab, headroom)) : 0.0;
+  const float bad = ab >= 0x7F800000u ? __uint_as_float(0x7FC00000u) : 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dsrc[i] = ab >= 0x7F800000u ? bad : (float)((double)(long long)acc[i] * inv);
 }
 
 __global__ void write_rows_kernel(float *__restrict__ E, int64_t w,
@@ -171,6 +223,96 @@ __global__ void gemm_finish_kernel(int64_t M, int64_t N, int splits, const float
   }
 }
 
+// BF16 MFMA GEMM with the FP32 path's interface and split-K: 64 x 64 tiles,
+// K in steps of 32; the operands are rounded to bf16 (round to nearest even)
+// while staged to LDS, each wave computes a 32 x 32 quarter as 2 x 2
+// v_mfma_f32_16x16x32_bf16 tiles with FP32 accumulation.  Operand layout of
+// the instruction: lane l holds row l % 16 of A and column l % 16 of B at
+// k = 8 (l / 16) .. 8 (l / 16) + 7; result element j of lane l is
+// (row 4 (l / 16) + j, column l % 16).
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+constexpr int TBM = 64, TBN = 64, TBK = 32, TPAD = TBK + 8;
+
+__device__ __forceinline__ uint16_t f2bf(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int64_t M, int64_t N, int64_t K,
+                                                        const float *__restrict__ A, int64_t sam,
+                                                        int64_t sak, const float *__restrict__ B,
+                                                        int64_t sbk, int64_t sbn,
+                                                        float *__restrict__ C, int64_t kchunk,
+                                                        int direct,
+                                                        const float *__restrict__ bias,
+                                                        int accumulate, int64_t ldc) {
+  __shared__ __attribute__((aligned(16))) uint16_t sA[TBM * TPAD];  // [m][k]
+  __shared__ __attribute__((aligned(16))) uint16_t sB[TBN * TPAD];  // [n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  const int64_t m0 = (int64_t)blockIdx.x * TBM, n0 = (int64_t)blockIdx.y * TBN;
+  const int64_t kb = (int64_t)blockIdx.z * kchunk;
+  const int64_t ke = kb + kchunk < K ? kb + kchunk : K;
+  f32x4_t acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+  for (int64_t k0 = kb; k0 < ke; k0 += TBK) {
+    for (int e = tid; e < TBM * TBK; e += 256) {
+      int kk, mm;
+      if (sak == 1) {
+        kk = e % TBK;
+        mm = e / TBK;
+      } else {
+        mm = e % TBM;
+        kk = e / TBM;
+      }
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      sA[mm * TPAD + kk] = f2bf((gm < M && gk < ke) ? A[gm * sam + gk * sak] : 0.0f);
+    }
+    for (int e = tid; e < TBN * TBK; e += 256) {
+      int kk, nn;
+      if (sbn == 1) {
+        nn = e % TBN;
+        kk = e / TBN;
+      } else {
+        kk = e % TBK;
+        nn = e / TBK;
+      }
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      sB[nn * TPAD + kk] = f2bf((gn < N && gk < ke) ? B[gk * sbk + gn * sbn] : 0.0f);
+    }
+    __syncthreads();
+    bf16x8_t a[2], b[2];
+    for (int i = 0; i < 2; i++) {
+      a[i] = *(const bf16x8_t *)(sA + (wm + i * 16 + r) * TPAD + kq);
+      b[i] = *(const bf16x8_t *)(sB + (wn + i * 16 + r) * TPAD + kq);
+    }
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int t = 0; t < 4; t++) {
+        const int64_t gm = m0 + wm + i * 16 + (lane >> 4) * 4 + t;
+        const int64_t gn = n0 + wn + j * 16 + (lane & 15);
+        if (gm >= M || gn >= N) continue;
+        float v = acc[i][j][t];
+        if (direct) {
+          if (bias) v += bias[gn];
+          float *o = C + gm * ldc + gn;
+          *o = accumulate ? *o + v : v;
+        } else {
+          C[(int64_t)blockIdx.z * M * N + gm * N + gn] = v;
+        }
+      }
+}
+
 // column sums of X (M x N): fixed-order partials per block of rows
 __global__ void colsum_part_kernel(int64_t M, int64_t N, const float *__restrict__ X,
                                    int64_t rows_per, float *__restrict__ part) {
@@ -277,8 +419,10 @@ __global__ __launch_bounds__(256) void attn_bwd_kernel(
       if (lane == 0) ds[r] = wr * (g - gbar);
       if (lane < d) dv[r * d + lane] = wr * go;
     }
-    if (lane == 0) atomicAdd(dnull_s, wn * (gn - gbar));
-    if (lane < d) atomicAdd(dnull_v + lane, wn * go);
+    // this item's null-token gradients: row i of a (n_items x (d + 1)) array
+    // summed in a fixed order afterwards
+    if (lane == 0) dnull_s[i * (d + 1)] = wn * (gn - gbar);
+    if (lane < d) dnull_s[i * (d + 1) + 1 + lane] = wn * go;
   }
 }
 
@@ -361,11 +505,25 @@ int latec_train_gather(const float *src, int64_t w, const int64_t *idx, int64_t 
 }
 
 int latec_train_scatter_add(const float *dout, int64_t w, const int64_t *idx, int64_t n,
-                            float *dsrc, void *stream) {
-  if (n <= 0 || w <= 0) return LATEC_OK;
-  hipLaunchKernelGGL(scatter_add_rows_kernel, dim3(grid1(n * w, 256)), dim3(256), 0,
-                     (hipStream_t)stream, dout, w, idx, n, dsrc);
-  return hip_ok(hipGetLastError(), "scatter_add_rows_kernel");
+                            float *dsrc, int64_t n_src, uint64_t *acc, void *stream) {
+  if (w <= 0 || n_src <= 0) return LATEC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  // acc: n_src * w words of 64 bits, then one word holding amax
+  const int64_t nacc = n_src * w;
+  uint32_t *amax = (uint32_t *)(acc + nacc);
+  int headroom = 1;
+  while (headroom < 62 && ((int64_t)1 << headroom) <= n) headroom++;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid1(2 * nacc + 2, 256)), dim3(256), 0, s,
+                     (uint32_t *)acc, 2 * nacc + 2);
+  if (n > 0) {
+    hipLaunchKernelGGL(absmax_kernel, dim3(grid1(n * w, 256)), dim3(256), 0, s, dout, n * w,
+                       amax);
+    hipLaunchKernelGGL(scatter_fixed_kernel, dim3(grid1(n * w, 256)), dim3(256), 0, s, dout, w,
+                       idx, n, amax, headroom, (unsigned long long *)acc);
+  }
+  hipLaunchKernelGGL(fixed_finish_kernel, dim3(grid1(nacc, 256)), dim3(256), 0, s,
+                     (const unsigned long long *)acc, nacc, amax, headroom, dsrc);
+  return hip_ok(hipGetLastError(), "scatter_fixed_kernel");
 }
 
 int latec_train_write_rows(float *E, int64_t w, const int64_t *idx, int64_t n,
@@ -377,10 +535,10 @@ int latec_train_write_rows(float *E, int64_t w, const int64_t *idx, int64_t n,
 }
 
 // C (M x N, row stride ldc) [+]= A * B (+ bias); work holds split partials.
-int latec_train_gemm(int64_t M, int64_t N, int64_t K, const float *A, int64_t sam, int64_t sak,
-                     const float *B, int64_t sbk, int64_t sbn, float *C, int64_t ldc,
-                     const float *bias, int accumulate, float *work, int64_t work_floats,
-                     void *stream) {
+static int train_gemm(bool bf16, int64_t M, int64_t N, int64_t K, const float *A, int64_t sam,
+                      int64_t sak, const float *B, int64_t sbk, int64_t sbn, float *C,
+                      int64_t ldc, const float *bias, int accumulate, float *work,
+                      int64_t work_floats, void *stream) {
   if (M <= 0 || N <= 0) return LATEC_OK;
   hipStream_t s = (hipStream_t)stream;
   const int64_t tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
@@ -397,18 +555,35 @@ int latec_train_gemm(int64_t M, int64_t N, int64_t K, const float *A, int64_t sa
   kchunk = ((kchunk + BK - 1) / BK) * BK;
   if (kchunk < BK) kchunk = BK;
   dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN), (unsigned)splits);
+  auto kern = bf16 ? gemm_bf16_kernel : gemm_kernel;
   if (splits == 1) {
-    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C,
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C,
                        kchunk, 1, bias, accumulate, ldc);
     return hip_ok(hipGetLastError(), "gemm_kernel");
   }
-  hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn,
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn,
                      work, kchunk, 0, (const float *)nullptr, 0, (int64_t)0);
   int rc = hip_ok(hipGetLastError(), "gemm_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(gemm_finish_kernel, dim3(grid1(M * N, 256)), dim3(256), 0, s, M, N, splits,
                      work, bias, accumulate, C, ldc);
   return hip_ok(hipGetLastError(), "gemm_finish_kernel");
+}
+
+int latec_train_gemm(int64_t M, int64_t N, int64_t K, const float *A, int64_t sam, int64_t sak,
+                     const float *B, int64_t sbk, int64_t sbn, float *C, int64_t ldc,
+                     const float *bias, int accumulate, float *work, int64_t work_floats,
+                     void *stream) {
+  return train_gemm(false, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, accumulate, work,
+                    work_floats, stream);
+}
+
+int latec_train_gemm_bf16(int64_t M, int64_t N, int64_t K, const float *A, int64_t sam,
+                          int64_t sak, const float *B, int64_t sbk, int64_t sbn, float *C,
+                          int64_t ldc, const float *bias, int accumulate, float *work,
+                          int64_t work_floats, void *stream) {
+  return train_gemm(true, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, accumulate, work,
+                    work_floats, stream);
 }
 
 int latec_train_colsum(int64_t M, int64_t N, const float *X, float *out, int accumulate,
@@ -463,17 +638,26 @@ int latec_train_attn(int64_t n_items, const int64_t *off, const float *s, const 
 
 int latec_train_attn_bwd(int64_t n_items, const int64_t *off, const float *v, int d,
                          const float *null_v, const float *w_rows, const float *w_null,
-                         const float *dout, float *ds, float *dv, float *dnull_s,
-                         float *dnull_v, void *stream) {
+                         const float *dout, float *ds, float *dv, float *dnull,
+                         float *work, int64_t work_floats, void *stream) {
   if (n_items <= 0) return LATEC_OK;
   if (d > 64) {
     latec::set_error("attention pool supports d <= 64");
     return LATEC_E_ARG;
   }
-  hipLaunchKernelGGL(attn_bwd_kernel, dim3(grid1(n_items * 64, 256)), dim3(256), 0,
-                     (hipStream_t)stream, n_items, off, v, d, null_v, w_rows, w_null, dout, ds,
-                     dv, dnull_s, dnull_v);
-  return hip_ok(hipGetLastError(), "attn_bwd_kernel");
+  const int64_t items = n_items * (d + 1);
+  if (work_floats < items + d + 1) {
+    latec::set_error("latec_train_attn_bwd: workspace too small");
+    return LATEC_E_CAPACITY;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(attn_bwd_kernel, dim3(grid1(n_items * 64, 256)), dim3(256), 0, s, n_items,
+                     off, v, d, null_v, w_rows, w_null, dout, ds, dv, work, nullptr);
+  int rc = hip_ok(hipGetLastError(), "attn_bwd_kernel");
+  if (rc) return rc;
+  // dnull[0] = the null score's gradient, dnull[1 + k] = the null value's
+  return latec_train_colsum(n_items, d + 1, work, dnull, 0, work + items, work_floats - items,
+                            stream);
 }
 
 int latec_train_nll(int64_t M, const float *o, const float *t, const float *eb, float mse_w,

@@ -46,11 +46,26 @@ def _work(n_floats: int):
     return w
 
 
-def _gemm(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias=None, accumulate=False):
+# GEMM arithmetic of every Linear, forward and both gradients: "fp32" or
+# "bf16" (operands rounded to bf16, FP32 accumulation on MFMA tiles; the FP32
+# master weights stay in the optimizer).  Set with set_precision or by
+# GnnTrainer(precision=...).
+_PRECISION = {"gemm": "fp32"}
+
+
+def set_precision(mode: str) -> None:
+    if mode not in ("fp32", "bf16"):
+        raise ValueError("precision must be 'fp32' or 'bf16'")
+    _PRECISION["gemm"] = mode
+
+
+def _gemm(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias=None, accumulate=False, prec=None):
     need = max(1, M * N) * 256 if M * N <= (1 << 20) else (1 << 20)
     w = _work(need)
-    check(lib().latec_train_gemm(M, N, K, ptr(A), sam, sak, ptr(B), sbk, sbn, ptr(C), ldc,
-                                 ptr(bias), int(accumulate), ptr(w), w.numel(), ptr(None)))
+    prec = prec or _PRECISION["gemm"]
+    fn = lib().latec_train_gemm_bf16 if prec == "bf16" else lib().latec_train_gemm
+    check(fn(M, N, K, ptr(A), sam, sak, ptr(B), sbk, sbn, ptr(C), ldc, ptr(bias),
+             int(accumulate), ptr(w), w.numel(), ptr(None)))
 
 
 def _colsum(X, out, accumulate=False):
@@ -74,7 +89,8 @@ def _functions():
             n, k = x.shape
             out = W.shape[0]
             y = T.empty(n, out, dtype=T.float32, device=x.device)
-            _gemm(n, out, k, x, k, 1, W, W.stride(1), W.stride(0), y, out, b)
+            ctx.prec = _PRECISION["gemm"]  # the backward's GEMMs use the forward's mode
+            _gemm(n, out, k, x, k, 1, W, W.stride(1), W.stride(0), y, out, b, prec=ctx.prec)
             ctx.save_for_backward(x, W)
             ctx.has_b = b is not None
             return y
@@ -88,10 +104,10 @@ def _functions():
             dx = dW = db = None
             if ctx.needs_input_grad[0]:
                 dx = T.empty(n, k, dtype=T.float32, device=dy.device)
-                _gemm(n, k, out, dy, out, 1, W, W.stride(0), W.stride(1), dx, k)
+                _gemm(n, k, out, dy, out, 1, W, W.stride(0), W.stride(1), dx, k, prec=ctx.prec)
             if ctx.needs_input_grad[1]:
                 dW = T.empty(out, k, dtype=T.float32, device=dy.device)
-                _gemm(out, k, n, dy, 1, out, x, k, 1, dW, k)
+                _gemm(out, k, n, dy, 1, out, x, k, 1, dW, k, prec=ctx.prec)
             if ctx.has_b and ctx.needs_input_grad[2]:
                 db = T.empty(out, dtype=T.float32, device=dy.device)
                 _colsum(dy, db)
@@ -132,9 +148,10 @@ def _functions():
             (idx,) = ctx.saved_tensors
             dout = dout.contiguous()
             w = dout.shape[1]
-            dsrc = T.zeros(ctx.n_src, w, dtype=T.float32, device=dout.device)
+            dsrc = T.empty(ctx.n_src, w, dtype=T.float32, device=dout.device)
+            acc = T.empty(ctx.n_src * w + 1, dtype=T.int64, device=dout.device)
             check(lib().latec_train_scatter_add(ptr(dout), w, ptr(idx), idx.numel(), ptr(dsrc),
-                                                ptr(None)))
+                                                ctx.n_src, ptr(acc), ptr(None)))
             return dsrc, None
 
     class WriteRows(T.autograd.Function):
@@ -191,13 +208,16 @@ def _functions():
             d = v.shape[1]
             ds = T.empty(max(ctx.n_s, 1), dtype=T.float32, device=v.device)
             dv = T.empty_like(v)
-            dns = T.zeros(1, dtype=T.float32, device=v.device)
-            dnv = T.zeros(d, dtype=T.float32, device=v.device)
-            check(lib().latec_train_attn_bwd(off.numel() - 1, ptr(off), ptr(v), d,
+            n_items = off.numel() - 1
+            dnull = T.zeros(d + 1, dtype=T.float32, device=v.device)
+            work = T.empty(n_items * (d + 1) + 1024 * (d + 1) + 16, dtype=T.float32,
+                           device=v.device)
+            check(lib().latec_train_attn_bwd(n_items, ptr(off), ptr(v), d,
                                              ptr(null_v.contiguous()), ptr(w_rows), ptr(w_null),
-                                             ptr(dout), ptr(ds), ptr(dv), ptr(dns), ptr(dnv),
-                                             ptr(None)))
-            return ds[: ctx.n_s], dv, (dns * ctx.scale).reshape(()), dnv, None, None
+                                             ptr(dout), ptr(ds), ptr(dv), ptr(dnull), ptr(work),
+                                             work.numel(), ptr(None)))
+            return (ds[: ctx.n_s], dv, (dnull[0] * ctx.scale).reshape(()), dnull[1:].clone(),
+                    None, None)
 
     class HeadLoss(T.autograd.Function):
         """Per-point bits of the head outputs (M, 2) at targets t, bounds eb."""
@@ -376,10 +396,16 @@ class GeometryCache:
 # ---- the model ----------------------------------------------------------------
 
 class GnnTrainer:
-    """FP32 master parameters + the training forward over a schedule."""
+    """FP32 master parameters + the training forward over a schedule.
+    ``precision``: the GEMM mode of run_stages, "fp32" or "bf16" (BF16 MFMA
+    operands, FP32 accumulation)."""
 
-    def __init__(self, d: int = 64, agg: int = 2, seed: int = 0, params: dict | None = None):
+    def __init__(self, d: int = 64, agg: int = 2, seed: int = 0, params: dict | None = None,
+                 precision: str = "fp32"):
         T = _torch()
+        if precision not in ("fp32", "bf16"):
+            raise ValueError("precision must be 'fp32' or 'bf16'")
+        self.precision = precision
         self.d, self.agg = d, agg
         P = params if params is not None else GM.init_params(d, agg, seed)
         self.params = {n: T.nn.Parameter(T.from_numpy(np.asarray(P[n], np.float32).copy())
@@ -479,6 +505,14 @@ class GnnTrainer:
         """Sum of bits over every predicted point of the batch x (B, N) in
         [0, 1] at bounds eb ((B,) or scalar), and the point count.  Returns
         (bits_sum, n_points, per-stage (mu, target) for diagnostics)."""
+        prev = _PRECISION["gemm"]
+        _PRECISION["gemm"] = self.precision
+        try:
+            return self._run_stages(x, shape, levels, eb, noise, closed_loop, mse_weight, gen)
+        finally:
+            _PRECISION["gemm"] = prev
+
+    def _run_stages(self, x, shape, levels, eb, noise, closed_loop, mse_weight, gen):
         T = _torch()
         B, N = x.shape
         K = len(shape)
@@ -544,13 +578,26 @@ def laplace_bits_reference(mu, log_b, target, eb):
     return -logp / math.log(2.0)
 
 
+# The training configuration of the benchmark's learned-predictor workload:
+# batches of 32 fields (16 x 2-D 256^2 with correlation 2..32, 16 x 4-D 16^4
+# with correlation 1..8), turbulence fraction 0.5, 3 discontinuities, stride 8
+# (levels 3), eb 1e-3, targets from the codec's own closed-loop reconstruction
+# (train_gnn.py:413-414, 446-450), BF16 GEMMs with FP32 master weights.
+C5 = {"batch2d": 16, "shape2d": (256, 256), "corr2d": (2.0, 32.0), "batch4d": 16,
+      "shape4d": (16, 16, 16, 16), "corr4d": (1.0, 8.0), "levels": 3,
+      "eb_range": (1e-3, 1e-3), "turbulence_frac": 0.5, "n_jumps": 3, "closed_loop": True}
+
+
 def train(trainer: GnnTrainer, steps: int, *, lr=2e-3, warmup=50, batch2d=8, shape2d=(128, 128),
           corr2d=(2.0, 32.0), batch4d=2, shape4d=(16, 16, 16, 16), corr4d=(1.0, 8.0),
           levels=3, eb_range=(1e-4, 1e-2), turbulence_frac=0.5, n_jumps=3, seed=0,
-          log=None, mse_weight=0.0):
+          log=None, mse_weight=0.0, closed_loop=False):
     """AdamW (weight decay 1e-3) with linear warm-up and cosine decay
-    (train_gnn.py:937-950) on synthetic 2-D and 4-D batches; teacher forcing
-    with eb drawn log-uniformly from ``eb_range`` per sample."""
+    (train_gnn.py:937-950) on synthetic 2-D and 4-D batches, eb drawn
+    log-uniformly from ``eb_range`` per sample; targets teacher-forced (the
+    truth plus U(-eb, eb) noise) or, with ``closed_loop``, each stage's
+    context the codec's own quantized reconstruction (``C5`` holds the
+    benchmark's configuration)."""
     from .synthetic import _gen, training_batch
 
     T = _torch()
@@ -574,7 +621,7 @@ def train(trainer: GnnTrainer, steps: int, *, lr=2e-3, warmup=50, batch2d=8, sha
             lo, hi = math.log(eb_range[0]), math.log(eb_range[1])
             eb = T.exp(lo + (hi - lo) * T.rand(batch, generator=gen, device=D.device()))
             bits, n, _ = trainer.run_stages(x, shape, levels, eb, noise=True, gen=gen,
-                                            mse_weight=mse_weight)
+                                            mse_weight=mse_weight, closed_loop=closed_loop)
             (bits / n).backward()
             total += float(bits.detach()) / n
             tot_pts += n
