@@ -1902,11 +1902,13 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
 #define PAIR_WAVES_PER_SIMD 1  // occupancy floor handed to the register allocator
 #endif
 // WPE: the occupancy floor in waves per SIMD (the register budget the
-// allocator works to); the "pair_waves" knob picks PAIR_WAVES_ALT for the
-// finest level's encode phases 2 and 3.
-#ifndef PAIR_WAVES_ALT
-#define PAIR_WAVES_ALT 8
-#endif
+// allocator works to); the "pair_waves" knob picks one wave per SIMD more than
+// the compiler's own choice for the finest level's encode phases 2 and 3
+// (phase 2: 6 -> 7 waves, 72 VGPRs; phase 3: 4 -> 5 waves, 96 VGPRs).
+template <int PH>
+struct PairWavesAlt {
+  static constexpr int value = PH == 3 ? 5 : 7;
+};
 template <int PH, bool ENC, bool AL, bool RND, bool BM, bool XD, int WPE = PAIR_WAVES_PER_SIMD>
 __global__ void __launch_bounds__(256, WPE) pair_kernel(PArgs A, const float *__restrict__ x,
                                                    float *__restrict__ rec,
@@ -3099,9 +3101,9 @@ static int64_t g_pair_quads = 2;
 // 2, 1 steps) until it has at least this many threads
 static int64_t g_min_threads = 1 << 19;
 // "pair_waves": 1 runs the finest level's encode phases 2 and 3 (the launches
-// that read x at its own grid) as the variant compiled for at least
-// PAIR_WAVES_ALT waves per SIMD (a smaller register budget: more waves in
-// flight, more spills); 0 the compiler's own occupancy
+// that read x at its own grid) as the variants compiled for one more wave per
+// SIMD than the compiler's own choice (a smaller register budget: more waves
+// in flight, more spills); 0 the compiler's own occupancy
 static int64_t g_pair_waves = 0;
 static int64_t g_march_peel = 0;  // histogram: distinct codes per wave counted by ballot
 
@@ -3426,7 +3428,7 @@ struct PairLaunch {
                   uint32_t *out_count) {
     if constexpr (valid && E && XD && PH > 1) {
       if (g_pair_waves) {
-        hipLaunchKernelGGL((pair_kernel<PH, E, AL, RND, BM, XD, PAIR_WAVES_ALT>), grid, dim3(256),
+        hipLaunchKernelGGL((pair_kernel<PH, E, AL, RND, BM, XD, PairWavesAlt<PH>::value>), grid, dim3(256),
                            0, stream, a, x, recon, codes, hist, out_rank, out_val, out_count);
         return;
       }
