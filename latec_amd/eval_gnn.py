@@ -65,7 +65,7 @@ def compare(ckpt, x: np.ndarray, rel_eb: float = 1e-3, numerics: str = "bf16",
             "gnn_bytes": len(s_gnn), "interp_bytes": len(s_int),
             "gnn_compress_s": t_gnn, "gnn_decompress_s": t_gnn_dec, "interp_compress_s": t_int,
             "gnn_max_err": err_gnn, "interp_max_err": err_int, "eb_abs": eb_abs,
-            "gate_stats": codec.last_stats}
+            "gate_stats": {k: v for k, v in codec.last_stats.items() if k != "recon"}}
 
 
 def main(argv=None):
