@@ -152,35 +152,6 @@ static Layout make_layout(int d, int agg) {
   return L;
 }
 
-// One two-layer row MLP (message or mix): y = W2 gelu(W1 [a | b] + add) + b2.
-// W1 is stored (H, K1) with K1 = 2D (two D-wide input halves) or D; the
-// additive terms are per-state bias rows (S0/S1: ns x H), per-row scalar
-// coefficient columns (c0/c1: H) and the bias b1 (H).
-template <typename T>
-struct MlpW {
-  const T *w1;        // (H, K1) row-major
-  const T *w2;        // (D, H)
-  const float *b1;    // (H)
-  const float *b2;    // (D)
-  const float *s0;    // (ns, H) state bias of half 0 (or nullptr)
-  const float *s1;    // (ns, H) state bias of half 1 (or nullptr)
-  const float *c0;    // (H) coefficient of row scalar 0 (or nullptr)
-  const float *c1;    // (H) coefficient of row scalar 1 (or nullptr)
-  int k1;
-};
-
-struct PoolW {  // AttnPool in f32: WkT/WvT are (D, D) transposed (in, out)
-  const float *wk1t, *bk1, *wk2, *wvt, *bv, *null_v;
-  float bk2, null_k;
-};
-
-template <typename T>
-struct KeyW {  // line-pool key MLP for the message rows: Wk1 (out, in) in T
-  const T *wk1;
-  const float *bk1, *wk2;
-  float bk2;
-};
-
 }  // namespace gnn
 }  // namespace latec
 
@@ -195,7 +166,7 @@ struct latec_gnn {
   size_t dev_bytes = 0;
   // device pointers into `dev`
   void *bi_w1 = nullptr, *bi_w2 = nullptr, *dir_w1 = nullptr, *dir_w2 = nullptr;
-  void *mx_w1 = nullptr, *mx_w2 = nullptr, *lk_w1 = nullptr;
+  void *mx_w1 = nullptr, *mx_w2 = nullptr, *key_w = nullptr, *init_w2 = nullptr;
   float *f32 = nullptr;            // f32 arrays (biases, state rows, pools, head, init)
   std::vector<int64_t> f32_off;    // named offsets into f32 (see make_device)
   int8_t *d_dirs = nullptr, *d_sidx = nullptr;
@@ -214,13 +185,11 @@ namespace gnn {
 
 // f32 array slots inside latec_gnn::f32
 enum F32Slot {
-  F_BI_B1, F_BI_B2, F_BI_S0, F_BI_S1, F_BI_C0, F_BI_C1,
-  F_DIR_B1, F_DIR_B2, F_DIR_S, F_DIR_CSIGN, F_DIR_CLOGD,
-  F_MX_B1, F_MX_B2,
-  F_LK_B1, F_LK_W2, F_LK_WVT, F_LK_BV, F_LK_NULLV, F_LK_W1T,
-  F_AP_W1T, F_AP_B1, F_AP_W2, F_AP_WVT, F_AP_BV, F_AP_NULLV,
-  F_HD_W1T, F_HD_COND, F_HD_B1, F_HD_W2, F_HD_B2,
-  F_IN_W0, F_IN_B0, F_IN_W2T, F_IN_B2,
+  F_BI_B2, F_DIR_B2, F_MX_B2,
+  F_LK_B1, F_LK_W2, F_LK_WV, F_LK_BV, F_LK_NULLV,
+  F_AP_WKV, F_AP_B1, F_AP_W2, F_AP_BV, F_AP_NULLV,
+  F_HD_W1, F_HD_COND, F_HD_B1, F_HD_W2, F_HD_B2,
+  F_IN_W0, F_IN_B0, F_IN_B2,
   F_SCAL,  // lk_bk2, lk_null_k, ap_bk2, ap_null_k
   F_NSLOTS
 };
@@ -481,78 +450,112 @@ __global__ void pair_write_kernel(const int32_t *__restrict__ steps, int64_t n_p
 }
 
 // ---------------------------------------------------------------------------
-// row MLP kernel (message / mix) on MFMA
+// MFMA tiles in the transposed form
 // ---------------------------------------------------------------------------
+//
+// A 16x16 MFMA tile computes D = A B + C.  Every layer below is evaluated
+// transposed, Y^T = W X^T: A is a 16-row block of the weight matrix (16
+// output units), B holds 16 data rows as its columns.  Lane l = 16 g + r then
+// always works on data row r: it supplies its row's inputs for the k-set of
+// lane group g and receives units 16 ob + 4 g + v (v = 0..3) of its row.  The
+// next layer's B operand is made from those registers directly, so hidden
+// activations never leave the register file; a layer fed from registers has
+// its weights stored with the input index in that register order:
+//   bf16 (v_mfma_f32_16x16x32_bf16): k-step of 32, group g supplies 8
+//        consecutive k; register position 32 s + 8 g + i holds unit
+//        16 (2 s + i / 4) + 4 g + i % 4;
+//   f32  (v_mfma_f32_16x16x4f32): k-step of 4, group g supplies one k;
+//        register position 4 t + g holds unit 16 (t / 4) + 4 g + t % 4.
+// Rows gathered from memory in f32 use the same f32 order (a lane loads four
+// consecutive values at 16 q + 4 g and feeds them to steps 4 q .. 4 q + 3);
+// bf16 rows gathered from memory are in natural order.
 
-struct RowArgs {
-  int32_t K, L, ns, pad;
-  const int32_t *n_items_dev;  // message items (pairs), read on device
-  int64_t n_rows_host;         // finalize rows (M * K)
-  const PairRec *rec;          // messages: per list entry
-  const int16_t *lw;           // messages: line (| side << 14)
-  const int8_t *sidx;          // (L, K, 2) state index per line, axis, side
-  const void *E;               // (frame rows, K, D)
-  void *out;                   // msg (slots, K, D) / E for finalize
-  float *score;                // (slots, K) for messages
-  int64_t slot_base;           // slot of list[0] (0 for bidir)
-  const int32_t *slot_base_dev;
-  // finalize
-  const void *ctx;             // (M, K, D)
-  const void *vemb;            // (M, D)
-  const int64_t *qout;         // (M) frame index of each finalized point
-  float score_scale;
-  int32_t pad2;
-};
-
-template <int D, typename T>
-struct RowCfg {
-  static constexpr int H = 2 * D;
-  static constexpr bool BF = sizeof(T) == 2;
-  static constexpr int KS = BF ? 32 : 4;            // MFMA K per instruction
-  static constexpr int W1P = 2 * D + (BF ? 8 : 4);  // padded LDS rows (elements)
-  static constexpr int W2P = H + (BF ? 8 : 4);
-  static constexpr int WKP = D + (BF ? 8 : 4);
-  static constexpr int HP = H + (BF ? 8 : 4);
-  static constexpr int MP = D + (BF ? 8 : 4);
-  static constexpr int WAVES = 8;
-  static constexpr int ROWS = 16;                   // rows per wave per iteration
-  static constexpr int THREADS = 64 * WAVES;
-};
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
-__device__ __forceinline__ void mfma_step(f32x4 &acc, const T *a, const T *b);
-
-// bf16: a, b point to 8 consecutive bf16 each
+struct Mf;
 template <>
-__device__ __forceinline__ void mfma_step<uint16_t>(f32x4 &acc, const uint16_t *a,
-                                                    const uint16_t *b) {
-  bf16x8 va, vb;
-  memcpy(&va, a, 16);
-  memcpy(&vb, b, 16);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc, 0, 0, 0);
-}
+struct Mf<uint16_t> {
+  static constexpr int KS = 32, KL = 8, NX = 1, PE = 8;  // NX: k-steps of additive terms
+  typedef bf16x8 frag;
+  typedef bf16x4 quad;
+};
 template <>
-__device__ __forceinline__ void mfma_step<float>(f32x4 &acc, const float *a, const float *b) {
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(*a, *b, acc, 0, 0, 0);
+struct Mf<float> {
+  static constexpr int KS = 4, KL = 1, NX = 4, PE = 4;
+  typedef float frag;
+  typedef f32x4 quad;
+};
+
+// register position -> unit of a hidden vector (see above)
+__host__ __device__ inline int reg_unit(bool bf, int kk) {
+  if (bf) {
+    const int s = kk / 32, g = (kk % 32) / 8, i = kk % 8;
+    return 16 * (2 * s + i / 4) + 4 * g + i % 4;
+  }
+  const int t = kk / 4, g = kk % 4;
+  return 16 * (t / 4) + 4 * g + t % 4;
 }
 
-// GELU of the bf16 path: x * Phi(x) with erf from Abramowitz & Stegun
-// 7.1.26 (|error| < 1.5e-7, far below the bf16 rounding of the result).
-__device__ __forceinline__ float gelu_bf(float x) {
-  const float z = fabsf(x) * 0.70710678118654752f;
-  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * z);
-  float y = 1.061405429f;
-  y = y * t - 1.453152027f;
-  y = y * t + 1.421413741f;
-  y = y * t - 0.284496736f;
-  y = y * t + 0.254829592f;
-  const float e = 1.0f - y * t * __expf(-z * z);  // erf(|x| / sqrt 2)
-  return 0.5f * x * (1.0f + copysignf(e, x));
+__device__ __forceinline__ void mfma(f32x4 &acc, bf16x8 a, bf16x8 b) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ void mfma(f32x4 &acc, float a, float b) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ typename Mf<T>::frag ldfrag(const T *p) {
+  if constexpr (sizeof(T) == 2)
+    return *(const bf16x8 *)p;
+  else
+    return *p;
+}
+
+// B operand of k-step s from the quads (16-unit blocks) of one row
+template <typename T, int NQ>
+__device__ __forceinline__ typename Mf<T>::frag bfrag(const typename Mf<T>::quad (&q)[NQ],
+                                                      int s) {
+  if constexpr (sizeof(T) == 2)
+    return __builtin_shufflevector(q[2 * s], q[2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7);
+  else
+    return q[s / 4][s % 4];
+}
+
+template <typename T>
+__device__ __forceinline__ typename Mf<T>::quad to_quad(f32x4 v) {
+  if constexpr (sizeof(T) == 2)
+    return __builtin_convertvector(v, bf16x4);
+  else
+    return v;
+}
+
+// GELU of the bf16 path: x Phi(x), Phi(x) = 1/2 + xc P(xc^2) with xc = x
+// clamped to [-4.5, 4.5] and P of degree 8 from a least-squares fit of the
+// normal CDF on [-4.5, 4.5] with Phi(+-4.5) pinned to 1 and 0.  Evaluated in
+// float32 the error is below 1.1e-4 absolute and 4.4e-4 relative over the
+// whole line (the result is rounded to bf16, half an ulp of which is 2e-3).
+__device__ __forceinline__ float gelu_fast(float x) {
+  const float xc = __builtin_amdgcn_fmed3f(x, -4.5f, 4.5f);
+  const float t = xc * xc;
+  float p = 3.722450820e-11f;
+  p = __builtin_fmaf(p, t, -3.938617876e-09f);
+  p = __builtin_fmaf(p, t, 1.826758336e-07f);
+  p = __builtin_fmaf(p, t, -4.930116575e-06f);
+  p = __builtin_fmaf(p, t, 8.704744687e-05f);
+  p = __builtin_fmaf(p, t, -1.076017623e-03f);
+  p = __builtin_fmaf(p, t, 9.730072692e-03f);
+  p = __builtin_fmaf(p, t, -6.624759734e-02f);
+  p = __builtin_fmaf(p, t, 3.988744617e-01f);
+  return x * __builtin_fmaf(xc, p, 0.5f);
 }
 
 template <typename T>
 __device__ __forceinline__ float gelu_t(float x) {
-  return sizeof(T) == 2 ? gelu_bf(x) : gelu(x);
+  if constexpr (sizeof(T) == 2)
+    return gelu_fast(x);
+  else
+    return gelu(x);
 }
 
 // Wave-local LDS hand-off: every lane's LDS writes are complete before any
@@ -563,391 +566,564 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Row MLP: for each row r (item i = r / K, axis k = r % K):
-//   x = [half0 | half1]  (D or 2D wide, T)
-//   h = gelu(W1 x + b1 + S0[st0] + S1[st1] + c0 * z0 + c1 * z1)   (H)
-//   y = W2 h + b2                                                  (D)
-// messages also get score = (wk2 . gelu(Wk1 y + bk1) + bk2) * scale.
-// Each wave walks its own 16-row groups (no workgroup barrier after the
-// weights are staged) and prefetches the next group's pair records.
-// MFMA fragment layouts (16x16 tiles, lane l): A row l%16, k = KS/4*(l/16)..;
-// B col l%16, same k; D rows 4*(l/16)..+3, col l%16.
+// block -> logical block so that each XCD (blocks are dispatched round-robin
+// over 8 of them) works on one contiguous range
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// message / mix rows on MFMA
+// ---------------------------------------------------------------------------
+
+struct RowArgs {
+  int32_t K, L;
+  const int32_t *n_items_dev;  // message items (pairs), read on device
+  int64_t n_rows_host;         // finalize rows (M * K)
+  const PairRec *rec;          // messages: per list entry
+  const int16_t *lw;           // messages: line (| side << 14)
+  const int8_t *sidx;          // (L, K, 2) state index per line, axis, side
+  const void *E;               // (frame rows, K, D)
+  void *out;                   // msg (slots, K, D) / E for finalize
+  float *score;                // (slots, K) for messages
+  int64_t slot_base;           // slot of list[0] (0 for bidir)
+  const int32_t *slot_base_dev;
+  const void *ctx;             // finalize: (M, K, D)
+  const void *vemb;            // finalize: (M, D)
+  const int64_t *qout;         // finalize: frame index of each finalized point
+  float score_scale;
+};
+
+// Weights of one row MLP, as staged: W1 (H, K1X) with the first-layer input
+// in register order and NX k-steps of additive terms appended; W2 (D, H) and
+// the key's Wk (D, D) with their inputs in register order.
+template <typename T>
+struct RowW {
+  const T *w1, *w2, *wk;
+  const float *b2, *bk1, *wk2;
+  float bk2;
+};
+
 template <int D, typename T, int MODE>
-__global__ __launch_bounds__(512) void row_mlp_kernel(RowArgs a, MlpW<T> w, KeyW<T> pk) {
-  using C = RowCfg<D, T>;
-  constexpr int H = C::H, KS = C::KS, KL = KS / 4;
-  constexpr int K1 = MODE == ROW_DIR ? D : 2 * D;
-  constexpr int NW = C::BF ? H * C::W1P + D * C::W2P + D * C::WKP : 0;
+struct RowShape {
+  typedef Mf<T> F;
+  static constexpr int H = 2 * D;
+  static constexpr int K1 = MODE == ROW_DIR ? D : 2 * D;
+  static constexpr int K1X = K1 + F::NX * F::KS;
+  static constexpr int P1 = K1X + F::PE, P2 = H + F::PE, PK = D + F::PE;
+  static constexpr bool KEY = MODE != ROW_FIN;
+  static size_t smem(int L, int K) {
+    return sizeof(T) * ((size_t)H * P1 + (size_t)D * P2 + (KEY ? (size_t)D * PK : 0)) +
+           sizeof(float) * 3 * D + (size_t)((L * K * 2 + 15) & ~15);
+  }
+};
+
+// Row r of the layer-1 input: its two source rows (or one), the two per-row
+// scalars and the state index of the additive terms, and where the output
+// row goes.
+template <typename T, int NK1>
+struct RowIn {
+  typename Mf<T>::frag x[NK1];
+  float z0, z1;
+  int st;
+  int64_t dst;  // output row (element offset / D); -1 if the row is past the end
+};
+
+// Additive-term k-step(s) of layer 1 (the B side; W1's matching columns hold
+// b1, the scalar coefficients and the state bias rows):
+//   bf16, one step of 32: group 0 [1, 1, z0h, z0h, z0l, z1h, z1h, z1l] against
+//     [b1h, b1l, c0h, c0l, c0h, c1h, c1l, c1h]; groups 1 and 2 one-hot(st) over
+//     8 states against the state rows' high and low bf16 parts; group 3 zero
+//     (z = zh + zl and every weight = hi + lo split, so the terms keep ~16 bits);
+//   f32, four steps of 4 (position e = 4 t + g): [1, z0, z1, one-hot(st) x 13].
+template <typename T>
+__device__ __forceinline__ typename Mf<T>::frag extra_frag(float z0, float z1, int st, int g,
+                                                           int t) {
+  if constexpr (sizeof(T) == 2) {
+    const __bf16 z0h = (__bf16)z0, z1h = (__bf16)z1;
+    const __bf16 z0l = (__bf16)(z0 - (float)z0h), z1l = (__bf16)(z1 - (float)z1h);
+    const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
+    bf16x8 v0 = {one, one, z0h, z0h, z0l, z1h, z1h, z1l};
+    bf16x8 v1;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v1[i] = st == i ? one : zero;
+    bf16x8 vz = {zero, zero, zero, zero, zero, zero, zero, zero};
+    return g == 0 ? v0 : (g < 3 ? v1 : vz);
+  } else {
+    const int e = 4 * t + g;
+    return e == 0 ? 1.0f : e == 1 ? z0 : e == 2 ? z1 : (e - 3 == st ? 1.0f : 0.0f);
+  }
+}
+
+// Row MLP: for each row r (item i = r / K, axis k = r % K)
+//   h = gelu(W1 [x0 | x1] + b1 + S[st] + c0 z0 + c1 z1)   (H)
+//   y = W2 h + b2                                         (D)
+// messages also get score = (wk2 . gelu(Wk y + bk1) + bk2) * scale.
+// Each wave walks groups of NT 16-row tiles; the next group's inputs are
+// gathered while this group's second layer and key run.
+template <int D, typename T, int MODE, int NT>
+__global__ __launch_bounds__(256) void msg_kernel(RowArgs a, RowW<T> w) {
+  typedef RowShape<D, T, MODE> S;
+  typedef Mf<T> F;
+  typedef typename F::frag frag;
+  typedef typename F::quad quad;
+  constexpr bool BF = sizeof(T) == 2;
+  constexpr int KS = F::KS, KL = F::KL, H = S::H, K1 = S::K1, K1X = S::K1X;
+  constexpr int NK1 = K1 / KS, NKX = F::NX, NK2 = H / KS, NK3 = D / KS;
+  constexpr int P1 = S::P1, P2 = S::P2, PK = S::PK;
+  constexpr int NB1 = H / 16, NB2 = D / 16;
+  constexpr bool KEY = S::KEY;
+  constexpr int ROWS = 16 * NT;
   extern __shared__ __align__(16) unsigned char smem[];
   T *sW1 = (T *)smem;
-  T *sW2 = sW1 + H * C::W1P;
-  T *sWk = sW2 + D * C::W2P;
-  float *tb = (float *)(smem + sizeof(T) * (size_t)NW);
-  float *tb1 = tb, *tc0 = tb1 + H, *tc1 = tc0 + H, *tb2 = tc1 + H, *tbk1 = tb2 + D;
-  float *twk2 = tbk1 + D, *ts0 = twk2 + D, *ts1 = ts0 + a.ns * H;
-  int8_t *tsx = (int8_t *)(ts1 + a.ns * H);                      // L*K*2, padded to 1024
-  unsigned char *pw = (unsigned char *)tsx + 1024;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  constexpr size_t PER_WAVE = sizeof(T) * (C::ROWS * C::HP + C::ROWS * C::MP) + 16 * 4 * 4;
-  T *sH = (T *)(pw + wv * PER_WAVE);
-  T *sM = sH + C::ROWS * C::HP;
-  float *add = (float *)(sM + C::ROWS * C::MP);
-  if constexpr (C::BF) {
-    for (int i = tid; i < H * K1; i += C::THREADS) sW1[(i / K1) * C::W1P + i % K1] = w.w1[i];
-    for (int i = tid; i < D * H; i += C::THREADS) sW2[(i / H) * C::W2P + i % H] = w.w2[i];
-    if constexpr (MODE != ROW_FIN)
-      for (int i = tid; i < D * D; i += C::THREADS) sWk[(i / D) * C::WKP + i % D] = pk.wk1[i];
-  }
-  for (int i = tid; i < H; i += C::THREADS) {
-    tb1[i] = w.b1[i];
-    tc0[i] = w.c0 ? w.c0[i] : 0.f;
-    tc1[i] = w.c1 ? w.c1[i] : 0.f;
-  }
-  for (int i = tid; i < D; i += C::THREADS) {
+  T *sW2 = sW1 + H * P1;
+  T *sWk = sW2 + D * P2;
+  float *tb2 = (float *)(sWk + (KEY ? D * PK : 0));
+  float *tbk1 = tb2 + D, *twk2 = tbk1 + D;
+  int8_t *tsx = (int8_t *)(twk2 + D);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int i = tid; i < H * K1X; i += nthr) sW1[(i / K1X) * P1 + i % K1X] = w.w1[i];
+  for (int i = tid; i < D * H; i += nthr) sW2[(i / H) * P2 + i % H] = w.w2[i];
+  if constexpr (KEY)
+    for (int i = tid; i < D * D; i += nthr) sWk[(i / D) * PK + i % D] = w.wk[i];
+  for (int i = tid; i < D; i += nthr) {
     tb2[i] = w.b2[i];
-    if constexpr (MODE != ROW_FIN) {
-      tbk1[i] = pk.bk1[i];
-      twk2[i] = pk.wk2[i];
+    if constexpr (KEY) {
+      tbk1[i] = w.bk1[i];
+      twk2[i] = w.wk2[i];
     }
   }
-  for (int i = tid; i < a.ns * H; i += C::THREADS) {
-    ts0[i] = w.s0 ? w.s0[i] : 0.f;
-    ts1[i] = w.s1 ? w.s1[i] : 0.f;
-  }
-  if constexpr (MODE != ROW_FIN)
-    for (int i = tid; i < a.L * a.K * 2; i += C::THREADS) tsx[i] = a.sidx[i];
+  if constexpr (KEY)
+    for (int i = tid; i < a.L * a.K * 2; i += nthr) tsx[i] = a.sidx[i];
   __syncthreads();
 
-  const T *gW1 = C::BF ? sW1 : w.w1;
-  const T *gW2 = C::BF ? sW2 : w.w2;
-  const T *gWk = C::BF ? sWk : pk.wk1;
-  constexpr int P1 = C::BF ? C::W1P : K1, P2 = C::BF ? C::W2P : H, PK = C::BF ? C::WKP : D;
-  const int64_t n_rows = MODE == ROW_FIN ? a.n_rows_host : (int64_t)a.n_items_dev[0] * a.K;
+  const int lane = tid & 63, wv = tid >> 6, r = lane & 15, g = lane >> 4;
+  const int waves = nthr >> 6;
+  const int K = a.K;
+  const int64_t n_rows = MODE == ROW_FIN ? a.n_rows_host : (int64_t)a.n_items_dev[0] * K;
   const int64_t slot_base = a.slot_base_dev ? (int64_t)a.slot_base_dev[0] : a.slot_base;
   const T *E = (const T *)a.E;
-  const int64_t gstep = (int64_t)gridDim.x * C::WAVES;
-  const int ri = lane & 15, kq = (lane >> 4) * KL;
-  constexpr int NK1 = K1 / KS, NK2 = H / KS, NK3 = D / KS;
+  const T *CTX = (const T *)a.ctx;
+  const T *VEMB = (const T *)a.vemb;
+  const int64_t gstep = (int64_t)gridDim.x * waves;
+  int64_t grp = (int64_t)xcd_block() * waves + wv;
 
-  int64_t grp = (int64_t)blockIdx.x * C::WAVES + wv;
-  // metadata of this lane's row in group g
-  PairRec rc{0, 0, 0.f, 0.f};
-  int16_t lc = 0;
-  auto fetch = [&](int64_t g, PairRec &r, int16_t &lw) {
-    const int64_t row = g * C::ROWS + ri;
-    if constexpr (MODE != ROW_FIN) {
+  // gather the layer-1 inputs of group gi
+  auto load = [&](int64_t gi, RowIn<T, NK1> (&in)[NT]) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int64_t row = gi * ROWS + 16 * t + r;
+      const T *s0, *s1;
+      float z0 = 0.f, z1 = 0.f;
+      int st = 0;
+      int64_t dst = -1;
       if (row < n_rows) {
-        const int64_t item = row / a.K;
-        r = a.rec[item];
-        lw = a.lw[item];
+        const int32_t rw = (int32_t)row;
+        const int32_t item = rw / K, k = rw - item * K;
+        if constexpr (MODE == ROW_FIN) {
+          s0 = CTX + (int64_t)rw * D;
+          s1 = VEMB + (int64_t)item * D;
+          dst = a.qout[item] * K + k;
+        } else {
+          const PairRec rc = a.rec[item];
+          const int16_t lw = a.lw[item];
+          const int l = lw & 0x3fff;
+          s0 = E + ((int64_t)rc.ra * K + k) * D;
+          s1 = MODE == ROW_BI ? E + ((int64_t)rc.rb * K + k) * D : s0;
+          st = tsx[(l * K + k) * 2 + (MODE == ROW_DIR ? (lw >> 14) & 1 : 0)];
+          z0 = rc.za;
+          z1 = rc.zb;
+          dst = slot_base * K + row;
+        }
+      } else {
+        s0 = s1 = MODE == ROW_FIN ? CTX : E;  // row 0: a valid address, result unused
       }
+      if constexpr (BF) {
+#pragma unroll
+        for (int s = 0; s < NK1; s++) {
+          const bool h1 = MODE != ROW_DIR && s * KS >= D;
+          const T *src = h1 ? s1 : s0;
+          in[t].x[s] = *(const bf16x8 *)(src + (s * KS - (h1 ? D : 0)) + 8 * g);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < K1 / 16; q++) {
+          const bool h1 = MODE != ROW_DIR && q * 16 >= D;
+          const T *src = h1 ? s1 : s0;
+          const f32x4 v = *(const f32x4 *)(src + (q * 16 - (h1 ? D : 0)) + 4 * g);
+#pragma unroll
+          for (int e = 0; e < 4; e++) in[t].x[4 * q + e] = v[e];
+        }
+      }
+      in[t].z0 = z0;
+      in[t].z1 = z1;
+      in[t].st = st;
+      in[t].dst = dst;
     }
   };
-  if (grp * C::ROWS < n_rows) fetch(grp, rc, lc);
-  for (; grp * C::ROWS < n_rows; grp += gstep) {
-    // prefetch the next group's records
-    PairRec rn{0, 0, 0.f, 0.f};
-    int16_t ln = 0;
-    if ((grp + gstep) * C::ROWS < n_rows) fetch(grp + gstep, rn, ln);
-    const int64_t r0 = grp * C::ROWS;
-    const int64_t r = r0 + ri;
-    const bool live = r < n_rows;
-    const int64_t item = live ? r / a.K : 0;
-    const int k = live ? (int)(r % a.K) : 0;
-    const T *src0 = nullptr, *src1 = nullptr;
-    float z0 = 0.f, z1 = 0.f;
-    int st0 = 0, st1 = 0;
-    if (live) {
-      if constexpr (MODE == ROW_FIN) {
-        src0 = (const T *)a.ctx + (item * a.K + k) * D;
-        src1 = (const T *)a.vemb + item * D;
-      } else if constexpr (MODE == ROW_BI) {
-        src0 = E + ((int64_t)rc.ra * a.K + k) * D;  // negative side first
-        src1 = E + ((int64_t)rc.rb * a.K + k) * D;
-        const int l = lc & 0x3fff;
-        st0 = tsx[(l * a.K + k) * 2 + 0];
-        st1 = tsx[(l * a.K + k) * 2 + 1];
-        z0 = rc.za;
-        z1 = rc.zb;
-      } else {
-        src0 = E + ((int64_t)rc.ra * a.K + k) * D;
-        const int l = lc & 0x3fff, side = (lc >> 14) & 1;
-        st0 = tsx[(l * a.K + k) * 2 + side];
-        z0 = rc.za;
-        z1 = rc.zb;
-      }
-    }
-    if (lane < 16) {
-      add[ri * 4 + 0] = z0;
-      add[ri * 4 + 1] = z1;
-      add[ri * 4 + 2] = __int_as_float(st0);
-      add[ri * 4 + 3] = __int_as_float(st1);
-    }
-    T afr[NK1][KL];
+
+  RowIn<T, NK1> in[NT];
+  if (grp * ROWS < n_rows) load(grp, in);
+  for (; grp * ROWS < n_rows; grp += gstep) {
+    // --- layer 1 (+ additive terms), GELU, kept as the next layer's B operand
+    frag xe[NT][NKX];
+    int64_t dst[NT];
 #pragma unroll
-    for (int ks = 0; ks < NK1; ks++) {
-      const int kk = ks * KS + kq;
-      const T *sp_ = (MODE == ROW_DIR || kk < D) ? src0 : src1;
-      const int ko = (MODE == ROW_DIR || kk < D) ? kk : kk - D;
-      if (live) {
-        if constexpr (C::BF) {
-          memcpy(&afr[ks][0], sp_ + ko, 16);
-        } else {
+    for (int t = 0; t < NT; t++) {
+      dst[t] = in[t].dst;
 #pragma unroll
-          for (int e = 0; e < KL; e++) afr[ks][e] = sp_[ko + e];
-        }
-      } else {
-#pragma unroll
-        for (int e = 0; e < KL; e++) afr[ks][e] = T(0);
-      }
+      for (int s = 0; s < NKX; s++) xe[t][s] = extra_frag<T>(in[t].z0, in[t].z1, in[t].st, g, s);
     }
-    wave_lds_sync();
-    // --- layer 1
+    // layer-1 blocks in groups that form one layer-2 k-step group each (bf16:
+    // two blocks = one step of 32; f32: one block = four steps of 4), so the
+    // hidden vector is consumed as it is made
+    constexpr int GB = BF ? 2 : 1, SPG = BF ? 1 : 4;
+    f32x4 acc2[NT][NB2];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int ob = 0; ob < NB2; ob++) acc2[t][ob] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
-    for (int cb = 0; cb < H / 16; cb++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      const T *bw = gW1 + (cb * 16 + (lane & 15)) * P1 + kq;
+    for (int j = 0; j < NB1 / GB; j++) {
+      quad hq[NT][GB];
 #pragma unroll
-      for (int ks = 0; ks < NK1; ks++) mfma_step<T>(acc, afr[ks], bw + ks * KS);
-      const int col = cb * 16 + (lane & 15);
-      const float b1 = tb1[col], c0 = tc0[col], c1 = tc1[col];
+      for (int b = 0; b < GB; b++) {
+        f32x4 acc[NT];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int row = (lane >> 4) * 4 + j;
-        const float *ad = add + row * 4;
-        float v = acc[j] + b1;
-        if (MODE != ROW_FIN) v += ts0[__float_as_int(ad[2]) * H + col];
-        if (MODE == ROW_BI) v += ts1[__float_as_int(ad[3]) * H + col];
-        v += c0 * ad[0];
-        v += c1 * ad[1];
-        sH[row * C::HP + col] = st<T>(gelu_t<T>(v));
-      }
-    }
-    wave_lds_sync();
-    // --- layer 2
-    T hfr[NK2][KL];
-    const T *hsrc = sH + ri * C::HP + kq;
+        for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const T *wr = sW1 + (16 * (j * GB + b) + r) * P1 + g * KL;
 #pragma unroll
-    for (int ks = 0; ks < NK2; ks++)
+        for (int s = 0; s < NK1 + NKX; s++) {
+          const frag af = ldfrag<T>(wr + s * KS);
 #pragma unroll
-      for (int e = 0; e < KL; e++) hfr[ks][e] = hsrc[ks * KS + e];
-#pragma unroll 1
-    for (int cb = 0; cb < D / 16; cb++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      const T *bw = gW2 + (cb * 16 + (lane & 15)) * P2 + kq;
-#pragma unroll
-      for (int ks = 0; ks < NK2; ks++) mfma_step<T>(acc, hfr[ks], bw + ks * KS);
-      const int col = cb * 16 + (lane & 15);
-      const float b2 = tb2[col];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int row = (lane >> 4) * 4 + j;
-        sM[row * C::MP + col] = st<T>(acc[j] + b2);
-      }
-    }
-    wave_lds_sync();
-    // --- coalesced store of the 16 output rows
-    {
-      T *out = (T *)a.out;
-      for (int e = lane; e < C::ROWS * D; e += 64) {
-        const int row = e / D, col = e % D;
-        const int64_t rr = r0 + row;
-        if (rr >= n_rows) continue;
-        int64_t dst;
-        if constexpr (MODE == ROW_FIN) {
-          const int64_t it = rr / a.K;
-          dst = (a.qout[it] * a.K + rr % a.K) * D + col;
-        } else {
-          dst = (slot_base * a.K + rr) * D + col;
+          for (int t = 0; t < NT; t++) mfma(acc[t], af, s < NK1 ? in[t].x[s] : xe[t][s - NK1]);
         }
-        out[dst] = sM[row * C::MP + col];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          f32x4 hv;
+#pragma unroll
+          for (int v = 0; v < 4; v++) hv[v] = gelu_t<T>(acc[t][v]);
+          hq[t][b] = to_quad<T>(hv);
+        }
+      }
+#pragma unroll
+      for (int ob = 0; ob < NB2; ob++) {
+        const T *wr = sW2 + (16 * ob + r) * P2 + g * KL + j * SPG * KS;
+#pragma unroll
+        for (int ss = 0; ss < SPG; ss++) {
+          const frag af = ldfrag<T>(wr + ss * KS);
+#pragma unroll
+          for (int t = 0; t < NT; t++) mfma(acc2[t][ob], af, bfrag<T, GB>(hq[t], ss));
+        }
+      }
+    }
+    // the next group's gathers fly during the epilogue and the key
+    if ((grp + gstep) * ROWS < n_rows) load(grp + gstep, in);
+    // --- layer 2 epilogue (+ b2): the output rows
+    quad mq[NT][NB2];
+    T *out = (T *)a.out;
+#pragma unroll
+    for (int ob = 0; ob < NB2; ob++) {
+      const f32x4 b2 = *(const f32x4 *)(tb2 + 16 * ob + 4 * g);
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        mq[t][ob] = to_quad<T>(acc2[t][ob] + b2);
+        if (dst[t] >= 0) *(quad *)(out + dst[t] * D + 16 * ob + 4 * g) = mq[t][ob];
       }
     }
     // --- attention key of each message row
-    if constexpr (MODE != ROW_FIN) {
-      T mfr[NK3][KL];
-      const T *msrc = sM + ri * C::MP + kq;
+    if constexpr (KEY) {
+      float part[NT];
 #pragma unroll
-      for (int ks = 0; ks < NK3; ks++)
-#pragma unroll
-        for (int e = 0; e < KL; e++) mfr[ks][e] = msrc[ks * KS + e];
-      float part[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < NT; t++) part[t] = 0.f;
 #pragma unroll 1
-      for (int cb = 0; cb < D / 16; cb++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        const T *bw = gWk + (cb * 16 + (lane & 15)) * PK + kq;
+      for (int ob = 0; ob < NB2; ob++) {
+        f32x4 acc[NT];
 #pragma unroll
-        for (int ks = 0; ks < NK3; ks++) mfma_step<T>(acc, mfr[ks], bw + ks * KS);
-        const int col = cb * 16 + (lane & 15);
-        const float bk1 = tbk1[col], wk2 = twk2[col];
+        for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const T *wr = sWk + (16 * ob + r) * PK + g * KL;
 #pragma unroll
-        for (int j = 0; j < 4; j++) part[j] += wk2 * gelu_t<T>(acc[j] + bk1);
+        for (int s = 0; s < NK3; s++) {
+          const frag af = ldfrag<T>(wr + s * KS);
+#pragma unroll
+          for (int t = 0; t < NT; t++) mfma(acc[t], af, bfrag<T, NB2>(mq[t], s));
+        }
+        const f32x4 bk = *(const f32x4 *)(tbk1 + 16 * ob + 4 * g);
+        const f32x4 wk = *(const f32x4 *)(twk2 + 16 * ob + 4 * g);
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) part[t] = __builtin_fmaf(wk[v], gelu_t<T>(acc[t][v] + bk[v]), part[t]);
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        float v = part[j];
-        v += __shfl_xor(v, 1);
-        v += __shfl_xor(v, 2);
-        v += __shfl_xor(v, 4);
-        v += __shfl_xor(v, 8);
-        part[j] = v;
+      for (int t = 0; t < NT; t++) {
+        float v = part[t];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (g == 0 && dst[t] >= 0) a.score[dst[t]] = (v + w.bk2) * a.score_scale;
       }
-      if ((lane & 15) == 0) {
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// line pool: softmax over each (point, axis) item's live lines + null token,
+// value projection on an f32 MFMA tile
+// ---------------------------------------------------------------------------
+
+struct PoolArgs {
+  const float *wv;      // (D, D) value projection, natural order
+  const float *bv, *null_v;
+  float null_score;     // null_k * scale
+};
+
+constexpr int POOL_WAVES = 4;
+
+template <int D>
+static size_t pool_smem() {
+  return sizeof(float) * ((size_t)D * (D + 4) + 2 * (size_t)D +
+                          POOL_WAVES * (16 * (size_t)(D + 4) + 16));
+}
+
+// 16 items per wave: 4 lanes per item for the softmax-weighted sum of its
+// messages (D / 4 columns each), then ctx^T = Wv P^T for the 16 items on
+// v_mfma_f32_16x16x4f32 (P through LDS), + (1 - w_null) bv + w_null null_v.
+template <int D, typename T>
+__global__ __launch_bounds__(64 * POOL_WAVES) void pool_kernel(
+    int64_t n_items, int K, int L, const int32_t *__restrict__ slot, const T *__restrict__ msg,
+    const float *__restrict__ score, PoolArgs pa, T *__restrict__ ctx, int64_t item_base) {
+  constexpr int PW = D + 4, DC = D / 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *sWv = (float *)smem;
+  float *sbv = sWv + D * PW, *snv = sbv + D;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  float *sP = snv + D + wv * (16 * PW + 16);
+  float *sWn = sP + 16 * PW;
+  for (int i = tid; i < D * D; i += 64 * POOL_WAVES) sWv[(i / D) * PW + i % D] = pa.wv[i];
+  for (int i = tid; i < D; i += 64 * POOL_WAVES) {
+    sbv[i] = pa.bv[i];
+    snv[i] = pa.null_v[i];
+  }
+  __syncthreads();
+  const int ii = lane >> 2, c4 = lane & 3;  // softmax phase: item, column quarter
+  const int r = lane & 15, g = lane >> 4;   // MFMA phase: item, k group
+  const int64_t gstep = (int64_t)gridDim.x * POOL_WAVES * 16;
+  for (int64_t it0 = ((int64_t)xcd_block() * POOL_WAVES + wv) * 16; it0 < n_items; it0 += gstep) {
+    {
+      int64_t it = it0 + ii;
+      if (it >= n_items) it = n_items - 1;
+      const int64_t m = it / K;
+      const int k = (int)(it - m * K);
+      const int32_t *sl = slot + m * L;
+      float mx = pa.null_score;
+      for (int l = 0; l < L; l++) {
+        const int32_t s = sl[l];
+        if (s >= 0) mx = fmaxf(mx, score[(int64_t)s * K + k]);
+      }
+      float den = 0.f, acc[DC];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int64_t rr = r0 + (lane >> 4) * 4 + j;
-          if (rr < n_rows) a.score[slot_base * a.K + rr] = (part[j] + pk.bk2) * a.score_scale;
+      for (int j = 0; j < DC; j++) acc[j] = 0.f;
+      for (int l = 0; l < L; l++) {
+        const int32_t s = sl[l];
+        if (s < 0) continue;
+        const float e = __expf(score[(int64_t)s * K + k] - mx);
+        den += e;
+        const T *mp = msg + ((int64_t)s * K + k) * D + c4 * DC;
+        if constexpr (sizeof(T) == 2) {
+#pragma unroll
+          for (int j = 0; j < DC; j += 8) {
+            const bf16x8 v = *(const bf16x8 *)(mp + j);
+#pragma unroll
+            for (int e2 = 0; e2 < 8; e2++) acc[j + e2] = __builtin_fmaf(e, (float)v[e2], acc[j + e2]);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < DC; j += 4) {
+            const f32x4 v = *(const f32x4 *)(mp + j);
+#pragma unroll
+            for (int e2 = 0; e2 < 4; e2++) acc[j + e2] = __builtin_fmaf(e, v[e2], acc[j + e2]);
+          }
         }
       }
+      const float en = __expf(pa.null_score - mx);
+      den += en;
+      const float inv = 1.0f / den;
+#pragma unroll
+      for (int j = 0; j < DC; j++) sP[ii * PW + c4 * DC + j] = acc[j] * inv;
+      if (c4 == 0) sWn[ii] = en * inv;
     }
     wave_lds_sync();
-    rc = rn;
-    lc = ln;
-  }
-}
-
-template <int D, typename T>
-static size_t row_mlp_smem(int ns) {
-  using C = RowCfg<D, T>;
-  const size_t nw = C::BF ? (size_t)C::H * C::W1P + (size_t)D * C::W2P + (size_t)D * C::WKP : 0;
-  const size_t tables = sizeof(float) * (3 * (size_t)C::H + 3 * (size_t)D + 2 * (size_t)ns * C::H);
-  const size_t per_wave = sizeof(T) * (C::ROWS * C::HP + C::ROWS * C::MP) + 16 * 4 * 4;
-  return sizeof(T) * nw + tables + 1024 + C::WAVES * per_wave;
-}
-
-// ---------------------------------------------------------------------------
-// line pool: per (query, axis) softmax over live lines + null, value projection
-// ---------------------------------------------------------------------------
-
-// One wave per (m, k) item; lane f < D holds feature f.
-template <int D, typename T>
-__global__ __launch_bounds__(256) void line_pool_kernel(
-    int64_t n_items, int K, int L, const int32_t *__restrict__ slot,
-    const T *__restrict__ msg, const float *__restrict__ score, PoolW pw, float scale,
-    T *__restrict__ ctx, int64_t item_base) {
-  __shared__ float sWvT[D * D];
-  __shared__ float sP[4][64];
-  for (int i = threadIdx.x; i < D * D; i += 256) sWvT[i] = pw.wvt[i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t per = (int64_t)gridDim.x * 4;
-  for (int64_t b0 = (int64_t)blockIdx.x * 4; b0 < n_items; b0 += per) {
-    const int64_t it0 = b0 + wv;
-    const bool act = it0 < n_items;
-    const int64_t it = act ? it0 : n_items - 1;
-    const int64_t m = it / K;
-    const int k = (int)(it % K);
-    const int32_t *sl = slot + m * L;
-    const float sn = pw.null_k * scale;
-    float mx = sn;
-    for (int l = 0; l < L; l++) {
-      const int32_t s = sl[l];
-      if (s >= 0) mx = fmaxf(mx, score[(int64_t)s * K + k]);
+    const float wn = sWn[r];
+    const int64_t item = it0 + r;
+#pragma unroll
+    for (int ob = 0; ob < D / 16; ob++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      const float *wr = sWv + (16 * ob + r) * PW + g;
+      const float *pr = sP + r * PW + g;
+#pragma unroll
+      for (int t = 0; t < D / 4; t++) mfma(acc, wr[4 * t], pr[4 * t]);
+      const int o = 16 * ob + 4 * g;
+      f32x4 y;
+#pragma unroll
+      for (int v = 0; v < 4; v++) y[v] = acc[v] + (1.0f - wn) * sbv[o + v] + wn * snv[o + v];
+      if (item < n_items)
+        *(typename Mf<T>::quad *)(ctx + (item_base * K + item) * D + o) = to_quad<T>(y);
     }
-    float den = 0.f, acc = 0.f;
-    for (int l = 0; l < L; l++) {
-      const int32_t s = sl[l];
-      if (s < 0) continue;
-      const float e = expf(score[(int64_t)s * K + k] - mx);
-      den += e;
-      if (lane < D) acc += e * ld<T>(msg + ((int64_t)s * K + k) * D + lane);
-    }
-    const float en = expf(sn - mx);
-    den += en;
-    const float inv = 1.0f / den;
-    const float wnull = en * inv;
-    if (lane < D) sP[wv][lane] = acc * inv;
-    __syncthreads();
-    if (act && lane < D) {
-      float o = 0.f;
-      for (int j = 0; j < D; j++) o += sWvT[j * D + lane] * sP[wv][j];
-      o += (1.0f - wnull) * pw.bv[lane];
-      o += wnull * pw.null_v[lane];
-      ctx[((item_base + m) * K + k) * D + lane] = st<T>(o);
-    }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
 // ---------------------------------------------------------------------------
-// head: axis pool + PredHead, one wave per point (FP32)
+// head: axis pool + PredHead on f32 MFMA tiles, 16 points per wave
 // ---------------------------------------------------------------------------
 
 struct HeadArgs {
   int64_t M;
   int K;
   float scale, log_eb, vmin, span, vmax;
-  const float *w1t;   // (D, H) head.net.0 weight transposed (in, out), cond column apart
-  const float *cond;  // (H) head.net.0 weight of the log-eb column
-  const float *b1;    // (H)
-  const float *w2;    // (2, H)
-  const float *b2;    // (2)
+  const float *wkv;     // (2D, D): axis-pool key layer 1 over value projection, input in f32 order
+  const float *bk1, *wk2, *bv, *null_v;
+  float bk2, null_k;
+  const float *w1;      // (H, D) head.net.0 weight without its log-eb column
+  const float *cond;    // (H) head.net.0 weight of the log-eb column
+  const float *b1;      // (H)
+  const float *w2;      // (2, H)
+  const float *b2;      // (2)
 };
 
+constexpr int HEAD_WAVES = 8;
+
+template <int D>
+static size_t head_smem() {
+  constexpr int H = 2 * D, PW = D + 4;
+  return sizeof(float) * ((size_t)2 * D * PW + (size_t)H * PW + 4 * (size_t)D + 3 * (size_t)H +
+                          HEAD_WAVES * (2 * 16 * (size_t)PW + 32));
+}
+
+// Phase A per 16-row tile of (point, axis) rows (16 / K points): keys and
+// values of the axis pool in one (2D x D) GEMM, the softmax over the point's
+// axes + null through LDS, the pooled context of each point into LDS.
+// Phase B over the 16 pooled points: z = gelu(W1 p + b1 + cond log_eb) and
+// (mu, log_b) = W2 z + b2.
 template <int D, typename T>
-__global__ __launch_bounds__(256) void head_kernel(const T *__restrict__ ctx, PoolW pw,
-                                                   HeadArgs h, float *__restrict__ pred,
-                                                   float *__restrict__ scl,
-                                                   float *__restrict__ mu_out,
-                                                   float *__restrict__ lb_out) {
-  constexpr int H = 2 * D;
-  __shared__ float sC[4][LATEC_MAX_DIM][D];
-  __shared__ float sP[4][D];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t per = (int64_t)gridDim.x * 4;
-  for (int64_t m0 = (int64_t)blockIdx.x * 4; m0 < h.M; m0 += per) {
-    const int64_t m = m0 + wv;
-    const bool act = m < h.M;
-    for (int k = 0; k < h.K; k++)
-      if (lane < D) sC[wv][k][lane] = act ? ld<T>(ctx + (m * h.K + k) * D + lane) : 0.f;
-    __syncthreads();
-    float s[LATEC_MAX_DIM], v[LATEC_MAX_DIM];
-    for (int k = 0; k < h.K; k++) {
-      float hk = 0.f, vk = 0.f;
-      if (lane < D) {
-        for (int i = 0; i < D; i++) {
-          const float c = sC[wv][k][i];
-          hk += pw.wk1t[i * D + lane] * c;
-          vk += pw.wvt[i * D + lane] * c;
+__global__ __launch_bounds__(64 * HEAD_WAVES) void head2_kernel(const T *__restrict__ ctx,
+                                                                HeadArgs h,
+                                                                float *__restrict__ pred,
+                                                                float *__restrict__ scl,
+                                                                float *__restrict__ mu_out,
+                                                                float *__restrict__ lb_out) {
+  constexpr int H = 2 * D, PW = D + 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *sWkv = (float *)smem;
+  float *sW1 = sWkv + 2 * D * PW;
+  float *sbk1 = sW1 + H * PW, *swk2 = sbk1 + D, *sbv = swk2 + D, *snv = sbv + D;
+  float *sb1 = snv + D, *sw20 = sb1 + H, *sw21 = sw20 + H;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, r = lane & 15, g = lane >> 4;
+  float *sV = sw21 + H + wv * (2 * 16 * PW + 32);
+  float *sPool = sV + 16 * PW;
+  float *sS = sPool + 16 * PW;  // 16 scores, 16 null weights
+  for (int i = tid; i < 2 * D * D; i += 64 * HEAD_WAVES) sWkv[(i / D) * PW + i % D] = h.wkv[i];
+  for (int i = tid; i < H * D; i += 64 * HEAD_WAVES) sW1[(i / D) * PW + i % D] = h.w1[i];
+  for (int i = tid; i < D; i += 64 * HEAD_WAVES) {
+    sbk1[i] = h.bk1[i];
+    swk2[i] = h.wk2[i];
+    sbv[i] = h.bv[i];
+    snv[i] = h.null_v[i];
+  }
+  for (int i = tid; i < H; i += 64 * HEAD_WAVES) {
+    sb1[i] = h.b1[i] + h.cond[i] * h.log_eb;
+    sw20[i] = h.w2[i];
+    sw21[i] = h.w2[H + i];
+  }
+  __syncthreads();
+  const int K = h.K, PPT = 16 / K, NTA = (16 + PPT - 1) / PPT;
+  const float sn = h.null_k * h.scale;
+  const int64_t gstep = (int64_t)gridDim.x * HEAD_WAVES * 16;
+  for (int64_t m0 = ((int64_t)xcd_block() * HEAD_WAVES + wv) * 16; m0 < h.M; m0 += gstep) {
+    for (int ta = 0; ta < NTA; ta++) {
+      const int pl = ta * PPT + r / K, k = r % K;  // this lane's row: local point, axis
+      const bool rv = r < PPT * K && pl < 16 && m0 + pl < h.M;
+      const int64_t row = rv ? (m0 + pl) * K + k : 0;
+      f32x4 x[D / 16];
+      const T *src = ctx + row * D + 4 * g;
+#pragma unroll
+      for (int q = 0; q < D / 16; q++) {
+        if constexpr (sizeof(T) == 2) {
+          const bf16x4 v = *(const bf16x4 *)(src + 16 * q);
+#pragma unroll
+          for (int e = 0; e < 4; e++) x[q][e] = (float)v[e];
+        } else {
+          x[q] = *(const f32x4 *)(src + 16 * q);
         }
-        hk = gelu(hk + pw.bk1[lane]) * pw.wk2[lane];
-        vk += pw.bv[lane];
       }
-      for (int o = 32; o >= 1; o >>= 1) hk += __shfl_xor(hk, o);
-      s[k] = (hk + pw.bk2) * h.scale;
-      v[k] = vk;
+      float part = 0.f;
+#pragma unroll 1
+      for (int ob = 0; ob < D / 16; ob++) {  // key hidden units
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        const float *wr = sWkv + (16 * ob + r) * PW + g;
+#pragma unroll
+        for (int t = 0; t < D / 4; t++) mfma(acc, wr[4 * t], x[t / 4][t % 4]);
+        const int o = 16 * ob + 4 * g;
+#pragma unroll
+        for (int v = 0; v < 4; v++) part += swk2[o + v] * gelu(acc[v] + sbk1[o + v]);
+      }
+      part += __shfl_xor(part, 16);
+      part += __shfl_xor(part, 32);
+      if (g == 0) sS[r] = (part + h.bk2) * h.scale;
+      wave_lds_sync();
+      // softmax over the point's axes + null
+      float mx = sn;
+      const int p0 = (r / K) * K;
+      for (int kk = 0; kk < K; kk++) mx = fmaxf(mx, sS[p0 + kk < 16 ? p0 + kk : 15]);
+      float den = __expf(sn - mx);
+      for (int kk = 0; kk < K; kk++) den += __expf(sS[p0 + kk < 16 ? p0 + kk : 15] - mx);
+      const float inv = 1.0f / den;
+      const float wr_ = __expf(sS[r] - mx) * inv;
+#pragma unroll 1
+      for (int q = 0; q < D / 16; q++) {  // value units, weighted
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        const float *wr = sWkv + (16 * (D / 16 + q) + r) * PW + g;
+#pragma unroll
+        for (int t = 0; t < D / 4; t++) mfma(acc, wr[4 * t], x[t / 4][t % 4]);
+        const int o = 16 * q + 4 * g;
+#pragma unroll
+        for (int v = 0; v < 4; v++) sV[r * PW + o + v] = wr_ * (acc[v] + sbv[o + v]);
+      }
+      if (g == 0 && k == 0) sS[16 + r / K] = __expf(sn - mx) * inv;
+      wave_lds_sync();
+      for (int idx = lane; idx < PPT * D; idx += 64) {
+        const int pp = idx / D, o = idx % D, plo = ta * PPT + pp;
+        if (plo >= 16) continue;
+        float s = sS[16 + pp] * snv[o];
+        for (int kk = 0; kk < K; kk++) s += sV[(pp * K + kk) * PW + o];
+        sPool[plo * PW + o] = s;
+      }
+      wave_lds_sync();
     }
-    const float sn = pw.null_k * h.scale;
-    float mx = sn;
-    for (int k = 0; k < h.K; k++) mx = fmaxf(mx, s[k]);
-    float den = 0.f, e[LATEC_MAX_DIM];
-    for (int k = 0; k < h.K; k++) {
-      e[k] = expf(s[k] - mx);
-      den += e[k];
-    }
-    const float en = expf(sn - mx);
-    den += en;
-    float pooled = 0.f;
-    if (lane < D) {
-      for (int k = 0; k < h.K; k++) pooled += (e[k] / den) * v[k];
-      pooled += (en / den) * pw.null_v[lane];
-      sP[wv][lane] = pooled;
-    }
-    __syncthreads();
+    // phase B
     float o0 = 0.f, o1 = 0.f;
-    for (int jj = lane; jj < H; jj += 64) {
-      float z = 0.f;
-      for (int i = 0; i < D; i++) z += h.w1t[i * H + jj] * sP[wv][i];
-      z += h.cond[jj] * h.log_eb;
-      z = gelu(z + h.b1[jj]);
-      o0 += h.w2[jj] * z;
-      o1 += h.w2[H + jj] * z;
+#pragma unroll 1
+    for (int ob = 0; ob < H / 16; ob++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      const float *wr = sW1 + (16 * ob + r) * PW + g;
+      const float *pr = sPool + r * PW + g;
+#pragma unroll
+      for (int t = 0; t < D / 4; t++) mfma(acc, wr[4 * t], pr[4 * t]);
+      const int o = 16 * ob + 4 * g;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const float z = gelu(acc[v] + sb1[o + v]);
+        o0 += sw20[o + v] * z;
+        o1 += sw21[o + v] * z;
+      }
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-      o0 += __shfl_xor(o0, o);
-      o1 += __shfl_xor(o1, o);
-    }
-    if (act && lane == 0) {
+    o0 += __shfl_xor(o0, 16);
+    o0 += __shfl_xor(o0, 32);
+    o1 += __shfl_xor(o1, 16);
+    o1 += __shfl_xor(o1, 32);
+    const int64_t m = m0 + r;
+    if (g == 0 && m < h.M) {
       o0 += h.b2[0];
       o1 += h.b2[1];
       const float mu = 1.0f / (1.0f + expf(-o0));
@@ -959,69 +1135,89 @@ __global__ __launch_bounds__(256) void head_kernel(const T *__restrict__ ctx, Po
       if (mu_out) mu_out[m] = mu;
       if (lb_out) lb_out[m] = lb;
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
 // ---------------------------------------------------------------------------
-// value embedding (InitEmbed of the clamped, normalized recon value)
+// InitEmbed of the clamped, normalized recon value on an MFMA tile
 // ---------------------------------------------------------------------------
 
+template <typename T>
+struct InitW {
+  const float *w0, *b0;  // (D) first layer (one input)
+  const T *w2;           // (D, D) natural order
+  const float *b2;       // (D)
+};
+
 template <int D, typename T>
-__global__ __launch_bounds__(256) void value_embed_kernel(
-    int64_t M, const float *__restrict__ recon, const int64_t *__restrict__ gidx,
-    float vmin, float vmax, float span, const float *__restrict__ w0,
-    const float *__restrict__ b0, const float *__restrict__ w2t,
-    const float *__restrict__ b2, T *__restrict__ out) {
-  __shared__ float sh[4][D];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t per = (int64_t)gridDim.x * 4;
-  for (int64_t m0 = (int64_t)blockIdx.x * 4; m0 < M; m0 += per) {
-    const int64_t m = m0 + wv;
-    const bool act = m < M;
-    float val = 0.f;
-    if (act) {
-      const float r = recon[gidx[m]];
-      val = (fminf(fmaxf(r, vmin), vmax) - vmin) / span;
-    }
-    if (lane < D) sh[wv][lane] = gelu(w0[lane] * val + b0[lane]);
-    __syncthreads();
-    if (act && lane < D) {
-      float o = 0.f;
-      for (int j = 0; j < D; j++) o += w2t[j * D + lane] * sh[wv][j];
-      out[m * D + lane] = st<T>(o + b2[lane]);
-    }
-    __syncthreads();
-  }
+static size_t init_smem() {
+  return sizeof(T) * (size_t)D * (D + Mf<T>::PE) + sizeof(float) * 3 * (size_t)D;
 }
 
-// halo cells: E[row, k, :] = InitEmbed(value) for every axis k
+// 16 points per wave: lane (r, g) evaluates gelu(w0 v + b0) for the units of
+// its k-set (natural order), then out^T = W2 h^T; out rows go to out[m] (the
+// value embedding) or, with rows != nullptr, to E[rows[m], k] for every axis
+// k < K (halo cells).
 template <int D, typename T>
-__global__ __launch_bounds__(256) void halo_embed_kernel(
-    int64_t M, int K, const float *__restrict__ recon, const int64_t *__restrict__ gidx,
-    const int64_t *__restrict__ rows, float vmin, float vmax, float span,
-    const float *__restrict__ w0, const float *__restrict__ b0,
-    const float *__restrict__ w2t, const float *__restrict__ b2, T *__restrict__ E) {
-  __shared__ float sh[4][D];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t per = (int64_t)gridDim.x * 4;
-  for (int64_t m0 = (int64_t)blockIdx.x * 4; m0 < M; m0 += per) {
-    const int64_t m = m0 + wv;
-    const bool act = m < M;
+__global__ __launch_bounds__(256) void init_embed_kernel(int64_t M, int K,
+                                                         const float *__restrict__ recon,
+                                                         const int64_t *__restrict__ gidx,
+                                                         const int64_t *__restrict__ rows,
+                                                         float vmin, float vmax, float span,
+                                                         InitW<T> w, T *__restrict__ out) {
+  typedef Mf<T> F;
+  constexpr int KS = F::KS, KL = F::KL, P = D + F::PE;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *sW = (T *)smem;
+  float *sw0 = (float *)(sW + D * P), *sb0 = sw0 + D, *sb2 = sb0 + D;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, r = lane & 15, g = lane >> 4;
+  for (int i = tid; i < D * D; i += 256) sW[(i / D) * P + i % D] = w.w2[i];
+  for (int i = tid; i < D; i += 256) {
+    sw0[i] = w.w0[i];
+    sb0[i] = w.b0[i];
+    sb2[i] = w.b2[i];
+  }
+  __syncthreads();
+  const int64_t gstep = (int64_t)gridDim.x * 4 * 16;
+  for (int64_t m0 = ((int64_t)blockIdx.x * 4 + wv) * 16; m0 < M; m0 += gstep) {
+    const int64_t m = m0 + r;
     float val = 0.f;
-    if (act) {
-      const float r = recon[gidx[m]];
-      val = (fminf(fmaxf(r, vmin), vmax) - vmin) / span;
+    if (m < M) {
+      const float x = recon[gidx[m]];
+      val = (fminf(fmaxf(x, vmin), vmax) - vmin) / span;
     }
-    if (lane < D) sh[wv][lane] = gelu(w0[lane] * val + b0[lane]);
-    __syncthreads();
-    if (act && lane < D) {
-      float o = 0.f;
-      for (int j = 0; j < D; j++) o += w2t[j * D + lane] * sh[wv][j];
-      const T v = st<T>(o + b2[lane]);
-      for (int k = 0; k < K; k++) E[(rows[m] * K + k) * D + lane] = v;
+    typename F::frag hb[D / KS];
+#pragma unroll
+    for (int s = 0; s < D / KS; s++) {
+      if constexpr (sizeof(T) == 2) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const int u = s * KS + 8 * g + i;
+          hb[s][i] = (__bf16)gelu(__builtin_fmaf(sw0[u], val, sb0[u]));
+        }
+      } else {
+        const int u = s * KS + g;
+        hb[s] = gelu(__builtin_fmaf(sw0[u], val, sb0[u]));
+      }
     }
-    __syncthreads();
+#pragma unroll
+    for (int ob = 0; ob < D / 16; ob++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      const T *wr = sW + (16 * ob + r) * P + g * KL;
+#pragma unroll
+      for (int s = 0; s < D / KS; s++) mfma(acc, ldfrag<T>(wr + s * KS), hb[s]);
+      const int o = 16 * ob + 4 * g;
+      const f32x4 b2 = *(const f32x4 *)(sb2 + o);
+      const typename F::quad y = to_quad<T>(acc + b2);
+      if (m < M) {
+        if (rows) {
+          for (int k = 0; k < K; k++) *(typename F::quad *)(out + (rows[m] * K + k) * D + o) = y;
+        } else {
+          *(typename F::quad *)(out + m * D + o) = y;
+        }
+      }
+    }
   }
 }
 
@@ -1164,12 +1360,27 @@ using latec::ClassDesc;
 
 namespace {
 
-// Packs the model for the device: GEMM operands in the storage type, state
-// rows folded into bias rows, pool matrices transposed, all in one block.
+// Packs the model for the device.  Message / mix MLPs: W1 with its input in
+// the kernel's register order and the additive-term columns appended (b1,
+// the two scalar coefficients, the state bias rows; bf16 as hi + lo parts),
+// W2 and the key's Wk with their inputs in register order, all in the
+// storage type.  Pools and head in f32.  Everything in one device block.
 int make_device(latec_gnn *g, const float *P) {
   const Layout &Ly = g->lay;
-  const int d = Ly.d, h = Ly.h, de = Ly.de, ds = Ly.ds, ns = Ly.ns;
+  const int d = Ly.d, h = Ly.h, de = Ly.de, ds = Ly.ds, ns = Ly.ns, agg = g->agg;
   const bool bf = g->numerics == NUM_BF16;
+  const int nxc = bf ? 32 : 16;  // additive-term columns
+  if (ns > (bf ? 8 : 13)) {
+    latec::set_error("GNN kernels support at most " + std::to_string(bf ? 8 : 13) +
+                     " direction states, got " + std::to_string(ns));
+    return LATEC_E_ARG;
+  }
+  // a two-sided pair's states are mirror images: st1 = 2 agg - st0
+  for (size_t i = 0; i + 1 < g->sidx.size(); i += 2)
+    if (g->sidx[i + 1] != 2 * agg - g->sidx[i]) {
+      latec::set_error("direction states of a line are not mirrored");
+      return LATEC_E_ARG;
+    }
   std::vector<unsigned char> host;
   size_t off = 0;
   auto mat = [&](const std::vector<float> &v) -> void * {
@@ -1195,28 +1406,14 @@ int make_device(latec_gnn *g, const float *P) {
     }
     return v;
   };
-  // message layer 1 (bidir): [E_neg | E_pos] columns
-  const int bi_in = 2 * de + 2;
-  std::vector<float> w_bi = cat2(rows(Ly.bi0w, h, bi_in, 0, d), d, rows(Ly.bi0w, h, bi_in, de, d), d, h);
-  void *o_bi_w1 = mat(w_bi);
-  void *o_bi_w2 = mat(rows(Ly.bi2w, d, h, 0, h));
-  const int dir_in = de + 2;
-  void *o_dir_w1 = mat(rows(Ly.dir0w, h, dir_in, 0, d));
-  void *o_dir_w2 = mat(rows(Ly.dir2w, d, h, 0, h));
-  void *o_mx_w1 = mat(rows(Ly.mx0w, h, 2 * d, 0, 2 * d));
-  void *o_mx_w2 = mat(rows(Ly.mx2w, d, h, 0, h));
-  void *o_lk_w1 = mat(rows(Ly.lp[0], d, d, 0, d));
-  // f32 arrays
-  std::vector<std::vector<float>> f(F_NSLOTS);
-  auto state_rows = [&](int64_t wbase, int n_in, int col0) {
-    std::vector<float> v((size_t)ns * h);
-    for (int s = 0; s < ns; s++)
-      for (int o = 0; o < h; o++) {
-        double acc = 0.0;
-        for (int j = 0; j < ds; j++)
-          acc += (double)P[wbase + (int64_t)o * n_in + col0 + j] * (double)P[Ly.table + s * ds + j];
-        v[(size_t)s * h + o] = (float)acc;
-      }
+  // input index -> register order; `gathered` inputs come from memory rows
+  // (natural order in bf16, the f32 order in f32), the others from a
+  // transposed layer's registers
+  auto reg_in = [&](const std::vector<float> &W, int n_out, int n_in, bool gathered, bool bfm) {
+    std::vector<float> v(W.size());
+    for (int o = 0; o < n_out; o++)
+      for (int kk = 0; kk < n_in; kk++)
+        v[(size_t)o * n_in + kk] = W[(size_t)o * n_in + ((bfm && gathered) ? kk : reg_unit(bfm, kk))];
     return v;
   };
   auto vec = [&](int64_t base, int n) { return std::vector<float>(P + base, P + base + n); };
@@ -1225,45 +1422,109 @@ int make_device(latec_gnn *g, const float *P) {
     for (int o = 0; o < n_out; o++) v[o] = P[wbase + (int64_t)o * n_in + c];
     return v;
   };
-  auto transpose = [&](int64_t base, int n_out, int n_in, int in_used) {
-    std::vector<float> v((size_t)in_used * n_out);
-    for (int o = 0; o < n_out; o++)
-      for (int i = 0; i < in_used; i++) v[(size_t)i * n_out + o] = P[base + (int64_t)o * n_in + i];
+  // DirState rows folded through the first layer: (ns, h), f64
+  auto state_rows = [&](int64_t wbase, int n_in, int col0) {
+    std::vector<double> v((size_t)ns * h);
+    for (int s = 0; s < ns; s++)
+      for (int o = 0; o < h; o++) {
+        double acc = 0.0;
+        for (int j = 0; j < ds; j++)
+          acc += (double)P[wbase + (int64_t)o * n_in + col0 + j] * (double)P[Ly.table + s * ds + j];
+        v[(size_t)s * h + o] = acc;
+      }
     return v;
   };
-  f[F_BI_B1] = vec(Ly.bi0b, h);
+  auto hi = [](float x) { return from_bf16(to_bf16(x)); };
+  // (h, K1 + nxc): W1 (register order) then the additive-term columns
+  auto w1x = [&](const std::vector<float> &W1, int K1, const std::vector<float> &b1,
+                 const std::vector<float> *c0, const std::vector<float> *c1,
+                 const std::vector<double> *S) {
+    const int n = K1 + nxc;
+    std::vector<float> v((size_t)h * n, 0.f);
+    const std::vector<float> Wr = reg_in(W1, h, K1, true, bf);
+    for (int o = 0; o < h; o++) {
+      for (int c = 0; c < K1; c++) v[(size_t)o * n + c] = Wr[(size_t)o * K1 + c];
+      float *e = &v[(size_t)o * n + K1];
+      if (bf) {
+        e[0] = hi(b1[o]);
+        e[1] = b1[o] - hi(b1[o]);
+        if (c0) {
+          e[2] = hi((*c0)[o]);
+          e[3] = (*c0)[o] - hi((*c0)[o]);
+          e[4] = hi((*c0)[o]);
+        }
+        if (c1) {
+          e[5] = hi((*c1)[o]);
+          e[6] = (*c1)[o] - hi((*c1)[o]);
+          e[7] = hi((*c1)[o]);
+        }
+        if (S)
+          for (int st = 0; st < ns; st++) {
+            const float sv = (float)(*S)[(size_t)st * h + o];
+            e[8 + st] = hi(sv);
+            e[16 + st] = sv - hi(sv);
+          }
+      } else {
+        e[0] = b1[o];
+        if (c0) e[1] = (*c0)[o];
+        if (c1) e[2] = (*c1)[o];
+        if (S)
+          for (int st = 0; st < ns; st++) e[3 + st] = (float)(*S)[(size_t)st * h + o];
+      }
+    }
+    return v;
+  };
+  // message layer 1 (bidir): [E_neg | E_pos] columns, scalars log d_neg, log d_pos
+  const int bi_in = 2 * de + 2;
+  const std::vector<float> w_bi =
+      cat2(rows(Ly.bi0w, h, bi_in, 0, d), d, rows(Ly.bi0w, h, bi_in, de, d), d, h);
+  const std::vector<double> s0 = state_rows(Ly.bi0w, bi_in, d), s1 = state_rows(Ly.bi0w, bi_in, de + d);
+  std::vector<double> s_bi((size_t)ns * h);
+  for (int st = 0; st < ns; st++)
+    for (int o = 0; o < h; o++)
+      s_bi[(size_t)st * h + o] = s0[(size_t)st * h + o] + s1[(size_t)(2 * agg - st) * h + o];
+  const std::vector<float> bi_c0 = col(Ly.bi0w, h, bi_in, 2 * de), bi_c1 = col(Ly.bi0w, h, bi_in, 2 * de + 1);
+  void *o_bi_w1 = mat(w1x(w_bi, 2 * d, vec(Ly.bi0b, h), &bi_c0, &bi_c1, &s_bi));
+  void *o_bi_w2 = mat(reg_in(rows(Ly.bi2w, d, h, 0, h), d, h, false, bf));
+  // one-sided: E_side, scalars sign and log d
+  const int dir_in = de + 2;
+  const std::vector<double> s_dir = state_rows(Ly.dir0w, dir_in, d);
+  const std::vector<float> dir_c0 = col(Ly.dir0w, h, dir_in, de), dir_c1 = col(Ly.dir0w, h, dir_in, de + 1);
+  void *o_dir_w1 = mat(w1x(rows(Ly.dir0w, h, dir_in, 0, d), d, vec(Ly.dir0b, h), &dir_c0, &dir_c1, &s_dir));
+  void *o_dir_w2 = mat(reg_in(rows(Ly.dir2w, d, h, 0, h), d, h, false, bf));
+  // mix: [ctx | value embedding]
+  void *o_mx_w1 = mat(w1x(rows(Ly.mx0w, h, 2 * d, 0, 2 * d), 2 * d, vec(Ly.mx0b, h), nullptr, nullptr, nullptr));
+  void *o_mx_w2 = mat(reg_in(rows(Ly.mx2w, d, h, 0, h), d, h, false, bf));
+  // line-pool key layer 1 on the message registers
+  void *o_key_w = mat(reg_in(rows(Ly.lp[0], d, d, 0, d), d, d, false, bf));
+  // InitEmbed layer 2, natural order
+  void *o_init_w2 = mat(rows(Ly.init2w, d, d, 0, d));
+  // f32 arrays
+  std::vector<std::vector<float>> f(F_NSLOTS);
   f[F_BI_B2] = vec(Ly.bi2b, d);
-  f[F_BI_S0] = state_rows(Ly.bi0w, bi_in, d);
-  f[F_BI_S1] = state_rows(Ly.bi0w, bi_in, de + d);
-  f[F_BI_C0] = col(Ly.bi0w, h, bi_in, 2 * de);
-  f[F_BI_C1] = col(Ly.bi0w, h, bi_in, 2 * de + 1);
-  f[F_DIR_B1] = vec(Ly.dir0b, h);
   f[F_DIR_B2] = vec(Ly.dir2b, d);
-  f[F_DIR_S] = state_rows(Ly.dir0w, dir_in, d);
-  f[F_DIR_CSIGN] = col(Ly.dir0w, h, dir_in, de);
-  f[F_DIR_CLOGD] = col(Ly.dir0w, h, dir_in, de + 1);
-  f[F_MX_B1] = vec(Ly.mx0b, h);
   f[F_MX_B2] = vec(Ly.mx2b, d);
   f[F_LK_B1] = vec(Ly.lp[1], d);
   f[F_LK_W2] = vec(Ly.lp[2], d);
-  f[F_LK_WVT] = transpose(Ly.lp[4], d, d, d);
+  f[F_LK_WV] = rows(Ly.lp[4], d, d, 0, d);
   f[F_LK_BV] = vec(Ly.lp[5], d);
   f[F_LK_NULLV] = vec(Ly.lp[7], d);
-  f[F_LK_W1T] = transpose(Ly.lp[0], d, d, d);
-  f[F_AP_W1T] = transpose(Ly.ap[0], d, d, d);
+  {
+    std::vector<float> kv = rows(Ly.ap[0], d, d, 0, d), v2 = rows(Ly.ap[4], d, d, 0, d);
+    kv.insert(kv.end(), v2.begin(), v2.end());
+    f[F_AP_WKV] = reg_in(kv, 2 * d, d, true, false);
+  }
   f[F_AP_B1] = vec(Ly.ap[1], d);
   f[F_AP_W2] = vec(Ly.ap[2], d);
-  f[F_AP_WVT] = transpose(Ly.ap[4], d, d, d);
   f[F_AP_BV] = vec(Ly.ap[5], d);
   f[F_AP_NULLV] = vec(Ly.ap[7], d);
-  f[F_HD_W1T] = transpose(Ly.hd0w, h, d + 1, d);
+  f[F_HD_W1] = rows(Ly.hd0w, h, d + 1, 0, d);
   f[F_HD_COND] = col(Ly.hd0w, h, d + 1, d);
   f[F_HD_B1] = vec(Ly.hd0b, h);
   f[F_HD_W2] = vec(Ly.hd2w, 2 * h);
   f[F_HD_B2] = vec(Ly.hd2b, 2);
   f[F_IN_W0] = vec(Ly.init0w, d);
   f[F_IN_B0] = vec(Ly.init0b, d);
-  f[F_IN_W2T] = transpose(Ly.init2w, d, d, d);
   f[F_IN_B2] = vec(Ly.init2b, d);
   f[F_SCAL] = {P[Ly.lp[3]], P[Ly.lp[6]], P[Ly.ap[3]], P[Ly.ap[6]]};
   for (int i = 0; i < 4; i++) g->scal[i] = f[F_SCAL][i];
@@ -1295,7 +1556,8 @@ int make_device(latec_gnn *g, const float *P) {
   g->dir_w2 = at(o_dir_w2);
   g->mx_w1 = at(o_mx_w1);
   g->mx_w2 = at(o_mx_w2);
-  g->lk_w1 = at(o_lk_w1);
+  g->key_w = at(o_key_w);
+  g->init_w2 = at(o_init_w2);
   g->f32 = (float *)at(o_f32);
   g->d_dirs = (int8_t *)at(o_dirs);
   g->d_sidx = (int8_t *)at(o_sidx);
@@ -1304,30 +1566,6 @@ int make_device(latec_gnn *g, const float *P) {
 }
 
 const float *fs(const latec_gnn *g, int slot) { return g->f32 + g->f32_off[slot]; }
-
-PoolW pool_w(const latec_gnn *g, bool line) {
-  PoolW p{};
-  if (line) {
-    p.wk1t = fs(g, F_LK_W1T);
-    p.bk1 = fs(g, F_LK_B1);
-    p.wk2 = fs(g, F_LK_W2);
-    p.wvt = fs(g, F_LK_WVT);
-    p.bv = fs(g, F_LK_BV);
-    p.null_v = fs(g, F_LK_NULLV);
-    p.bk2 = g->scal[0];
-    p.null_k = g->scal[1];
-  } else {
-    p.wk1t = fs(g, F_AP_W1T);
-    p.bk1 = fs(g, F_AP_B1);
-    p.wk2 = fs(g, F_AP_W2);
-    p.wvt = fs(g, F_AP_WVT);
-    p.bv = fs(g, F_AP_BV);
-    p.null_v = fs(g, F_AP_NULLV);
-    p.bk2 = g->scal[2];
-    p.null_k = g->scal[3];
-  }
-  return p;
-}
 
 int ensure_work(latec_gnn *g, size_t bytes) {
   if (g->work_bytes >= bytes) return LATEC_OK;
@@ -1344,42 +1582,53 @@ int ensure_work(latec_gnn *g, size_t bytes) {
   return rc;
 }
 
+// dynamic LDS attribute + workgroups per CU that the LDS allows (1..4)
+int lds_setup(const void *fn, size_t smem, int *per_cu) {
+  int rc = hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
+                  "hipFuncSetAttribute");
+  const int n = (int)((size_t)160 * 1024 / (smem ? smem : 1));
+  *per_cu = n < 1 ? 1 : n > 4 ? 4 : n;
+  return rc;
+}
+
+template <int D, typename T, int MODE>
+int launch_rows_m(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
+  constexpr int NT = sizeof(T) == 2 ? 2 : 1;
+  const size_t smem = RowShape<D, T, MODE>::smem(a.L, a.K);
+  const void *fn = (const void *)msg_kernel<D, T, MODE, NT>;
+  int per_cu = 1;
+  int rc = lds_setup(fn, smem, &per_cu);
+  if (rc) return rc;
+  hipLaunchKernelGGL((msg_kernel<D, T, MODE, NT>), dim3(g->n_cu * per_cu), dim3(256), smem, s,
+                     a, w);
+  return hip_ok(hipGetLastError(), "msg_kernel launch");
+}
+
 template <int D, typename T>
 int launch_rows(latec_gnn *g, int mode, RowArgs &a, hipStream_t s) {
-  MlpW<T> w{};
-  KeyW<T> pk{(const T *)g->lk_w1, fs(g, F_LK_B1), fs(g, F_LK_W2), g->scal[0]};
-  if (mode == ROW_BI) {
-    w = {(const T *)g->bi_w1, (const T *)g->bi_w2, fs(g, F_BI_B1), fs(g, F_BI_B2),
-         fs(g, F_BI_S0), fs(g, F_BI_S1), fs(g, F_BI_C0), fs(g, F_BI_C1), 2 * D};
-  } else if (mode == ROW_DIR) {
-    w = {(const T *)g->dir_w1, (const T *)g->dir_w2, fs(g, F_DIR_B1), fs(g, F_DIR_B2),
-         fs(g, F_DIR_S), nullptr, fs(g, F_DIR_CSIGN), fs(g, F_DIR_CLOGD), D};
-  } else {
-    w = {(const T *)g->mx_w1, (const T *)g->mx_w2, fs(g, F_MX_B1), fs(g, F_MX_B2),
-         nullptr, nullptr, nullptr, nullptr, 2 * D};
-  }
-  a.ns = g->lay.ns;
+  RowW<T> w{};
+  w.wk = (const T *)g->key_w;
+  w.bk1 = fs(g, F_LK_B1);
+  w.wk2 = fs(g, F_LK_W2);
+  w.bk2 = g->scal[0];
   a.L = g->L;
   a.sidx = g->d_sidx;
-  const size_t smem = row_mlp_smem<D, T>(a.ns);
-  const int grid = g->n_cu;
-  const void *fn = mode == ROW_BI    ? (const void *)row_mlp_kernel<D, T, ROW_BI>
-                   : mode == ROW_DIR ? (const void *)row_mlp_kernel<D, T, ROW_DIR>
-                                     : (const void *)row_mlp_kernel<D, T, ROW_FIN>;
-  int rc = hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
-                  "hipFuncSetAttribute(row_mlp_kernel)");
-  if (rc) return rc;
-  using C = RowCfg<D, T>;
-  if (mode == ROW_BI)
-    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_BI>), dim3(grid), dim3(C::THREADS), smem, s, a,
-                       w, pk);
-  else if (mode == ROW_DIR)
-    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_DIR>), dim3(grid), dim3(C::THREADS), smem, s, a,
-                       w, pk);
-  else
-    hipLaunchKernelGGL((row_mlp_kernel<D, T, ROW_FIN>), dim3(grid), dim3(C::THREADS), smem, s, a,
-                       w, pk);
-  return hip_ok(hipGetLastError(), "row_mlp_kernel launch");
+  if (mode == ROW_BI) {
+    w.w1 = (const T *)g->bi_w1;
+    w.w2 = (const T *)g->bi_w2;
+    w.b2 = fs(g, F_BI_B2);
+    return launch_rows_m<D, T, ROW_BI>(g, a, w, s);
+  }
+  if (mode == ROW_DIR) {
+    w.w1 = (const T *)g->dir_w1;
+    w.w2 = (const T *)g->dir_w2;
+    w.b2 = fs(g, F_DIR_B2);
+    return launch_rows_m<D, T, ROW_DIR>(g, a, w, s);
+  }
+  w.w1 = (const T *)g->mx_w1;
+  w.w2 = (const T *)g->mx_w2;
+  w.b2 = fs(g, F_MX_B2);
+  return launch_rows_m<D, T, ROW_FIN>(g, a, w, s);
 }
 
 }  // namespace
@@ -1478,8 +1727,12 @@ int embed_t(latec_gnn *g, const void *E, const int64_t *frame_shape, const int32
   int16_t *lw_dir = (int16_t *)((unsigned char *)lw_bi + b_lw);
   T *msg = (T *)((unsigned char *)lw_dir + b_lw);
   float *score = (float *)((unsigned char *)msg + b_msg);
-  const PoolW pw = pool_w(g, true);
   const float scale = 1.0f / std::sqrt((float)D);
+  const PoolArgs pa{fs(g, F_LK_WV), fs(g, F_LK_BV), fs(g, F_LK_NULLV), g->scal[1] * scale};
+  const size_t psm = pool_smem<D>();
+  int pool_per_cu = 1;
+  rc = lds_setup((const void *)pool_kernel<D, T>, psm, &pool_per_cu);
+  if (rc) return rc;
   for (int64_t m0 = 0; m0 < M; m0 += MT) {
     const int64_t mt = M - m0 < MT ? M - m0 : MT;
     const int64_t n_pairs = mt * L;
@@ -1513,11 +1766,12 @@ int embed_t(latec_gnn *g, const void *E, const int64_t *frame_shape, const int32
     rc = launch_rows<D, T>(g, ROW_DIR, a, s);
     if (rc) return rc;
     const int64_t items = mt * K;
-    int grid = (int)((items + 3) / 4);
-    if (grid > 8 * g->n_cu) grid = 8 * g->n_cu;
-    hipLaunchKernelGGL((line_pool_kernel<D, T>), dim3(grid), dim3(256), 0, s, items, K, L,
-                       slot, msg, score, pw, scale, (T *)ctx, m0);
-    rc = hip_ok(hipGetLastError(), "line_pool_kernel launch");
+    const int64_t waves = (items + 15) / 16;
+    int64_t grid = (waves + POOL_WAVES - 1) / POOL_WAVES;
+    if (grid > (int64_t)g->n_cu * pool_per_cu * 4) grid = (int64_t)g->n_cu * pool_per_cu * 4;
+    hipLaunchKernelGGL((pool_kernel<D, T>), dim3((unsigned)grid), dim3(64 * POOL_WAVES), psm, s,
+                       items, K, L, slot, msg, score, pa, (T *)ctx, m0);
+    rc = hip_ok(hipGetLastError(), "pool_kernel launch");
     if (rc) return rc;
   }
   return LATEC_OK;
@@ -1536,17 +1790,44 @@ int head_t(latec_gnn *g, const void *ctx, int64_t M, double eb, float vmin, floa
   h.vmin = vmin;
   h.vmax = vmax;
   h.span = vmax - vmin;
-  h.w1t = fs(g, F_HD_W1T);
+  h.wkv = fs(g, F_AP_WKV);
+  h.bk1 = fs(g, F_AP_B1);
+  h.wk2 = fs(g, F_AP_W2);
+  h.bv = fs(g, F_AP_BV);
+  h.null_v = fs(g, F_AP_NULLV);
+  h.bk2 = g->scal[2];
+  h.null_k = g->scal[3];
+  h.w1 = fs(g, F_HD_W1);
   h.cond = fs(g, F_HD_COND);
   h.b1 = fs(g, F_HD_B1);
   h.w2 = fs(g, F_HD_W2);
   h.b2 = fs(g, F_HD_B2);
-  int grid = (int)((M + 3) / 4);
-  if (grid > 8 * g->n_cu) grid = 8 * g->n_cu;
+  const size_t smem = head_smem<D>();
+  int per_cu = 1;
+  int rc = lds_setup((const void *)head2_kernel<D, T>, smem, &per_cu);
+  if (rc) return rc;
+  const int64_t waves = (M + 15) / 16;
+  int64_t grid = (waves + HEAD_WAVES - 1) / HEAD_WAVES;
+  if (grid > (int64_t)g->n_cu * per_cu) grid = (int64_t)g->n_cu * per_cu;
+  hipLaunchKernelGGL((head2_kernel<D, T>), dim3((unsigned)grid), dim3(64 * HEAD_WAVES), smem, s,
+                     (const T *)ctx, h, pred, scale, mu, logb);
+  return hip_ok(hipGetLastError(), "head2_kernel launch");
+}
+
+template <int D, typename T>
+int init_embed(latec_gnn *g, int64_t M, const float *recon, const int64_t *gidx,
+               const int64_t *rows, float vmin, float vmax, T *out, hipStream_t s) {
+  const InitW<T> w{fs(g, F_IN_W0), fs(g, F_IN_B0), (const T *)g->init_w2, fs(g, F_IN_B2)};
+  const size_t smem = init_smem<D, T>();
+  int per_cu = 1;
+  int rc = lds_setup((const void *)init_embed_kernel<D, T>, smem, &per_cu);
+  if (rc) return rc;
+  int64_t grid = (M + 63) / 64;
+  if (grid > (int64_t)g->n_cu * per_cu) grid = (int64_t)g->n_cu * per_cu;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((head_kernel<D, T>), dim3(grid), dim3(256), 0, s, (const T *)ctx,
-                     pool_w(g, false), h, pred, scale, mu, logb);
-  return hip_ok(hipGetLastError(), "head_kernel launch");
+  hipLaunchKernelGGL((init_embed_kernel<D, T>), dim3((unsigned)grid), dim3(256), smem, s, M,
+                     g->ndim, recon, gidx, rows, vmin, vmax, vmax - vmin, w, out);
+  return hip_ok(hipGetLastError(), "init_embed_kernel launch");
 }
 
 template <int D, typename T>
@@ -1556,13 +1837,7 @@ int finalize_t(latec_gnn *g, void *E, const void *ctx, const int64_t *qidx,
   int rc = ensure_work(g, (size_t)M * D * sizeof(T) + 256);
   if (rc) return rc;
   T *vemb = (T *)g->work;
-  int grid = (int)((M + 3) / 4);
-  if (grid > 8 * g->n_cu) grid = 8 * g->n_cu;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((value_embed_kernel<D, T>), dim3(grid), dim3(256), 0, s, M, recon, gidx,
-                     vmin, vmax, vmax - vmin, fs(g, F_IN_W0), fs(g, F_IN_B0), fs(g, F_IN_W2T),
-                     fs(g, F_IN_B2), vemb);
-  rc = hip_ok(hipGetLastError(), "value_embed_kernel launch");
+  rc = init_embed<D, T>(g, M, recon, gidx, nullptr, vmin, vmax, vemb, s);
   if (rc) return rc;
   RowArgs a{};
   a.K = g->ndim;
@@ -1577,13 +1852,7 @@ int finalize_t(latec_gnn *g, void *E, const void *ctx, const int64_t *qidx,
 template <int D, typename T>
 int halo_t(latec_gnn *g, void *E, const int64_t *rows, const int64_t *gidx,
            const float *recon, int64_t M, float vmin, float vmax, hipStream_t s) {
-  int grid = (int)((M + 3) / 4);
-  if (grid > 8 * g->n_cu) grid = 8 * g->n_cu;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((halo_embed_kernel<D, T>), dim3(grid), dim3(256), 0, s, M, g->ndim,
-                     recon, gidx, rows, vmin, vmax, vmax - vmin, fs(g, F_IN_W0),
-                     fs(g, F_IN_B0), fs(g, F_IN_W2T), fs(g, F_IN_B2), (T *)E);
-  return hip_ok(hipGetLastError(), "halo_embed_kernel launch");
+  return init_embed<D, T>(g, M, recon, gidx, rows, vmin, vmax, (T *)E, s);
 }
 
 #define GNN_DISPATCH(FN, ...)                                                      \

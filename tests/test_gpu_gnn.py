@@ -241,6 +241,17 @@ def test_multichunk_halo_geometry_matches_scan(tmp_path, shape, levels, edges):
         pr._wave = None
 
 
+def _gate_costs_flat(W):
+    """The selection's candidate costs (every (row, T) split of the 12
+    candidate rows) followed by the no-gate total, as _gate_select ranks them."""
+    cums = np.cumsum(W, axis=-1)
+    base, ci = cums[0], cums[1:]
+    tot = base[-1]
+    low = tot - base[:-1] + ci[:, :-1]
+    high = base[:-1] + (ci[:, -1:] - ci[:, :-1])
+    return np.concatenate([np.concatenate([low, high]).reshape(-1), [tot]])
+
+
 @pytest.mark.parametrize("shape", [(40, 36, 44), (16, 18, 16, 14)])
 def test_gate_costs_match_oracle_on_real_stages(monkeypatch, shape):
     """Every gate evaluation of a real encode (committed checkpoint): the
@@ -269,7 +280,12 @@ def test_gate_costs_match_oracle_on_real_stages(monkeypatch, shape):
     for i, (xs, p, sc, ips, eb, radius, W) in enumerate(seen):
         ref = G.gate_costs(xs, p, sc, ips, eb, radius)
         np.testing.assert_allclose(W, ref, rtol=1e-9, atol=1e-6, err_msg=f"gate {i}")
-        assert GC._gate_select(W) == GC._gate_select(ref), i
+        if GC._gate_select(W) != GC._gate_select(ref):
+            # a near-tie: the device's choice must cost the same under the
+            # oracle's own sums, to the sums' rounding
+            cw, cr = _gate_costs_flat(W), _gate_costs_flat(ref)
+            fw, fr = int(np.argmin(cw)), int(np.argmin(cr))
+            assert abs(cr[fw] - cr[fr]) <= 1e-6 + 1e-9 * abs(cr[fr]), (i, cr[fw], cr[fr])
     y = codec.uncompress(stream).numpy()
     assert np.abs(y - x).max() <= 1e-3 * float(x.max() - x.min()) + GC.roundtrip_slack(
         x.min(), x.max())
