@@ -77,6 +77,14 @@ struct Model {
   std::vector<uint64_t> dlut;  // 2^(PREC - DLUT_SHIFT) entries
 };
 
+// floor(2^64 / p) for 1 < p < 2^32 (0 for p <= 1): a power of two exactly,
+// otherwise floor((2^64 - 1) / p), which is the same when p does not divide 2^64
+static inline uint64_t rcp64(uint32_t p) {
+  if (p <= 1) return 0;
+  if ((p & (p - 1)) == 0) return (uint64_t)1 << (64 - __builtin_ctz(p));
+  return ~(uint64_t)0 / p;
+}
+
 static void build_model(const uint32_t *cdf, int64_t A, Model &m, bool for_decode) {
   m.A = A;
   m.left.resize(A + 1);
@@ -94,7 +102,7 @@ static void build_model(const uint32_t *cdf, int64_t A, Model &m, bool for_decod
     SymInfo &s = m.sym[k];
     s.left = m.left[k];
     s.prob = m.left[k + 1] - m.left[k];
-    s.rcp = s.prob > 1 ? (uint64_t)((((unsigned __int128)1) << 64) / s.prob) : 0;
+    s.rcp = rcp64(s.prob);
   }
   if (for_decode) {
     m.dlut.assign((size_t)1 << (PREC - DLUT_SHIFT), 0);
@@ -200,31 +208,59 @@ int64_t rans_encode(const uint32_t *syms, int64_t n, const uint32_t *cdf, int64_
   return nw;
 }
 
+// Stable counting sort of positions 0..n-1 by level: order[start[l] ..
+// start[l + 1]) holds the positions of level l in increasing order.  False on a
+// level >= 64.
+static bool group_by_level(const uint8_t *levels, int64_t n, int64_t start[65],
+                           std::unique_ptr<uint32_t[]> *order, const uint32_t *vals,
+                           std::unique_ptr<uint32_t[]> *sorted) {
+  int64_t cnt[65] = {0};
+  for (int64_t i = 0; i < n; i++) cnt[levels[i] < 64 ? levels[i] + 1 : 64]++;
+  for (int64_t i = 0; i < n; i++)
+    if (levels[i] >= 64) return false;
+  start[0] = 0;
+  for (int l = 0; l < 64; l++) start[l + 1] = start[l] + cnt[l + 1];
+  int64_t pos[64];
+  for (int l = 0; l < 64; l++) pos[l] = start[l];
+  if (order) {
+    order->reset(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
+    for (int64_t i = 0; i < n; i++) (*order)[pos[levels[i]]++] = (uint32_t)i;
+  } else {
+    sorted->reset(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
+    for (int64_t i = 0; i < n; i++) (*sorted)[pos[levels[i]]++] = vals[i];
+  }
+  return true;
+}
+
 int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t n,
                            const uint32_t *cdfs, int64_t A, uint32_t *words, int64_t cap) {
-  // groups by level, levels pushed 63..0, each group in reverse (encode_reverse)
-  std::vector<std::vector<int64_t>> idx(64);
-  for (int64_t i = 0; i < n; i++) {
-    if (levels[i] >= 64) {
-      set_error("scale level exceeds table count");
-      return LATEC_E_ARG;
-    }
-    idx[levels[i]].push_back(i);
+  // groups by level, levels pushed 63..0, each group in reverse (encode_reverse):
+  // the level-sorted symbols from last to first
+  if (n > (int64_t)UINT32_MAX) {
+    set_error("stage too large for the rANS coder");
+    return LATEC_E_ARG;
+  }
+  int64_t start[65];
+  std::unique_ptr<uint32_t[]> sorted;
+  if (!group_by_level(levels, n, start, nullptr, syms, &sorted)) {
+    set_error("scale level exceeds table count");
+    return LATEC_E_ARG;
   }
   std::vector<uint32_t> tmp(enc_cap(n));
   uint64_t state = 0;
   int64_t nw = 0;
   for (int l = 63; l >= 0; l--) {
-    if (idx[l].empty()) continue;
+    if (start[l + 1] == start[l]) continue;
     Model m;
     build_model(cdfs + (int64_t)l * (A + 1), A, m, false);
-    for (int64_t j = (int64_t)idx[l].size() - 1; j >= 0; j--) {
-      const uint32_t s = syms[idx[l][j]];
+    const SymInfo *si = m.sym.data();
+    for (int64_t j = start[l + 1] - 1; j >= start[l]; j--) {
+      const uint32_t s = sorted[j];
       if (s >= (uint64_t)A) {
         set_error("code outside the narrowed rANS alphabet");
         return LATEC_E_ARG;
       }
-      enc_put(state, m.sym[s], tmp.data(), nw);
+      enc_put(state, si[s], tmp.data(), nw);
     }
   }
   nw = enc_flush(state, tmp.data(), nw);
@@ -269,19 +305,21 @@ int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
     set_error("malformed rANS payload");
     return LATEC_E_FORMAT;
   }
-  std::vector<std::vector<int64_t>> idx(64);
-  for (int64_t i = 0; i < n; i++) {
-    if (levels[i] >= 64) {
-      set_error("scale level exceeds table count");
-      return LATEC_E_ARG;
-    }
-    idx[levels[i]].push_back(i);
+  if (n > (int64_t)UINT32_MAX) {
+    set_error("stage too large for the rANS coder");
+    return LATEC_E_ARG;
+  }
+  int64_t start[65];
+  std::unique_ptr<uint32_t[]> order;
+  if (!group_by_level(levels, n, start, &order, nullptr, nullptr)) {
+    set_error("scale level exceeds table count");
+    return LATEC_E_ARG;
   }
   for (int l = 0; l < 64; l++) {
-    if (idx[l].empty()) continue;
+    if (start[l + 1] == start[l]) continue;
     Model m;
     build_model(cdfs + (int64_t)l * (A + 1), A, m, true);
-    for (int64_t i : idx[l]) syms[i] = dec_get(state, m, words, top);
+    for (int64_t j = start[l]; j < start[l + 1]; j++) syms[order[j]] = dec_get(state, m, words, top);
   }
   if (top != 0 || state != 0) {
     set_error("trailing data in rANS payload");

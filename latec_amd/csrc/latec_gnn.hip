@@ -531,22 +531,21 @@ __device__ __forceinline__ typename Mf<T>::quad to_quad(f32x4 v) {
 }
 
 // GELU of the bf16 path: x Phi(x), Phi(x) = 1/2 + xc P(xc^2) with xc = x
-// clamped to [-4.5, 4.5] and P of degree 8 from a least-squares fit of the
-// normal CDF on [-4.5, 4.5] with Phi(+-4.5) pinned to 1 and 0.  Evaluated in
-// float32 the error is below 1.1e-4 absolute and 4.4e-4 relative over the
-// whole line (the result is rounded to bf16, half an ulp of which is 2e-3).
+// clamped to [-4, 4] and P of degree 7 from a least-squares fit of the normal
+// CDF on [-4, 4] with Phi(+-4) pinned to 1 and 0.  Evaluated in float32 the
+// error is below 1.5e-4 absolute and 5.7e-4 relative over the whole line
+// (the result is rounded to bf16, half an ulp of which is 2e-3).
 __device__ __forceinline__ float gelu_fast(float x) {
-  const float xc = __builtin_amdgcn_fmed3f(x, -4.5f, 4.5f);
+  const float xc = __builtin_amdgcn_fmed3f(x, -4.0f, 4.0f);
   const float t = xc * xc;
-  float p = 3.722450820e-11f;
-  p = __builtin_fmaf(p, t, -3.938617876e-09f);
-  p = __builtin_fmaf(p, t, 1.826758336e-07f);
-  p = __builtin_fmaf(p, t, -4.930116575e-06f);
-  p = __builtin_fmaf(p, t, 8.704744687e-05f);
-  p = __builtin_fmaf(p, t, -1.076017623e-03f);
-  p = __builtin_fmaf(p, t, 9.730072692e-03f);
-  p = __builtin_fmaf(p, t, -6.624759734e-02f);
-  p = __builtin_fmaf(p, t, 3.988744617e-01f);
+  float p = -1.491892165e-09f;
+  p = __builtin_fmaf(p, t, 1.169115293e-07f);
+  p = __builtin_fmaf(p, t, -3.999352884e-06f);
+  p = __builtin_fmaf(p, t, 7.960923540e-05f);
+  p = __builtin_fmaf(p, t, -1.042630989e-03f);
+  p = __builtin_fmaf(p, t, 9.651600383e-03f);
+  p = __builtin_fmaf(p, t, -6.616644561e-02f);
+  p = __builtin_fmaf(p, t, 3.988512158e-01f);
   return x * __builtin_fmaf(xc, p, 0.5f);
 }
 
@@ -593,15 +592,17 @@ struct RowArgs {
   const void *vemb;            // finalize: (M, D)
   const int64_t *qout;         // finalize: frame index of each finalized point
   float score_scale;
+  uint32_t kmagic;              // ceil(2^16 / K): n / K = (n * kmagic) >> 16 for n < 2^10
 };
 
 // Weights of one row MLP, as staged: W1 (H, K1X) with the first-layer input
-// in register order and NX k-steps of additive terms appended; W2 (D, H) and
-// the key's Wk (D, D) with their inputs in register order.
+// in register order and NX k-steps of additive terms appended; W2 (D, H + KS)
+// and the key's Wk (D, D + KS) with their inputs in register order and one
+// bias k-step appended (bf16: high and low parts in its first two columns).
 template <typename T>
 struct RowW {
   const T *w1, *w2, *wk;
-  const float *b2, *bk1, *wk2;
+  const float *wk2;
   float bk2;
 };
 
@@ -610,12 +611,13 @@ struct RowShape {
   typedef Mf<T> F;
   static constexpr int H = 2 * D;
   static constexpr int K1 = MODE == ROW_DIR ? D : 2 * D;
-  static constexpr int K1X = K1 + F::NX * F::KS;
-  static constexpr int P1 = K1X + F::PE, P2 = H + F::PE, PK = D + F::PE;
+  static constexpr int K1X = K1 + F::NX * F::KS;  // + additive terms
+  static constexpr int K2X = H + F::KS, K3X = D + F::KS;  // + one bias step
+  static constexpr int P1 = K1X + F::PE, P2 = K2X + F::PE, PK = K3X + F::PE;
   static constexpr bool KEY = MODE != ROW_FIN;
   static size_t smem(int L, int K) {
     return sizeof(T) * ((size_t)H * P1 + (size_t)D * P2 + (KEY ? (size_t)D * PK : 0)) +
-           sizeof(float) * 3 * D + (size_t)((L * K * 2 + 15) & ~15);
+           sizeof(float) * D + (size_t)((L * K * 2 + 15) & ~15);
   }
 };
 
@@ -630,6 +632,13 @@ struct RowIn {
   int64_t dst;  // output row (element offset / D); -1 if the row is past the end
 };
 
+// two floats -> two bf16 (round to nearest even) in one word, a in the low half
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
 // Additive-term k-step(s) of layer 1 (the B side; W1's matching columns hold
 // b1, the scalar coefficients and the state bias rows):
 //   bf16, one step of 32: group 0 [1, 1, z0h, z0h, z0l, z1h, z1h, z1l] against
@@ -641,36 +650,57 @@ template <typename T>
 __device__ __forceinline__ typename Mf<T>::frag extra_frag(float z0, float z1, int st, int g,
                                                            int t) {
   if constexpr (sizeof(T) == 2) {
-    const __bf16 z0h = (__bf16)z0, z1h = (__bf16)z1;
-    const __bf16 z0l = (__bf16)(z0 - (float)z0h), z1l = (__bf16)(z1 - (float)z1h);
-    const __bf16 one = (__bf16)1.0f, zero = (__bf16)0.0f;
-    bf16x8 v0 = {one, one, z0h, z0h, z0l, z1h, z1h, z1l};
-    bf16x8 v1;
-#pragma unroll
-    for (int i = 0; i < 8; i++) v1[i] = st == i ? one : zero;
-    bf16x8 vz = {zero, zero, zero, zero, zero, zero, zero, zero};
-    return g == 0 ? v0 : (g < 3 ? v1 : vz);
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const uint32_t h0 = pk_bf16(z0, z0), h1 = pk_bf16(z1, z1);
+    const float l0 = z0 - __builtin_bit_cast(float, h0 << 16);
+    const float l1 = z1 - __builtin_bit_cast(float, h1 << 16);
+    // words (element 0 in the low half): [1 1] [z0h z0h] [z0l z1h] [z1h z1l]
+    const uint32_t w2 = pk_bf16(l0, z1), w3 = pk_bf16(z1, l1);
+    const uint32_t one = 0x3F80u << ((st & 1) * 16), pr = (uint32_t)st >> 1;
+    u32x4 v;
+    if (g == 0) {
+      v = u32x4{0x3F803F80u, h0, w2, w3};
+    } else if (g < 3) {
+      v = u32x4{pr == 0 ? one : 0u, pr == 1 ? one : 0u, pr == 2 ? one : 0u, pr == 3 ? one : 0u};
+    } else {
+      v = u32x4{0u, 0u, 0u, 0u};
+    }
+    return __builtin_bit_cast(bf16x8, v);
   } else {
     const int e = 4 * t + g;
     return e == 0 ? 1.0f : e == 1 ? z0 : e == 2 ? z1 : (e - 3 == st ? 1.0f : 0.0f);
   }
 }
 
+// B side of a bias k-step: 1 at the two positions of group 0 (bf16: the bias'
+// high and low parts) or the first (f32), 0 elsewhere.
+template <typename T>
+__device__ __forceinline__ typename Mf<T>::frag bias_frag(int g) {
+  if constexpr (sizeof(T) == 2) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = {g == 0 ? 0x3F803F80u : 0u, 0u, 0u, 0u};
+    return __builtin_bit_cast(bf16x8, v);
+  } else {
+    return g == 0 ? 1.0f : 0.0f;
+  }
+}
+
 // Row MLP: for each row r (item i = r / K, axis k = r % K)
 //   h = gelu(W1 [x0 | x1] + b1 + S[st] + c0 z0 + c1 z1)   (H)
 //   y = W2 h + b2                                         (D)
-// messages also get score = (wk2 . gelu(Wk y + bk1) + bk2) * scale.
-// Each wave walks groups of NT 16-row tiles; the next group's inputs are
-// gathered while this group's second layer and key run.
-template <int D, typename T, int MODE, int NT>
-__global__ __launch_bounds__(256) void msg_kernel(RowArgs a, RowW<T> w) {
+// messages also get score = (wk2 . gelu(Wk y + bk1) + bk2) * scale.  The
+// biases ride on MFMA k-steps.  Each wave walks groups of NT 16-row tiles; the
+// next group's inputs are gathered while this group's epilogue and key run.
+template <int D, typename T, int MODE, int NT, int WV>
+__global__ __launch_bounds__(64 * WV) void msg_kernel(RowArgs a, RowW<T> w) {
   typedef RowShape<D, T, MODE> S;
   typedef Mf<T> F;
   typedef typename F::frag frag;
   typedef typename F::quad quad;
   constexpr bool BF = sizeof(T) == 2;
   constexpr int KS = F::KS, KL = F::KL, H = S::H, K1 = S::K1, K1X = S::K1X;
-  constexpr int NK1 = K1 / KS, NKX = F::NX, NK2 = H / KS, NK3 = D / KS;
+  constexpr int K2X = S::K2X, K3X = S::K3X;
+  constexpr int NK1 = K1 / KS, NKX = F::NX, NK3 = D / KS;
   constexpr int P1 = S::P1, P2 = S::P2, PK = S::PK;
   constexpr int NB1 = H / 16, NB2 = D / 16;
   constexpr bool KEY = S::KEY;
@@ -679,50 +709,48 @@ __global__ __launch_bounds__(256) void msg_kernel(RowArgs a, RowW<T> w) {
   T *sW1 = (T *)smem;
   T *sW2 = sW1 + H * P1;
   T *sWk = sW2 + D * P2;
-  float *tb2 = (float *)(sWk + (KEY ? D * PK : 0));
-  float *tbk1 = tb2 + D, *twk2 = tbk1 + D;
+  float *twk2 = (float *)(sWk + (KEY ? D * PK : 0));
   int8_t *tsx = (int8_t *)(twk2 + D);
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  for (int i = tid; i < H * K1X; i += nthr) sW1[(i / K1X) * P1 + i % K1X] = w.w1[i];
-  for (int i = tid; i < D * H; i += nthr) sW2[(i / H) * P2 + i % H] = w.w2[i];
-  if constexpr (KEY)
-    for (int i = tid; i < D * D; i += nthr) sWk[(i / D) * PK + i % D] = w.wk[i];
-  for (int i = tid; i < D; i += nthr) {
-    tb2[i] = w.b2[i];
-    if constexpr (KEY) {
-      tbk1[i] = w.bk1[i];
-      twk2[i] = w.wk2[i];
-    }
+  const int tid = threadIdx.x;
+  constexpr int NTHR = 64 * WV;
+  for (int i = tid; i < H * K1X; i += NTHR) sW1[(i / K1X) * P1 + i % K1X] = w.w1[i];
+  for (int i = tid; i < D * K2X; i += NTHR) sW2[(i / K2X) * P2 + i % K2X] = w.w2[i];
+  if constexpr (KEY) {
+    for (int i = tid; i < D * K3X; i += NTHR) sWk[(i / K3X) * PK + i % K3X] = w.wk[i];
+    for (int i = tid; i < D; i += NTHR) twk2[i] = w.wk2[i];
+    for (int i = tid; i < a.L * a.K * 2; i += NTHR) tsx[i] = a.sidx[i];
   }
-  if constexpr (KEY)
-    for (int i = tid; i < a.L * a.K * 2; i += nthr) tsx[i] = a.sidx[i];
   __syncthreads();
 
   const int lane = tid & 63, wv = tid >> 6, r = lane & 15, g = lane >> 4;
-  const int waves = nthr >> 6;
   const int K = a.K;
   const int64_t n_rows = MODE == ROW_FIN ? a.n_rows_host : (int64_t)a.n_items_dev[0] * K;
   const int64_t slot_base = a.slot_base_dev ? (int64_t)a.slot_base_dev[0] : a.slot_base;
   const T *E = (const T *)a.E;
   const T *CTX = (const T *)a.ctx;
   const T *VEMB = (const T *)a.vemb;
-  const int64_t gstep = (int64_t)gridDim.x * waves;
-  int64_t grp = (int64_t)xcd_block() * waves + wv;
+  const int64_t gstep = (int64_t)gridDim.x * WV;
+  int64_t grp = (int64_t)xcd_block() * WV + wv;
+  const frag bfr = bias_frag<T>(g);
 
   // gather the layer-1 inputs of group gi
   auto load = [&](int64_t gi, RowIn<T, NK1> (&in)[NT]) {
+    // row = base + o (o < ROWS): item = ib + (kb + o) / K, k = (kb + o) % K
+    const int64_t base = gi * ROWS;
+    const int32_t ib = (int32_t)(base / K), kb = (int32_t)(base - (int64_t)ib * K);
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      const int64_t row = gi * ROWS + 16 * t + r;
+      const int64_t row = base + 16 * t + r;
       const T *s0, *s1;
       float z0 = 0.f, z1 = 0.f;
       int st = 0;
       int64_t dst = -1;
       if (row < n_rows) {
-        const int32_t rw = (int32_t)row;
-        const int32_t item = rw / K, k = rw - item * K;
+        const int32_t o = kb + 16 * t + r;  // < K + ROWS
+        const int32_t qo = (int32_t)(((uint32_t)o * a.kmagic) >> 16);
+        const int32_t item = ib + qo, k = o - qo * K;
         if constexpr (MODE == ROW_FIN) {
-          s0 = CTX + (int64_t)rw * D;
+          s0 = CTX + row * D;
           s1 = VEMB + (int64_t)item * D;
           dst = a.qout[item] * K + k;
         } else {
@@ -766,7 +794,6 @@ __global__ __launch_bounds__(256) void msg_kernel(RowArgs a, RowW<T> w) {
   RowIn<T, NK1> in[NT];
   if (grp * ROWS < n_rows) load(grp, in);
   for (; grp * ROWS < n_rows; grp += gstep) {
-    // --- layer 1 (+ additive terms), GELU, kept as the next layer's B operand
     frag xe[NT][NKX];
     int64_t dst[NT];
 #pragma unroll
@@ -777,13 +804,18 @@ __global__ __launch_bounds__(256) void msg_kernel(RowArgs a, RowW<T> w) {
     }
     // layer-1 blocks in groups that form one layer-2 k-step group each (bf16:
     // two blocks = one step of 32; f32: one block = four steps of 4), so the
-    // hidden vector is consumed as it is made
+    // hidden vector is consumed as it is made; layer 2 starts from its bias step
     constexpr int GB = BF ? 2 : 1, SPG = BF ? 1 : 4;
     f32x4 acc2[NT][NB2];
 #pragma unroll
-    for (int t = 0; t < NT; t++)
+    for (int ob = 0; ob < NB2; ob++) {
+      const frag af = ldfrag<T>(sW2 + (16 * ob + r) * P2 + g * KL + H);
 #pragma unroll
-      for (int ob = 0; ob < NB2; ob++) acc2[t][ob] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < NT; t++) {
+        acc2[t][ob] = f32x4{0.f, 0.f, 0.f, 0.f};
+        mfma(acc2[t][ob], af, bfr);
+      }
+    }
 #pragma unroll 1
     for (int j = 0; j < NB1 / GB; j++) {
       quad hq[NT][GB];
@@ -820,18 +852,16 @@ __global__ __launch_bounds__(256) void msg_kernel(RowArgs a, RowW<T> w) {
     }
     // the next group's gathers fly during the epilogue and the key
     if ((grp + gstep) * ROWS < n_rows) load(grp + gstep, in);
-    // --- layer 2 epilogue (+ b2): the output rows
+    // --- the output rows
     quad mq[NT][NB2];
     T *out = (T *)a.out;
 #pragma unroll
-    for (int ob = 0; ob < NB2; ob++) {
-      const f32x4 b2 = *(const f32x4 *)(tb2 + 16 * ob + 4 * g);
+    for (int ob = 0; ob < NB2; ob++)
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        mq[t][ob] = to_quad<T>(acc2[t][ob] + b2);
+        mq[t][ob] = to_quad<T>(acc2[t][ob]);
         if (dst[t] >= 0) *(quad *)(out + dst[t] * D + 16 * ob + 4 * g) = mq[t][ob];
       }
-    }
     // --- attention key of each message row
     if constexpr (KEY) {
       float part[NT];
@@ -840,21 +870,26 @@ __global__ __launch_bounds__(256) void msg_kernel(RowArgs a, RowW<T> w) {
 #pragma unroll 1
       for (int ob = 0; ob < NB2; ob++) {
         f32x4 acc[NT];
-#pragma unroll
-        for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
         const T *wr = sWk + (16 * ob + r) * PK + g * KL;
+        {
+          const frag af = ldfrag<T>(wr + D);
+#pragma unroll
+          for (int t = 0; t < NT; t++) {
+            acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            mfma(acc[t], af, bfr);
+          }
+        }
 #pragma unroll
         for (int s = 0; s < NK3; s++) {
           const frag af = ldfrag<T>(wr + s * KS);
 #pragma unroll
           for (int t = 0; t < NT; t++) mfma(acc[t], af, bfrag<T, NB2>(mq[t], s));
         }
-        const f32x4 bk = *(const f32x4 *)(tbk1 + 16 * ob + 4 * g);
         const f32x4 wk = *(const f32x4 *)(twk2 + 16 * ob + 4 * g);
 #pragma unroll
         for (int t = 0; t < NT; t++)
 #pragma unroll
-          for (int v = 0; v < 4; v++) part[t] = __builtin_fmaf(wk[v], gelu_t<T>(acc[t][v] + bk[v]), part[t]);
+          for (int v = 0; v < 4; v++) part[t] = __builtin_fmaf(wk[v], gelu_t<T>(acc[t][v]), part[t]);
       }
 #pragma unroll
       for (int t = 0; t < NT; t++) {
@@ -1435,6 +1470,17 @@ int make_device(latec_gnn *g, const float *P) {
     return v;
   };
   auto hi = [](float x) { return from_bf16(to_bf16(x)); };
+  // (n_out, n_in + ks): W then one bias k-step (bf16: high and low parts)
+  const int ks = bf ? 32 : 4;
+  auto with_bias = [&](const std::vector<float> &W, int n_out, int n_in, const std::vector<float> &b) {
+    std::vector<float> v((size_t)n_out * (n_in + ks), 0.f);
+    for (int o = 0; o < n_out; o++) {
+      for (int c = 0; c < n_in; c++) v[(size_t)o * (n_in + ks) + c] = W[(size_t)o * n_in + c];
+      v[(size_t)o * (n_in + ks) + n_in] = bf ? hi(b[o]) : b[o];
+      if (bf) v[(size_t)o * (n_in + ks) + n_in + 1] = b[o] - hi(b[o]);
+    }
+    return v;
+  };
   // (h, K1 + nxc): W1 (register order) then the additive-term columns
   auto w1x = [&](const std::vector<float> &W1, int K1, const std::vector<float> &b1,
                  const std::vector<float> *c0, const std::vector<float> *c1,
@@ -1485,18 +1531,18 @@ int make_device(latec_gnn *g, const float *P) {
       s_bi[(size_t)st * h + o] = s0[(size_t)st * h + o] + s1[(size_t)(2 * agg - st) * h + o];
   const std::vector<float> bi_c0 = col(Ly.bi0w, h, bi_in, 2 * de), bi_c1 = col(Ly.bi0w, h, bi_in, 2 * de + 1);
   void *o_bi_w1 = mat(w1x(w_bi, 2 * d, vec(Ly.bi0b, h), &bi_c0, &bi_c1, &s_bi));
-  void *o_bi_w2 = mat(reg_in(rows(Ly.bi2w, d, h, 0, h), d, h, false, bf));
+  void *o_bi_w2 = mat(with_bias(reg_in(rows(Ly.bi2w, d, h, 0, h), d, h, false, bf), d, h, vec(Ly.bi2b, d)));
   // one-sided: E_side, scalars sign and log d
   const int dir_in = de + 2;
   const std::vector<double> s_dir = state_rows(Ly.dir0w, dir_in, d);
   const std::vector<float> dir_c0 = col(Ly.dir0w, h, dir_in, de), dir_c1 = col(Ly.dir0w, h, dir_in, de + 1);
   void *o_dir_w1 = mat(w1x(rows(Ly.dir0w, h, dir_in, 0, d), d, vec(Ly.dir0b, h), &dir_c0, &dir_c1, &s_dir));
-  void *o_dir_w2 = mat(reg_in(rows(Ly.dir2w, d, h, 0, h), d, h, false, bf));
+  void *o_dir_w2 = mat(with_bias(reg_in(rows(Ly.dir2w, d, h, 0, h), d, h, false, bf), d, h, vec(Ly.dir2b, d)));
   // mix: [ctx | value embedding]
   void *o_mx_w1 = mat(w1x(rows(Ly.mx0w, h, 2 * d, 0, 2 * d), 2 * d, vec(Ly.mx0b, h), nullptr, nullptr, nullptr));
-  void *o_mx_w2 = mat(reg_in(rows(Ly.mx2w, d, h, 0, h), d, h, false, bf));
+  void *o_mx_w2 = mat(with_bias(reg_in(rows(Ly.mx2w, d, h, 0, h), d, h, false, bf), d, h, vec(Ly.mx2b, d)));
   // line-pool key layer 1 on the message registers
-  void *o_key_w = mat(reg_in(rows(Ly.lp[0], d, d, 0, d), d, d, false, bf));
+  void *o_key_w = mat(with_bias(reg_in(rows(Ly.lp[0], d, d, 0, d), d, d, false, bf), d, d, vec(Ly.lp[1], d)));
   // InitEmbed layer 2, natural order
   void *o_init_w2 = mat(rows(Ly.init2w, d, d, 0, d));
   // f32 arrays
@@ -1582,56 +1628,91 @@ int ensure_work(latec_gnn *g, size_t bytes) {
   return rc;
 }
 
-// dynamic LDS attribute + workgroups per CU that the LDS allows (1..4)
-int lds_setup(const void *fn, size_t smem, int *per_cu) {
+// dynamic LDS attribute + resident workgroups per CU (1..8)
+int lds_setup(const void *fn, int threads, size_t smem, int *per_cu) {
   int rc = hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem),
                   "hipFuncSetAttribute");
-  const int n = (int)((size_t)160 * 1024 / (smem ? smem : 1));
-  *per_cu = n < 1 ? 1 : n > 4 ? 4 : n;
+  if (rc) return rc;
+  int n = 0;
+  rc = hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, smem),
+              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  *per_cu = n < 1 ? 1 : n > 8 ? 8 : n;
   return rc;
+}
+
+// launch shape of the message / mix rows (knob gnn_rows): 16-row tiles per
+// wave group and waves per workgroup
+int g_gnn_rows = 0;
+constexpr int GNN_ROW_SHAPES = 4;
+
+template <int D, typename T, int MODE, int NT, int WV>
+int launch_rows_s(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
+  const size_t smem = RowShape<D, T, MODE>::smem(a.L, a.K);
+  const void *fn = (const void *)msg_kernel<D, T, MODE, NT, WV>;
+  int per_cu = 1;
+  int rc = lds_setup(fn, 64 * WV, smem, &per_cu);
+  if (rc) return rc;
+  hipLaunchKernelGGL((msg_kernel<D, T, MODE, NT, WV>), dim3(g->n_cu * per_cu), dim3(64 * WV),
+                     smem, s, a, w);
+  return hip_ok(hipGetLastError(), "msg_kernel launch");
 }
 
 template <int D, typename T, int MODE>
 int launch_rows_m(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
-  constexpr int NT = sizeof(T) == 2 ? 2 : 1;
-  const size_t smem = RowShape<D, T, MODE>::smem(a.L, a.K);
-  const void *fn = (const void *)msg_kernel<D, T, MODE, NT>;
-  int per_cu = 1;
-  int rc = lds_setup(fn, smem, &per_cu);
-  if (rc) return rc;
-  hipLaunchKernelGGL((msg_kernel<D, T, MODE, NT>), dim3(g->n_cu * per_cu), dim3(256), smem, s,
-                     a, w);
-  return hip_ok(hipGetLastError(), "msg_kernel launch");
+  if (sizeof(T) == 4) return launch_rows_s<D, T, MODE, 1, 4>(g, a, w, s);
+  switch (g_gnn_rows) {
+    case 1: return launch_rows_s<D, T, MODE, 1, 4>(g, a, w, s);
+    case 2: return launch_rows_s<D, T, MODE, 1, 8>(g, a, w, s);
+    case 3: return launch_rows_s<D, T, MODE, 2, 8>(g, a, w, s);
+    default: return launch_rows_s<D, T, MODE, 2, 4>(g, a, w, s);
+  }
 }
 
 template <int D, typename T>
 int launch_rows(latec_gnn *g, int mode, RowArgs &a, hipStream_t s) {
   RowW<T> w{};
   w.wk = (const T *)g->key_w;
-  w.bk1 = fs(g, F_LK_B1);
   w.wk2 = fs(g, F_LK_W2);
   w.bk2 = g->scal[0];
   a.L = g->L;
   a.sidx = g->d_sidx;
+  a.kmagic = (65536u + (uint32_t)a.K - 1) / (uint32_t)a.K;
   if (mode == ROW_BI) {
     w.w1 = (const T *)g->bi_w1;
     w.w2 = (const T *)g->bi_w2;
-    w.b2 = fs(g, F_BI_B2);
     return launch_rows_m<D, T, ROW_BI>(g, a, w, s);
   }
   if (mode == ROW_DIR) {
     w.w1 = (const T *)g->dir_w1;
     w.w2 = (const T *)g->dir_w2;
-    w.b2 = fs(g, F_DIR_B2);
     return launch_rows_m<D, T, ROW_DIR>(g, a, w, s);
   }
   w.w1 = (const T *)g->mx_w1;
   w.w2 = (const T *)g->mx_w2;
-  w.b2 = fs(g, F_MX_B2);
   return launch_rows_m<D, T, ROW_FIN>(g, a, w, s);
 }
 
 }  // namespace
+
+namespace latec {
+int gnn_tune_set(const char *name, int64_t v) {
+  if (!strcmp(name, "gnn_rows")) {
+    if (v < 0 || v >= GNN_ROW_SHAPES) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_gnn_rows = (int)v;
+    return LATEC_OK;
+  }
+  set_error(std::string("unknown tuning knob ") + name);
+  return LATEC_E_ARG;
+}
+int64_t gnn_tune_get(const char *name) {
+  if (!strcmp(name, "gnn_rows")) return g_gnn_rows;
+  set_error(std::string("unknown tuning knob ") + name);
+  return LATEC_E_ARG;
+}
+}  // namespace latec
 
 namespace {
 
@@ -1731,7 +1812,7 @@ int embed_t(latec_gnn *g, const void *E, const int64_t *frame_shape, const int32
   const PoolArgs pa{fs(g, F_LK_WV), fs(g, F_LK_BV), fs(g, F_LK_NULLV), g->scal[1] * scale};
   const size_t psm = pool_smem<D>();
   int pool_per_cu = 1;
-  rc = lds_setup((const void *)pool_kernel<D, T>, psm, &pool_per_cu);
+  rc = lds_setup((const void *)pool_kernel<D, T>, 64 * POOL_WAVES, psm, &pool_per_cu);
   if (rc) return rc;
   for (int64_t m0 = 0; m0 < M; m0 += MT) {
     const int64_t mt = M - m0 < MT ? M - m0 : MT;
@@ -1804,7 +1885,7 @@ int head_t(latec_gnn *g, const void *ctx, int64_t M, double eb, float vmin, floa
   h.b2 = fs(g, F_HD_B2);
   const size_t smem = head_smem<D>();
   int per_cu = 1;
-  int rc = lds_setup((const void *)head2_kernel<D, T>, smem, &per_cu);
+  int rc = lds_setup((const void *)head2_kernel<D, T>, 64 * HEAD_WAVES, smem, &per_cu);
   if (rc) return rc;
   const int64_t waves = (M + 15) / 16;
   int64_t grid = (waves + HEAD_WAVES - 1) / HEAD_WAVES;
@@ -1820,7 +1901,7 @@ int init_embed(latec_gnn *g, int64_t M, const float *recon, const int64_t *gidx,
   const InitW<T> w{fs(g, F_IN_W0), fs(g, F_IN_B0), (const T *)g->init_w2, fs(g, F_IN_B2)};
   const size_t smem = init_smem<D, T>();
   int per_cu = 1;
-  int rc = lds_setup((const void *)init_embed_kernel<D, T>, smem, &per_cu);
+  int rc = lds_setup((const void *)init_embed_kernel<D, T>, 256, smem, &per_cu);
   if (rc) return rc;
   int64_t grid = (M + 63) / 64;
   if (grid > (int64_t)g->n_cu * per_cu) grid = (int64_t)g->n_cu * per_cu;

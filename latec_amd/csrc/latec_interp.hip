@@ -3114,6 +3114,8 @@ static int64_t tune_epoch() { return g_tune_epoch; }
 // count once, when recorded)
 static std::atomic<int64_t> g_quad_launches{0};
 
+int gnn_tune_set(const char *name, int64_t v);  // latec_gnn.hip
+int64_t gnn_tune_get(const char *name);
 int tune_set(const char *name, int64_t v) {
   if (!strcmp(name, "quad_launches")) {
     g_quad_launches = v;
@@ -3184,6 +3186,7 @@ int tune_set(const char *name, int64_t v) {
     g_march_peel = v;
     return LATEC_OK;
   }
+  if (!strncmp(name, "gnn_", 4)) return gnn_tune_set(name, v);
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -3198,6 +3201,7 @@ int64_t tune_get(const char *name) {
   if (!strcmp(name, "pair_quads")) return g_pair_quads;
   if (!strcmp(name, "quad_launches")) return g_quad_launches;
   if (!strcmp(name, "pair_waves")) return g_pair_waves;
+  if (!strncmp(name, "gnn_", 4)) return gnn_tune_get(name);
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }

@@ -338,24 +338,30 @@ def choose_shape(codes, levels64, tables: RansTables, kexp: int, *,
     codes = np.asarray(codes, np.uint32).ravel()
     if len(codes) == 0:
         return 0
-    syms = _narrow(codes, tables.radius, kexp)
-    levels = np.asarray(levels64, np.uint8).ravel().astype(np.int64)
-    if syms.shape != levels.shape:
+    levels = np.asarray(levels64, np.uint8).ravel()
+    if codes.shape != levels.shape:
         raise ValueError("codes and levels must have the same flattened length")
+    # the stride sample first (narrowing is elementwise, so this is the
+    # reference's sample of the narrowed symbols)
     if sample < 1.0:
-        step = max(1, len(syms) // max(_SELECT_MIN, int(len(syms) * sample)))
+        step = max(1, len(codes) // max(_SELECT_MIN, int(len(codes) * sample)))
         if step > 1:
-            syms, levels = syms[::step], levels[::step]
+            codes, levels = codes[::step], levels[::step]
+    syms = _narrow(codes, tables.radius, kexp)
+    levels = levels.astype(np.int64)
     weights = _level_weights(int(kexp))
     floor = 1.0 / tables.total
     idx = np.arange(N_LEVELS)
+    p0 = weights[levels, syms]
+    broad_of = {}  # level offset -> gathered broad weights (shared by shapes)
     best_id, best_cost = 0, np.inf
     for shape_id in _shape_ids(int(kexp)):
         d, w = SHAPES[shape_id]
-        p = weights[levels, syms]
+        p = p0
         if w:
-            broad = weights[np.minimum(idx + d, N_LEVELS - 1)[levels], syms]
-            p = (1.0 - w) * p + w * broad
+            if d not in broad_of:
+                broad_of[d] = weights[np.minimum(idx + d, N_LEVELS - 1)[levels], syms]
+            p = (1.0 - w) * p + w * broad_of[d]
         cost = -np.log2(np.maximum(p, floor)).sum()
         if cost < best_cost:
             best_id, best_cost = shape_id, float(cost)

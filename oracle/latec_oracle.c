@@ -691,6 +691,43 @@ int64_t orc_rans_encode(const uint32_t *syms, int64_t n, const uint32_t *cdf,
   return nw;
 }
 
+/* Per-point levels (latec/rans.py:546-578): one stack coder; for level
+ * 63, 62, ..., 0 the symbols of that level, in position order, are pushed with
+ * encode_reverse (last first) under that level's CDF row.  Returns the word
+ * count or -1. */
+int64_t orc_rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t n,
+                               const uint32_t *cdfs, int64_t A, uint32_t *words,
+                               int64_t cap) {
+  uint32_t *left = (uint32_t *)malloc(sizeof(uint32_t) * (A + 1));
+  uint64_t state = 0;
+  int64_t nw = 0;
+  for (int level = 63; level >= 0; level--) {
+    categorical_fast(cdfs + (int64_t)level * (A + 1), A, left);
+    for (int64_t i = n - 1; i >= 0; i--) {
+      if (levels[i] != level) continue;
+      uint32_t s = syms[i];
+      uint64_t lo = left[s], pr = left[s + 1] - left[s];
+      if ((state >> (64 - PRECISION)) >= pr) {
+        if (nw >= cap) {
+          free(left);
+          return -1;
+        }
+        words[nw++] = (uint32_t)state;
+        state >>= 32;
+      }
+      uint64_t pre = state / pr, rem = state % pr;
+      state = (pre << PRECISION) | (lo + rem);
+    }
+  }
+  free(left);
+  if (state) {
+    if (nw + 2 > cap) return -1;
+    words[nw++] = (uint32_t)state;
+    if (state >> 32) words[nw++] = (uint32_t)(state >> 32);
+  }
+  return nw;
+}
+
 /* Returns 0 ok, -1 trailing data / malformed. */
 int orc_rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf,
                     int64_t A, uint32_t *syms, int64_t n) {

@@ -64,6 +64,7 @@ def lib():
             "orc_choose_const_level": (C.c_float, [p, i64, i32, i32, p, p, p]),
             "orc_scale_to_level": (None, [p, i64, f64, p]),
             "orc_rans_encode": (i64, [p, i64, p, i64, p, i64]),
+            "orc_rans_encode_levels": (i64, [p, p, i64, p, i64, p, i64]),
             "orc_rans_decode": (i32, [p, i64, p, i64, p, i64]),
             "orc_pack_stage": (i64, [p, i64, p, i64, i64, i32, i32, p, i64]),
             "orc_unpack_stage": (i64, [p, i64, i64, i32, p, i64, p, i64, p, p]),
@@ -226,6 +227,20 @@ def rans_encode_syms(syms, cdf):
     cap = 2 * len(syms) + 8
     words = np.empty(cap, np.uint32)
     nw = lib().orc_rans_encode(_p(syms), len(syms), _p(cdf), len(cdf) - 1, _p(words), cap)
+    if nw < 0:
+        raise ValueError("rANS capacity")
+    return words[:nw].copy()
+
+
+def rans_encode_levels(syms, levels, cdfs):
+    """Per-point levels: cdfs is (64, A + 1)."""
+    syms = np.ascontiguousarray(syms, np.uint32)
+    levels = np.ascontiguousarray(levels, np.uint8)
+    cdfs = np.ascontiguousarray(cdfs, np.uint32)
+    cap = 2 * len(syms) + 8
+    words = np.empty(cap, np.uint32)
+    nw = lib().orc_rans_encode_levels(_p(syms), _p(levels), len(syms), _p(cdfs),
+                                      cdfs.shape[1] - 1, _p(words), cap)
     if nw < 0:
         raise ValueError("rANS capacity")
     return words[:nw].copy()

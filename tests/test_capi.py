@@ -45,6 +45,39 @@ def test_rans_words_match_oracle():
         assert np.array_equal(out, syms)
 
 
+@pytest.mark.parametrize("kexp,n,nlev", [(1, 7, 3), (4, 5000, 64), (7, 40000, 12), (10, 20000, 40)])
+def test_rans_levels_words_match_oracle(kexp, n, nlev):
+    """Per-point levels (the GNN stages' coder): the library's words equal the
+    oracle's level loop, decode back, and with one model for every level equal
+    the single-model coder on the symbols taken level by level (stable)."""
+    rng = np.random.RandomState(kexp + n)
+    cdfs = O.build_cdfs(kexp, 2, 24, 1)
+    A = cdfs.shape[1] - 1
+    levels = rng.randint(0, 64, n).astype(np.uint8) % nlev * (64 // nlev)
+    syms = np.empty(n, np.uint32)
+    for l in np.unique(levels):
+        pmf = np.diff(cdfs[l].astype(np.int64))
+        sel = levels == l
+        syms[sel] = rng.choice(A, int(sel.sum()), p=pmf / pmf.sum())
+    ref = O.rans_encode_levels(syms, levels, cdfs)
+    words = np.empty(n + 2, np.uint32)
+    nw = _lib.check(_lib.lib().latec_rans_encode_levels(
+        _lib.ptr(syms), _lib.ptr(levels), n, _lib.ptr(cdfs), A, _lib.ptr(words), len(words)))
+    assert np.array_equal(words[:nw], ref)
+    out = np.empty(n, np.uint32)
+    _lib.check(_lib.lib().latec_rans_decode_levels(_lib.ptr(ref), len(ref), _lib.ptr(levels),
+                                                   _lib.ptr(cdfs), A, _lib.ptr(out), n))
+    assert np.array_equal(out, syms)
+    one = np.ascontiguousarray(np.repeat(cdfs[20:21], 64, axis=0))
+    ok = syms < A
+    same = O.rans_encode_levels(syms, levels, one)
+    order = np.argsort(levels, kind="stable")
+    assert np.array_equal(same, O.rans_encode_syms(syms[order], one[0])), "grouping order"
+    nw = _lib.check(_lib.lib().latec_rans_encode_levels(
+        _lib.ptr(syms), _lib.ptr(levels), n, _lib.ptr(one), A, _lib.ptr(words), len(words)))
+    assert ok.all() and np.array_equal(words[:nw], same)
+
+
 def test_rans_trailing_data_rejected():
     cdf = O.build_cdfs(2, 0)[10]
     w = O.rans_encode_syms(np.array([1, 2, 3], np.uint32), cdf)

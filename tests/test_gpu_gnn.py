@@ -289,3 +289,30 @@ def test_gate_costs_match_oracle_on_real_stages(monkeypatch, shape):
     y = codec.uncompress(stream).numpy()
     assert np.abs(y - x).max() <= 1e-3 * float(x.max() - x.min()) + GC.roundtrip_slack(
         x.min(), x.max())
+
+
+@pytest.mark.parametrize("numerics", ["bf16", "fp32"])
+def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
+    """Every launch shape of the message / mix rows (knob gnn_rows: 16-row
+    tiles per wave group, waves per workgroup) codes the same stream and
+    decodes it to the same values: a row's result does not depend on the
+    tile it shares."""
+    from latec_amd import _lib
+    from latec_amd.gnn_codec import GNNCodec
+
+    ckpt = _ckpt(tmp_path, 64, 2, 5)
+    x = _field((18, 20, 17, 19), 9)
+    codec = GNNCodec(ckpt, numerics=numerics)
+    L = _lib.lib()
+    base = L.latec_tune_get(b"gnn_rows")
+    streams, decoded = [], []
+    try:
+        for v in range(4):
+            assert L.latec_tune_set(b"gnn_rows", v) == 0
+            streams.append(codec.compress(x, 1e-3))
+            decoded.append(codec.uncompress(streams[0]).numpy())
+    finally:
+        L.latec_tune_set(b"gnn_rows", base)
+    assert all(s == streams[0] for s in streams[1:])
+    assert all(np.array_equal(d, decoded[0]) for d in decoded[1:])
+    assert L.latec_tune_set(b"gnn_rows", 4) != 0
