@@ -395,6 +395,16 @@ int latec_gnn_halo(latec_gnn *g, void *d_E, const int64_t *d_rows,
                    const int64_t *d_gidx, const float *d_recon, int64_t M,
                    float vmin, float vmax, void *stream);
 
+/* All halo rows of a chunk frame at once (_wave_fill_halo,
+ * gnn_predictor.py:1716-1735, with the usable cells of chunk_halo_info
+ * :1338-1356): every frame cell outside the chunk (`p`: the chunk's local
+ * plan) that lies inside the tensor and in a coded chunk or on the anchor
+ * lattice gets E[row, k] = init(clamp(recon[cell])) for every axis k.  No-op
+ * for a frame without a halo. */
+int latec_gnn_halo_frame(latec_gnn *g, latec_plan *p, const latec_gnn_frame *fr,
+                         void *d_E, const float *d_recon, float vmin, float vmax,
+                         void *stream);
+
 /* Gate rate proxy (_laplace_bits_t + the bucket sums of _gate_select_t,
  * gnn_codec.py:777-797, 920-971): costs[c * 13 + b] = summed bits of
  * candidate c in scale bucket b (c = 0: the model; c = 1 + 4 * kind + s:

@@ -113,6 +113,7 @@ SIGNATURES = {
     "latec_gnn_head": (i32, [p, p, i64, f64, f32, f32, p, p, p, p, p]),
     "latec_gnn_finalize": (i32, [p, p, p, p, p, p, i64, f32, f32, p]),
     "latec_gnn_halo": (i32, [p, p, p, p, p, i64, f32, f32, p]),
+    "latec_gnn_halo_frame": (i32, [p, p, p, p, p, f32, f32, p]),
     "latec_gnn_gate_costs": (i32, [p, p, p, p, p, i32, i64, f64, u32, p, p, p, p]),
     "latec_gnn_gate_apply": (i32, [p, p, p, i64, f64, f32, i32, p]),
     "latec_gather_f32": (i32, [p, p, i64, p, p]),

@@ -313,6 +313,63 @@ __global__ void geometry_kernel(const ClassDesc *__restrict__ cls, GeomArgs g,
 // live-pair compaction (deterministic, pair order)
 // ---------------------------------------------------------------------------
 
+// Halo cells of a chunk frame (the frame is the chunk grown by `pad` cells on
+// every side): one thread per frame cell; a cell outside the chunk is usable
+// when it lies inside the tensor and in a coded chunk or on the anchor
+// lattice (gnn_predictor.py:1338-1356).  Usable cells are appended to
+// (frame row, tensor index) lists in any order: each row is written once.
+struct HaloArgs {
+  int32_t ndim, stride;
+  int64_t pad, n_frame;
+  int64_t fshape[LATEC_MAX_DIM], cshape[LATEC_MAX_DIM], origin[LATEC_MAX_DIM];
+  int64_t tshape[LATEC_MAX_DIM], edges[LATEC_MAX_DIM], grid[LATEC_MAX_DIM];
+  const uint8_t *coded;
+};
+
+__global__ __launch_bounds__(256) void halo_list_kernel(HaloArgs h, int64_t *__restrict__ frows,
+                                                         int64_t *__restrict__ gflat,
+                                                         uint32_t *__restrict__ count) {
+  const int64_t gstep = (int64_t)gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < h.n_frame; i0 += gstep) {
+    const int64_t i = i0 + threadIdx.x;
+    bool ok = i < h.n_frame;
+    int64_t r = ok ? i : 0, gf = 0, cid = 0, tstr = 1;
+    bool inside = true, anchor = true;
+    int64_t gc[LATEC_MAX_DIM];
+    for (int j = h.ndim - 1; j >= 0; j--) {
+      const int64_t c = r % h.fshape[j] - h.pad;
+      r /= h.fshape[j];
+      inside &= c >= 0 && c < h.cshape[j];
+      gc[j] = c + h.origin[j];
+      ok &= gc[j] >= 0 && gc[j] < h.tshape[j];
+    }
+    ok &= !inside;
+    if (ok) {
+      for (int j = 0; j < h.ndim; j++) {
+        cid = cid * h.grid[j] + gc[j] / h.edges[j];
+        anchor &= gc[j] % h.stride == 0;
+      }
+      for (int j = h.ndim - 1; j >= 0; j--) {
+        gf += gc[j] * tstr;
+        tstr *= h.tshape[j];
+      }
+      ok = anchor || h.coded[cid];
+    }
+    const uint64_t m = __ballot(ok);
+    if (!m) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (ok) {
+      const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      frows[k] = i;
+      gflat[k] = gf;
+    }
+  }
+}
+
 constexpr int PAIR_BLOCK = 1024;
 
 __device__ __forceinline__ int pair_kind(int32_t st) {
@@ -2129,6 +2186,57 @@ int latec_gnn_halo(latec_gnn *g, void *d_E, const int64_t *d_rows, const int64_t
   if (M == 0) return LATEC_OK;
   hipStream_t s = (hipStream_t)stream;
   return GNN_DISPATCH(halo_t, d_E, d_rows, d_gidx, d_recon, M, vmin, vmax, s);
+}
+
+int latec_gnn_halo_frame(latec_gnn *g, latec_plan *p, const latec_gnn_frame *fr, void *d_E,
+                         const float *d_recon, float vmin, float vmax, void *stream) {
+  if (!g || !p || !fr || !d_E || !d_recon || !(vmax > vmin)) {
+    latec::set_error("latec_gnn_halo_frame: bad argument");
+    return LATEC_E_ARG;
+  }
+  if (p->ndim != g->ndim || p->block != 1 || (1 << p->levels) != p->stride) {
+    latec::set_error("GNN halo needs a block-1 plan of the model's rank with stride 2^levels");
+    return LATEC_E_ARG;
+  }
+  if (!fr->has_halo || fr->pad <= 0) return LATEC_OK;
+  if (!fr->d_coded) {
+    latec::set_error("a halo frame needs the coded-chunk flags");
+    return LATEC_E_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  HaloArgs h{};
+  h.ndim = p->ndim;
+  h.stride = p->stride;
+  h.pad = fr->pad;
+  h.n_frame = 1;
+  for (int j = 0; j < p->ndim; j++) {
+    h.cshape[j] = p->shape[j];
+    h.fshape[j] = p->shape[j] + 2 * (int64_t)fr->pad;
+    h.n_frame *= h.fshape[j];
+    h.origin[j] = fr->origin[j];
+    h.tshape[j] = fr->tshape[j];
+    h.edges[j] = fr->edges[j] > 0 ? fr->edges[j] : 1;
+    h.grid[j] = fr->grid[j] > 0 ? fr->grid[j] : 1;
+  }
+  h.coded = fr->d_coded;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t bl = al(sizeof(int64_t) * (size_t)h.n_frame);
+  int rc = ensure_work(g, 2 * bl + 256);
+  if (rc) return rc;
+  int64_t *frows = (int64_t *)g->work;
+  int64_t *gflat = (int64_t *)((unsigned char *)g->work + bl);
+  uint32_t *cnt = (uint32_t *)((unsigned char *)g->work + 2 * bl);
+  hipLaunchKernelGGL((zero_kernel<uint32_t>), dim3(1), dim3(64), 0, s, (int64_t)1, cnt);
+  int64_t blocks = (h.n_frame + 255) / 256;
+  if (blocks > (int64_t)g->n_cu * 16) blocks = (int64_t)g->n_cu * 16;
+  hipLaunchKernelGGL(halo_list_kernel, dim3((unsigned)blocks), dim3(256), 0, s, h, frows, gflat,
+                     cnt);
+  rc = hip_ok(hipGetLastError(), "halo_list_kernel launch");
+  uint32_t n = 0;
+  if (!rc) rc = hip_ok(hipMemcpyAsync(&n, cnt, sizeof(n), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (rc || n == 0) return rc;
+  return GNN_DISPATCH(halo_t, d_E, frows, gflat, d_recon, (int64_t)n, vmin, vmax, s);
 }
 
 int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,

@@ -1436,12 +1436,21 @@ __device__ __forceinline__ double pair_combine(const double *v, int nodd, int ce
   return center == 1 ? v[0] : v[nodd - 1];
 }
 
-// One stage's quantizer constants.
+// One stage's quantizer constants.  hlo = 0.5 - 1e-12 (R + 1): a product
+// t = d / w rounded through 1/w whose distance to the nearest integer is
+// below hlo rounds like the division (the margin is the general rule's
+// 1e-12 max(|t|, 1) at its largest |t| that can give an in-range code; a
+// |t| above R + 1 is out of range either way), so the fallback needs one
+// compare.
 struct PQ {
   double w, inv_w, Rd;
   float feb;
   uint32_t R;
+  double hlo;
 };
+__device__ __forceinline__ PQ make_pq(const L3Stage &S) {
+  return PQ{S.w, S.inv_w, (double)S.R, S.feb, S.R, 0.5 - 1e-12 * ((double)S.R + 1.0)};
+}
 
 // quantize_one (encode): code and reconstruction of x against prediction p.
 // Every lane of the wave calls it (the exact-division fallback is a vote).
@@ -1452,13 +1461,14 @@ __device__ __forceinline__ void pair_quant(const PQ &Q, int round_mode, double s
   const double d = (double)xv - pd;
   const double tq = d * Q.inv_w;
   double qd = rint(tq);
-  const bool fb = !(0.5 - fabs(tq - qd) > 1e-12 * fmax(fabs(tq), 1.0));
+  const bool fb = !(fabs(tq - qd) < Q.hlo);
   if (__any(fb)) {
     if (fb) qd = rint(d / Q.w);
   }
   const bool inr = fabs(qd) < Q.Rd;
-  const int qi = inr ? (int)qd : 0;
-  const float cand = (float)(pd + Q.w * (double)qi);
+  const int qi = (int)qd;  // used only when in range
+  // qd + 0.0: the integer's double (+0 for a -0 quotient), as (double)qi
+  const float cand = (float)(pd + Q.w * (qd + 0.0));
   float chk = cand;
   if (RND) {
     if (round_mode == 1)
@@ -1480,16 +1490,16 @@ __device__ __forceinline__ void pair_quant2(const PQ &Q, int round_mode, double 
   const double d0 = (double)x0 - pd0, d1 = (double)x1 - pd1;
   const double t0 = d0 * Q.inv_w, t1 = d1 * Q.inv_w;
   double q0 = rint(t0), q1 = rint(t1);
-  const bool fb0 = !(0.5 - fabs(t0 - q0) > 1e-12 * fmax(fabs(t0), 1.0));
-  const bool fb1 = !(0.5 - fabs(t1 - q1) > 1e-12 * fmax(fabs(t1), 1.0));
+  const bool fb0 = !(fabs(t0 - q0) < Q.hlo);
+  const bool fb1 = !(fabs(t1 - q1) < Q.hlo);
   if (__any(fb0 || fb1)) {
     if (fb0) q0 = rint(d0 / Q.w);
     if (fb1) q1 = rint(d1 / Q.w);
   }
   auto finish = [&](double pd, float xv, double qd, uint32_t &code, float &r) {
     const bool inr = fabs(qd) < Q.Rd;
-    const int qi = inr ? (int)qd : 0;
-    const float cand = (float)(pd + Q.w * (double)qi);
+    const int qi = (int)qd;  // used only when in range
+    const float cand = (float)(pd + Q.w * (qd + 0.0));
     float chk = cand;
     if (RND) {
       if (round_mode == 1)
@@ -1619,8 +1629,8 @@ __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
   const int cubic = A.cubic, end_mode = A.end_mode, center = A.center;
   const int round_mode = A.round_mode;
   const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
-  const PQ QE = {SE.w, SE.inv_w, (double)SE.R, SE.feb, SE.R};
-  const PQ QO = {SO.w, SO.inv_w, (double)SO.R, SO.feb, SO.R};
+  const PQ QE = make_pq(SE);
+  const PQ QO = make_pq(SO);
   const uint32_t wlo_e = SE.R > HIST_WIN / 2 ? SE.R - HIST_WIN / 2 : 0;
   const uint32_t wlo_o = SO.R > HIST_WIN / 2 ? SO.R - HIST_WIN / 2 : 0;
   uint32_t *gh_e = ENC ? hist + (int64_t)SE.gs * A.hist_words : nullptr;
@@ -1987,8 +1997,8 @@ __device__ __forceinline__ void quad_set(const PArgs &A, const PSet &T,
   const int cubic = A.cubic, end_mode = A.end_mode, center = A.center;
   const int round_mode = A.round_mode;
   const L3Stage &SE = A.st[T.ev.stage], &SO = A.st[T.od.stage];
-  const PQ QE = {SE.w, SE.inv_w, (double)SE.R, SE.feb, SE.R};
-  const PQ QO = {SO.w, SO.inv_w, (double)SO.R, SO.feb, SO.R};
+  const PQ QE = make_pq(SE);
+  const PQ QO = make_pq(SO);
   const uint32_t wlo_e = SE.R > HIST_WIN / 2 ? SE.R - HIST_WIN / 2 : 0;
   const uint32_t wlo_o = SO.R > HIST_WIN / 2 ? SO.R - HIST_WIN / 2 : 0;
   uint32_t *gh_e = ENC ? hist + (int64_t)SE.gs * A.hist_words : nullptr;

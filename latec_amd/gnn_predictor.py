@@ -200,25 +200,6 @@ class ChunkedGNNPredictor:
         fr.d_coded = ptr(self._coded_dev).value if pad else None
         return fr
 
-    def _halo_cells(self, origin, cshape, pad):
-        """Frame rows and tensor indices of the usable halo cells: inside the
-        tensor, outside the chunk, in a coded chunk or on the anchor lattice."""
-        nd = len(self.shape)
-        fshape = tuple(n + 2 * pad for n in cshape)
-        idx = np.indices(fshape).reshape(nd, -1).T - pad
-        inside = np.all((idx >= 0) & (idx < np.asarray(cshape)), axis=1)
-        g = idx + np.asarray(origin)
-        intensor = np.all((g >= 0) & (g < np.asarray(self.shape)), axis=1)
-        keep = ~inside & intensor
-        idx, g = idx[keep], g[keep]
-        cid = np.ravel_multi_index(tuple((g // np.asarray(self.edges)).T), self.grid)
-        anchor = np.all(g % self.anchor_stride == 0, axis=1)
-        ok = self.coded[cid] | anchor
-        idx, g = idx[ok], g[ok]
-        frows = np.ravel_multi_index(tuple((idx + pad).T), fshape).astype(np.int64)
-        gflat = np.ravel_multi_index(tuple(g.T), self.shape).astype(np.int64)
-        return frows, gflat
-
     def start_wave(self, chunk_ids, recon):
         from .predictor import get_plan
 
@@ -243,12 +224,8 @@ class ChunkedGNNPredictor:
             if not pad:
                 self._geom_cache = {key: geoms}
         if pad:
-            frows, gflat = self._halo_cells(origin, cshape, pad)
-            if len(frows):
-                dr, dg = D.to_device(frows), D.to_device(gflat)
-                check(lib().latec_gnn_halo(m.handle, ptr(E), ptr(dr), ptr(dg), ptr(recon),
-                                           len(frows), 0.0 + self.vmin, self.vmax, ptr(None)))
-                T.cuda.synchronize()
+            check(lib().latec_gnn_halo_frame(m.handle, plan.handle, ctypes.byref(frame), ptr(E),
+                                             ptr(recon), 0.0 + self.vmin, self.vmax, ptr(None)))
         self._wave = {
             "ci": ci, "plan": plan, "E": E, "geoms": geoms, "fshape": np.asarray(fshape, np.int64),
             "chain": [0] + [s for s in range(1, plan.n_stages) if plan.counts[s]],

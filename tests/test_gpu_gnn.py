@@ -241,6 +241,65 @@ def test_multichunk_halo_geometry_matches_scan(tmp_path, shape, levels, edges):
         pr._wave = None
 
 
+def _host_halo_cells(pr, origin, cshape, pad):
+    """Frame rows and tensor indices of the usable halo cells, enumerated on
+    the host: outside the chunk, inside the tensor, in a coded chunk or on
+    the anchor lattice (gnn_predictor.py:1338-1356)."""
+    nd = len(pr.shape)
+    fshape = tuple(n + 2 * pad for n in cshape)
+    idx = np.indices(fshape).reshape(nd, -1).T - pad
+    inside = np.all((idx >= 0) & (idx < np.asarray(cshape)), axis=1)
+    g = idx + np.asarray(origin)
+    intensor = np.all((g >= 0) & (g < np.asarray(pr.shape)), axis=1)
+    keep = ~inside & intensor
+    idx, g = idx[keep], g[keep]
+    cid = np.ravel_multi_index(tuple((g // np.asarray(pr.edges)).T), pr.grid)
+    ok = pr.coded[cid] | np.all(g % pr.anchor_stride == 0, axis=1)
+    idx, g = idx[ok], g[ok]
+    return (np.ravel_multi_index(tuple((idx + pad).T), fshape).astype(np.int64),
+            np.ravel_multi_index(tuple(g.T), pr.shape).astype(np.int64))
+
+
+@pytest.mark.parametrize("numerics", ["bf16", "fp32"])
+@pytest.mark.parametrize("shape,levels,edges", [
+    ((37, 41), 2, (16, 12)),
+    ((21, 18, 17), 2, (8, 8, 8)),
+    ((9, 10, 8, 9), 1, (4, 4, 4, 4)),
+])
+def test_halo_frame_equals_listed_halo_rows(tmp_path, shape, levels, edges, numerics):
+    """The device halo fill of every chunk frame equals the halo rows of the
+    host-enumerated usable cells written through latec_gnn_halo, and leaves
+    every other row zero."""
+    from latec_amd._lib import check, lib, ptr
+    from latec_amd.gnn_predictor import ChunkedGNNPredictor
+
+    path = _ckpt(tmp_path, 32, 2, 0)
+    stride = 1 << levels
+    pr = ChunkedGNNPredictor(path, 0.0, 1.0, levels=levels, anchor_stride=stride,
+                             numerics=numerics)
+    pr.begin(shape, edges)
+    recon = torch.from_numpy(_field(shape, 4)).cuda()[None]
+    m = pr.model
+    for ci in range(pr.n_chunks):
+        pr.coded[:] = False
+        pr.coded[:ci] = True
+        pr._coded_dev.zero_()
+        pr._coded_dev[:ci] = 1
+        sls = pr.chunk_slices(ci)
+        origin = tuple(sl.start for sl in sls)
+        cshape = tuple(sl.stop - sl.start for sl in sls)
+        w = pr.start_wave([ci], recon)
+        ref = torch.zeros_like(w["E"])
+        frows, gflat = _host_halo_cells(pr, origin, cshape, stride)
+        if len(frows):
+            dr, dg = torch.from_numpy(frows).cuda(), torch.from_numpy(gflat).cuda()
+            check(lib().latec_gnn_halo(m.handle, ptr(ref), ptr(dr), ptr(dg), ptr(recon),
+                                       len(frows), 0.0, 1.0, ptr(None)))
+        torch.cuda.synchronize()
+        assert torch.equal(w["E"], ref), (shape, ci)
+        pr._wave = None
+
+
 def _gate_costs_flat(W):
     """The selection's candidate costs (every (row, T) split of the 12
     candidate rows) followed by the no-gate total, as _gate_select ranks them."""
