@@ -1,6 +1,6 @@
 """Autotuner for the plain path's launch shapes.
 
-The 3-D level kernels have eight launch-shape knobs, set process-wide with
+The 3-D level kernels have eleven launch-shape knobs, set process-wide with
 ``latec_tune_set``:
 
 * ``march_blocks`` -- workgroups per parity class (with 4 waves each, this is
@@ -26,7 +26,15 @@ The 3-D level kernels have eight launch-shape knobs, set process-wide with
 * ``pair_waves`` -- the waves-per-SIMD dial of the finest level's encode
   phases 2 and 3: 0 the compiler's own register budget (77 / 103 VGPRs, six
   and four waves per SIMD), 1 the variant compiled for at least eight waves
-  per SIMD (64 VGPRs, with spills).
+  per SIMD (64 VGPRs, with spills);
+* ``sheet`` -- whether the 3-D levels above the first whose lines have an
+  even length of at most 512 run as two sheet launches (even sheets, then odd
+  sheets, each marching along the rows of whole lines) instead of the three
+  pair phases: bit 0 for the encoder, bit 1 for the decoder;
+* ``sheet_blocks`` -- a sheet launch halves its march segment (16, 8, 4 odd
+  rows per lane) while it would have fewer workgroups than this;
+* ``sheet_min_points`` -- the smallest level (points of its compact grid) that
+  runs as sheet launches.
 
 The slab kernel and histogram peeling are measured slower than their
 alternatives on every workload tried so far, so the default candidates leave
@@ -51,7 +59,7 @@ import numpy as np
 from ._lib import check, lib, ptr
 
 KNOBS = ("march_blocks", "march_hist_peel", "slab", "pair", "pair_lines", "pair_quads",
-         "min_threads", "pair_waves")
+         "min_threads", "pair_waves", "sheet", "sheet_blocks", "sheet_min_points")
 CANDIDATES = {"march_blocks": (1024, 2048, 4096),
               "march_hist_peel": (0,),
               "slab": (0,),
@@ -59,7 +67,10 @@ CANDIDATES = {"march_blocks": (1024, 2048, 4096),
               "pair_lines": (0, 1),
               "pair_quads": (2, 3),
               "min_threads": (1 << 18, 1 << 19),
-              "pair_waves": (0, 1)}
+              "pair_waves": (0, 1),
+              "sheet": (0, 3),
+              "sheet_blocks": (1024,),
+              "sheet_min_points": (1 << 25,)}
 
 
 def get() -> dict:

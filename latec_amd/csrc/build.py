@@ -23,7 +23,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ZSTD_INC = os.environ.get("LATEC_ZSTD_INC", "/opt/conda/include")
 ARCH = os.environ.get("LATEC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["latec_interp.hip", "latec_tables.hip", "latec_gnn.hip", "latec_train.hip",
+HIP_SOURCES = ["latec_interp.hip", "latec_sheet.hip", "latec_tables.hip", "latec_gnn.hip", "latec_train.hip",
                "latec_mlrans.hip",
                "latec_capi.cpp"]
 CXX_SOURCES = ["latec_plan.cpp", "latec_entropy.cpp"]

@@ -30,7 +30,9 @@
 #include <type_traits>
 #include <vector>
 
+#include "latec_level.h"
 #include "latec_plan.h"
+#include "latec_sheet.h"
 
 namespace latec {
 
@@ -234,31 +236,7 @@ __device__ __forceinline__ uint32_t quantize_one(float x, float p, const GridArg
   return code;
 }
 
-#define HIST_WIN 1024
-#define HIST_COPIES 256
-// Per-stage histogram layout: [2R main bins][HIST_COPIES x HIST_WIN copies of
-// the central window].  A block flushes its LDS window into copy
-// (block id % HIST_COPIES), so thousands of blocks do not serialize their
-// atomics on the same few bins; the fit folds the copies into the main bins.
-__device__ __forceinline__ void flush_hist(const uint32_t *lhist, uint32_t *__restrict__ ghist,
-                                           uint32_t R) {
-  const uint32_t bid = blockIdx.y * gridDim.x + blockIdx.x;
-  uint32_t *gwin = ghist + 2 * (size_t)R + (size_t)(bid % HIST_COPIES) * HIST_WIN;
-  for (int i = threadIdx.x; i < HIST_WIN; i += blockDim.x)
-    if (lhist[i]) atomicAdd(&gwin[i], lhist[i]);
-}
-
-// Histogram counts into a block's LDS window (workgroup scope) or straight
-// into the global bins (device scope).  The distinct scopes keep the two
-// atomics of a window-or-global branch apart: merged, they would become one
-// flat atomic on a selected pointer, which is slower and makes every later
-// wait on vector-memory loads wait for it too.
-__device__ __forceinline__ void lds_add(uint32_t *p, uint32_t k) {
-  __hip_atomic_fetch_add(p, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void gbl_add(uint32_t *p, uint32_t k) {
-  __hip_atomic_fetch_add(p, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// (histogram windows, level stencils, quantizer and emission helpers: latec_level.h)
 
 template <typename CodeT>
 __device__ __forceinline__ void emit(CodeT *__restrict__ codes, int64_t slot,
@@ -468,14 +446,6 @@ __global__ void __launch_bounds__(256) decode_stage_kernel(
 // quantize_one), so codes, outliers and recon are bit-identical.
 // ---------------------------------------------------------------------------
 
-struct L3Stage {
-  double w, inv_w;
-  float feb;
-  uint32_t R;
-  int64_t code_base;  // codes + code_base + rank
-  int64_t out_lo;     // outlier arrays offset of the stage
-  int32_t gs, pad;    // global stage index (hist, out_count)
-};
 struct L3Class {
   int64_t offset;
   int64_t start[3], rstr[3];
@@ -978,22 +948,6 @@ struct MArgs {
   MClass cl[3];
 };
 
-// interp_axis on taps already read (A = c-3s, B = c-s, C = c+s, D = c+3s).
-__device__ __forceinline__ double interp4(double A, double B, double C, double D, bool vm3,
-                                          bool vp1, bool vp3, int cubic, int end_mode) {
-  if (!vp1) {
-    if ((end_mode & 2) && vm3) return 1.5 * B - 0.5 * A;
-    return B;
-  }
-  if (cubic) {
-    if (vm3 && vp3) return (((-A + 9.0 * B) + 9.0 * C) - D) * 0.0625;
-    if (end_mode & 1) {
-      if (vp3) return ((3.0 * B + 6.0 * C) - D) * 0.125;
-      if (vm3) return ((-A + 6.0 * B) + 3.0 * C) * 0.125;
-    }
-  }
-  return 0.5 * (B + C);
-}
 
 __device__ __forceinline__ void hist_add(uint32_t *lhist, uint32_t *__restrict__ ghist,
                                          uint32_t win_lo, uint32_t code, uint32_t k) {
@@ -1043,49 +997,6 @@ __device__ __forceinline__ void emit_wave(uint16_t *__restrict__ codes, uint32_t
   }
 }
 
-// Loads and stores at a 32-bit byte offset from a uniform base (SGPR base +
-// VGPR offset addressing; every level grid here is below 2^30 values).
-template <typename T>
-__device__ __forceinline__ T ldo(const T *__restrict__ base, uint32_t idx) {
-  return *(const T *)((const char *)base + idx * (uint32_t)sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ void sto(T *__restrict__ base, uint32_t idx, T v) {
-  *(T *)((char *)base + idx * (uint32_t)sizeof(T)) = v;
-}
-
-// RN(x / 3) without the division sequence.  Let c = RN(1/3) = (1/3)(1 - 2^-54)
-// (1/3 = 0.0101...b, and the double below it is the nearest), q0 = RN(x c) and
-// q = RN(q0 + r c) with r = x - 3 q0 from one FMA.  Then
-//   |q0 - x/3| <= |q0 - x c| + |x c - x/3| <= ulp(x/3)/2 + 2^-54 |x/3| < ulp(x/3),
-// because ulp(t) > 2^-53 |t| for every normal t: q0 is within one ulp of x/3.
-// A quotient within one ulp leaves a remainder x - 3 q0 that is a double, so
-// the FMA computes r exactly, and with c within half an ulp of 1/3 Markstein's
-// theorem (P. Markstein, IBM J. Res. Dev. 34(1), 1990, Theorem 1; restated in
-// Muller et al., Handbook of Floating-Point Arithmetic, the division chapter)
-// gives q = RN(x / 3) exactly.  Both steps assume that q0 and r are normal
-// and that nothing overflows: for |x| >= 2^-968, q0 >= 2^-970 and r (a
-// multiple of ulp(q0) >= 2^-1022) is zero or normal; |x| <= 2^1020 keeps 3 q0
-// finite; the correction is one FMA, so r c is never rounded on its own.
-// Other values (tiny, huge, non-finite) take the division itself.  (Also
-// checked bit for bit against the division on 4.6e8 operands across the
-// range, including multiples of 3 and their neighbours.)  Every lane of the
-// wave calls it.
-__device__ __forceinline__ double div3(double x) {
-  const double c = 1.0 / 3.0;
-  double q = x * c;
-  q = fma(fma(-q, 3.0, x), c, q);
-  const bool ok = fabs(x) >= 0x1p-968 && fabs(x) <= 0x1p1020;
-  if (__any(!ok)) {
-    if (!ok) q = x / 3.0;
-  }
-  return q;
-}
-
-// Cubic interior stencil of interp4 (all four taps valid).
-__device__ __forceinline__ double cubic4(double A, double B, double C, double D) {
-  return (((-A + 9.0 * B) + 9.0 * C) - D) * 0.0625;
-}
 
 template <int P, bool ENC>
 __device__ __forceinline__ void march_class(const MArgs &A, const MClass &K,
@@ -1425,187 +1336,6 @@ struct PArgs {
   PSet set[2];
 };
 
-// the centre rule over the per-axis predictions v[0..nodd) (ascending axes)
-__device__ __forceinline__ double pair_combine(const double *v, int nodd, int center) {
-  if (nodd == 1) return v[0];
-  if (center == 0) {
-    double total = v[0] + v[1];
-    if (nodd == 3) return div3(total + v[2]);
-    return total * 0.5;
-  }
-  return center == 1 ? v[0] : v[nodd - 1];
-}
-
-// One stage's quantizer constants.  hlo = 0.5 - 1e-12 (R + 1): a product
-// t = d / w rounded through 1/w whose distance to the nearest integer is
-// below hlo rounds like the division (the margin is the general rule's
-// 1e-12 max(|t|, 1) at its largest |t| that can give an in-range code; a
-// |t| above R + 1 is out of range either way), so the fallback needs one
-// compare.
-struct PQ {
-  double w, inv_w, Rd;
-  float feb;
-  uint32_t R;
-  double hlo;
-};
-__device__ __forceinline__ PQ make_pq(const L3Stage &S) {
-  return PQ{S.w, S.inv_w, (double)S.R, S.feb, S.R, 0.5 - 1e-12 * ((double)S.R + 1.0)};
-}
-
-// quantize_one (encode): code and reconstruction of x against prediction p.
-// Every lane of the wave calls it (the exact-division fallback is a vote).
-template <bool RND>  // RND false: round_mode is 0 (no output rounding check)
-__device__ __forceinline__ void pair_quant(const PQ &Q, int round_mode, double span,
-                                           double offset, double pd, float xv, uint32_t &code,
-                                           float &r) {
-  const double d = (double)xv - pd;
-  const double tq = d * Q.inv_w;
-  double qd = rint(tq);
-  const bool fb = !(fabs(tq - qd) < Q.hlo);
-  if (__any(fb)) {
-    if (fb) qd = rint(d / Q.w);
-  }
-  const bool inr = fabs(qd) < Q.Rd;
-  const int qi = (int)qd;  // used only when in range
-  // qd + 0.0: the integer's double (+0 for a -0 quotient), as (double)qi
-  const float cand = (float)(pd + Q.w * (qd + 0.0));
-  float chk = cand;
-  if (RND) {
-    if (round_mode == 1)
-      chk = rintf(cand);
-    else if (round_mode == 2)
-      chk = (float)((rint((double)cand * span + offset) - offset) / span);
-  }
-  const bool ok = inr && fabsf(xv - chk) <= Q.feb;
-  code = ok ? (uint32_t)(qi + (int)Q.R) : 0u;
-  r = ok ? cand : xv;
-}
-
-// pair_quant of two points with one wave vote for the exact-division fallback.
-template <bool RND>
-__device__ __forceinline__ void pair_quant2(const PQ &Q, int round_mode, double span,
-                                            double offset, double pd0, float x0, double pd1,
-                                            float x1, uint32_t &code0, float &r0,
-                                            uint32_t &code1, float &r1) {
-  const double d0 = (double)x0 - pd0, d1 = (double)x1 - pd1;
-  const double t0 = d0 * Q.inv_w, t1 = d1 * Q.inv_w;
-  double q0 = rint(t0), q1 = rint(t1);
-  const bool fb0 = !(fabs(t0 - q0) < Q.hlo);
-  const bool fb1 = !(fabs(t1 - q1) < Q.hlo);
-  if (__any(fb0 || fb1)) {
-    if (fb0) q0 = rint(d0 / Q.w);
-    if (fb1) q1 = rint(d1 / Q.w);
-  }
-  auto finish = [&](double pd, float xv, double qd, uint32_t &code, float &r) {
-    const bool inr = fabs(qd) < Q.Rd;
-    const int qi = (int)qd;  // used only when in range
-    const float cand = (float)(pd + Q.w * (qd + 0.0));
-    float chk = cand;
-    if (RND) {
-      if (round_mode == 1)
-        chk = rintf(cand);
-      else if (round_mode == 2)
-        chk = (float)((rint((double)cand * span + offset) - offset) / span);
-    }
-    const bool ok = inr && fabsf(xv - chk) <= Q.feb;
-    code = ok ? (uint32_t)(qi + (int)Q.R) : 0u;
-    r = ok ? cand : xv;
-  };
-  finish(pd0, x0, q0, code0, r0);
-  finish(pd1, x1, q1, code1, r1);
-}
-
-// Decode: reconstruction from a code; code 0 looks the value up among the
-// stage's outliers (sorted by rank), a missing one gives NaN.
-__device__ __forceinline__ float pair_dequant(const PQ &Q, double pd, uint32_t code, bool active,
-                                              uint32_t rank, const uint32_t *__restrict__ orank,
-                                              const float *__restrict__ oval, int64_t nout) {
-  float r = (float)(pd + Q.w * (double)((int)code - (int)Q.R));
-  if (__any(active && code == 0)) {
-    if (active && code == 0) {
-      int64_t lo = 0, hi = nout;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)orank[mid] < (int64_t)rank) lo = mid + 1; else hi = mid;
-      }
-      r = (lo < nout && (int64_t)orank[lo] == (int64_t)rank) ? oval[lo]
-                                                             : __int_as_float(0x7fc00000);
-    }
-  }
-  return r;
-}
-
-// a pair (k, k + 1) at element idx; when the pair has no odd point the odd
-// lane reads idx again (in bounds, value unused)
-__device__ __forceinline__ float2 pair_ld(const float *__restrict__ b, uint32_t idx, bool al,
-                                          bool odd_ok) {
-  if (al) return ldo((const float2 *)b, idx >> 1);
-  return make_float2(ldo(b, idx), ldo(b, odd_ok ? idx + 1 : idx));
-}
-
-// Masked stores as one unconditional buffer instruction: a lane that must
-// not store writes to byte offset BUF_OFF_NONE, beyond the resource's range,
-// which the hardware drops.  With no branch around them the number of memory
-// operations a step issues is fixed, so the wait for the next step's loads
-// need not wait for this step's stores.  (gfx9 buffer resource: 32-bit data
-// format in the last word; 2^31 - 1 byte range, every array here is smaller.)
-typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-#define BUF_OFF_NONE 0x7fffffffu
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, 0x7fffffff,
-                                           0x00020000);
-}
-__device__ __forceinline__ void buf_st32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on,
-                                         float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, on ? byte_off : BUF_OFF_NONE, 0,
-                                        0);
-}
-__device__ __forceinline__ void buf_st64(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on,
-                                         float a, float b) {
-  const u32x2_t v = {__float_as_uint(a), __float_as_uint(b)};
-  __builtin_amdgcn_raw_buffer_store_b64(v, r, on ? byte_off : BUF_OFF_NONE, 0, 0);
-}
-__device__ __forceinline__ void buf_st16(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, bool on,
-                                         uint16_t v) {
-  __builtin_amdgcn_raw_buffer_store_b16(v, r, on ? byte_off : BUF_OFF_NONE, 0, 0);
-}
-
-// emit() of the pair kernels in two halves, so that a step votes once for all
-// its points: emit_common stores the code and counts an in-window code in LDS
-// (branch-free: a lane whose code is outside the window, or that is inactive,
-// adds 0 to the window's spare word HIST_WIN) and says whether the point also
-// needs emit_rare (a code outside the window, counted globally, or an
-// outlier, appended).  emit_rare runs under a wave vote.
-__device__ __forceinline__ bool emit_common(__amdgpu_buffer_rsrc_t codes, uint32_t cbase,
-                                            uint32_t slot, uint32_t code, uint32_t *lhist,
-                                            uint32_t win_lo, bool active) {
-  buf_st16(codes, cbase + 2u * slot, active, (uint16_t)code);
-  const uint32_t b = code - win_lo;
-  const bool inwin = code != 0 && b < HIST_WIN;
-  lds_add(lhist + (inwin ? b : (uint32_t)HIST_WIN), active && inwin ? 1u : 0u);
-  return active && !inwin;
-}
-__device__ __forceinline__ void emit_rare(bool rare, uint32_t slot, uint32_t code, float x,
-                                          uint32_t *__restrict__ out_rank,
-                                          float *__restrict__ out_val,
-                                          uint32_t *__restrict__ out_count,
-                                          uint32_t *__restrict__ ghist) {
-  if (rare) gbl_add(ghist + code, 1u);
-  const bool out = rare && code == 0;
-  const uint64_t m = __ballot(out);
-  if (m) {
-    const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(out_count, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (out) {
-      const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-      out_rank[k] = slot;
-      out_val[k] = x;
-    }
-  }
-}
 
 template <int PH, bool ENC, bool AL, bool RND, bool BM, bool XD>
 __device__ __forceinline__ void pair_set(const PArgs &A, const PSet &T,
@@ -3116,6 +2846,19 @@ static int64_t g_min_threads = 1 << 19;
 // in flight, more spills); 0 the compiler's own occupancy
 static int64_t g_pair_waves = 0;
 static int64_t g_march_peel = 0;  // histogram: distinct codes per wave counted by ballot
+// "sheet": bit 0 (encode) and bit 1 (decode) run a 3-D level above the first
+// as two sheet launches (even sheets, then odd sheets; latec_sheet.hip) when
+// its lines have an even length of at most 512 points
+static int64_t g_sheet = 3;
+// "sheet_blocks": a sheet launch halves its march segment (16, 8, 4 odd
+// rows) while it would have fewer workgroups than this
+static int64_t g_sheet_blocks = 1024;
+// "sheet_min_points": the smallest level (points of its compact grid) that
+// runs as sheet launches; smaller levels keep the pair launches, which
+// measure faster there (more parallelism per point)
+static int64_t g_sheet_min = 1ll << 25;
+// "sheet_launches" (read, or set to reset): sheet launches made so far
+static std::atomic<int64_t> g_sheet_launches{0};
 
 static int64_t g_tune_epoch = 0;  // bumped by every knob change
 static int64_t tune_epoch() { return g_tune_epoch; }
@@ -3131,7 +2874,35 @@ int tune_set(const char *name, int64_t v) {
     g_quad_launches = v;
     return LATEC_OK;
   }
+  if (!strcmp(name, "sheet_launches")) {
+    g_sheet_launches = v;
+    return LATEC_OK;
+  }
   g_tune_epoch++;
+  if (!strcmp(name, "sheet")) {
+    if (v < 0 || v > 3) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_sheet = v;
+    return LATEC_OK;
+  }
+  if (!strcmp(name, "sheet_min_points")) {
+    if (v < 0 || v > (1ll << 40)) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_sheet_min = v;
+    return LATEC_OK;
+  }
+  if (!strcmp(name, "sheet_blocks")) {
+    if (v < 1 || v > (1 << 20)) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_sheet_blocks = v;
+    return LATEC_OK;
+  }
   if (!strcmp(name, "march_blocks")) {
     if (v < 8 || v > (1 << 20)) {
       set_error(std::string("tuning value out of range for ") + name);
@@ -3211,6 +2982,10 @@ int64_t tune_get(const char *name) {
   if (!strcmp(name, "pair_quads")) return g_pair_quads;
   if (!strcmp(name, "quad_launches")) return g_quad_launches;
   if (!strcmp(name, "pair_waves")) return g_pair_waves;
+  if (!strcmp(name, "sheet")) return g_sheet;
+  if (!strcmp(name, "sheet_blocks")) return g_sheet_blocks;
+  if (!strcmp(name, "sheet_min_points")) return g_sheet_min;
+  if (!strcmp(name, "sheet_launches")) return g_sheet_launches;
   if (!strncmp(name, "gnn_", 4)) return gnn_tune_get(name);
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
@@ -3669,6 +3444,122 @@ static int launch_pair(const latec_plan *gp, const latec_plan *q, const latec_pl
   return hip_check(hipGetLastError(), "pair_kernel");
 }
 
+// Sheet launches for a 3-D level above the first (one channel, u16 codes,
+// lines of an even length of at most 512): the even sheets, then the odd
+// sheets (latec_sheet.hip).  Reads the coarse points from the level below's
+// compact grid `coarse` (shape qc).  Returns 1, before launching anything, if
+// the level's layout is not the one the kernels assume.
+static bool sheet_ok(const latec_plan *gp, const latec_plan *q, int k,
+                     const latec_interp_opts *o, bool encode) {
+  return ((g_sheet >> (encode ? 0 : 1)) & 1) && q->npoints >= g_sheet_min && o->cubic == 1 &&
+         o->center >= 0 &&
+         o->center <= 2 && k >= 2 && gp->ndim == 3 && gp->channels == 1 &&
+         o->code_bytes == 2 && q->stages.size() == 6 && q->npoints < (1ll << 29) &&
+         q->shape[2] % 2 == 0 && q->shape[2] <= 512 && q->shape[0] >= 2 && q->shape[1] >= 2;
+}
+
+static int launch_sheets(const latec_plan *gp, const latec_plan *q, const latec_plan *qc, int g0,
+                         bool encode, const float *x, int64_t xstep, const float *coarse,
+                         float *recon, void *codes, uint32_t *hist, uint32_t *out_rank,
+                         float *out_val, uint32_t *out_count, const double *ebs,
+                         const latec_interp_opts *o, hipStream_t stream) {
+  for (int j = 0; j < 3; j++)
+    if (qc->shape[j] != (q->shape[j] + 1) / 2) return 1;
+  // the sub-plan's classes by parity, in their stages: 100, 010, 001 alone,
+  // then the weight-2 stage, then 111
+  static const int want_stage[8] = {0, 1, 2, 4, 3, 4, 4, 5};
+  const ClassDesc *cls[8] = {};
+  for (int js = 1; js <= 5; js++)
+    for (const ClassDesc &c : q->stages[js].classes) {
+      if (c.parity < 1 || c.parity > 7 || c.ndim != 3 || c.step != 2 || (c.flags & 1) ||
+          want_stage[c.parity] != js || cls[c.parity])
+        return 1;
+      for (int j = 0; j < 3; j++)
+        if (c.starts[j] != ((c.parity >> j) & 1)) return 1;
+      cls[c.parity] = &c;
+    }
+  for (int pb = 1; pb < 8; pb++)
+    if (!cls[pb]) return 1;
+  const int64_t n[3] = {q->shape[0], q->shape[1], q->shape[2]};
+  if (n[2] % 2 || n[2] > 512 || n[0] < 2 || n[1] < 2) return 1;
+  if (gp->npoints >= (1ll << 30)) return 1;
+  SheetArgs A[2];
+  // stage slots (fixed: the kernels index them at compile time) and classes by
+  // (row parity, k parity); the even sheets' (e,e) class is the coarse copy
+  static const int slot_js[2][3] = {{2, 3, 4}, {1, 4, 5}};
+  static const int cls_par[2][4] = {{0, 4, 2, 6}, {1, 5, 3, 7}};
+  static const int cls_slot[2][4] = {{0, 1, 0, 2}, {0, 1, 1, 2}};
+  const uint32_t npairs = (uint32_t)(n[2] / 2);
+  const int64_t xst[3] = {xstep * gp->shape[1] * gp->shape[2], xstep * gp->shape[2], xstep};
+  for (int j = 0; j < 3; j++) {
+    if (xstep == 1 && gp->shape[j] != n[j]) return 1;
+    if ((n[j] - 1) * xstep >= gp->shape[j] || xst[j] >= (1ll << 30)) return 1;
+  }
+  for (int od = 0; od < 2; od++) {
+    SheetArgs &a = A[od];
+    memset(&a, 0, sizeof a);
+    for (int j = 0; j < 3; j++) {
+      a.n[j] = (int32_t)n[j];
+      a.cn[j] = (int32_t)qc->shape[j];
+      a.xs[j] = (uint32_t)xst[j];
+    }
+    a.coarse = coarse;
+    a.cubic = o->cubic;
+    a.center = o->center;
+    a.end_mode = o->end_mode;
+    a.round_mode = o->round_mode;
+    a.span = o->span;
+    a.offset = o->offset;
+    a.hist_words = hist_words(o->radius);
+    a.npairs = npairs;
+    a.nsheets = (uint32_t)(od ? n[0] / 2 : (n[0] + 1) / 2);
+    // wave runs of 61 pairs over the sheets' lines, four waves per workgroup
+    a.nruns = (uint32_t)(((int64_t)a.nsheets * npairs + 60) / 61);
+    a.rgroups = (a.nruns + 3) / 4;
+    const uint32_t no = (uint32_t)(n[1] / 2);
+    uint32_t seg = SHEET_SEG;
+    while (seg > 4 && (int64_t)a.rgroups * ((no + seg - 1) / seg) < g_sheet_blocks) seg >>= 1;
+    a.seg = seg;
+    a.nseg = (no + seg - 1) / seg;
+    for (int sl = 0; sl < 3; sl++) {
+      const int js = slot_js[od][sl];
+      const GridArgs g = make_args(q, js, ebs[g0 + js - 1], o);
+      L3Stage &S = a.st[sl];
+      S.w = g.w;
+      S.inv_w = g.inv_w;
+      S.feb = g.feb;
+      S.R = g.R;
+      S.code_base = (int64_t)gp->channels * gp->stages[g0 + js - 1].offset;
+      S.out_lo = S.code_base;
+      S.gs = g0 + js - 1;
+    }
+    for (int c = 0; c < 4; c++) {
+      SheetClass &K = a.cl[c];
+      K.stage = cls_slot[od][c];
+      const int par = cls_par[od][c];
+      if (!par) continue;
+      if (slot_js[od][K.stage] != want_stage[par]) return 1;
+      K.offset = cls[par]->offset;
+      for (int j = 0; j < 3; j++) K.rstr[j] = cls[par]->rstr[j];
+    }
+    if ((int64_t)a.rgroups * a.nseg >= (1ll << 31)) return 1;
+  }
+  for (int od = 0; od < 2; od++) {
+    SheetArgs &a = A[od];
+    for (int sl = 0; sl < 3; sl++) a.hoff[sl] = (int64_t)a.st[sl].gs * a.hist_words;
+  }
+  // layout bits of the kernel variant (decode: no rounding check, x unused)
+  const int bits = (encode ? 1 : 0) | (encode && o->round_mode != 0 ? 2 : 0) |
+                   (!encode || xstep == 1 ? 4 : 0);
+  for (int od = 0; od < 2; od++) {
+    g_sheet_launches++;
+    const int rc = sheet_launch(A[od], od == 1, bits, encode ? x : nullptr, recon,
+                                (uint16_t *)codes, hist, out_rank, out_val, out_count, stream);
+    if (rc) return rc < 0 ? rc : LATEC_E_HIP;
+  }
+  return 0;
+}
+
 static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon, void *codes,
                        uint32_t *hist, uint32_t *out_rank, float *out_val,
                        uint32_t *out_count, const double *ebs, const latec_interp_opts *o,
@@ -3704,6 +3595,14 @@ static int pyramid_run(latec_plan *p, bool encode, const float *x, float *recon,
     if (slab_ok(p, q, k, o)) {  // reads level k-1 itself: no lattice scatter
       rc = launch_slab(p, q, d->sub[k - 1], g0, encode, X(k), d->R[k - 1], R(k), codes, hist,
                        out_rank, out_val, out_count, ebs, o, stream);
+      if (rc < 0) return rc;
+      if (rc == 0) continue;
+    }
+    if (sheet_ok(p, q, k, o, encode)) {  // reads level k-1 itself: no lattice scatter
+      const bool dx = (direct >> k) & 1;
+      rc = launch_sheets(p, q, d->sub[k - 1], g0, encode, dx ? x : X(k), dx ? xstep(k) : 1,
+                         d->R[k - 1], R(k), codes, hist, out_rank, out_val, out_count, ebs, o,
+                         stream);
       if (rc < 0) return rc;
       if (rc == 0) continue;
     }

@@ -12,12 +12,18 @@ def test_knobs_round_trip_and_validation():
     start = autotune.get()
     try:
         autotune.set_knobs(march_blocks=1000, march_hist_peel=3, slab=0, pair=0,
-                           min_threads=12345, pair_lines=0, pair_quads=0, pair_waves=1)
+                           min_threads=12345, pair_lines=0, pair_quads=0, pair_waves=1,
+                           sheet=2, sheet_blocks=77, sheet_min_points=5)
         assert autotune.get() == {"march_blocks": 1000, "march_hist_peel": 3, "slab": 0,
                                   "pair": 0, "pair_lines": 0, "pair_quads": 0,
-                                  "min_threads": 12345, "pair_waves": 1}
+                                  "min_threads": 12345, "pair_waves": 1, "sheet": 2,
+                                  "sheet_blocks": 77, "sheet_min_points": 5}
         with pytest.raises(ValueError):
             autotune.set_knobs(pair_waves=2)
+        with pytest.raises(ValueError):
+            autotune.set_knobs(sheet=4)
+        with pytest.raises(ValueError):
+            autotune.set_knobs(sheet_blocks=0)
         with pytest.raises(ValueError):
             autotune.set_knobs(pair=2)
         with pytest.raises(ValueError):
