@@ -534,6 +534,16 @@ def main():
         k_dec.append(ev[2].elapsed_time(ev[3]) * 1e-3)
     t_kenc = float(np.mean(k_enc))
     t_kdec = float(np.mean(k_dec)) if k_dec else float("nan")
+    # which level kernels one encode runs (launch counters of a direct encode)
+    check(lib().latec_tune_set(b"sheet_launches", 0))
+    codec.encode_device(xd, plan, ebs, o, graph=False, ws=ws)
+    torch.cuda.synchronize()
+    n_sheet_levels = int(lib().latec_tune_get(b"sheet_launches")) // 2
+    n_pyr = max(0, levels - 1)  # levels above the first
+    enc_desc = (f"one encode, one hipGraph: x pyramid gather, anchors encode_stage_kernel, "
+                f"first level march_kernel x3, {n_pyr - n_sheet_levels} level(s) as "
+                f"pair_kernel phases 1-3, {n_sheet_levels} level(s) as sheet_kernel "
+                f"(even sheets, odd sheets); 10 B/value + 4 B/outlier")
     enc_bytes = 10 * n_values + 4 * n_out
     dec_bytes = 6 * n_values + 4 * n_out
     ach_enc = enc_bytes / t_kenc
@@ -593,10 +603,7 @@ def main():
                      "unit": "GB/s", "frac": ach_enc / HBM_PEAK,
                      "traffic": profiled_traffic(shape), "algorithmic_bytes": enc_bytes,
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
-                     "kernel": "one encode: x pyramid lattice_kernel x4, anchors "
-                               "encode_stage_kernel x1, first level march_kernel x3, "
-                               "levels 2-5 pair_kernel x12 (three phases per level), one "
-                               "hipGraph (10 B/value + 4 B/outlier)",
+                     "kernel": enc_desc,
                      "decode": {"achieved": ach_dec / 1e9, "frac": ach_dec / HBM_PEAK}},
         "cpu_baseline": cpu,
         "autotune": None if tuned is None else {"knobs": tuned["knobs"], "ms": tuned["ms"],
