@@ -55,9 +55,16 @@ int latec_abi_version(void);
  * workgroup), "pair_quads" (bits: line quads for encode / decode, and for
  * phase 2 alone), "min_threads" (march shortening threshold) and
  * "pair_waves" (1: the finest level's encode phases 2 and 3 compiled for
- * at least 8 waves per SIMD).  "quad_launches" reads (or, set, resets) a
- * count of line-quad launches.  Process-wide; no knob changes a code or a
- * reconstructed value. */
+ * at least 8 waves per SIMD), "sheet" (bits: encode / decode run large
+ * cubic 3-D levels as even-sheet and odd-sheet launches), "sheet_blocks"
+ * (workgroup target of a sheet launch, 1..2^20) and "sheet_min_points"
+ * (smallest level that runs as sheets).  "quad_launches" and
+ * "sheet_launches" read (or, set, reset) counts of line-quad and sheet
+ * launches.  GNN kernels: "gnn_rows" (0..3: 16-row tiles per wave group and
+ * waves per workgroup of the message / mix rows) and "gnn_pool" (0: line
+ * pool line by line; 1, 2: loads issued ahead, 8 or 4 lines per batch).
+ * Process-wide; no knob changes a code, a reconstructed value or a GNN
+ * prediction. */
 int latec_tune_set(const char *name, int64_t value);
 int64_t latec_tune_get(const char *name);
 
@@ -322,7 +329,8 @@ int64_t latec_zstd_decompress(const uint8_t *src, int64_t n, uint8_t *dst,
 typedef struct latec_gnn latec_gnn;
 
 #define LATEC_GNN_FP32 0 /* FP32 MFMA GEMMs, FP32 pools and head */
-#define LATEC_GNN_BF16 1 /* BF16 MFMA GEMMs (FP32 accumulate), FP32 head */
+#define LATEC_GNN_BF16 1 /* BF16 MFMA GEMMs (FP32 accumulate); pools and head \
+                            in FP32, their GEMMs on split-bf16 MFMA */
 
 /* Floats in the packed parameter block of a (d, agg_level) model: the
  * reference checkpoint's state_dict tensors flattened in module order

@@ -163,6 +163,15 @@ def lib() -> C.CDLL:
                 f"{LIB_PATH} is missing: build it with `python -m latec_amd.csrc.build` "
                 "(the HIP path has no CPU fallback)"
             )
+        # torch's wheel ships its own HIP runtime, which its libraries load
+        # as "libamdhip64.so"; loaded first, this library would pull in
+        # /opt/rocm's copy under "libamdhip64.so.7" and torch a second one,
+        # and torch's device pointers would mean nothing to this library's
+        # runtime.  Importing torch first makes both use torch's copy.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "latec_plan.h"
@@ -167,6 +168,9 @@ struct latec_gnn {
   // device pointers into `dev`
   void *bi_w1 = nullptr, *bi_w2 = nullptr, *dir_w1 = nullptr, *dir_w2 = nullptr;
   void *mx_w1 = nullptr, *mx_w2 = nullptr, *key_w = nullptr, *init_w2 = nullptr;
+  // bf16 mode: head weights as high and low bf16 parts, natural order
+  void *hd_wkv_hi = nullptr, *hd_wkv_lo = nullptr, *hd_w1_hi = nullptr, *hd_w1_lo = nullptr;
+  void *lk_wv_hi = nullptr, *lk_wv_lo = nullptr;  // line pool value projection
   float *f32 = nullptr;            // f32 arrays (biases, state rows, pools, head, init)
   std::vector<int64_t> f32_off;    // named offsets into f32 (see make_device)
   int8_t *d_dirs = nullptr, *d_sidx = nullptr;
@@ -968,38 +972,123 @@ struct PoolArgs {
   const float *wv;      // (D, D) value projection, natural order
   const float *bv, *null_v;
   float null_score;     // null_k * scale
+  const uint16_t *wv_hi, *wv_lo;  // bf16 mode: wv as high + low bf16 parts
 };
 
 constexpr int POOL_WAVES = 4;
 
-template <int D>
-static size_t pool_smem() {
-  return sizeof(float) * ((size_t)D * (D + 4) + 2 * (size_t)D +
-                          POOL_WAVES * (16 * (size_t)(D + 4) + 16));
+// LDS of the pool kernels: the value projection (bf16 mode: high and low
+// bf16 parts, rows of D + 8; fp32: rows of D + 4 floats), bv, null_v, then
+// per wave 16 pooled rows and 16 null weights.
+template <int D, typename T>
+struct PoolLds {
+  static constexpr bool BF = sizeof(T) == 2;
+  static constexpr int PW = D + 4, PB = D + 8;
+  static constexpr size_t WBYTES = BF ? 2 * sizeof(uint16_t) * D * PB : sizeof(float) * D * PW;
+  static size_t bytes() {
+    return WBYTES + sizeof(float) * (2 * (size_t)D + POOL_WAVES * (16 * (size_t)PW + 16));
+  }
+};
+
+template <int D, typename T>
+__device__ __forceinline__ void pool_load_weights(unsigned char *smem, const PoolArgs &pa,
+                                                  int tid) {
+  typedef PoolLds<D, T> S;
+  float *sbv = (float *)(smem + S::WBYTES), *snv = sbv + D;
+  if constexpr (S::BF) {
+    uint16_t *sWh = (uint16_t *)smem, *sWl = sWh + D * S::PB;
+    for (int i = tid; i < D * D; i += 64 * POOL_WAVES) {
+      sWh[(i / D) * S::PB + i % D] = pa.wv_hi[i];
+      sWl[(i / D) * S::PB + i % D] = pa.wv_lo[i];
+    }
+  } else {
+    float *sWv = (float *)smem;
+    for (int i = tid; i < D * D; i += 64 * POOL_WAVES) sWv[(i / D) * S::PW + i % D] = pa.wv[i];
+  }
+  for (int i = tid; i < D; i += 64 * POOL_WAVES) {
+    sbv[i] = pa.bv[i];
+    snv[i] = pa.null_v[i];
+  }
+}
+
+// split x = hi + lo into two bf16 fragments (hi = RNE(x), lo = RNE(x - hi))
+__device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8 &hi, bf16x8 &lo) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const __bf16 h = (__bf16)x[i];
+    hi[i] = h;
+    lo[i] = (__bf16)(x[i] - (float)h);
+  }
+}
+
+// Value projection of a wave's 16 pooled items (rows of sP, null weights in
+// sWn): ctx^T = Wv P^T + (1 - w_null) bv + w_null null_v.  fp32 mode on
+// v_mfma_f32_16x16x4f32; bf16 mode on v_mfma_f32_16x16x32_bf16 with split
+// operands, Wv_hi P_hi + Wv_lo P_hi + Wv_hi P_lo (about 16 bits of each
+// operand; the result is stored as bf16).
+template <int D, typename T>
+__device__ __forceinline__ void pool_project(const unsigned char *smem, const float *sP,
+                                             const float *sWn, int lane, int64_t it0,
+                                             int64_t n_items, int64_t item_base, int K,
+                                             T *__restrict__ ctx) {
+  typedef PoolLds<D, T> S;
+  constexpr int PW = S::PW, PB = S::PB;
+  const float *sbv = (const float *)(smem + S::WBYTES), *snv = sbv + D;
+  const int r = lane & 15, g = lane >> 4;  // item, k group
+  const float wn = sWn[r];
+  const int64_t item = it0 + r;
+  [[maybe_unused]] bf16x8 ph[D / 32], pl[D / 32];
+  if constexpr (S::BF) {
+#pragma unroll
+    for (int q = 0; q < D / 32; q++) {
+      float pv[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) pv[i] = sP[r * PW + 32 * q + 8 * g + i];
+      split_bf16(pv, ph[q], pl[q]);
+    }
+  }
+#pragma unroll
+  for (int ob = 0; ob < D / 16; ob++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (S::BF) {
+      const __bf16 *wr = (const __bf16 *)smem + (16 * ob + r) * PB + 8 * g;
+#pragma unroll
+      for (int q = 0; q < D / 32; q++) {
+        const bf16x8 a_hi = *(const bf16x8 *)(wr + 32 * q);
+        const bf16x8 a_lo = *(const bf16x8 *)(wr + D * PB + 32 * q);
+        mfma(acc, a_hi, ph[q]);
+        mfma(acc, a_lo, ph[q]);
+        mfma(acc, a_hi, pl[q]);
+      }
+    } else {
+      const float *wr = (const float *)smem + (16 * ob + r) * PW + g;
+      const float *pr = sP + r * PW + g;
+#pragma unroll
+      for (int t = 0; t < D / 4; t++) mfma(acc, wr[4 * t], pr[4 * t]);
+    }
+    const int o = 16 * ob + 4 * g;
+    f32x4 y;
+#pragma unroll
+    for (int v = 0; v < 4; v++) y[v] = acc[v] + (1.0f - wn) * sbv[o + v] + wn * snv[o + v];
+    if (item < n_items)
+      *(typename Mf<T>::quad *)(ctx + (item_base * K + item) * D + o) = to_quad<T>(y);
+  }
 }
 
 // 16 items per wave: 4 lanes per item for the softmax-weighted sum of its
-// messages (D / 4 columns each), then ctx^T = Wv P^T for the 16 items on
-// v_mfma_f32_16x16x4f32 (P through LDS), + (1 - w_null) bv + w_null null_v.
+// messages (D / 4 columns each), line by line, then the value projection.
 template <int D, typename T>
 __global__ __launch_bounds__(64 * POOL_WAVES) void pool_kernel(
     int64_t n_items, int K, int L, const int32_t *__restrict__ slot, const T *__restrict__ msg,
     const float *__restrict__ score, PoolArgs pa, T *__restrict__ ctx, int64_t item_base) {
   constexpr int PW = D + 4, DC = D / 4;
   extern __shared__ __align__(16) unsigned char smem[];
-  float *sWv = (float *)smem;
-  float *sbv = sWv + D * PW, *snv = sbv + D;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  float *sP = snv + D + wv * (16 * PW + 16);
+  float *sP = (float *)(smem + PoolLds<D, T>::WBYTES) + 2 * D + wv * (16 * PW + 16);
   float *sWn = sP + 16 * PW;
-  for (int i = tid; i < D * D; i += 64 * POOL_WAVES) sWv[(i / D) * PW + i % D] = pa.wv[i];
-  for (int i = tid; i < D; i += 64 * POOL_WAVES) {
-    sbv[i] = pa.bv[i];
-    snv[i] = pa.null_v[i];
-  }
+  pool_load_weights<D, T>(smem, pa, tid);
   __syncthreads();
   const int ii = lane >> 2, c4 = lane & 3;  // softmax phase: item, column quarter
-  const int r = lane & 15, g = lane >> 4;   // MFMA phase: item, k group
   const int64_t gstep = (int64_t)gridDim.x * POOL_WAVES * 16;
   for (int64_t it0 = ((int64_t)xcd_block() * POOL_WAVES + wv) * 16; it0 < n_items; it0 += gstep) {
     {
@@ -1046,22 +1135,100 @@ __global__ __launch_bounds__(64 * POOL_WAVES) void pool_kernel(
       if (c4 == 0) sWn[ii] = en * inv;
     }
     wave_lds_sync();
-    const float wn = sWn[r];
-    const int64_t item = it0 + r;
+    pool_project<D, T>(smem, sP, sWn, lane, it0, n_items, item_base, K, ctx);
+    wave_lds_sync();
+  }
+}
+
+// The same pool for items of at most POOL_LMAX lines, with every load issued
+// ahead of its use: the item's slots, then the scores of all its lines, then
+// the messages of LB lines at a time (dead lines load slot 0's and select
+// zeros); bf16 with LB = 4 holds its registers to four waves per SIMD.  Each live line adds e * m in line order exactly as in
+// pool_kernel (a dead line adds +0, which leaves a sum that starts at +0
+// unchanged), so both kernels give the same bits.
+constexpr int POOL_LMAX = 16;
+
+template <int D, typename T, int LB>
+__global__ __launch_bounds__(64 * POOL_WAVES)
+__attribute__((amdgpu_waves_per_eu(LB == 4 && sizeof(T) == 2 ? 4 : 1))) void pool_lines_kernel(
+    int64_t n_items, int K, int L, const int32_t *__restrict__ slot, const T *__restrict__ msg,
+    const float *__restrict__ score, PoolArgs pa, T *__restrict__ ctx, int64_t item_base) {
+  constexpr int PW = D + 4, DC = D / 4;
+  constexpr bool BF = sizeof(T) == 2;
+  static_assert(POOL_LMAX % LB == 0, "lines per batch of message loads");
+  constexpr int NV = BF ? DC / 8 : DC / 4;          // vector loads per line
+  typedef typename std::conditional<BF, bf16x8, f32x4>::type vec;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  float *sP = (float *)(smem + PoolLds<D, T>::WBYTES) + 2 * D + wv * (16 * PW + 16);
+  float *sWn = sP + 16 * PW;
+  pool_load_weights<D, T>(smem, pa, tid);
+  __syncthreads();
+  const int ii = lane >> 2, c4 = lane & 3;
+  const int64_t gstep = (int64_t)gridDim.x * POOL_WAVES * 16;
+  for (int64_t it0 = ((int64_t)xcd_block() * POOL_WAVES + wv) * 16; it0 < n_items; it0 += gstep) {
+    {
+      int64_t it = it0 + ii;
+      if (it >= n_items) it = n_items - 1;
+      const int64_t m = it / K;
+      const int k = (int)(it - m * K);
+      const int32_t *sl = slot + m * L;
+      int32_t s[POOL_LMAX];
 #pragma unroll
-    for (int ob = 0; ob < D / 16; ob++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      const float *wr = sWv + (16 * ob + r) * PW + g;
-      const float *pr = sP + r * PW + g;
+      for (int l = 0; l < POOL_LMAX; l++) s[l] = l < L ? sl[l] : -1;
+      float sc[POOL_LMAX];
 #pragma unroll
-      for (int t = 0; t < D / 4; t++) mfma(acc, wr[4 * t], pr[4 * t]);
-      const int o = 16 * ob + 4 * g;
-      f32x4 y;
+      for (int l = 0; l < POOL_LMAX; l++) sc[l] = score[(int64_t)(s[l] < 0 ? 0 : s[l]) * K + k];
+      float mx = pa.null_score;
 #pragma unroll
-      for (int v = 0; v < 4; v++) y[v] = acc[v] + (1.0f - wn) * sbv[o + v] + wn * snv[o + v];
-      if (item < n_items)
-        *(typename Mf<T>::quad *)(ctx + (item_base * K + item) * D + o) = to_quad<T>(y);
+      for (int l = 0; l < POOL_LMAX; l++)
+        if (s[l] >= 0) mx = fmaxf(mx, sc[l]);
+      float den = 0.f, acc[DC];
+#pragma unroll
+      for (int j = 0; j < DC; j++) acc[j] = 0.f;
+#pragma unroll
+      for (int l0 = 0; l0 < POOL_LMAX; l0 += LB) {
+        vec v[LB][NV];
+#pragma unroll
+        for (int b = 0; b < LB; b++) {
+          const int32_t q = s[l0 + b] < 0 ? 0 : s[l0 + b];
+          const T *mp = msg + ((int64_t)q * K + k) * D + c4 * DC;
+#pragma unroll
+          for (int u = 0; u < NV; u++) v[b][u] = *(const vec *)(mp + u * (BF ? 8 : 4));
+        }
+#pragma unroll
+        for (int b = 0; b < LB; b++) {
+          const bool live = s[l0 + b] >= 0;
+          const float e = live ? __expf(sc[l0 + b] - mx) : 0.f;
+          den += e;
+#pragma unroll
+          for (int u = 0; u < NV; u++) {
+            if constexpr (BF) {
+              typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+              u32x4 w = __builtin_bit_cast(u32x4, v[b][u]);
+#pragma unroll
+              for (int z = 0; z < 4; z++) w[z] = live ? w[z] : 0u;
+              const bf16x8 vv = __builtin_bit_cast(bf16x8, w);
+#pragma unroll
+              for (int e2 = 0; e2 < 8; e2++)
+                acc[8 * u + e2] = __builtin_fmaf(e, (float)vv[e2], acc[8 * u + e2]);
+            } else {
+#pragma unroll
+              for (int e2 = 0; e2 < 4; e2++)
+                acc[4 * u + e2] = __builtin_fmaf(e, live ? v[b][u][e2] : 0.f, acc[4 * u + e2]);
+            }
+          }
+        }
+      }
+      const float en = __expf(pa.null_score - mx);
+      den += en;
+      const float inv = 1.0f / den;
+#pragma unroll
+      for (int j = 0; j < DC; j++) sP[ii * PW + c4 * DC + j] = acc[j] * inv;
+      if (c4 == 0) sWn[ii] = en * inv;
     }
+    wave_lds_sync();
+    pool_project<D, T>(smem, sP, sWn, lane, it0, n_items, item_base, K, ctx);
     wave_lds_sync();
   }
 }
@@ -1082,6 +1249,10 @@ struct HeadArgs {
   const float *b1;      // (H)
   const float *w2;      // (2, H)
   const float *b2;      // (2)
+  // bf16 mode (head_split_kernel): the two GEMM weights as high + low bf16
+  // parts, inputs in natural order
+  const uint16_t *wkv_hi, *wkv_lo;  // (2D, D)
+  const uint16_t *w1_hi, *w1_lo;    // (H, D)
 };
 
 constexpr int HEAD_WAVES = 8;
@@ -1206,6 +1377,170 @@ __global__ __launch_bounds__(64 * HEAD_WAVES) void head2_kernel(const T *__restr
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const float z = gelu(acc[v] + sb1[o + v]);
+        o0 += sw20[o + v] * z;
+        o1 += sw21[o + v] * z;
+      }
+    }
+    o0 += __shfl_xor(o0, 16);
+    o0 += __shfl_xor(o0, 32);
+    o1 += __shfl_xor(o1, 16);
+    o1 += __shfl_xor(o1, 32);
+    const int64_t m = m0 + r;
+    if (g == 0 && m < h.M) {
+      o0 += h.b2[0];
+      o1 += h.b2[1];
+      const float mu = 1.0f / (1.0f + expf(-o0));
+      const float lb = h.log_eb + fminf(fmaxf(o1, -4.0f), 12.0f);
+      float p = mu * h.span + h.vmin;
+      p = fminf(fmaxf(p, h.vmin), h.vmax);
+      pred[m] = p;
+      scl[m] = exp2f(lb) * h.span;
+      if (mu_out) mu_out[m] = mu;
+      if (lb_out) lb_out[m] = lb;
+    }
+    wave_lds_sync();
+  }
+}
+
+template <int D>
+static size_t head_split_smem() {
+  constexpr int H = 2 * D, PB = D + 8, PW = D + 4;
+  return sizeof(uint16_t) * 2 * ((size_t)2 * D * PB + (size_t)H * PB) +
+         sizeof(float) * (4 * (size_t)D + 3 * (size_t)H + HEAD_WAVES * (2 * 16 * (size_t)PW + 32));
+}
+
+// head2_kernel for the bf16 mode, both GEMMs on v_mfma_f32_16x16x32_bf16 with
+// split operands: the axis pool's (2D x D) product takes the bf16 context rows
+// as they are against W = W_hi + W_lo (two products), PredHead's first layer
+// takes p = p_hi + p_lo against W1 = W1_hi + W1_lo (three products: the
+// lo x lo term is below 2^-16 of the result).  The products keep about 16
+// bits of each operand; the GELUs are gelu_fast.  Same phases, softmax and
+// pooling order as head2_kernel.
+template <int D>
+__global__ __launch_bounds__(64 * HEAD_WAVES) void head_split_kernel(
+    const uint16_t *__restrict__ ctx, HeadArgs h, float *__restrict__ pred,
+    float *__restrict__ scl, float *__restrict__ mu_out, float *__restrict__ lb_out) {
+  constexpr int H = 2 * D, PB = D + 8, PW = D + 4, NS = D / 32;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16 *sKh = (__bf16 *)smem, *sKl = sKh + 2 * D * PB;
+  __bf16 *s1h = sKl + 2 * D * PB, *s1l = s1h + H * PB;
+  float *sbk1 = (float *)(s1l + H * PB);
+  float *swk2 = sbk1 + D, *sbv = swk2 + D, *snv = sbv + D;
+  float *sb1 = snv + D, *sw20 = sb1 + H, *sw21 = sw20 + H;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, r = lane & 15, g = lane >> 4;
+  float *sV = sw21 + H + wv * (2 * 16 * PW + 32);
+  float *sPool = sV + 16 * PW;
+  float *sS = sPool + 16 * PW;  // 16 scores, 16 null weights
+  const __bf16 *wkh = (const __bf16 *)h.wkv_hi, *wkl = (const __bf16 *)h.wkv_lo;
+  const __bf16 *w1h = (const __bf16 *)h.w1_hi, *w1l = (const __bf16 *)h.w1_lo;
+  for (int i = tid; i < 2 * D * D; i += 64 * HEAD_WAVES) {
+    sKh[(i / D) * PB + i % D] = wkh[i];
+    sKl[(i / D) * PB + i % D] = wkl[i];
+  }
+  for (int i = tid; i < H * D; i += 64 * HEAD_WAVES) {
+    s1h[(i / D) * PB + i % D] = w1h[i];
+    s1l[(i / D) * PB + i % D] = w1l[i];
+  }
+  for (int i = tid; i < D; i += 64 * HEAD_WAVES) {
+    sbk1[i] = h.bk1[i];
+    swk2[i] = h.wk2[i];
+    sbv[i] = h.bv[i];
+    snv[i] = h.null_v[i];
+  }
+  for (int i = tid; i < H; i += 64 * HEAD_WAVES) {
+    sb1[i] = h.b1[i] + h.cond[i] * h.log_eb;
+    sw20[i] = h.w2[i];
+    sw21[i] = h.w2[H + i];
+  }
+  __syncthreads();
+  const int K = h.K, PPT = 16 / K, NTA = (16 + PPT - 1) / PPT;
+  const float sn = h.null_k * h.scale;
+  const int64_t gstep = (int64_t)gridDim.x * HEAD_WAVES * 16;
+  for (int64_t m0 = ((int64_t)xcd_block() * HEAD_WAVES + wv) * 16; m0 < h.M; m0 += gstep) {
+    for (int ta = 0; ta < NTA; ta++) {
+      const int pl = ta * PPT + r / K, k = r % K;  // this lane's row: local point, axis
+      const bool rv = r < PPT * K && pl < 16 && m0 + pl < h.M;
+      const int64_t row = rv ? (m0 + pl) * K + k : 0;
+      bf16x8 x[NS];
+#pragma unroll
+      for (int q = 0; q < NS; q++) x[q] = *(const bf16x8 *)(ctx + row * D + 32 * q + 8 * g);
+      float part = 0.f;
+#pragma unroll 1
+      for (int ob = 0; ob < D / 16; ob++) {  // key hidden units
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        const __bf16 *wr = sKh + (16 * ob + r) * PB + 8 * g;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+          mfma(acc, *(const bf16x8 *)(wr + 32 * q), x[q]);
+          mfma(acc, *(const bf16x8 *)(wr + 2 * D * PB + 32 * q), x[q]);
+        }
+        const int o = 16 * ob + 4 * g;
+#pragma unroll
+        for (int v = 0; v < 4; v++) part += swk2[o + v] * gelu_fast(acc[v] + sbk1[o + v]);
+      }
+      part += __shfl_xor(part, 16);
+      part += __shfl_xor(part, 32);
+      if (g == 0) sS[r] = (part + h.bk2) * h.scale;
+      wave_lds_sync();
+      // softmax over the point's axes + null
+      float mx = sn;
+      const int p0 = (r / K) * K;
+      for (int kk = 0; kk < K; kk++) mx = fmaxf(mx, sS[p0 + kk < 16 ? p0 + kk : 15]);
+      float den = __expf(sn - mx);
+      for (int kk = 0; kk < K; kk++) den += __expf(sS[p0 + kk < 16 ? p0 + kk : 15] - mx);
+      const float inv = 1.0f / den;
+      const float wr_ = __expf(sS[r] - mx) * inv;
+#pragma unroll 1
+      for (int q = 0; q < D / 16; q++) {  // value units, weighted
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        const __bf16 *wr = sKh + (16 * (D / 16 + q) + r) * PB + 8 * g;
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+          mfma(acc, *(const bf16x8 *)(wr + 32 * t), x[t]);
+          mfma(acc, *(const bf16x8 *)(wr + 2 * D * PB + 32 * t), x[t]);
+        }
+        const int o = 16 * q + 4 * g;
+#pragma unroll
+        for (int v = 0; v < 4; v++) sV[r * PW + o + v] = wr_ * (acc[v] + sbv[o + v]);
+      }
+      if (g == 0 && k == 0) sS[16 + r / K] = __expf(sn - mx) * inv;
+      wave_lds_sync();
+      for (int idx = lane; idx < PPT * D; idx += 64) {
+        const int pp = idx / D, o = idx % D, plo = ta * PPT + pp;
+        if (plo >= 16) continue;
+        float sacc = sS[16 + pp] * snv[o];
+        for (int kk = 0; kk < K; kk++) sacc += sV[(pp * K + kk) * PW + o];
+        sPool[plo * PW + o] = sacc;
+      }
+      wave_lds_sync();
+    }
+    // phase B: p of pooled point r, units 32 q + 8 g + i, split once
+    bf16x8 ph[NS], plo[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      float pv[8];
+      const float *pr = sPool + r * PW + 32 * q + 8 * g;
+#pragma unroll
+      for (int i = 0; i < 8; i++) pv[i] = pr[i];
+      split_bf16(pv, ph[q], plo[q]);
+    }
+    float o0 = 0.f, o1 = 0.f;
+#pragma unroll 1
+    for (int ob = 0; ob < H / 16; ob++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      const __bf16 *wr = s1h + (16 * ob + r) * PB + 8 * g;
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        const bf16x8 a_hi = *(const bf16x8 *)(wr + 32 * q);
+        const bf16x8 a_lo = *(const bf16x8 *)(wr + H * PB + 32 * q);
+        mfma(acc, a_hi, ph[q]);
+        mfma(acc, a_lo, ph[q]);
+        mfma(acc, a_hi, plo[q]);
+      }
+      const int o = 16 * ob + 4 * g;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const float z = gelu_fast(acc[v] + sb1[o + v]);
         o0 += sw20[o + v] * z;
         o1 += sw21[o + v] * z;
       }
@@ -1602,6 +1937,27 @@ int make_device(latec_gnn *g, const float *P) {
   void *o_key_w = mat(with_bias(reg_in(rows(Ly.lp[0], d, d, 0, d), d, d, false, bf), d, d, vec(Ly.lp[1], d)));
   // InitEmbed layer 2, natural order
   void *o_init_w2 = mat(rows(Ly.init2w, d, d, 0, d));
+  // bf16 mode: axis-pool key + value layers (2d, d), PredHead's first layer
+  // (h, d) without its log-eb column and the line pool's value projection
+  // (d, d), high and low parts
+  void *o_hd[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (bf) {
+    std::vector<float> kv = rows(Ly.ap[0], d, d, 0, d), v2 = rows(Ly.ap[4], d, d, 0, d);
+    kv.insert(kv.end(), v2.begin(), v2.end());
+    const std::vector<float> w1 = rows(Ly.hd0w, h, d + 1, 0, d);
+    auto lo = [&](const std::vector<float> &W) {
+      std::vector<float> v(W.size());
+      for (size_t i = 0; i < W.size(); i++) v[i] = W[i] - hi(W[i]);
+      return v;
+    };
+    o_hd[0] = mat(kv);
+    o_hd[1] = mat(lo(kv));
+    o_hd[2] = mat(w1);
+    o_hd[3] = mat(lo(w1));
+    const std::vector<float> wv = rows(Ly.lp[4], d, d, 0, d);
+    o_hd[4] = mat(wv);
+    o_hd[5] = mat(lo(wv));
+  }
   // f32 arrays
   std::vector<std::vector<float>> f(F_NSLOTS);
   f[F_BI_B2] = vec(Ly.bi2b, d);
@@ -1661,6 +2017,14 @@ int make_device(latec_gnn *g, const float *P) {
   g->mx_w2 = at(o_mx_w2);
   g->key_w = at(o_key_w);
   g->init_w2 = at(o_init_w2);
+  if (bf) {
+    g->hd_wkv_hi = at(o_hd[0]);
+    g->hd_wkv_lo = at(o_hd[1]);
+    g->hd_w1_hi = at(o_hd[2]);
+    g->hd_w1_lo = at(o_hd[3]);
+    g->lk_wv_hi = at(o_hd[4]);
+    g->lk_wv_lo = at(o_hd[5]);
+  }
   g->f32 = (float *)at(o_f32);
   g->d_dirs = (int8_t *)at(o_dirs);
   g->d_sidx = (int8_t *)at(o_sidx);
@@ -1701,6 +2065,10 @@ int lds_setup(const void *fn, int threads, size_t smem, int *per_cu) {
 // wave group and waves per workgroup
 int g_gnn_rows = 0;
 constexpr int GNN_ROW_SHAPES = 4;
+// line pool kernel (knob gnn_pool): 0 line by line; 1 and 2 loads issued
+// ahead, message loads of 8 (1) or 4 (2) lines at a time (items of at most
+// POOL_LMAX lines); all give the same bits
+int g_gnn_pool = 2;
 
 template <int D, typename T, int MODE, int NT, int WV>
 int launch_rows_s(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
@@ -1761,11 +2129,20 @@ int gnn_tune_set(const char *name, int64_t v) {
     g_gnn_rows = (int)v;
     return LATEC_OK;
   }
+  if (!strcmp(name, "gnn_pool")) {
+    if (v < 0 || v > 2) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_gnn_pool = (int)v;
+    return LATEC_OK;
+  }
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
 int64_t gnn_tune_get(const char *name) {
   if (!strcmp(name, "gnn_rows")) return g_gnn_rows;
+  if (!strcmp(name, "gnn_pool")) return g_gnn_pool;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -1866,10 +2243,15 @@ int embed_t(latec_gnn *g, const void *E, const int64_t *frame_shape, const int32
   T *msg = (T *)((unsigned char *)lw_dir + b_lw);
   float *score = (float *)((unsigned char *)msg + b_msg);
   const float scale = 1.0f / std::sqrt((float)D);
-  const PoolArgs pa{fs(g, F_LK_WV), fs(g, F_LK_BV), fs(g, F_LK_NULLV), g->scal[1] * scale};
-  const size_t psm = pool_smem<D>();
+  const PoolArgs pa{fs(g, F_LK_WV), fs(g, F_LK_BV), fs(g, F_LK_NULLV), g->scal[1] * scale,
+                    (const uint16_t *)g->lk_wv_hi, (const uint16_t *)g->lk_wv_lo};
+  const size_t psm = PoolLds<D, T>::bytes();
+  const int pool_lines = L <= POOL_LMAX ? g_gnn_pool : 0;
+  const void *pool_fn = pool_lines == 1   ? (const void *)pool_lines_kernel<D, T, 8>
+                        : pool_lines == 2 ? (const void *)pool_lines_kernel<D, T, 4>
+                                          : (const void *)pool_kernel<D, T>;
   int pool_per_cu = 1;
-  rc = lds_setup((const void *)pool_kernel<D, T>, 64 * POOL_WAVES, psm, &pool_per_cu);
+  rc = lds_setup(pool_fn, 64 * POOL_WAVES, psm, &pool_per_cu);
   if (rc) return rc;
   for (int64_t m0 = 0; m0 < M; m0 += MT) {
     const int64_t mt = M - m0 < MT ? M - m0 : MT;
@@ -1907,8 +2289,17 @@ int embed_t(latec_gnn *g, const void *E, const int64_t *frame_shape, const int32
     const int64_t waves = (items + 15) / 16;
     int64_t grid = (waves + POOL_WAVES - 1) / POOL_WAVES;
     if (grid > (int64_t)g->n_cu * pool_per_cu * 4) grid = (int64_t)g->n_cu * pool_per_cu * 4;
-    hipLaunchKernelGGL((pool_kernel<D, T>), dim3((unsigned)grid), dim3(64 * POOL_WAVES), psm, s,
-                       items, K, L, slot, msg, score, pa, (T *)ctx, m0);
+    if (pool_lines == 1)
+      hipLaunchKernelGGL((pool_lines_kernel<D, T, 8>), dim3((unsigned)grid),
+                         dim3(64 * POOL_WAVES), psm, s, items, K, L, slot, msg, score, pa,
+                         (T *)ctx, m0);
+    else if (pool_lines == 2)
+      hipLaunchKernelGGL((pool_lines_kernel<D, T, 4>), dim3((unsigned)grid),
+                         dim3(64 * POOL_WAVES), psm, s, items, K, L, slot, msg, score, pa,
+                         (T *)ctx, m0);
+    else
+      hipLaunchKernelGGL((pool_kernel<D, T>), dim3((unsigned)grid), dim3(64 * POOL_WAVES), psm,
+                         s, items, K, L, slot, msg, score, pa, (T *)ctx, m0);
     rc = hip_ok(hipGetLastError(), "pool_kernel launch");
     if (rc) return rc;
   }
@@ -1940,16 +2331,30 @@ int head_t(latec_gnn *g, const void *ctx, int64_t M, double eb, float vmin, floa
   h.b1 = fs(g, F_HD_B1);
   h.w2 = fs(g, F_HD_W2);
   h.b2 = fs(g, F_HD_B2);
-  const size_t smem = head_smem<D>();
+  constexpr bool BF = sizeof(T) == 2;
+  h.wkv_hi = (const uint16_t *)g->hd_wkv_hi;
+  h.wkv_lo = (const uint16_t *)g->hd_wkv_lo;
+  h.w1_hi = (const uint16_t *)g->hd_w1_hi;
+  h.w1_lo = (const uint16_t *)g->hd_w1_lo;
+  const size_t smem = BF ? head_split_smem<D>() : head_smem<D>();
+  const void *fn;
+  if constexpr (BF)
+    fn = (const void *)head_split_kernel<D>;
+  else
+    fn = (const void *)head2_kernel<D, T>;
   int per_cu = 1;
-  int rc = lds_setup((const void *)head2_kernel<D, T>, 64 * HEAD_WAVES, smem, &per_cu);
+  int rc = lds_setup(fn, 64 * HEAD_WAVES, smem, &per_cu);
   if (rc) return rc;
   const int64_t waves = (M + 15) / 16;
   int64_t grid = (waves + HEAD_WAVES - 1) / HEAD_WAVES;
   if (grid > (int64_t)g->n_cu * per_cu) grid = (int64_t)g->n_cu * per_cu;
-  hipLaunchKernelGGL((head2_kernel<D, T>), dim3((unsigned)grid), dim3(64 * HEAD_WAVES), smem, s,
-                     (const T *)ctx, h, pred, scale, mu, logb);
-  return hip_ok(hipGetLastError(), "head2_kernel launch");
+  if constexpr (BF)
+    hipLaunchKernelGGL((head_split_kernel<D>), dim3((unsigned)grid), dim3(64 * HEAD_WAVES), smem,
+                       s, (const uint16_t *)ctx, h, pred, scale, mu, logb);
+  else
+    hipLaunchKernelGGL((head2_kernel<D, T>), dim3((unsigned)grid), dim3(64 * HEAD_WAVES), smem,
+                       s, (const T *)ctx, h, pred, scale, mu, logb);
+  return hip_ok(hipGetLastError(), "head kernel launch");
 }
 
 template <int D, typename T>

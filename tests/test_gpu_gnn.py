@@ -78,6 +78,41 @@ def test_forward_matches_oracle(tmp_path, shape, levels, d, agg, numerics):
             assert np.sqrt(np.mean((lb - rl) ** 2)) <= 1e-1, i
 
 
+@pytest.mark.parametrize("d,ndim", [(64, 4), (32, 3), (64, 2)])
+@pytest.mark.parametrize("numerics", ["fp32", "bf16"])
+def test_head_matches_oracle_on_same_context(d, ndim, numerics):
+    """latec_gnn_head (axis pool + PredHead) against the FP64 oracle's head_of
+    on one context tensor (stored as the mode's type): FP32 mu within 5e-5;
+    bf16 (GEMMs on split-bf16 MFMA, the polynomial GELU) mu within 1e-3 and
+    log_b within 1e-2, with M not a multiple of the 16-point tile."""
+    from latec_amd import _lib
+    from latec_amd.gnn_predictor import GnnModel
+    from oracle import gnn_oracle as G
+
+    P = _params(d, 2, 7)
+    m = GnnModel(d, 2, P, ndim, numerics)
+    M, eb, vmin, vmax = 1237, 3e-3, -2.0, 5.0
+    rng = np.random.default_rng(11)
+    ctx = torch.from_numpy(rng.standard_normal((M, ndim, d)).astype(np.float32) * 1.5).cuda()
+    ctx = ctx.to(m.dtype)
+    out = [torch.empty(M, dtype=torch.float32, device="cuda") for _ in range(4)]
+    _lib.check(_lib.lib().latec_gnn_head(m.handle, ctx.data_ptr(), M, eb, vmin, vmax,
+                                         *[o.data_ptr() for o in out], None))
+    torch.cuda.synchronize()
+    pred, scale, mu, lb = (o.cpu().numpy().astype(np.float64) for o in out)
+    Pd = {k: torch.from_numpy(np.asarray(v, np.float64)) for k, v in P.items()}
+    rmu, rlb = G.head_of(Pd, ctx.float().cpu().double()[None], eb)
+    rmu, rlb = rmu.numpy()[0], rlb.numpy()[0]
+    tol_mu, tol_lb = (5e-5, 5e-4) if numerics == "fp32" else (1e-3, 1e-2)
+    np.testing.assert_allclose(mu, rmu, atol=tol_mu, rtol=0)
+    np.testing.assert_allclose(lb, rlb, atol=tol_lb, rtol=0)
+    span = vmax - vmin
+    np.testing.assert_allclose(pred, np.clip(mu * span + vmin, vmin, vmax), rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(scale, np.exp2(lb) * span, rtol=1e-5)
+    assert np.std(rmu) > 1e-2 and np.std(rlb) > 1e-2  # the head is not flat on this input
+    m.close()
+
+
 def test_geometry_matches_scan(tmp_path):
     """Stage geometry of the kernel == the oracle's exhaustive scan."""
     from latec_amd.gnn_predictor import ChunkedGNNPredictor
@@ -353,9 +388,10 @@ def test_gate_costs_match_oracle_on_real_stages(monkeypatch, shape):
 @pytest.mark.parametrize("numerics", ["bf16", "fp32"])
 def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
     """Every launch shape of the message / mix rows (knob gnn_rows: 16-row
-    tiles per wave group, waves per workgroup) codes the same stream and
-    decodes it to the same values: a row's result does not depend on the
-    tile it shares."""
+    tiles per wave group, waves per workgroup) and both line-pool kernels
+    (knob gnn_pool: line by line, loads issued ahead for 8 or 4 lines) code the same stream
+    and decode it to the same values: a row's result does not depend on the
+    tile it shares, and both pools add the lines in the same order."""
     from latec_amd import _lib
     from latec_amd.gnn_codec import GNNCodec
 
@@ -363,15 +399,18 @@ def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
     x = _field((18, 20, 17, 19), 9)
     codec = GNNCodec(ckpt, numerics=numerics)
     L = _lib.lib()
-    base = L.latec_tune_get(b"gnn_rows")
+    base = L.latec_tune_get(b"gnn_rows"), L.latec_tune_get(b"gnn_pool")
     streams, decoded = [], []
     try:
-        for v in range(4):
-            assert L.latec_tune_set(b"gnn_rows", v) == 0
+        for rows, pool in [(v, 1) for v in range(4)] + [(0, 0), (0, 2)]:
+            assert L.latec_tune_set(b"gnn_rows", rows) == 0
+            assert L.latec_tune_set(b"gnn_pool", pool) == 0
             streams.append(codec.compress(x, 1e-3))
             decoded.append(codec.uncompress(streams[0]).numpy())
     finally:
-        L.latec_tune_set(b"gnn_rows", base)
+        L.latec_tune_set(b"gnn_rows", base[0])
+        L.latec_tune_set(b"gnn_pool", base[1])
     assert all(s == streams[0] for s in streams[1:])
     assert all(np.array_equal(d, decoded[0]) for d in decoded[1:])
     assert L.latec_tune_set(b"gnn_rows", 4) != 0
+    assert L.latec_tune_set(b"gnn_pool", 3) != 0
