@@ -60,7 +60,7 @@ int latec_abi_version(void);
  * (workgroup target of a sheet launch, 1..2^20) and "sheet_min_points"
  * (smallest level that runs as sheets).  "quad_launches" and
  * "sheet_launches" read (or, set, reset) counts of line-quad and sheet
- * launches.  GNN kernels: "gnn_rows" (0..3: 16-row tiles per wave group and
+ * launches.  GNN kernels: "gnn_rows" (0..5: 16-row tiles per wave group and
  * waves per workgroup of the message / mix rows) and "gnn_pool" (0: line
  * pool line by line; 1, 2: loads issued ahead, 8 or 4 lines per batch).
  * Process-wide; no knob changes a code, a reconstructed value or a GNN

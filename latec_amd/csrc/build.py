@@ -30,8 +30,12 @@ CXX_SOURCES = ["latec_plan.cpp", "latec_entropy.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ZSTD_INC}",
           f"-I{os.path.join(os.path.dirname(PKG), 'include')}"]
+# MFMA results in VGPRs (not AGPRs): the GNN kernels apply GELU / softmax to
+# every accumulator, which an AGPR destination would first copy out with one
+# v_accvgpr_read each
 HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-                      "-packed-fp32-ops", "-munsafe-fp-atomics"]
+                      "-packed-fp32-ops", "-munsafe-fp-atomics",
+                      "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _compile(src: str) -> str:

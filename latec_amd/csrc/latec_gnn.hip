@@ -2063,8 +2063,8 @@ int lds_setup(const void *fn, int threads, size_t smem, int *per_cu) {
 
 // launch shape of the message / mix rows (knob gnn_rows): 16-row tiles per
 // wave group and waves per workgroup
-int g_gnn_rows = 0;
-constexpr int GNN_ROW_SHAPES = 4;
+int g_gnn_rows = 3;
+constexpr int GNN_ROW_SHAPES = 6;
 // line pool kernel (knob gnn_pool): 0 line by line; 1 and 2 loads issued
 // ahead, message loads of 8 (1) or 4 (2) lines at a time (items of at most
 // POOL_LMAX lines); all give the same bits
@@ -2089,6 +2089,8 @@ int launch_rows_m(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
     case 1: return launch_rows_s<D, T, MODE, 1, 4>(g, a, w, s);
     case 2: return launch_rows_s<D, T, MODE, 1, 8>(g, a, w, s);
     case 3: return launch_rows_s<D, T, MODE, 2, 8>(g, a, w, s);
+    case 4: return launch_rows_s<D, T, MODE, 2, 6>(g, a, w, s);
+    case 5: return launch_rows_s<D, T, MODE, 1, 6>(g, a, w, s);
     default: return launch_rows_s<D, T, MODE, 2, 4>(g, a, w, s);
   }
 }

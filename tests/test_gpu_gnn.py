@@ -402,7 +402,7 @@ def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
     base = L.latec_tune_get(b"gnn_rows"), L.latec_tune_get(b"gnn_pool")
     streams, decoded = [], []
     try:
-        for rows, pool in [(v, 1) for v in range(4)] + [(0, 0), (0, 2)]:
+        for rows, pool in [(v, 2) for v in range(6)] + [(0, 0), (0, 1)]:
             assert L.latec_tune_set(b"gnn_rows", rows) == 0
             assert L.latec_tune_set(b"gnn_pool", pool) == 0
             streams.append(codec.compress(x, 1e-3))
@@ -412,5 +412,5 @@ def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
         L.latec_tune_set(b"gnn_pool", base[1])
     assert all(s == streams[0] for s in streams[1:])
     assert all(np.array_equal(d, decoded[0]) for d in decoded[1:])
-    assert L.latec_tune_set(b"gnn_rows", 4) != 0
+    assert L.latec_tune_set(b"gnn_rows", 6) != 0
     assert L.latec_tune_set(b"gnn_pool", 3) != 0
