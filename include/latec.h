@@ -61,8 +61,10 @@ int latec_abi_version(void);
  * (smallest level that runs as sheets).  "quad_launches" and
  * "sheet_launches" read (or, set, reset) counts of line-quad and sheet
  * launches.  GNN kernels: "gnn_rows" (0..5: 16-row tiles per wave group and
- * waves per workgroup of the message / mix rows) and "gnn_pool" (0: line
- * pool line by line; 1, 2: loads issued ahead, 8 or 4 lines per batch).
+ * waves per workgroup of the message / mix rows), "gnn_pool" (0: line
+ * pool line by line; 1, 2: loads issued ahead, 8 or 4 lines per batch) and
+ * "gnn_geo_table" (1: stage geometry through per-stage lookup tables of the
+ * first known cell along each line; 0: a scan of the cells).
  * Process-wide; no knob changes a code, a reconstructed value or a GNN
  * prediction. */
 int latec_tune_set(const char *name, int64_t value);

@@ -178,6 +178,8 @@ struct latec_gnn {
   float scal[4] = {0, 0, 0, 0};    // line pool bk2, null_k; axis pool bk2, null_k
   int n_cu = 256;
   int64_t *d_line_off = nullptr;   // frame offsets of the lines (cached per frame)
+  // geometry lookup tables (geo_table_kernel), per (stage, log2 stride)
+  std::vector<std::pair<int64_t, uint8_t *>> geo_tabs;
   int64_t line_off_key[LATEC_MAX_DIM] = {0};
   // workspace
   void *work = nullptr;
@@ -262,14 +264,44 @@ __device__ __forceinline__ int cell_state(const GeomArgs &g, const int64_t *c) {
   return (st == 0 || g.coded[cid]) ? 2 : 1;
 }
 
+// Whether a cell is known before a stage depends only on its coordinates'
+// residues mod the stride (stage_of), so the first known cell along a line
+// from a point is a function of the point's residues, the line and the side:
+// tab[(r * L + l) * 2 + side] = the first t in 1..limit with a known cell at
+// r + sg t dir_l (0 if none), r the residues packed S bits per axis (axis 0
+// highest).  One thread per (residues, line, side).
+__global__ void geo_table_kernel(GeomArgs g, const int8_t *__restrict__ dirs, int64_t n,
+                                 uint8_t *__restrict__ tab) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int side = (int)(i & 1), l = (int)((i >> 1) % g.L);
+  const int64_t r = (i >> 1) / g.L;
+  const int S = g.levels_log2;
+  const int64_t m = ((int64_t)1 << S) - 1;
+  int64_t c[LATEC_MAX_DIM], nb[LATEC_MAX_DIM];
+  for (int j = 0; j < g.ndim; j++) c[j] = (r >> (S * (g.ndim - 1 - j))) & m;
+  const int sg = side ? -1 : 1;
+  int hit = 0;
+  for (int t = 1; t <= g.limit; t++) {
+    for (int j = 0; j < g.ndim; j++) nb[j] = c[j] + (int64_t)sg * t * dirs[l * g.ndim + j];
+    if (stage_of(nb, g.ndim, S) < g.stage) {
+      hit = t;
+      break;
+    }
+  }
+  tab[i] = (uint8_t)hit;
+}
+
 // One thread per query point of the stage (enumerated class by class).
 // Writes qidx[rank] (frame flat index), gidx[rank] (tensor flat index) and
 // steps[rank * L + l] = sp | sn << 16 (0 = no usable sample on that side).
+// With `tab` (geo_table_kernel) each line and side costs one lookup and the
+// usability of the cell found; without it, a scan of up to `limit` cells.
 __global__ void geometry_kernel(const ClassDesc *__restrict__ cls, GeomArgs g,
                                 const int8_t *__restrict__ dirs, int64_t total,
                                 const int64_t *__restrict__ cls_start,
                                 int64_t *__restrict__ qidx, int64_t *__restrict__ gidx,
-                                int32_t *__restrict__ steps) {
+                                int32_t *__restrict__ steps, const uint8_t *__restrict__ tab) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   int ci = 0;
@@ -292,6 +324,27 @@ __global__ void geometry_kernel(const ClassDesc *__restrict__ cls, GeomArgs g,
   qidx[rank] = qf;
   gidx[rank] = gf;
   int64_t nb[LATEC_MAX_DIM];
+  if (tab) {
+    const int S = g.levels_log2;
+    const int64_t mk = ((int64_t)1 << S) - 1;
+    int64_t r = 0;
+    for (int j = 0; j < g.ndim; j++) r = (r << S) | (coord[j] & mk);
+    const uint8_t *tr = tab + r * g.L * 2;
+    for (int l = 0; l < g.L; l++) {
+      int res[2];
+      for (int side = 0; side < 2; side++) {
+        const int t = tr[2 * l + side];
+        res[side] = 0;
+        if (t) {
+          const int sg = side ? -1 : 1;
+          for (int j = 0; j < g.ndim; j++) nb[j] = coord[j] + (int64_t)sg * t * dirs[l * g.ndim + j];
+          res[side] = cell_state(g, nb) == 2 ? t : 0;
+        }
+      }
+      steps[rank * g.L + l] = res[0] | (res[1] << 16);
+    }
+    return;
+  }
   for (int l = 0; l < g.L; l++) {
     int res[2];
     for (int side = 0; side < 2; side++) {
@@ -1659,45 +1712,60 @@ struct GateArgs {
   const float *x, *pred, *scale;
   const float *ip[3];
   const float *thr;            // 63 f32 thresholds of the 64-level scale grid
-  const double *grid;          // 64 grid scales
+  const double *lvk;           // (3, 64) per grid level: bits0, eb / (b ln 2), bits2
   double bound[12];            // eb * 2^T, T = 2..13
   double *partial;             // (blocks, n_cand, 13)
 };
 
-__device__ __forceinline__ double gate_bits(double absr, double b, const GateArgs &g) {
+// The reference's rate proxy (gnn_codec.py:777-797) at a snapped scale b:
+// with k = rint(|r| / 2eb),
+//   k = 0: bits = -log2(-expm1(-eb / b))
+//   k > 0: bits = -log2(0.5 exp(-(2k - 1) eb / b) (-expm1(-2 eb / b)))
+//               = 1 + (2k - 1) eb / (b ln 2) - log2(-expm1(-2 eb / b)),
+// clamped to 24 bits, 56 past the radius.  b takes one of 64 grid values, so
+// the two logarithms and eb / (b ln 2) are per-level constants (lvk, built on
+// the host in float64); the rewrite moves each point's bits by about an ulp
+// (the sums stay within 1e-12 of the reference's).
+__device__ __forceinline__ double gate_bits(double absr, int lev, const GateArgs &g,
+                                            const double *lvk) {
   const double k = rint(absr / g.two_eb);
-  double p;
-  if (k == 0.0)
-    p = -expm1(-g.eb / b);
-  else
-    p = 0.5 * exp(-((2.0 * k - 1.0) * g.eb / b)) * -expm1(-2.0 * g.eb / b);
-  double bits = -log2(p);
+  double bits = k == 0.0 ? lvk[lev] : 1.0 + (2.0 * k - 1.0) * lvk[64 + lev] + lvk[128 + lev];
   if (bits > 24.0) bits = 24.0;
   if (k >= (double)g.radius) bits = 56.0;
   return bits;
 }
 
-__device__ __forceinline__ double snap_scale(float b, const GateArgs &g) {
-  int lo = 0, hi = 63;  // bucketize(right=True): count of thresholds <= b
+// bucketize(right=True) of b against the 63 thresholds (in LDS): the grid level
+__device__ __forceinline__ int snap_level(float b, const float *thr) {
+  int lo = 0, hi = 63;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (g.thr[mid] <= b)
+    if (thr[mid] <= b)
       lo = mid + 1;
     else
       hi = mid;
   }
-  return g.grid[lo];
+  return lo;
 }
 
 constexpr int GATE_BLOCK = 256;
 constexpr int GATE_NB = 13;
+constexpr int GATE_NC = 13;  // 1 + 4 * 3 candidates at most
 
-// partial[b][c][bucket]: sums over the points of block b (fixed tree order).
+// partial[b][c][bucket]: sums over the points of block b.  Each thread sums
+// its points in order, a butterfly of lane shuffles sums each wave, and the
+// four wave sums are added in wave order: a fixed order for a given n.
 __global__ __launch_bounds__(GATE_BLOCK) void gate_cost_kernel(GateArgs g) {
-  __shared__ double red[GATE_BLOCK];
+  __shared__ double wsum[GATE_BLOCK / 64][GATE_NC][GATE_NB];
+  __shared__ double lvk[3 * 64];
+  __shared__ float thr[64];
+  for (int i = threadIdx.x; i < 3 * 64; i += GATE_BLOCK) lvk[i] = g.lvk[i];
+  if (threadIdx.x < 63) thr[threadIdx.x] = g.thr[threadIdx.x];
+  __syncthreads();
   const int64_t per_block = (g.n + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per_block;
   const int64_t hi = lo + per_block < g.n ? lo + per_block : g.n;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (uint32_t c = 0; c < g.n_cand; c++) {
     double acc[GATE_NB];
 #pragma unroll
@@ -1710,21 +1778,36 @@ __global__ __launch_bounds__(GATE_BLOCK) void gate_cost_kernel(GateArgs g) {
       int bucket = 0;
       while (bucket < 12 && g.bound[bucket] < b) bucket++;
       const float bs = c ? sc * exp2f(-(float)sh) : sc;
-      const double bits = gate_bits(fabs((double)g.x[i] - (double)ref[i]), snap_scale(bs, g), g);
+      const double bits =
+          gate_bits(fabs((double)g.x[i] - (double)ref[i]), snap_level(bs, thr), g, lvk);
 #pragma unroll
       for (int bb = 0; bb < GATE_NB; bb++)
         if (bb == bucket) acc[bb] += bits;
     }
+#pragma unroll
     for (int b = 0; b < GATE_NB; b++) {
-      red[threadIdx.x] = acc[b];
-      __syncthreads();
-      for (int o = GATE_BLOCK / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) g.partial[((int64_t)blockIdx.x * g.n_cand + c) * GATE_NB + b] = red[0];
-      __syncthreads();
+      double v = acc[b];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) wsum[wv][c][b] = v;
     }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < (int)g.n_cand * GATE_NB; t += GATE_BLOCK) {
+    const int c = t / GATE_NB, b = t % GATE_NB;
+    double v = wsum[0][c][b];
+    for (int w = 1; w < GATE_BLOCK / 64; w++) v += wsum[w][c][b];
+    g.partial[((int64_t)blockIdx.x * g.n_cand + c) * GATE_NB + b] = v;
+  }
+}
+
+// out[i] = sum over blocks b, in block order, of partial[b][i]
+__global__ void gate_sum_kernel(const double *__restrict__ partial, int blocks, int width,
+                                double *__restrict__ out) {
+  for (int i = threadIdx.x; i < width; i += blockDim.x) {
+    double v = 0.0;
+    for (int b = 0; b < blocks; b++) v += partial[(int64_t)b * width + i];
+    out[i] = v;
   }
 }
 
@@ -2069,6 +2152,8 @@ constexpr int GNN_ROW_SHAPES = 6;
 // ahead, message loads of 8 (1) or 4 (2) lines at a time (items of at most
 // POOL_LMAX lines); all give the same bits
 int g_gnn_pool = 2;
+// geometry through per-stage lookup tables (knob gnn_geo_table; same steps)
+int g_gnn_geo_table = 1;
 
 template <int D, typename T, int MODE, int NT, int WV>
 int launch_rows_s(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
@@ -2131,6 +2216,14 @@ int gnn_tune_set(const char *name, int64_t v) {
     g_gnn_rows = (int)v;
     return LATEC_OK;
   }
+  if (!strcmp(name, "gnn_geo_table")) {
+    if (v < 0 || v > 1) {
+      set_error(std::string("tuning value out of range for ") + name);
+      return LATEC_E_ARG;
+    }
+    g_gnn_geo_table = (int)v;
+    return LATEC_OK;
+  }
   if (!strcmp(name, "gnn_pool")) {
     if (v < 0 || v > 2) {
       set_error(std::string("tuning value out of range for ") + name);
@@ -2145,6 +2238,7 @@ int gnn_tune_set(const char *name, int64_t v) {
 int64_t gnn_tune_get(const char *name) {
   if (!strcmp(name, "gnn_rows")) return g_gnn_rows;
   if (!strcmp(name, "gnn_pool")) return g_gnn_pool;
+  if (!strcmp(name, "gnn_geo_table")) return g_gnn_geo_table;
   set_error(std::string("unknown tuning knob ") + name);
   return LATEC_E_ARG;
 }
@@ -2474,6 +2568,7 @@ void latec_gnn_destroy(latec_gnn *g) {
   if (g->dev) (void)hipFree(g->dev);
   if (g->work) (void)hipFree(g->work);
   if (g->d_line_off) (void)hipFree(g->d_line_off);
+  for (auto &t : g->geo_tabs) (void)hipFree(t.second);
   delete g;
 }
 
@@ -2541,8 +2636,27 @@ int latec_gnn_geometry(latec_gnn *g, latec_plan *p, int stage, const latec_gnn_f
                                       hipMemcpyHostToDevice, s), "hipMemcpyAsync");
   if (rc) return rc;
   const int64_t total = starts[nc];
+  // lookup table of the stage: (2^levels)^ndim residue tuples x L lines x 2
+  // sides, built once per handle (kept up to 2^16 residue tuples)
+  uint8_t *tab = nullptr;
+  if (g_gnn_geo_table && p->levels * p->ndim <= 16 && a.limit <= 255) {
+    const int64_t key = ((int64_t)stage << 8) | p->levels;
+    for (auto &t : g->geo_tabs)
+      if (t.first == key) tab = t.second;
+    if (!tab) {
+      const int64_t n = ((int64_t)1 << (p->levels * p->ndim)) * g->L * 2;
+      rc = hip_ok(hipMalloc(&tab, (size_t)n), "hipMalloc(geometry table)");
+      if (rc) return rc;
+      g->geo_tabs.push_back({key, tab});
+      hipLaunchKernelGGL(geo_table_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, a,
+                         (const int8_t *)g->d_dirs, n, tab);
+      rc = hip_ok(hipGetLastError(), "geo_table_kernel launch");
+      if (rc) return rc;
+    }
+  }
   hipLaunchKernelGGL(geometry_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, dcls, a,
-                     (const int8_t *)g->d_dirs, total, dstart, d_qidx, d_gidx, d_steps);
+                     (const int8_t *)g->d_dirs, total, dstart, d_qidx, d_gidx, d_steps,
+                     (const uint8_t *)tab);
   rc = hip_ok(hipGetLastError(), "geometry_kernel launch");
   if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die here
   return rc;
@@ -2658,18 +2772,26 @@ int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
   for (int i = 0; i < nc * GATE_NB; i++) costs[i] = 0.0;
   if (n == 0) return LATEC_OK;
   hipStream_t s = (hipStream_t)stream;
-  int blocks = (int)((n + 4095) / 4096);
+  int blocks = (int)((n + 1023) / 1024);
   if (blocks > 1024) blocks = 1024;
-  const size_t bpart = sizeof(double) * (size_t)blocks * nc * GATE_NB;
-  int rc = ensure_work(g, bpart + 1024);
+  const size_t bpart = sizeof(double) * ((size_t)blocks + 1) * nc * GATE_NB;
+  int rc = ensure_work(g, bpart + 256 + 256 + sizeof(double) * 3 * 64);
   if (rc) return rc;
   double *part = (double *)g->work;
   float *dthr = (float *)((unsigned char *)g->work + ((bpart + 255) & ~(size_t)255));
-  double *dgrid = (double *)((unsigned char *)dthr + 256);
+  double *dlvk = (double *)((unsigned char *)dthr + 256);
+  // per grid level: -log2(-expm1(-eb/b)), eb / (b ln 2), -log2(-expm1(-2 eb/b))
+  double lvk[3 * 64];
+  for (int l = 0; l < 64; l++) {
+    const double b = grid64[l];
+    lvk[l] = -std::log2(-std::expm1(-eb / b));
+    lvk[64 + l] = eb / b / std::log(2.0);
+    lvk[128 + l] = -std::log2(-std::expm1(-2.0 * eb / b));
+  }
   rc = hip_ok(hipMemcpyAsync(dthr, thr63, sizeof(float) * 63, hipMemcpyHostToDevice, s),
               "hipMemcpyAsync");
-  if (!rc) rc = hip_ok(hipMemcpyAsync(dgrid, grid64, sizeof(double) * 64,
-                                      hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  if (!rc) rc = hip_ok(hipMemcpyAsync(dlvk, lvk, sizeof(lvk), hipMemcpyHostToDevice, s),
+                       "hipMemcpyAsync");
   if (rc) return rc;
   GateArgs a{};
   a.n = n;
@@ -2682,19 +2804,20 @@ int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
   a.scale = d_scale;
   for (int i = 0; i < n_interp; i++) a.ip[i] = d_ip[i];
   a.thr = dthr;
-  a.grid = dgrid;
+  a.lvk = dlvk;
   for (int t = 0; t < 12; t++) a.bound[t] = eb * std::ldexp(1.0, t + 2);
   a.partial = part;
   hipLaunchKernelGGL(gate_cost_kernel, dim3(blocks), dim3(GATE_BLOCK), 0, s, a);
   rc = hip_ok(hipGetLastError(), "gate_cost_kernel launch");
   if (rc) return rc;
-  std::vector<double> h((size_t)blocks * nc * GATE_NB);
-  rc = hip_ok(hipMemcpyAsync(h.data(), part, bpart, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  double *sums = part + (size_t)blocks * nc * GATE_NB;
+  hipLaunchKernelGGL(gate_sum_kernel, dim3(1), dim3(256), 0, s, (const double *)part, blocks,
+                     nc * GATE_NB, sums);
+  rc = hip_ok(hipGetLastError(), "gate_sum_kernel launch");
+  if (!rc) rc = hip_ok(hipMemcpyAsync(costs, sums, sizeof(double) * nc * GATE_NB,
+                                      hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
   if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
-  if (rc) return rc;
-  for (int b = 0; b < blocks; b++)
-    for (int i = 0; i < nc * GATE_NB; i++) costs[i] += h[(size_t)b * nc * GATE_NB + i];
-  return LATEC_OK;
+  return rc;
 }
 
 int latec_gnn_gate_apply(float *d_pred, float *d_scale, const float *d_ip, int64_t n,

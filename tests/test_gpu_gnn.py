@@ -21,6 +21,18 @@ def _params(d, agg, seed):
     return P
 
 
+@pytest.fixture(params=[1, 0], ids=["table", "scan"])
+def geo_table(request):
+    """Geometry through the per-stage lookup tables (1) or the cell scan (0)."""
+    from latec_amd import _lib
+
+    L = _lib.lib()
+    base = L.latec_tune_get(b"gnn_geo_table")
+    assert L.latec_tune_set(b"gnn_geo_table", request.param) == 0
+    yield request.param
+    L.latec_tune_set(b"gnn_geo_table", base)
+
+
 def _ckpt(tmp_path, d=32, agg=2, seed=0, name="m.pt"):
     from latec_amd import gnn_model as GM
 
@@ -113,8 +125,9 @@ def test_head_matches_oracle_on_same_context(d, ndim, numerics):
     m.close()
 
 
-def test_geometry_matches_scan(tmp_path):
-    """Stage geometry of the kernel == the oracle's exhaustive scan."""
+def test_geometry_matches_scan(tmp_path, geo_table):
+    """Stage geometry of the kernel (lookup tables or cell scan) == the
+    oracle's exhaustive scan."""
     from latec_amd.gnn_predictor import ChunkedGNNPredictor
     from oracle import gnn_oracle as G
 
@@ -232,10 +245,11 @@ def _stage_tile(ndim, levels, stride):
     ((21, 18, 17), 2, (8, 8, 8)),
     ((9, 10, 8, 9), 1, (4, 4, 4, 4)),
 ])
-def test_multichunk_halo_geometry_matches_scan(tmp_path, shape, levels, edges):
+def test_multichunk_halo_geometry_matches_scan(tmp_path, shape, levels, edges, geo_table):
     """Every chunk of an explicit chunk grid (extended blocks, halo frames):
-    the kernel's frame indices, tensor indices and steps equal an exhaustive
-    scan of the padded frame with the earlier chunks coded."""
+    the kernel's frame indices, tensor indices and steps (lookup tables or
+    cell scan) equal an exhaustive scan of the padded frame with the earlier
+    chunks coded."""
     from latec_amd.gnn_predictor import ChunkedGNNPredictor
     from oracle import gnn_oracle as G
     from oracle import oracle as O
@@ -388,10 +402,12 @@ def test_gate_costs_match_oracle_on_real_stages(monkeypatch, shape):
 @pytest.mark.parametrize("numerics", ["bf16", "fp32"])
 def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
     """Every launch shape of the message / mix rows (knob gnn_rows: 16-row
-    tiles per wave group, waves per workgroup) and both line-pool kernels
-    (knob gnn_pool: line by line, loads issued ahead for 8 or 4 lines) code the same stream
-    and decode it to the same values: a row's result does not depend on the
-    tile it shares, and both pools add the lines in the same order."""
+    tiles per wave group, waves per workgroup), both line-pool kernels (knob
+    gnn_pool: line by line, loads issued ahead for 8 or 4 lines) and the
+    geometry with or without its lookup tables (knob gnn_geo_table) code the
+    same stream and decode it to the same values: a row's result does not
+    depend on the tile it shares, both pools add the lines in the same order
+    and both geometry paths find the same cells."""
     from latec_amd import _lib
     from latec_amd.gnn_codec import GNNCodec
 
@@ -399,18 +415,20 @@ def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
     x = _field((18, 20, 17, 19), 9)
     codec = GNNCodec(ckpt, numerics=numerics)
     L = _lib.lib()
-    base = L.latec_tune_get(b"gnn_rows"), L.latec_tune_get(b"gnn_pool")
+    knobs = (b"gnn_rows", b"gnn_pool", b"gnn_geo_table")
+    base = [L.latec_tune_get(k) for k in knobs]
     streams, decoded = [], []
     try:
-        for rows, pool in [(v, 2) for v in range(6)] + [(0, 0), (0, 1)]:
-            assert L.latec_tune_set(b"gnn_rows", rows) == 0
-            assert L.latec_tune_set(b"gnn_pool", pool) == 0
+        for vals in [(v, 2, 1) for v in range(6)] + [(0, 0, 1), (0, 1, 1), (3, 2, 0)]:
+            for k, v in zip(knobs, vals):
+                assert L.latec_tune_set(k, v) == 0
             streams.append(codec.compress(x, 1e-3))
             decoded.append(codec.uncompress(streams[0]).numpy())
     finally:
-        L.latec_tune_set(b"gnn_rows", base[0])
-        L.latec_tune_set(b"gnn_pool", base[1])
+        for k, v in zip(knobs, base):
+            L.latec_tune_set(k, v)
     assert all(s == streams[0] for s in streams[1:])
     assert all(np.array_equal(d, decoded[0]) for d in decoded[1:])
     assert L.latec_tune_set(b"gnn_rows", 6) != 0
     assert L.latec_tune_set(b"gnn_pool", 3) != 0
+    assert L.latec_tune_set(b"gnn_geo_table", 2) != 0
