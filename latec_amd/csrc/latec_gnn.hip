@@ -180,6 +180,8 @@ struct latec_gnn {
   int64_t *d_line_off = nullptr;   // frame offsets of the lines (cached per frame)
   // geometry lookup tables (geo_table_kernel), per (stage, log2 stride)
   std::vector<std::pair<int64_t, uint8_t *>> geo_tabs;
+  // gate tables (thresholds, per-level constants) per eb
+  std::vector<std::pair<uint64_t, void *>> gate_tabs;
   int64_t line_off_key[LATEC_MAX_DIM] = {0};
   // workspace
   void *work = nullptr;
@@ -2569,6 +2571,7 @@ void latec_gnn_destroy(latec_gnn *g) {
   if (g->work) (void)hipFree(g->work);
   if (g->d_line_off) (void)hipFree(g->d_line_off);
   for (auto &t : g->geo_tabs) (void)hipFree(t.second);
+  for (auto &t : g->gate_tabs) (void)hipFree(t.second);
   delete g;
 }
 
@@ -2760,39 +2763,67 @@ int latec_gnn_halo_frame(latec_gnn *g, latec_plan *p, const latec_gnn_frame *fr,
   return GNN_DISPATCH(halo_t, d_E, frows, gflat, d_recon, (int64_t)n, vmin, vmax, s);
 }
 
-int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
-                         const float *d_scale, const float *const *d_ip, int n_interp,
-                         int64_t n, double eb, uint32_t radius, const float *thr63,
-                         const double *grid64, double *costs, void *stream) {
-  if (!g || n_interp < 0 || n_interp > 3 || !thr63 || !grid64 || !costs) {
-    latec::set_error("latec_gnn_gate_costs: bad argument");
-    return LATEC_E_ARG;
-  }
-  const int nc = 1 + 4 * n_interp;
-  for (int i = 0; i < nc * GATE_NB; i++) costs[i] = 0.0;
-  if (n == 0) return LATEC_OK;
-  hipStream_t s = (hipStream_t)stream;
-  int blocks = (int)((n + 1023) / 1024);
-  if (blocks > 1024) blocks = 1024;
-  const size_t bpart = sizeof(double) * ((size_t)blocks + 1) * nc * GATE_NB;
-  int rc = ensure_work(g, bpart + 256 + 256 + sizeof(double) * 3 * 64);
-  if (rc) return rc;
-  double *part = (double *)g->work;
-  float *dthr = (float *)((unsigned char *)g->work + ((bpart + 255) & ~(size_t)255));
-  double *dlvk = (double *)((unsigned char *)dthr + 256);
+}  // extern "C"
+
+namespace {
+
+// Device copies of a bound's gate tables (63 thresholds, then the per-level
+// constants of gate_bits), uploaded once per distinct eb and handle: the
+// thresholds and grid a caller passes are functions of eb
+// (_gate_scale_thresholds).
+int gate_tables(latec_gnn *g, double eb, const float *thr63, const double *grid64,
+                const float **d_thr, const double **d_lvk) {
+  uint64_t key;
+  memcpy(&key, &eb, 8);
+  for (auto &t : g->gate_tabs)
+    if (t.first == key) {
+      *d_thr = (const float *)t.second;
+      *d_lvk = (const double *)((unsigned char *)t.second + 256);
+      return LATEC_OK;
+    }
+  std::vector<unsigned char> h(256 + sizeof(double) * 3 * 64, 0);
+  memcpy(h.data(), thr63, sizeof(float) * 63);
+  double *lvk = (double *)(h.data() + 256);
   // per grid level: -log2(-expm1(-eb/b)), eb / (b ln 2), -log2(-expm1(-2 eb/b))
-  double lvk[3 * 64];
   for (int l = 0; l < 64; l++) {
     const double b = grid64[l];
     lvk[l] = -std::log2(-std::expm1(-eb / b));
     lvk[64 + l] = eb / b / std::log(2.0);
     lvk[128 + l] = -std::log2(-std::expm1(-2.0 * eb / b));
   }
-  rc = hip_ok(hipMemcpyAsync(dthr, thr63, sizeof(float) * 63, hipMemcpyHostToDevice, s),
-              "hipMemcpyAsync");
-  if (!rc) rc = hip_ok(hipMemcpyAsync(dlvk, lvk, sizeof(lvk), hipMemcpyHostToDevice, s),
-                       "hipMemcpyAsync");
+  void *d = nullptr;
+  int rc = hip_ok(hipMalloc(&d, h.size()), "hipMalloc(gate tables)");
   if (rc) return rc;
+  rc = hip_ok(hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice), "hipMemcpy(gate tables)");
+  if (rc) {
+    (void)hipFree(d);
+    return rc;
+  }
+  g->gate_tabs.push_back({key, d});
+  *d_thr = (const float *)d;
+  *d_lvk = (const double *)((unsigned char *)d + 256);
+  return LATEC_OK;
+}
+
+// gate_cost_kernel + gate_sum_kernel: the bucket sums ((1 + 4 n_interp) * 13
+// doubles) at byte sums_off of the handle's workspace, the block partials
+// before them.
+int gate_costs_launch(latec_gnn *g, const float *d_x, const float *d_pred, const float *d_scale,
+                      const float *const *d_ip, int n_interp, int64_t n, double eb,
+                      uint32_t radius, const float *thr63, const double *grid64,
+                      size_t sums_off, hipStream_t s) {
+  const int nc = 1 + 4 * n_interp;
+  int blocks = (int)((n + 1023) / 1024);
+  if (blocks > 1024) blocks = 1024;
+  const size_t bpart = sizeof(double) * (size_t)blocks * nc * GATE_NB;
+  const float *dthr;
+  const double *dlvk;
+  int rc = gate_tables(g, eb, thr63, grid64, &dthr, &dlvk);
+  if (rc) return rc;
+  rc = ensure_work(g, (bpart > sums_off ? bpart : sums_off) + sizeof(double) * nc * GATE_NB);
+  if (rc) return rc;
+  double *part = (double *)g->work;
+  double *d_sums = (double *)((unsigned char *)g->work + sums_off);
   GateArgs a{};
   a.n = n;
   a.eb = eb;
@@ -2810,12 +2841,34 @@ int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
   hipLaunchKernelGGL(gate_cost_kernel, dim3(blocks), dim3(GATE_BLOCK), 0, s, a);
   rc = hip_ok(hipGetLastError(), "gate_cost_kernel launch");
   if (rc) return rc;
-  double *sums = part + (size_t)blocks * nc * GATE_NB;
   hipLaunchKernelGGL(gate_sum_kernel, dim3(1), dim3(256), 0, s, (const double *)part, blocks,
-                     nc * GATE_NB, sums);
-  rc = hip_ok(hipGetLastError(), "gate_sum_kernel launch");
-  if (!rc) rc = hip_ok(hipMemcpyAsync(costs, sums, sizeof(double) * nc * GATE_NB,
-                                      hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+                     nc * GATE_NB, d_sums);
+  return hip_ok(hipGetLastError(), "gate_sum_kernel launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
+                         const float *d_scale, const float *const *d_ip, int n_interp,
+                         int64_t n, double eb, uint32_t radius, const float *thr63,
+                         const double *grid64, double *costs, void *stream) {
+  if (!g || n_interp < 0 || n_interp > 3 || !thr63 || !grid64 || !costs) {
+    latec::set_error("latec_gnn_gate_costs: bad argument");
+    return LATEC_E_ARG;
+  }
+  const int nc = 1 + 4 * n_interp;
+  for (int i = 0; i < nc * GATE_NB; i++) costs[i] = 0.0;
+  if (n == 0) return LATEC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t sums_off = ((sizeof(double) * 1025 * nc * GATE_NB + 255) / 256) * 256;
+  int rc = gate_costs_launch(g, d_x, d_pred, d_scale, d_ip, n_interp, n, eb, radius, thr63,
+                             grid64, sums_off, s);
+  if (rc) return rc;
+  rc = hip_ok(hipMemcpyAsync(costs, (unsigned char *)g->work + sums_off,
+                             sizeof(double) * nc * GATE_NB, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
   if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
   return rc;
 }
