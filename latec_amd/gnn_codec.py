@@ -271,6 +271,33 @@ class _ChunkSchedule:
                         out[cl.offset:], cl.sizes, [8 * x for x in cl.strides], writeable=True)
                     dst[...] = v
                 self.keep[s] = out
+        self._dkeep = {}
+
+    def device_keep(self, s):
+        """keep[s] on the device (None when the whole stage is coded), uploaded
+        once."""
+        if self.keep[s] is None:
+            return None
+        d = self._dkeep.get(s)
+        if d is None:
+            d = self._dkeep[s] = D.to_device(self.keep[s])
+        return d
+
+
+_SCHEDULES: dict = {}
+
+
+def _chunk_schedule(cshape, levels, stride, low_axes) -> _ChunkSchedule:
+    """_ChunkSchedule, kept per (chunk shape, levels, stride, low axes): a
+    chunked tensor repeats a few chunk shapes, and every call on a tensor of
+    the same shape repeats them all."""
+    key = (tuple(int(n) for n in cshape), int(levels), int(stride), tuple(low_axes))
+    sched = _SCHEDULES.get(key)
+    if sched is None:
+        if len(_SCHEDULES) >= 256:
+            _SCHEDULES.clear()
+        sched = _SCHEDULES[key] = _ChunkSchedule(cshape, levels, stride, low_axes)
+    return sched
 
 
 class GNNCompressorCodec:
@@ -456,7 +483,7 @@ class GNNCompressorCodec:
         for ci in range(n_chunks):
             sls = pred_obj.chunk_slices(ci)
             cshape = tuple(sl.stop - sl.start for sl in sls)
-            sched = _ChunkSchedule(cshape, levels, stride, pred_obj.low_axes(ci))
+            sched = _chunk_schedule(cshape, levels, stride, pred_obj.low_axes(ci))
             plan = get_plan(cshape, 1, levels, stride, 1)
             single = n_chunks == 1
             pred_obj.start_wave([ci], recon)
@@ -466,8 +493,7 @@ class GNNCompressorCodec:
                     futures.append(pool.submit(pack_stage, *empty[:2], rans_levels=empty[2],
                                                rans_tables=tables[s]))
                     continue
-                keep = sched.keep[s]
-                dkeep = D.to_device(keep) if keep is not None else None
+                dkeep = sched.device_keep(s)
                 e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
                 e0.record()
                 p_full, sc_full = pred_obj.predict_wave_stage(s, recon, ebs[s])
@@ -589,7 +615,7 @@ class GNNCompressorCodec:
         for ci in range(pred_obj.n_chunks):
             sls = pred_obj.chunk_slices(ci)
             cshape = tuple(sl.stop - sl.start for sl in sls)
-            sched = _ChunkSchedule(cshape, levels, stride, pred_obj.low_axes(ci))
+            sched = _chunk_schedule(cshape, levels, stride, pred_obj.low_axes(ci))
             plan = get_plan(cshape, 1, levels, stride, 1)
             single = pred_obj.n_chunks == 1
             pred_obj.start_wave([ci], recon)
@@ -599,8 +625,7 @@ class GNNCompressorCodec:
                     _, _, off = unpack_stage(payload, off, rans_levels=np.zeros(0, np.uint8),
                                              rans_tables=tables[s], radius=radius)
                     continue
-                keep = sched.keep[s]
-                dkeep = D.to_device(keep) if keep is not None else None
+                dkeep = sched.device_keep(s)
                 p_full, sc_full = pred_obj.predict_wave_stage(s, recon, ebs[s])
                 gidx = pred_obj.stage_gidx(s)
                 if dkeep is not None:
@@ -643,12 +668,7 @@ class GNNCompressorCodec:
                 if len(codes) != n:
                     raise ValueError("stage code count mismatch")
                 zeros = np.flatnonzero(codes == 0).astype(np.uint32)
-                hc = codes.astype(np.uint16 if o.code_bytes == 2 else np.uint32)
-                d_codes = D.to_device(hc)
-                d_rank = D.to_device(zeros if len(zeros) else np.zeros(1, np.uint32))
-                d_val = D.to_device(np.ascontiguousarray(outl, np.float32)
-                                    if len(outl) else np.zeros(1, np.float32))
-                d_cnt = D.to_device(np.array([len(zeros)], np.uint32))
+                d_codes, d_rank, d_val, d_cnt = _stage_upload(codes, zeros, outl, o.code_bytes)
                 rec = T.empty(n, dtype=T.float32, device=D.device())
                 check(lib().latec_dequantize(ptr(p), ptr(d_codes), n, ptr(d_rank), ptr(d_val),
                                              ptr(d_cnt), float(ebs[s]), _optp(o), ptr(rec),
@@ -671,6 +691,29 @@ class GNNCompressorCodec:
 
 def _optp(o):
     return ctypes.cast(ctypes.pointer(o), ctypes.c_void_p)
+
+
+def _stage_upload(codes, zeros, outl, code_bytes):
+    """A decoded stage's codes, outlier ranks, outlier values and outlier
+    count as device views of one host-to-device copy from the thread's
+    pinned staging buffer (the caller's next device-to-host read orders the
+    buffer's reuse after the copy)."""
+    T = D.torch()
+    al = lambda b: (b + 255) & ~255  # noqa: E731
+    n, k = len(codes), len(zeros)
+    o1 = al(n * code_bytes)
+    o2 = o1 + al(4 * max(k, 1))
+    o3 = o2 + al(4 * max(k, 1))
+    tot = o3 + 16
+    hv, ha = D.staging(tot, np.uint8)
+    ha[:n * code_bytes] = codes.astype(np.uint16 if code_bytes == 2 else np.uint32).view(np.uint8)
+    ha[o1:o1 + 4 * k] = zeros.view(np.uint8)
+    ha[o2:o2 + 4 * k] = np.ascontiguousarray(outl, np.float32).view(np.uint8)
+    ha[o3:o3 + 4] = np.array([k], np.uint32).view(np.uint8)
+    dbuf = T.empty(tot, dtype=T.uint8, device=D.device())
+    dbuf.copy_(hv, non_blocking=True)
+    return (dbuf[:n * code_bytes], dbuf[o1:o1 + 4 * max(k, 1)], dbuf[o2:o2 + 4 * max(k, 1)],
+            dbuf[o3:o3 + 4])
 
 
 GNNCodec = GNNCompressorCodec
