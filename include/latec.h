@@ -60,8 +60,8 @@ int latec_abi_version(void);
  * (workgroup target of a sheet launch, 1..2^20) and "sheet_min_points"
  * (smallest level that runs as sheets).  "quad_launches" and
  * "sheet_launches" read (or, set, reset) counts of line-quad and sheet
- * launches.  GNN kernels: "gnn_rows" (0..5: 16-row tiles per wave group and
- * waves per workgroup of the message / mix rows), "gnn_pool" (0: line
+ * launches.  GNN kernels: "gnn_rows" (0..7: 16-row tiles per wave group,
+ * waves per workgroup and minimum waves per SIMD of the message / mix rows), "gnn_pool" (0: line
  * pool line by line; 1, 2: loads issued ahead, 8 or 4 lines per batch) and
  * "gnn_geo_table" (1: stage geometry through per-stage lookup tables of the
  * first known cell along each line; 0: a scan of the cells).

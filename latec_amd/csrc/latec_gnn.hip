@@ -807,8 +807,9 @@ __device__ __forceinline__ typename Mf<T>::frag bias_frag(int g) {
 // messages also get score = (wk2 . gelu(Wk y + bk1) + bk2) * scale.  The
 // biases ride on MFMA k-steps.  Each wave walks groups of NT 16-row tiles; the
 // next group's inputs are gathered while this group's epilogue and key run.
-template <int D, typename T, int MODE, int NT, int WV>
-__global__ __launch_bounds__(64 * WV) void msg_kernel(RowArgs a, RowW<T> w) {
+template <int D, typename T, int MODE, int NT, int WV, int MINW = 1>
+__global__ __launch_bounds__(64 * WV) __attribute__((amdgpu_waves_per_eu(MINW))) void msg_kernel(
+    RowArgs a, RowW<T> w) {
   typedef RowShape<D, T, MODE> S;
   typedef Mf<T> F;
   typedef typename F::frag frag;
@@ -2149,7 +2150,7 @@ int lds_setup(const void *fn, int threads, size_t smem, int *per_cu) {
 // launch shape of the message / mix rows (knob gnn_rows): 16-row tiles per
 // wave group and waves per workgroup
 int g_gnn_rows = 3;
-constexpr int GNN_ROW_SHAPES = 6;
+constexpr int GNN_ROW_SHAPES = 8;
 // line pool kernel (knob gnn_pool): 0 line by line; 1 and 2 loads issued
 // ahead, message loads of 8 (1) or 4 (2) lines at a time (items of at most
 // POOL_LMAX lines); all give the same bits
@@ -2157,15 +2158,15 @@ int g_gnn_pool = 2;
 // geometry through per-stage lookup tables (knob gnn_geo_table; same steps)
 int g_gnn_geo_table = 1;
 
-template <int D, typename T, int MODE, int NT, int WV>
+template <int D, typename T, int MODE, int NT, int WV, int MINW = 1>
 int launch_rows_s(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
   const size_t smem = RowShape<D, T, MODE>::smem(a.L, a.K);
-  const void *fn = (const void *)msg_kernel<D, T, MODE, NT, WV>;
+  const void *fn = (const void *)msg_kernel<D, T, MODE, NT, WV, MINW>;
   int per_cu = 1;
   int rc = lds_setup(fn, 64 * WV, smem, &per_cu);
   if (rc) return rc;
-  hipLaunchKernelGGL((msg_kernel<D, T, MODE, NT, WV>), dim3(g->n_cu * per_cu), dim3(64 * WV),
-                     smem, s, a, w);
+  hipLaunchKernelGGL((msg_kernel<D, T, MODE, NT, WV, MINW>), dim3(g->n_cu * per_cu),
+                     dim3(64 * WV), smem, s, a, w);
   return hip_ok(hipGetLastError(), "msg_kernel launch");
 }
 
@@ -2178,6 +2179,8 @@ int launch_rows_m(latec_gnn *g, RowArgs &a, const RowW<T> &w, hipStream_t s) {
     case 3: return launch_rows_s<D, T, MODE, 2, 8>(g, a, w, s);
     case 4: return launch_rows_s<D, T, MODE, 2, 6>(g, a, w, s);
     case 5: return launch_rows_s<D, T, MODE, 1, 6>(g, a, w, s);
+    case 6: return launch_rows_s<D, T, MODE, 2, 8, 4>(g, a, w, s);
+    case 7: return launch_rows_s<D, T, MODE, 1, 8, 4>(g, a, w, s);
     default: return launch_rows_s<D, T, MODE, 2, 4>(g, a, w, s);
   }
 }

@@ -419,7 +419,7 @@ def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
     base = [L.latec_tune_get(k) for k in knobs]
     streams, decoded = [], []
     try:
-        for vals in [(v, 2, 1) for v in range(6)] + [(0, 0, 1), (0, 1, 1), (3, 2, 0)]:
+        for vals in [(v, 2, 1) for v in range(8)] + [(0, 0, 1), (0, 1, 1), (3, 2, 0)]:
             for k, v in zip(knobs, vals):
                 assert L.latec_tune_set(k, v) == 0
             streams.append(codec.compress(x, 1e-3))
@@ -429,6 +429,6 @@ def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
             L.latec_tune_set(k, v)
     assert all(s == streams[0] for s in streams[1:])
     assert all(np.array_equal(d, decoded[0]) for d in decoded[1:])
-    assert L.latec_tune_set(b"gnn_rows", 6) != 0
+    assert L.latec_tune_set(b"gnn_rows", 8) != 0
     assert L.latec_tune_set(b"gnn_pool", 3) != 0
     assert L.latec_tune_set(b"gnn_geo_table", 2) != 0
