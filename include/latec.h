@@ -61,8 +61,9 @@ int latec_abi_version(void);
  * (smallest level that runs as sheets).  "quad_launches" and
  * "sheet_launches" read (or, set, reset) counts of line-quad and sheet
  * launches.  GNN kernels: "gnn_rows" (0..7: 16-row tiles per wave group,
- * waves per workgroup and minimum waves per SIMD of the message / mix rows), "gnn_pool" (0: line
- * pool line by line; 1, 2: loads issued ahead, 8 or 4 lines per batch) and
+ * waves per workgroup and minimum waves per SIMD of the message / mix
+ * rows), "gnn_pool" (0: line pool line by line; 1, 2: loads issued ahead,
+ * 8 or 4 lines per batch) and
  * "gnn_geo_table" (1: stage geometry through per-stage lookup tables of the
  * first known cell along each line; 0: a scan of the cells).
  * Process-wide; no knob changes a code, a reconstructed value or a GNN
