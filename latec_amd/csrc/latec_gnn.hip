@@ -182,6 +182,13 @@ struct latec_gnn {
   std::vector<std::pair<int64_t, uint8_t *>> geo_tabs;
   // gate tables (thresholds, per-level constants) per eb
   std::vector<std::pair<uint64_t, void *>> gate_tabs;
+  // a stage's parity classes and their start indices on the device, per
+  // (plan shape, levels, stride, stage): the plan fixes them
+  struct ClassTab {
+    std::vector<int64_t> key;
+    void *dev;
+  };
+  std::vector<ClassTab> class_tabs;
   int64_t line_off_key[LATEC_MAX_DIM] = {0};
   // workspace
   void *work = nullptr;
@@ -2575,6 +2582,7 @@ void latec_gnn_destroy(latec_gnn *g) {
   if (g->d_line_off) (void)hipFree(g->d_line_off);
   for (auto &t : g->geo_tabs) (void)hipFree(t.second);
   for (auto &t : g->gate_tabs) (void)hipFree(t.second);
+  for (auto &t : g->class_tabs) (void)hipFree(t.dev);
   delete g;
 }
 
@@ -2626,21 +2634,40 @@ int latec_gnn_geometry(latec_gnn *g, latec_plan *p, int stage, const latec_gnn_f
     latec::set_error("a halo frame needs the coded-chunk flags");
     return LATEC_E_ARG;
   }
-  // classes of the stage and their start indices, uploaded once per call
+  // classes of the stage and their start indices on the device, uploaded
+  // once per (plan shape, levels, stride, stage) and handle
   const int nc = (int)st.classes.size();
   a.n_classes = nc;
   std::vector<int64_t> starts(nc + 1, 0);
   for (int c = 0; c < nc; c++) starts[c + 1] = starts[c] + st.classes[c].size;
   const size_t bcls = ((sizeof(ClassDesc) * nc + 255) / 256) * 256;
-  int rc = ensure_work(g, bcls + sizeof(int64_t) * (nc + 1));
-  if (rc) return rc;
-  ClassDesc *dcls = (ClassDesc *)g->work;
-  int64_t *dstart = (int64_t *)((unsigned char *)g->work + bcls);
-  rc = hip_ok(hipMemcpyAsync(dcls, st.classes.data(), sizeof(ClassDesc) * nc,
-                             hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-  if (!rc) rc = hip_ok(hipMemcpyAsync(dstart, starts.data(), sizeof(int64_t) * (nc + 1),
-                                      hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-  if (rc) return rc;
+  std::vector<int64_t> key = {p->ndim, p->levels, p->stride, p->block, stage};
+  key.insert(key.end(), p->shape, p->shape + p->ndim);
+  void *ctab = nullptr;
+  for (auto &t : g->class_tabs)
+    if (t.key == key) ctab = t.dev;
+  int rc = LATEC_OK;
+  if (!ctab) {
+    if (g->class_tabs.size() >= 1024) {  // many shapes: start over
+      rc = hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      if (rc) return rc;
+      for (auto &t : g->class_tabs) (void)hipFree(t.dev);
+      g->class_tabs.clear();
+    }
+    std::vector<unsigned char> h(bcls + sizeof(int64_t) * (nc + 1), 0);
+    memcpy(h.data(), st.classes.data(), sizeof(ClassDesc) * nc);
+    memcpy(h.data() + bcls, starts.data(), sizeof(int64_t) * (nc + 1));
+    rc = hip_ok(hipMalloc(&ctab, h.size()), "hipMalloc(class tables)");
+    if (rc) return rc;
+    rc = hip_ok(hipMemcpy(ctab, h.data(), h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) {
+      (void)hipFree(ctab);
+      return rc;
+    }
+    g->class_tabs.push_back({key, ctab});
+  }
+  const ClassDesc *dcls = (const ClassDesc *)ctab;
+  const int64_t *dstart = (const int64_t *)((unsigned char *)ctab + bcls);
   const int64_t total = starts[nc];
   // lookup table of the stage: (2^levels)^ndim residue tuples x L lines x 2
   // sides, built once per handle (kept up to 2^16 residue tuples)
@@ -2663,9 +2690,7 @@ int latec_gnn_geometry(latec_gnn *g, latec_plan *p, int stage, const latec_gnn_f
   hipLaunchKernelGGL(geometry_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, dcls, a,
                      (const int8_t *)g->d_dirs, total, dstart, d_qidx, d_gidx, d_steps,
                      (const uint8_t *)tab);
-  rc = hip_ok(hipGetLastError(), "geometry_kernel launch");
-  if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die here
-  return rc;
+  return hip_ok(hipGetLastError(), "geometry_kernel launch");
 }
 
 int latec_gnn_embed(latec_gnn *g, const void *d_E, const int64_t *frame_shape,
@@ -2784,6 +2809,12 @@ int gate_tables(latec_gnn *g, double eb, const float *thr63, const double *grid6
       *d_lvk = (const double *)((unsigned char *)t.second + 256);
       return LATEC_OK;
     }
+  if (g->gate_tabs.size() >= 64) {  // many bounds: start over (kernels may still read them)
+    int rc = hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (rc) return rc;
+    for (auto &t : g->gate_tabs) (void)hipFree(t.second);
+    g->gate_tabs.clear();
+  }
   std::vector<unsigned char> h(256 + sizeof(double) * 3 * 64, 0);
   memcpy(h.data(), thr63, sizeof(float) * 63);
   double *lvk = (double *)(h.data() + 256);

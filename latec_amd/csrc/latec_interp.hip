@@ -65,7 +65,13 @@ static int hip_check(hipError_t e, const char *what) {
   return LATEC_OK;
 }
 
+// Device copy of a plan's class tables, made on first use; callers on
+// several threads may share a plan, so the first use is serialized.
+static std::mutex g_upload_mu;
+
 int plan_upload(latec_plan *p) {
+  if (__atomic_load_n(&p->dev, __ATOMIC_ACQUIRE)) return LATEC_OK;
+  std::lock_guard<std::mutex> lk(g_upload_mu);
   if (p->dev) return LATEC_OK;
   auto *d = new DeviceBufs();
   std::vector<ClassDesc> cls;
@@ -93,7 +99,7 @@ int plan_upload(latec_plan *p) {
     delete d;
     return rc;
   }
-  p->dev = d;
+  __atomic_store_n(&p->dev, d, __ATOMIC_RELEASE);
   return LATEC_OK;
 }
 

@@ -35,7 +35,7 @@ import numpy as np
 
 from . import _device as D
 from . import rans as R
-from ._lib import InterpOpts, check, lib, ptr
+from ._lib import InterpOpts, check, lib, ptr, stream_ptr
 from .bitstream import pack_stage, unpack_stage, zstd_compress, zstd_decompress
 from .gnn_predictor import ChunkedGNNPredictor, load_model
 from .levels import _slice_count, stage_ebs, stage_layouts, stage_strides
@@ -217,7 +217,7 @@ def gate_costs_device(model, x, pred, scale, cands, n: int, eb: float, radius: i
     check(lib().latec_gnn_gate_costs(model.handle, ptr(x), ptr(pred), ptr(scale),
                                      ctypes.cast(ipa, ctypes.c_void_p), len(cands), int(n),
                                      float(eb), int(radius), ptr(thr), ptr(grid), ptr(W),
-                                     ptr(None)))
+                                     stream_ptr()))
     return W.reshape(-1, 13)
 
 
@@ -337,12 +337,17 @@ class GNNCompressorCodec:
         return edges
 
     def _predictor(self, levels: int, numerics: str) -> ChunkedGNNPredictor:
+        """The predictor of (levels, numerics), kept for the last four keys
+        (fields of different ranks and sizes take different levels; each
+        predictor holds its model handles and the last tensor's frame)."""
         key = (int(levels), numerics)
-        p = self._predictors.get(key)
+        p = self._predictors.pop(key, None)
         if p is None:
             p = ChunkedGNNPredictor(self.checkpoint_path, 0.0, 1.0, levels=levels,
                                     anchor_stride=1 << levels, numerics=numerics)
-            self._predictors = {key: p}
+            while len(self._predictors) >= 4:
+                self._predictors.pop(next(iter(self._predictors)))
+        self._predictors[key] = p  # most recent last
         return p
 
     # -- encode -----------------------------------------------------------------
@@ -444,7 +449,7 @@ class GNNCompressorCodec:
             o.code_bytes = 2
             buf = T.empty(M, dtype=T.float32, device=D.device())
             check(lib().latec_interp_predict(plan.handle, s, ptr(block), ptr(buf), _optp(o),
-                                             ptr(None)))
+                                             stream_ptr()))
             out[k] = buf if keep is None else buf[keep]
         return out
 
@@ -538,13 +543,13 @@ class GNNCompressorCodec:
                     if gt > 0:
                         check(lib().latec_gnn_gate_apply(ptr(p), ptr(sc), ptr(cands[gk]), n,
                                                          float(ebs[s]) * 2.0 ** gt,
-                                                         float(2.0 ** -gs), gd, ptr(None)))
+                                                         float(2.0 ** -gs), gd, stream_ptr()))
                 codes, rank, val, k, _ = quantize_device(x_st, p, n, ebs[s], o)
                 rec = T.empty(n, dtype=T.float32, device=D.device())
                 d_cnt = D.to_device(np.array([k], np.uint32))
                 check(lib().latec_dequantize(ptr(p), ptr(codes), n, ptr(rank), ptr(val),
                                              ptr(d_cnt), float(ebs[s]), _optp(o), ptr(rec),
-                                             ptr(None)))
+                                             stream_ptr()))
                 recon.view(-1)[gidx] = rec
                 lv = T.empty(n, dtype=T.uint8, device=D.device())
                 R.scale_to_level_device(sc, n, float(ebs[s]), lv)
@@ -558,7 +563,7 @@ class GNNCompressorCodec:
             pred_obj.finish_wave(recon)
         parts.extend(f.result() for f in futures)
         pool.shutdown()
-        T.cuda.synchronize()
+        T.cuda.current_stream().synchronize()
         stats["model_s"] = sum(a.elapsed_time(b) for a, b in events) * 1e-3
         stats["gate_words"] = len(gates or []) + len(fine_gates or [])
         stats["gates_fired"] = sum(1 for g in (gates or []) + (fine_gates or []) if g)
@@ -659,7 +664,7 @@ class GNNCompressorCodec:
                                                          dkeep, n)[gk]
                             check(lib().latec_gnn_gate_apply(ptr(p), ptr(sc), ptr(ip), n,
                                                              float(ebs[s]) * 2.0 ** gt,
-                                                             float(2.0 ** -gs), gd, ptr(None)))
+                                                             float(2.0 ** -gs), gd, stream_ptr()))
                 lv = T.empty(n, dtype=T.uint8, device=D.device())
                 R.scale_to_level_device(sc, n, float(ebs[s]), lv)
                 h_lv = D.to_host(lv, np.uint8)
@@ -672,7 +677,7 @@ class GNNCompressorCodec:
                 rec = T.empty(n, dtype=T.float32, device=D.device())
                 check(lib().latec_dequantize(ptr(p), ptr(d_codes), n, ptr(d_rank), ptr(d_val),
                                              ptr(d_cnt), float(ebs[s]), _optp(o), ptr(rec),
-                                             ptr(None)))
+                                             stream_ptr()))
                 recon.view(-1)[gidx] = rec
             pred_obj.finish_wave(recon)
         if off != len(payload):

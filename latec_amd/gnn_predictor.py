@@ -27,7 +27,7 @@ import numpy as np
 
 from . import _device as D
 from . import gnn_model as GM
-from ._lib import GnnFrame, check, lib, ptr
+from ._lib import GnnFrame, check, lib, ptr, stream_ptr
 
 NUMERICS = {"fp32": 0, "bf16": 1}
 
@@ -104,7 +104,7 @@ class StageGeom:
         if self.M:
             check(lib().latec_gnn_geometry(model.handle, plan.handle, stage,
                                            ctypes.byref(frame), ptr(self.qidx), ptr(self.gidx),
-                                           ptr(self.steps), ptr(None)))
+                                           ptr(self.steps), stream_ptr()))
 
 
 class ChunkedGNNPredictor:
@@ -225,7 +225,7 @@ class ChunkedGNNPredictor:
                 self._geom_cache = {key: geoms}
         if pad:
             check(lib().latec_gnn_halo_frame(m.handle, plan.handle, ctypes.byref(frame), ptr(E),
-                                             ptr(recon), 0.0 + self.vmin, self.vmax, ptr(None)))
+                                             ptr(recon), 0.0 + self.vmin, self.vmax, stream_ptr()))
         self._wave = {
             "ci": ci, "plan": plan, "E": E, "geoms": geoms, "fshape": np.asarray(fshape, np.int64),
             "chain": [0] + [s for s in range(1, plan.n_stages) if plan.counts[s]],
@@ -240,7 +240,7 @@ class ChunkedGNNPredictor:
         if geom.M:
             check(lib().latec_gnn_embed(m.handle, ptr(w["E"]), ptr(w["fshape"]),
                                         ptr(geom.steps), ptr(geom.qidx), geom.M, ptr(ctx),
-                                        ptr(None)))
+                                        stream_ptr()))
         return ctx
 
     def stage_flops(self, s: int) -> int:
@@ -280,7 +280,7 @@ class ChunkedGNNPredictor:
             ctx_p = w["ctx"] if w["ctx"] is not None else self._embed(gp)
             check(lib().latec_gnn_finalize(m.handle, ptr(w["E"]), ptr(ctx_p), ptr(gp.qidx),
                                            ptr(gp.gidx), ptr(recon), gp.M, self.vmin, self.vmax,
-                                           ptr(None)))
+                                           stream_ptr()))
         ctx = self._embed(gh)
         pred = T.empty(gh.M, dtype=T.float32, device=D.device())
         scale = T.empty(gh.M, dtype=T.float32, device=D.device())
@@ -290,7 +290,7 @@ class ChunkedGNNPredictor:
             lb = T.empty(gh.M, dtype=T.float32, device=D.device())
         check(lib().latec_gnn_head(m.handle, ptr(ctx), gh.M, float(eb) / self.span,
                                    self.vmin, self.vmax, ptr(pred), ptr(scale), ptr(mu), ptr(lb),
-                                   ptr(None)))
+                                   stream_ptr()))
         w["ctx"] = ctx
         w["pos"] = j
         if want_head:

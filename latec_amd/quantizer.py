@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _device as D
-from ._lib import InterpOpts, check, lib, ptr
+from ._lib import InterpOpts, check, lib, ptr, stream_ptr
 
 DEFAULT_RADIUS = 1 << 15
 
@@ -43,7 +43,7 @@ def quantize_device(dx, dpred, n: int, eb: float, o: InterpOpts, d_hist=None):
     cnt = D.zeros(1, np.uint32)
     hist = d_hist if d_hist is not None else D.zeros(lib().latec_hist_words(o.radius), np.uint32)
     check(lib().latec_quantize(ptr(dx), ptr(dpred), int(n), float(eb), _opts_ptr(o),
-                               ptr(codes), ptr(rank), ptr(val), ptr(cnt), ptr(hist), ptr(None)))
+                               ptr(codes), ptr(rank), ptr(val), ptr(cnt), ptr(hist), stream_ptr()))
     k = int(cnt.item())
     if k > 1:  # the kernel appends outliers in any order: list them by position
         nn = np.array([n], np.int64)
@@ -51,7 +51,7 @@ def quantize_device(dx, dpred, n: int, eb: float, o: InterpOpts, d_hist=None):
         scratch = D.empty(words, np.uint32)
         check(lib().latec_order_outliers(1, ptr(codes), o.code_bytes, ptr(np.zeros(1, np.int64)),
                                          ptr(nn), ptr(dx), ptr(rank), ptr(val), ptr(cnt),
-                                         ptr(scratch), words, ptr(None)))
+                                         ptr(scratch), words, stream_ptr()))
     return codes, rank, val, k, hist
 
 
@@ -91,6 +91,6 @@ def dequantize(pred, codes, outlier_vals, eb, radius=DEFAULT_RADIUS):
     d_val = D.to_device(outlier_vals if len(outlier_vals) else np.zeros(1, np.float32))
     cnt = D.to_device(np.array([len(zeros)], np.uint32))
     check(lib().latec_dequantize(ptr(d_pred), ptr(d_codes), len(pred), ptr(d_rank), ptr(d_val),
-                                 ptr(cnt), float(eb), _opts_ptr(o), ptr(out), ptr(None)))
-    D.torch().cuda.synchronize()
+                                 ptr(cnt), float(eb), _opts_ptr(o), ptr(out), stream_ptr()))
+    D.torch().cuda.current_stream().synchronize()
     return D.to_host(out)

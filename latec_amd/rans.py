@@ -16,7 +16,7 @@ from functools import lru_cache
 import numpy as np
 
 from . import _device as D
-from ._lib import Fit, check, lib, ptr
+from ._lib import Fit, check, lib, ptr, stream_ptr
 
 SCALE_LO_DIV = 16.0
 SCALE_HI_MULT = 4096.0
@@ -116,8 +116,8 @@ def _level_weights(kexp: int, n_levels: int = N_LEVELS) -> np.ndarray:
     """rans.py:247-267, built on the GPU."""
     out = D.empty(N_LEVELS * (2 << kexp), np.float64)
     with _table_lock:
-        check(lib().latec_level_weights(int(kexp), ptr(out), ptr(None)))
-    D.torch().cuda.synchronize()
+        check(lib().latec_level_weights(int(kexp), ptr(out), stream_ptr()))
+    D.torch().cuda.current_stream().synchronize()
     return D.to_host(out).reshape(N_LEVELS, 2 << kexp)
 
 
@@ -125,8 +125,8 @@ def _level_weights(kexp: int, n_levels: int = N_LEVELS) -> np.ndarray:
 def _model_weights(kexp: int, shape: int, owx: int) -> np.ndarray:
     out = D.empty(N_LEVELS * (2 << kexp), np.float64)
     with _table_lock:
-        check(lib().latec_model_weights(int(kexp), int(shape), int(owx), ptr(out), ptr(None)))
-    D.torch().cuda.synchronize()
+        check(lib().latec_model_weights(int(kexp), int(shape), int(owx), ptr(out), stream_ptr()))
+    D.torch().cuda.current_stream().synchronize()
     return D.to_host(out).reshape(N_LEVELS, 2 << kexp)
 
 
@@ -135,8 +135,8 @@ def _neg_log2_weights(kexp: int, shape: int, n_levels: int, precision: int, owx:
     out = D.empty(N_LEVELS * (2 << kexp), np.float32)
     with _table_lock:
         check(lib().latec_neg_log2_weights(int(kexp), int(shape), int(owx), int(precision),
-                                           ptr(out), ptr(None)))
-    D.torch().cuda.synchronize()
+                                           ptr(out), stream_ptr()))
+    D.torch().cuda.current_stream().synchronize()
     return D.to_host(out).reshape(N_LEVELS, 2 << kexp)
 
 
@@ -156,8 +156,8 @@ def build_cdfs(kexp: int, shape: int = 0, precision: int = DEFAULT_PRECISION, ow
         d = D.empty((hi - lo) * (A + 1), np.uint32)
         with _table_lock:
             check(lib().latec_build_cdfs(int(kexp), int(shape), int(precision), int(owx), lo, hi,
-                                         ptr(d), ptr(None)))
-        D.torch().cuda.synchronize()
+                                         ptr(d), stream_ptr()))
+        D.torch().cuda.current_stream().synchronize()
         h = D.to_host(d, np.uint32).reshape(hi - lo, A + 1)
         for r in range(lo, hi):
             _cdf_cache[(kexp, shape, precision, owx, r)] = h[r - lo].copy()
@@ -232,7 +232,7 @@ def scale_to_level_device(d_scale, n: int, eb: float, d_levels, stream=None) -> 
     scale_to_level_host on every value (asynchronous on the stream)."""
     thr = np.ascontiguousarray(scale_thresholds(eb))
     check(lib().latec_scale_to_level(ptr(d_scale), int(n), ptr(thr), ptr(d_levels),
-                                     ptr(stream)))
+                                     stream_ptr(stream)))
 
 
 def scale_to_level(scale, eb: float, n_levels: int = 64) -> np.ndarray:
@@ -245,7 +245,7 @@ def scale_to_level(scale, eb: float, n_levels: int = 64) -> np.ndarray:
     ds = D.to_device(s)
     out = D.empty(len(s), np.uint8)
     scale_to_level_device(ds, len(s), eb, out)
-    D.torch().cuda.synchronize()
+    D.torch().cuda.current_stream().synchronize()
     return D.to_host(out).reshape(np.shape(scale))
 
 
@@ -257,10 +257,10 @@ def fit_codes_device(d_codes, n: int, radius: int, code_bytes: int, d_hist=None,
     if d_hist is None:
         d_hist = D.zeros(lib().latec_hist_words(radius), np.uint32)
         check(lib().latec_code_hist(ptr(d_codes), int(n), code_bytes, int(radius), ptr(d_hist),
-                                    ptr(None)))
+                                    stream_ptr()))
     f = Fit()
     check(lib().latec_fit_codes(ptr(d_hist), ptr(d_codes), int(n), int(radius), code_bytes,
-                                precision, ptr_fit(f), ptr(None)))
+                                precision, ptr_fit(f), stream_ptr()))
     return f
 
 
