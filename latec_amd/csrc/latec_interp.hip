@@ -781,7 +781,7 @@ __global__ void __launch_bounds__(256) predict_stage_kernel(
 //   pred_sae = sum |x - p|, pred_sse = sum (x - p)^2,
 //   recon_sae = sum |x - r|, recon_sse = sum (x - r)^2, recon_max = max |x - r|
 // (NaN-sticky, as NumPy's max).  Each workgroup writes its five partials
-// (a fixed-shape LDS tree); stats_finish_kernel adds them in block order.
+// (a fixed-shape LDS tree); stats_finish_kernel combines them in a fixed order.
 constexpr int NSTAT = 5;
 __device__ __forceinline__ double nan_max(double a, double b) {
   return (b > a || b != b) && a == a ? b : a;
@@ -826,18 +826,31 @@ __global__ void __launch_bounds__(256) stats_stage_kernel(
     partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NSTAT + t] = red[t][0];
 }
 
-// out[s * 5 + k] = the stage's partials [off[s], off[s + 1]) summed in order
-// (one workgroup per stage; thread k < 5 walks moment k).
-__global__ void stats_finish_kernel(const double *__restrict__ partial,
-                                    const int64_t *__restrict__ off, double *__restrict__ out) {
-  const int st = blockIdx.x, k = threadIdx.x;
-  if (k >= NSTAT) return;
+// out[s * 5 + k] = the stage's partials [off[s], off[s + 1]) combined in a
+// fixed order: one workgroup per stage, STATS_SPLIT threads per moment k each
+// walk one contiguous run of the partials in order, then the run totals are
+// added in run order.
+constexpr int STATS_SPLIT = 64;
+__global__ void __launch_bounds__(NSTAT * STATS_SPLIT) stats_finish_kernel(
+    const double *__restrict__ partial, const int64_t *__restrict__ off,
+    double *__restrict__ out) {
+  __shared__ double run[NSTAT][STATS_SPLIT];
+  const int st = blockIdx.x, k = threadIdx.x / STATS_SPLIT, j = threadIdx.x % STATS_SPLIT;
+  const int64_t b0 = off[st], n = off[st + 1] - b0;
+  const int64_t per = (n + STATS_SPLIT - 1) / STATS_SPLIT;
+  const int64_t lo = b0 + j * per, hi = lo + per < b0 + n ? lo + per : b0 + n;
   double a = 0.0;
-  for (int64_t b = off[st]; b < off[st + 1]; b++) {
+  for (int64_t b = lo; b < hi; b++) {
     const double v = partial[b * NSTAT + k];
     a = k == 4 ? nan_max(a, v) : a + v;
   }
-  out[st * NSTAT + k] = a;
+  run[k][j] = a;
+  __syncthreads();
+  if (j == 0) {
+    double t = 0.0;
+    for (int r = 0; r < STATS_SPLIT; r++) t = k == 4 ? nan_max(t, run[k][r]) : t + run[k][r];
+    out[st * NSTAT + k] = t;
+  }
 }
 
 // Segment min / max of float32 values as NumPy's min() / max() give them: a
@@ -3793,7 +3806,8 @@ int interp_stats(latec_plan *p, const float *x, const float *recon,
     }
     if ((rc = hip_check(hipGetLastError(), "stats_stage_kernel"))) return rc;
   }
-  hipLaunchKernelGGL(stats_finish_kernel, dim3(ns), dim3(64), 0, stream, partial, d_off, out);
+  hipLaunchKernelGGL(stats_finish_kernel, dim3(ns), dim3(NSTAT * STATS_SPLIT), 0, stream,
+                     partial, d_off, out);
   if ((rc = hip_check(hipGetLastError(), "stats_finish_kernel"))) return rc;
   // the host offsets are read by the copy above: finish before they go
   return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
