@@ -428,6 +428,46 @@ int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
                          const float *thr63, const double *grid64,
                          double *costs, void *stream);
 
+/* One coded stage of the GNN encoder after the model's prediction (the body
+ * of the chunk loop, gnn_codec.py:1215-1300): the coded rows' prediction,
+ * scale and values; with `gate`, the three interpolation candidates
+ * (latec_interp_predict on d_block with cand_opts), the gate's bucket sums,
+ * its choice (_gate_select, gnn_codec.py:920-971, on the host in float64 in
+ * NumPy's order) and its application; the quantizer, the reconstruction
+ * written into d_recon at the points' tensor indices, the scale levels, and
+ * the codes, levels and outliers (by stage position) copied to the host.
+ * interp_kind 1..3 codes the stage with that interpolation candidate at scale
+ * eb instead of the model.  Two host waits: the gate's sums, the outputs. */
+typedef struct latec_gnn_stage_io {
+  int64_t M;                 /* points of the full local stage */
+  int64_t n;                 /* coded points (M, or the chunk's column split) */
+  const int64_t *d_keep;     /* n positions in the full stage; NULL if n == M */
+  const float *d_pred;       /* (M) model prediction */
+  const float *d_scale;      /* (M) model scale */
+  const int64_t *d_gidx;     /* (M) tensor index of each stage point */
+  const float *d_x;          /* normalized tensor, flat */
+  float *d_recon;            /* reconstruction, flat; the coded points written */
+  const float *d_block;      /* the chunk's block of the reconstruction */
+  const latec_interp_opts *opts;       /* quantizer options */
+  const latec_interp_opts *cand_opts;  /* 3: interpolation kinds 1, 2, 3 */
+  int32_t gate, interp_kind, reuse_choice, pad0;
+  int32_t *choice;           /* host int32[4] {kind, dir, T, shift}: written,
+                              * or read when reuse_choice */
+  uint32_t *gate_word;       /* host: the stage's gate word (0 none) or NULL */
+  double eb;
+  uint32_t radius, pad1;
+  const float *gate_thr;     /* host: the gate's 63 snapping thresholds of eb */
+  const double *gate_grid;   /* host: the gate's 64 grid scales of eb */
+  const float *level_thr;    /* host: the 63 scale-level thresholds of eb */
+  uint32_t *d_hist;          /* device histogram scratch of latec_quantize */
+  void *codes;               /* host out: n codes (opts->code_bytes each) */
+  uint8_t *levels;           /* host out: n scale levels */
+  float *outliers;           /* host out: outlier values by position (cap n) */
+  int64_t *n_out;            /* host out: outlier count */
+} latec_gnn_stage_io;
+int latec_gnn_code_stage(latec_gnn *g, latec_plan *p, int stage,
+                         const latec_gnn_stage_io *io, void *stream);
+
 /* _gate_apply_t (gnn_codec.py:974-995): where the scale is below (dir 0) or
  * at/above (dir 1) `threshold`, pred = ip and scale *= mult. */
 int latec_gnn_gate_apply(float *d_pred, float *d_scale, const float *d_ip,

@@ -59,6 +59,22 @@ class GnnFrame(C.Structure):
     ]
 
 
+class GnnStageIO(C.Structure):
+    _fields_ = [
+        ("M", C.c_int64), ("n", C.c_int64),
+        ("d_keep", C.c_void_p), ("d_pred", C.c_void_p), ("d_scale", C.c_void_p),
+        ("d_gidx", C.c_void_p), ("d_x", C.c_void_p), ("d_recon", C.c_void_p),
+        ("d_block", C.c_void_p), ("opts", C.c_void_p), ("cand_opts", C.c_void_p),
+        ("gate", C.c_int32), ("interp_kind", C.c_int32), ("reuse_choice", C.c_int32),
+        ("pad0", C.c_int32),
+        ("choice", C.c_void_p), ("gate_word", C.c_void_p),
+        ("eb", C.c_double), ("radius", C.c_uint32), ("pad1", C.c_uint32),
+        ("gate_thr", C.c_void_p), ("gate_grid", C.c_void_p), ("level_thr", C.c_void_p),
+        ("d_hist", C.c_void_p), ("codes", C.c_void_p), ("levels", C.c_void_p),
+        ("outliers", C.c_void_p), ("n_out", C.c_void_p),
+    ]
+
+
 p = C.c_void_p
 i64, i32, u32, f64, f32 = C.c_int64, C.c_int, C.c_uint32, C.c_double, C.c_float
 
@@ -116,6 +132,7 @@ SIGNATURES = {
     "latec_gnn_halo_frame": (i32, [p, p, p, p, p, f32, f32, p]),
     "latec_gnn_gate_costs": (i32, [p, p, p, p, p, i32, i64, f64, u32, p, p, p, p]),
     "latec_gnn_gate_apply": (i32, [p, p, p, i64, f64, f32, i32, p]),
+    "latec_gnn_code_stage": (i32, [p, p, i32, p, p]),
     "latec_gather_f32": (i32, [p, p, i64, p, p]),
     "latec_scatter_f32": (i32, [p, p, i64, p, p]),
     "latec_train_gather": (i32, [p, i64, p, i64, p, p]),

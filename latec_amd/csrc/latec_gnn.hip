@@ -30,6 +30,7 @@
 //     applied once per (point, axis) instead of once per line.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -189,6 +190,9 @@ struct latec_gnn {
     void *dev;
   };
   std::vector<ClassTab> class_tabs;
+  // latec_gnn_code_stage's device temporaries
+  void *stage_work = nullptr;
+  size_t stage_work_bytes = 0;
   int64_t line_off_key[LATEC_MAX_DIM] = {0};
   // workspace
   void *work = nullptr;
@@ -1847,6 +1851,30 @@ __global__ void scatter_kernel(int64_t n, const float *__restrict__ src,
   if (i < n) dst[idx[i]] = src[i];
 }
 
+__global__ void gather_i64_kernel(int64_t n, const int64_t *__restrict__ src,
+                                  const int64_t *__restrict__ idx, int64_t *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// rec[rank[j]] = val[j] for j < *count: the quantizer's outliers (listed in
+// any order) over a stage reconstruction
+__global__ void outlier_fix_kernel(const uint32_t *__restrict__ rank,
+                                   const float *__restrict__ val,
+                                   const uint32_t *__restrict__ count, int64_t cap,
+                                   float *__restrict__ rec) {
+  int64_t k = *count;
+  if (k > cap) k = cap;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < k;
+       j += (int64_t)gridDim.x * blockDim.x)
+    rec[rank[j]] = val[j];
+}
+
+__global__ void fill_f32_kernel(int64_t n, float v, float *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 template <typename T>
 __global__ void zero_kernel(int64_t n, T *p) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2583,6 +2611,7 @@ void latec_gnn_destroy(latec_gnn *g) {
   for (auto &t : g->geo_tabs) (void)hipFree(t.second);
   for (auto &t : g->gate_tabs) (void)hipFree(t.second);
   for (auto &t : g->class_tabs) (void)hipFree(t.dev);
+  if (g->stage_work) (void)hipFree(g->stage_work);
   delete g;
 }
 
@@ -2905,6 +2934,211 @@ int latec_gnn_gate_costs(latec_gnn *g, const float *d_x, const float *d_pred,
               "hipMemcpyAsync");
   if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
   return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// _gate_select (gnn_codec.py:920-971) on the host from the bucket sums W
+// ((1 + 4 n_interp) rows of 13): the same float64 operations in the same
+// order as the NumPy function (running sums along the buckets; the cost of
+// taking candidate row r below bucket j is (tot - base[j]) + ci[j], above it
+// base[j] + (ci[12] - ci[j]); the first minimum over [low rows; high rows]
+// wins if it is below tot; a NaN cost, which NumPy's argmin would return,
+// never gates).  out = {kind, dir, T, shift}.
+void gate_select_host(const double *W, int n_interp, int32_t out[4]) {
+  const int nrow = 4 * n_interp, NB = GATE_NB;
+  double base[GATE_NB], acc = 0.0;
+  for (int b = 0; b < NB; b++) base[b] = (acc += W[b]);
+  const double tot = base[NB - 1];
+  double best = 0.0;
+  int fi = -1;
+  bool nan = false;
+  for (int half = 0; half < 2; half++)
+    for (int r = 0; r < nrow; r++) {
+      double ci[GATE_NB], a2 = 0.0;
+      for (int b = 0; b < NB; b++) ci[b] = (a2 += W[(1 + r) * NB + b]);
+      for (int j = 0; j < NB - 1; j++) {
+        const double c = half == 0 ? (tot - base[j]) + ci[j] : base[j] + (ci[NB - 1] - ci[j]);
+        nan |= c != c;
+        if (fi < 0 || c < best) {
+          best = c;
+          fi = (half * nrow + r) * (NB - 1) + j;
+        }
+      }
+    }
+  out[0] = out[1] = out[2] = out[3] = 0;
+  if (fi < 0 || nan || !(best < tot)) return;
+  const int row = fi / (NB - 1), side_row = row % nrow;
+  out[0] = side_row / 4 + 1;
+  out[1] = row >= nrow;
+  out[2] = 2 + fi % (NB - 1);
+  out[3] = 2 * (side_row % 4);
+}
+
+int stage_work(latec_gnn *g, size_t bytes) {
+  if (g->stage_work_bytes >= bytes) return LATEC_OK;
+  if (g->stage_work) {
+    int rc = hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (rc) return rc;
+    (void)hipFree(g->stage_work);
+    g->stage_work = nullptr;
+    g->stage_work_bytes = 0;
+  }
+  bytes = (bytes * 5) / 4 + (1 << 20);
+  int rc = hip_ok(hipMalloc(&g->stage_work, bytes), "hipMalloc(stage workspace)");
+  if (!rc) g->stage_work_bytes = bytes;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int latec_gnn_code_stage(latec_gnn *g, latec_plan *p, int stage, const latec_gnn_stage_io *io,
+                         void *stream) {
+  if (!g || !p || !io || !io->d_pred || !io->d_scale || !io->d_gidx || !io->d_x ||
+      !io->d_recon || !io->opts || !io->level_thr || !io->codes || !io->levels ||
+      !io->outliers || !io->n_out) {
+    latec::set_error("latec_gnn_code_stage: null argument");
+    return LATEC_E_ARG;
+  }
+  if (stage < 1 || stage >= (int)p->stages.size() || io->n < 0 || io->n > io->M ||
+      (io->n < io->M && !io->d_keep)) {
+    latec::set_error("latec_gnn_code_stage: bad stage or counts");
+    return LATEC_E_ARG;
+  }
+  const int64_t M = io->M, n = io->n;
+  *io->n_out = 0;
+  if (io->gate_word) *io->gate_word = 0;
+  if (n == 0) return LATEC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const latec_interp_opts &o = *io->opts;
+  const int cb = o.code_bytes;
+  const bool gate = io->gate && !io->interp_kind;
+  const int ncand = gate ? 3 : (io->interp_kind ? 1 : 0);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  // p, sc, x, rec (n f32), gidx (n i64), codes (n), rank, val (n), cnt, levels (n),
+  // candidates (ncand x M full, ncand x n kept)
+  const size_t b_f = al(4 * (size_t)n), b_i = al(8 * (size_t)n), b_c = al((size_t)cb * n);
+  const size_t b_l = al((size_t)n), b_cf = al(4 * (size_t)M);
+  const size_t need = 6 * b_f + b_i + b_c + b_l + 256 + ncand * (b_cf + b_f);
+  int rc = stage_work(g, need);
+  if (rc) return rc;
+  unsigned char *w = (unsigned char *)g->stage_work;
+  float *dp = (float *)w, *dsc = (float *)(w + b_f), *dx = (float *)(w + 2 * b_f);
+  float *drec = (float *)(w + 3 * b_f);
+  uint32_t *drank = (uint32_t *)(w + 4 * b_f);
+  float *dval = (float *)(w + 5 * b_f);
+  int64_t *dgidx = (int64_t *)(w + 6 * b_f);
+  void *dcodes = w + 6 * b_f + b_i;
+  uint8_t *dlv = (uint8_t *)(w + 6 * b_f + b_i + b_c);
+  uint32_t *dcnt = (uint32_t *)(w + 6 * b_f + b_i + b_c + b_l);
+  float *dcf[3] = {nullptr, nullptr, nullptr}, *dck[3] = {nullptr, nullptr, nullptr};
+  for (int c = 0; c < ncand; c++) {
+    dcf[c] = (float *)(w + 6 * b_f + b_i + b_c + b_l + 256 + c * (b_cf + b_f));
+    dck[c] = (float *)((unsigned char *)dcf[c] + b_cf);
+  }
+  const int gb = grid_for(n, 256);
+  // the coded rows of the stage (the chunk's column split) and their values
+  if (io->d_keep) {
+    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, s, n, io->d_pred, io->d_keep, dp);
+    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, s, n, io->d_scale, io->d_keep,
+                       dsc);
+    hipLaunchKernelGGL(gather_i64_kernel, dim3(gb), dim3(256), 0, s, n, io->d_gidx, io->d_keep,
+                       dgidx);
+  } else {
+    rc = hip_ok(hipMemcpyAsync(dp, io->d_pred, 4 * n, hipMemcpyDeviceToDevice, s), "copy");
+    if (!rc) rc = hip_ok(hipMemcpyAsync(dsc, io->d_scale, 4 * n, hipMemcpyDeviceToDevice, s),
+                         "copy");
+    if (!rc) rc = hip_ok(hipMemcpyAsync(dgidx, io->d_gidx, 8 * n, hipMemcpyDeviceToDevice, s),
+                         "copy");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, s, n, io->d_x, dgidx, dx);
+  // interpolation candidates of the stage on the chunk's block (kinds 1..3, or
+  // the one kind of an interpolation-only encode)
+  for (int c = 0; c < ncand; c++) {
+    const int kind = gate ? c + 1 : io->interp_kind;
+    rc = latec_interp_predict(p, stage, io->d_block, dcf[c], &io->cand_opts[kind - 1], stream);
+    if (rc) return rc;
+    if (io->d_keep)
+      hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, s, n, (const float *)dcf[c],
+                         io->d_keep, dck[c]);
+    else
+      dck[c] = dcf[c];
+  }
+  if (io->interp_kind) {
+    rc = hip_ok(hipMemcpyAsync(dp, dck[0], 4 * n, hipMemcpyDeviceToDevice, s), "copy");
+    if (rc) return rc;
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(gb), dim3(256), 0, s, n, (float)io->eb, dsc);
+  } else if (gate) {
+    int32_t *ch = io->choice;
+    if (!io->reuse_choice) {
+      const int nc = 1 + 4 * 3;
+      const size_t sums_off = ((sizeof(double) * 1025 * nc * GATE_NB + 255) / 256) * 256;
+      const float *ips[3] = {dck[0], dck[1], dck[2]};
+      rc = gate_costs_launch(g, dx, dp, dsc, ips, 3, n, io->eb, io->radius, io->gate_thr,
+                             io->gate_grid, sums_off, s);
+      if (rc) return rc;
+      double W[13 * GATE_NB];
+      rc = hip_ok(hipMemcpyAsync(W, (unsigned char *)g->work + sums_off, sizeof(W),
+                                 hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+      if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+      if (rc) return rc;
+      gate_select_host(W, 3, ch);
+      if (io->gate_word)
+        *io->gate_word = (uint32_t)((ch[1] << 10) + ch[0] * 256 + ch[2] * 16 + ch[3]);
+    }
+    if (ch[2] > 0) {
+      const float *ip = dck[ch[0] - 1];
+      hipLaunchKernelGGL(gate_apply_kernel, dim3(gb), dim3(256), 0, s, n, dp, dsc, ip,
+                         io->eb * std::ldexp(1.0, ch[2]), (float)std::ldexp(1.0, -ch[3]),
+                         (int)ch[1]);
+    }
+  }
+  // quantize, the reconstruction (outliers written over it), into recon
+  rc = hip_ok(hipMemsetAsync(dcnt, 0, 4, s), "hipMemsetAsync");
+  if (rc) return rc;
+  rc = latec_quantize(dx, dp, n, io->eb, &o, dcodes, drank, dval, dcnt, io->d_hist, stream);
+  if (rc) return rc;
+  rc = latec_dequantize(dp, dcodes, n, drank, dval, dcnt, io->eb, &o, drec, stream);
+  if (rc) return rc;
+  int ob = gb < 1024 ? gb : 1024;
+  hipLaunchKernelGGL(outlier_fix_kernel, dim3(ob), dim3(256), 0, s, (const uint32_t *)drank,
+                     (const float *)dval, (const uint32_t *)dcnt, n, drec);
+  hipLaunchKernelGGL(scatter_kernel, dim3(gb), dim3(256), 0, s, n, (const float *)drec,
+                     (const int64_t *)dgidx, io->d_recon);
+  rc = latec_scale_to_level(dsc, n, io->level_thr, dlv, stream);
+  if (rc) return rc;
+  // codes, levels and the outliers by stage position to the host
+  uint32_t k = 0;
+  rc = hip_ok(hipMemcpyAsync(&k, dcnt, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  if (!rc) rc = hip_ok(hipMemcpyAsync(io->codes, dcodes, (size_t)cb * n, hipMemcpyDeviceToHost,
+                                      s), "hipMemcpyAsync");
+  if (!rc) rc = hip_ok(hipMemcpyAsync(io->levels, dlv, (size_t)n, hipMemcpyDeviceToHost, s),
+                       "hipMemcpyAsync");
+  if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (rc) return rc;
+  if (k > (uint32_t)n) {
+    latec::set_error("latec_gnn_code_stage: outlier count past the stage");
+    return LATEC_E_COUNT;
+  }
+  if (k) {
+    std::vector<uint32_t> r(k);
+    std::vector<float> v(k);
+    rc = hip_ok(hipMemcpy(r.data(), drank, 4 * (size_t)k, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!rc) rc = hip_ok(hipMemcpy(v.data(), dval, 4 * (size_t)k, hipMemcpyDeviceToHost),
+                         "hipMemcpy");
+    if (rc) return rc;
+    std::vector<uint32_t> ord(k);
+    for (uint32_t j = 0; j < k; j++) ord[j] = j;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return r[a] < r[b]; });
+    for (uint32_t j = 0; j < k; j++) io->outliers[j] = v[ord[j]];
+  }
+  *io->n_out = k;
+  return LATEC_OK;
 }
 
 int latec_gnn_gate_apply(float *d_pred, float *d_scale, const float *d_ip, int64_t n,

@@ -35,7 +35,7 @@ import numpy as np
 
 from . import _device as D
 from . import rans as R
-from ._lib import InterpOpts, check, lib, ptr, stream_ptr
+from ._lib import GnnStageIO, InterpOpts, check, lib, ptr, stream_ptr
 from .bitstream import pack_stage, unpack_stage, zstd_compress, zstd_decompress
 from .gnn_predictor import ChunkedGNNPredictor, load_model
 from .levels import _slice_count, stage_ebs, stage_layouts, stage_strides
@@ -321,6 +321,9 @@ class GNNCompressorCodec:
         self.last_stats: dict = {}
         self.last_decoded = None  # the last uncompress's normalized reconstruction (device)
         self.count_flops = False  # count the model's GEMM flops into last_stats (syncs)
+        # each stage after the model in one library call (latec_gnn_code_stage);
+        # False runs the same steps as separate calls from Python
+        self.native_stages = True
 
     @staticmethod
     def _chunk_edges(shape, anchor_stride: int, cs):
@@ -453,6 +456,43 @@ class GNNCompressorCodec:
             out[k] = buf if keep is None else buf[keep]
         return out
 
+    @staticmethod
+    def _code_stage(pred_obj, plan, s, n, dkeep, p_full, sc_full, xn, recon, block, o,
+                    cand_opts, gate, interp_kind, reuse, choice, eb, radius, gate_tab, hist):
+        """latec_gnn_code_stage: the stage after the model (gate, quantizer,
+        reconstruction, levels) in one library call; (codes, outlier values,
+        levels, outlier count, gate word)."""
+        io = GnnStageIO()
+        io.M, io.n = int(plan.counts[s]), int(n)
+        io.d_keep = None if dkeep is None else dkeep.data_ptr()
+        io.d_pred, io.d_scale = p_full[0].data_ptr(), sc_full[0].data_ptr()
+        io.d_gidx = pred_obj.stage_gidx(s).data_ptr()
+        io.d_x, io.d_recon = xn.data_ptr(), recon.data_ptr()
+        io.d_block = None if block is None else block.data_ptr()
+        io.opts = ctypes.addressof(o)
+        io.cand_opts = ctypes.addressof(cand_opts)
+        io.gate, io.interp_kind, io.reuse_choice = gate, int(interp_kind), int(reuse)
+        word = np.zeros(1, np.uint32)
+        io.choice, io.gate_word = choice.ctypes.data, word.ctypes.data
+        io.eb, io.radius = eb, radius
+        thr = grid = None
+        if gate_tab is not None:
+            thr, grid = gate_tab
+            io.gate_thr, io.gate_grid = thr.ctypes.data, grid.ctypes.data
+        lthr = np.ascontiguousarray(R.scale_thresholds(eb), np.float32)
+        io.level_thr = lthr.ctypes.data
+        io.d_hist = hist.data_ptr()
+        codes = np.empty(n, np.uint16 if o.code_bytes == 2 else np.uint32)
+        levels = np.empty(n, np.uint8)
+        outl = np.empty(max(n, 1), np.float32)
+        n_out = np.zeros(1, np.int64)
+        io.codes, io.levels = codes.ctypes.data, levels.ctypes.data
+        io.outliers, io.n_out = outl.ctypes.data, n_out.ctypes.data
+        check(lib().latec_gnn_code_stage(pred_obj.model.handle, plan.handle, s,
+                                         ctypes.byref(io), stream_ptr()))
+        k = int(n_out[0])
+        return codes, outl[:k].copy(), levels, k, int(word[0])
+
     def _encode(self, values, xn, norm, round_output, eb, ratio, edges, levels, radius,
                 gate, gate_fine, interp_kind, numerics, zstd_level):
         T = D.torch()
@@ -480,6 +520,8 @@ class GNNCompressorCodec:
         gate_tabs = {}
         gates, fine_gates = ([] if gate else None), ([] if gate and gate_fine else None)
         o = opts_for(radius, round_output)
+        cand_opts = _candidate_opts(ndim)
+        hist = D.zeros(lib().latec_hist_words(o.radius), np.uint32)
         empty = np.zeros(0, np.uint32), np.zeros(0, np.float32), np.zeros(0, np.uint8)
         pool = ThreadPoolExecutor(max_workers=8)
         futures = []
@@ -506,16 +548,38 @@ class GNNCompressorCodec:
                 events.append((e0, e1))
                 if self.count_flops:
                     stats["model_flops"] = stats.get("model_flops", 0) + pred_obj.stage_flops(s)
-                gidx = pred_obj.stage_gidx(s)
-                if dkeep is not None:
-                    p, sc, gidx = p_full[0][dkeep], sc_full[0][dkeep], gidx[dkeep]
-                else:
-                    p, sc = p_full[0].clone(), sc_full[0].clone()
                 n = int(sched.counts[s])
                 if n == 0:
                     futures.append(pool.submit(pack_stage, *empty[:2], rans_levels=empty[2],
                                                rans_tables=tables[s]))
                     continue
+                if self.native_stages:
+                    fine = bool(gate and gate_fine and is_finest[s])
+                    reuse = fine and fine_desc is not None
+                    if gate and not reuse and ebs[s] not in gate_tabs:
+                        gate_tabs[ebs[s]] = _gate_scale_thresholds(ebs[s])
+                    if fine and not reuse:
+                        fine_desc = np.zeros(4, np.int32)
+                    choice = fine_desc if fine else np.zeros(4, np.int32)
+                    block = None
+                    if gate or interp_kind:
+                        block = recon if single else recon[(slice(None), *sls)].contiguous()
+                    h_codes, h_val, h_lv, k, word = self._code_stage(
+                        pred_obj, plan, s, n, dkeep, p_full, sc_full, xn, recon, block, o,
+                        cand_opts, int(bool(gate)), interp_kind, reuse, choice, float(ebs[s]),
+                        radius, gate_tabs.get(ebs[s]), hist)
+                    if gate and not reuse:
+                        (fine_gates if fine else gates).append(word)
+                    stats["outliers"] += k
+                    stats["stage_codes"] += n
+                    futures.append(pool.submit(pack_stage, h_codes.astype(np.uint32), h_val,
+                                               rans_levels=h_lv, rans_tables=tables[s]))
+                    continue
+                gidx = pred_obj.stage_gidx(s)
+                if dkeep is not None:
+                    p, sc, gidx = p_full[0][dkeep], sc_full[0][dkeep], gidx[dkeep]
+                else:
+                    p, sc = p_full[0].clone(), sc_full[0].clone()
                 x_st = xn[gidx]
                 if interp_kind:
                     ip = self._interp_candidates(recon, sls, single, plan, s, (interp_kind,),
@@ -696,6 +760,20 @@ class GNNCompressorCodec:
 
 def _optp(o):
     return ctypes.cast(ctypes.pointer(o), ctypes.c_void_p)
+
+
+def _candidate_opts(ndim: int):
+    """Options of the gate's three interpolation candidates (kinds 1..3), as
+    _interp_candidates sets them."""
+    arr = (InterpOpts * 3)()
+    for k in (1, 2, 3):
+        o = arr[k - 1]
+        o.cubic = 1 if k == 1 else 0
+        o.center = default_interp_center(ndim) if k == 1 else (1 if k == 2 else 2)
+        o.end_mode = _GATE_END_MODE
+        o.radius = 1 << 15
+        o.code_bytes = 2
+    return arr
 
 
 def _stage_upload(codes, zeros, outl, code_bytes):

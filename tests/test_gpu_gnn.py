@@ -383,6 +383,10 @@ def test_gate_costs_match_oracle_on_real_stages(monkeypatch, shape):
     ckpt = GC.__file__.replace("gnn_codec.py", "checkpoints/gnn_d64_agg2.pt")
     x = _field(shape, 6)
     codec = GNNCodec(ckpt)
+    # the Python stage loop reads the sums back through gate_costs_device (the
+    # native loop runs the same kernels and codes the same bytes,
+    # test_native_stage_loop_equals_python_stage_loop)
+    codec.native_stages = False
     stream = codec.compress(x, 1e-3)
     assert len(seen) >= 3
     for i, (xs, p, sc, ips, eb, radius, W) in enumerate(seen):
@@ -432,3 +436,41 @@ def test_row_launch_shapes_give_identical_streams(tmp_path, numerics):
     assert L.latec_tune_set(b"gnn_rows", 8) != 0
     assert L.latec_tune_set(b"gnn_pool", 3) != 0
     assert L.latec_tune_set(b"gnn_geo_table", 2) != 0
+
+
+@pytest.mark.parametrize("case", [
+    dict(shape=(18, 20, 17, 19), levels=2, chunk=None, fine=False, pred="gnn", num="bf16",
+         radius=1 << 15),
+    dict(shape=(40, 36, 44), levels=2, chunk=16, fine=True, pred="gnn", num="bf16", radius=8),
+    dict(shape=(64, 60), levels=3, chunk=None, fine=False, pred="gnn", num="fp32", radius=16),
+    dict(shape=(24, 20, 28), levels=2, chunk=8, fine=False, pred="interp-cubic", num="bf16",
+         radius=8),
+    dict(shape=(30, 26), levels=2, chunk=8, fine=False, pred="gnn", num="bf16", radius=1 << 15,
+         gate=False),
+], ids=["4d", "3d-chunks-fine-r8", "2d-fp32-r16", "3d-interp-r8", "2d-chunks-nogate"])
+def test_native_stage_loop_equals_python_stage_loop(tmp_path, case):
+    """latec_gnn_code_stage (gate, quantizer, reconstruction, levels and the
+    outliers by position in one library call per stage) codes the same bytes
+    as the same steps called one by one from Python; small radii make many
+    outliers, explicit chunk sizes take the chunk's column split."""
+    from latec_amd import gnn_codec as GC
+    from latec_amd.gnn_codec import GNNCodec
+
+    ckpt = _ckpt(tmp_path, 32, 2, 4)
+    x = _field(case["shape"], 6)
+    codec = GNNCodec(ckpt, numerics=case["num"])
+    kw = dict(levels=case["levels"], chunk_size=case["chunk"], gate_fine=case["fine"],
+              predictor=case["pred"], radius=case["radius"], gate=case.get("gate", True))
+    codec.native_stages = False
+    ref = codec.compress(x, 1e-3, **kw)
+    ref_stats = dict(codec.last_stats)
+    codec.native_stages = True
+    got = codec.compress(x, 1e-3, **kw)
+    assert got == ref
+    assert codec.last_stats["outliers"] == ref_stats["outliers"]
+    assert codec.last_stats["stage_codes"] == ref_stats["stage_codes"]
+    if case["radius"] < 64:
+        assert ref_stats["outliers"] > 2
+    y = codec.uncompress(got).numpy()
+    assert np.abs(y - x).max() <= 1e-3 * float(x.max() - x.min()) + GC.roundtrip_slack(
+        x.min(), x.max())
