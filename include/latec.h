@@ -468,6 +468,36 @@ typedef struct latec_gnn_stage_io {
 int latec_gnn_code_stage(latec_gnn *g, latec_plan *p, int stage,
                          const latec_gnn_stage_io *io, void *stream);
 
+/* The decoder's stage after the model, in two calls around the host's rANS
+ * decode (gnn_codec.py:1430-1490).  begin: the coded rows' prediction and
+ * scale, the stream's gate word applied (or interpolation kind interp_kind
+ * at scale eb), and their scale levels copied to `levels` (host, n);
+ * end: the decoded codes and outlier values (host) dequantized against that
+ * prediction and written into d_recon at the rows' tensor indices
+ * ("outlier count mismatch" when the zero codes and the values disagree).
+ * The prediction and indices stay in the handle between the two calls. */
+typedef struct latec_gnn_dstage_io {
+  int64_t M;                 /* points of the full local stage */
+  int64_t n;                 /* coded points */
+  const int64_t *d_keep;     /* n positions in the full stage; NULL if n == M */
+  const float *d_pred;       /* (M) model prediction */
+  const float *d_scale;      /* (M) model scale */
+  const int64_t *d_gidx;     /* (M) tensor index of each stage point */
+  const float *d_block;      /* the chunk's block of the reconstruction */
+  const latec_interp_opts *cand_opts;  /* 3: interpolation kinds 1, 2, 3 */
+  int32_t interp_kind;       /* 0: the model (+ gate_word), 1..3: that kind */
+  uint32_t gate_word;        /* the stage's gate word, 0 for none */
+  double eb;
+  const float *level_thr;    /* host: the 63 scale-level thresholds of eb */
+  uint8_t *levels;           /* host out: n scale levels */
+} latec_gnn_dstage_io;
+int latec_gnn_decode_stage_begin(latec_gnn *g, latec_plan *p, int stage,
+                                 const latec_gnn_dstage_io *io, void *stream);
+int latec_gnn_decode_stage_end(latec_gnn *g, const void *codes,
+                               const float *outliers, int64_t n_out, double eb,
+                               const latec_interp_opts *o, float *d_recon,
+                               void *stream);
+
 /* _gate_apply_t (gnn_codec.py:974-995): where the scale is below (dir 0) or
  * at/above (dir 1) `threshold`, pred = ip and scale *= mult. */
 int latec_gnn_gate_apply(float *d_pred, float *d_scale, const float *d_ip,

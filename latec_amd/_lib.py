@@ -75,6 +75,16 @@ class GnnStageIO(C.Structure):
     ]
 
 
+class GnnDecodeStageIO(C.Structure):
+    _fields_ = [
+        ("M", C.c_int64), ("n", C.c_int64),
+        ("d_keep", C.c_void_p), ("d_pred", C.c_void_p), ("d_scale", C.c_void_p),
+        ("d_gidx", C.c_void_p), ("d_block", C.c_void_p), ("cand_opts", C.c_void_p),
+        ("interp_kind", C.c_int32), ("gate_word", C.c_uint32),
+        ("eb", C.c_double), ("level_thr", C.c_void_p), ("levels", C.c_void_p),
+    ]
+
+
 p = C.c_void_p
 i64, i32, u32, f64, f32 = C.c_int64, C.c_int, C.c_uint32, C.c_double, C.c_float
 
@@ -133,6 +143,8 @@ SIGNATURES = {
     "latec_gnn_gate_costs": (i32, [p, p, p, p, p, i32, i64, f64, u32, p, p, p, p]),
     "latec_gnn_gate_apply": (i32, [p, p, p, i64, f64, f32, i32, p]),
     "latec_gnn_code_stage": (i32, [p, p, i32, p, p]),
+    "latec_gnn_decode_stage_begin": (i32, [p, p, i32, p, p]),
+    "latec_gnn_decode_stage_end": (i32, [p, p, p, i64, f64, p, p, p]),
     "latec_gather_f32": (i32, [p, p, i64, p, p]),
     "latec_scatter_f32": (i32, [p, p, i64, p, p]),
     "latec_train_gather": (i32, [p, i64, p, i64, p, p]),

@@ -190,9 +190,13 @@ struct latec_gnn {
     void *dev;
   };
   std::vector<ClassTab> class_tabs;
-  // latec_gnn_code_stage's device temporaries
+  // latec_gnn_code_stage's / latec_gnn_decode_stage_*'s device temporaries
   void *stage_work = nullptr;
   size_t stage_work_bytes = 0;
+  // the decoder's pinned staging of a stage's codes and outliers
+  void *stage_host = nullptr;
+  size_t stage_host_bytes = 0;
+  int64_t dec_n = -1;  // coded points of the stage latec_gnn_decode_stage_begin set up
   int64_t line_off_key[LATEC_MAX_DIM] = {0};
   // workspace
   void *work = nullptr;
@@ -2612,6 +2616,7 @@ void latec_gnn_destroy(latec_gnn *g) {
   for (auto &t : g->gate_tabs) (void)hipFree(t.second);
   for (auto &t : g->class_tabs) (void)hipFree(t.dev);
   if (g->stage_work) (void)hipFree(g->stage_work);
+  if (g->stage_host) (void)hipHostFree(g->stage_host);
   delete g;
 }
 
@@ -3004,7 +3009,8 @@ int latec_gnn_code_stage(latec_gnn *g, latec_plan *p, int stage, const latec_gnn
     latec::set_error("latec_gnn_code_stage: null argument");
     return LATEC_E_ARG;
   }
-  if (stage < 1 || stage >= (int)p->stages.size() || io->n < 0 || io->n > io->M ||
+  if (stage < 1 || stage >= (int)p->stages.size() || p->channels != 1 ||
+      io->M != p->stages[stage].count || io->n < 0 || io->n > io->M ||
       (io->n < io->M && !io->d_keep)) {
     latec::set_error("latec_gnn_code_stage: bad stage or counts");
     return LATEC_E_ARG;
@@ -3139,6 +3145,180 @@ int latec_gnn_code_stage(latec_gnn *g, latec_plan *p, int stage, const latec_gnn
   }
   *io->n_out = k;
   return LATEC_OK;
+}
+
+// Decoder stage, first half: the coded rows' prediction and scale (gate
+// applied from the stream's word), their scale levels to the host.  The rows'
+// prediction and tensor indices stay in the stage workspace for
+// latec_gnn_decode_stage_end.
+int latec_gnn_decode_stage_begin(latec_gnn *g, latec_plan *p, int stage,
+                                 const latec_gnn_dstage_io *io, void *stream) {
+  if (!g || !p || !io || !io->d_pred || !io->d_scale || !io->d_gidx || !io->level_thr ||
+      !io->levels || !io->cand_opts) {
+    latec::set_error("latec_gnn_decode_stage_begin: null argument");
+    return LATEC_E_ARG;
+  }
+  if (stage < 1 || stage >= (int)p->stages.size() || p->channels != 1 ||
+      io->M != p->stages[stage].count || io->n < 0 || io->n > io->M ||
+      (io->n < io->M && !io->d_keep)) {
+    latec::set_error("latec_gnn_decode_stage_begin: bad stage or counts");
+    return LATEC_E_ARG;
+  }
+  const int64_t M = io->M, n = io->n;
+  g->dec_n = -1;
+  if (n == 0) return LATEC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t word = io->gate_word;
+  const int gk = (int)((word >> 8) & 3), gt = (int)((word >> 4) & 15), gs = (int)(word & 15);
+  const int gd = (int)((word >> 10) & 1);
+  const int kind = io->interp_kind ? io->interp_kind : (gt > 0 ? gk : 0);
+  if (kind < 0 || kind > 3 || (io->interp_kind == 0 && gt > 0 && gk == 0)) {
+    latec::set_error("latec_gnn_decode_stage_begin: bad gate word");
+    return LATEC_E_ARG;
+  }
+  if (kind && !io->d_block) {
+    latec::set_error("latec_gnn_decode_stage_begin: the gate needs the chunk block");
+    return LATEC_E_ARG;
+  }
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t b_f = al(4 * (size_t)n), b_i = al(8 * (size_t)n), b_cf = al(4 * (size_t)M);
+  // p, sc, rec (n f32), gidx (n i64), codes (4n), ranks (4n), vals (4n), cnt,
+  // levels (n), one candidate (M full, n kept)
+  const size_t need = 3 * b_f + b_i + 3 * b_f + 256 + al((size_t)n) + b_cf + b_f;
+  int rc = stage_work(g, need);
+  if (rc) return rc;
+  unsigned char *w = (unsigned char *)g->stage_work;
+  float *dp = (float *)w, *dsc = (float *)(w + b_f);
+  int64_t *dgidx = (int64_t *)(w + 3 * b_f);
+  uint8_t *dlv = (uint8_t *)(w + 3 * b_f + b_i + 3 * b_f + 256);
+  float *dcf = (float *)(w + 3 * b_f + b_i + 3 * b_f + 256 + al((size_t)n));
+  float *dck = (float *)((unsigned char *)dcf + b_cf);
+  const int gb = grid_for(n, 256);
+  if (io->d_keep) {
+    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, s, n, io->d_pred, io->d_keep, dp);
+    hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, s, n, io->d_scale, io->d_keep,
+                       dsc);
+    hipLaunchKernelGGL(gather_i64_kernel, dim3(gb), dim3(256), 0, s, n, io->d_gidx, io->d_keep,
+                       dgidx);
+  } else {
+    rc = hip_ok(hipMemcpyAsync(dp, io->d_pred, 4 * n, hipMemcpyDeviceToDevice, s), "copy");
+    if (!rc) rc = hip_ok(hipMemcpyAsync(dsc, io->d_scale, 4 * n, hipMemcpyDeviceToDevice, s),
+                         "copy");
+    if (!rc) rc = hip_ok(hipMemcpyAsync(dgidx, io->d_gidx, 8 * n, hipMemcpyDeviceToDevice, s),
+                         "copy");
+    if (rc) return rc;
+  }
+  if (kind) {
+    rc = latec_interp_predict(p, stage, io->d_block, dcf, &io->cand_opts[kind - 1], stream);
+    if (rc) return rc;
+    if (io->d_keep)
+      hipLaunchKernelGGL(gather_kernel, dim3(gb), dim3(256), 0, s, n, (const float *)dcf,
+                         io->d_keep, dck);
+    else
+      dck = dcf;
+    if (io->interp_kind) {
+      rc = hip_ok(hipMemcpyAsync(dp, dck, 4 * n, hipMemcpyDeviceToDevice, s), "copy");
+      if (rc) return rc;
+      hipLaunchKernelGGL(fill_f32_kernel, dim3(gb), dim3(256), 0, s, n, (float)io->eb, dsc);
+    } else {
+      hipLaunchKernelGGL(gate_apply_kernel, dim3(gb), dim3(256), 0, s, n, dp, dsc,
+                         (const float *)dck, io->eb * std::ldexp(1.0, gt),
+                         (float)std::ldexp(1.0, -gs), gd);
+    }
+  }
+  rc = latec_scale_to_level(dsc, n, io->level_thr, dlv, stream);
+  if (!rc) rc = hip_ok(hipMemcpyAsync(io->levels, dlv, (size_t)n, hipMemcpyDeviceToHost, s),
+                       "hipMemcpyAsync");
+  if (!rc) rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (!rc) g->dec_n = n;
+  return rc;
+}
+
+// Decoder stage, second half: the decoded codes (host, n of opts->code_bytes)
+// and outlier values (host, n_out, by stage position) dequantized against
+// the prediction latec_gnn_decode_stage_begin left, written into d_recon at
+// the rows' tensor indices.  Stream-ordered; the host arrays are copied to
+// pinned staging before return.
+int latec_gnn_decode_stage_end(latec_gnn *g, const void *codes, const float *outliers,
+                               int64_t n_out, double eb, const latec_interp_opts *o,
+                               float *d_recon, void *stream) {
+  if (!g || !o || !d_recon || (g->dec_n > 0 && !codes) || n_out < 0 ||
+      (n_out && !outliers)) {
+    latec::set_error("latec_gnn_decode_stage_end: bad argument");
+    return LATEC_E_ARG;
+  }
+  const int64_t n = g->dec_n;
+  if (n < 0) {
+    latec::set_error("latec_gnn_decode_stage_end without latec_gnn_decode_stage_begin");
+    return LATEC_E_ARG;
+  }
+  g->dec_n = -1;
+  if (n == 0) return LATEC_OK;
+  const int cb = o->code_bytes;
+  if (cb != 2 && cb != 4) {
+    latec::set_error("latec_gnn_decode_stage_end: code_bytes must be 2 or 4");
+    return LATEC_E_ARG;
+  }
+  // ranks of the zero codes, ascending (the order the outliers are stored in)
+  std::vector<uint32_t> zr;
+  for (int64_t i = 0; i < n; i++) {
+    const uint32_t c = cb == 2 ? ((const uint16_t *)codes)[i] : ((const uint32_t *)codes)[i];
+    if (c == 0) zr.push_back((uint32_t)i);
+  }
+  if ((int64_t)zr.size() != n_out) {
+    latec::set_error("outlier count mismatch: " + std::to_string(zr.size()) + " zeros vs " +
+                     std::to_string(n_out) + " values");
+    return LATEC_E_COUNT;
+  }
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t k = zr.size();
+  const size_t hb_c = al((size_t)cb * n), hb_k = al(4 * (k ? k : 1));
+  const size_t hbytes = hb_c + 2 * hb_k + 256;
+  hipStream_t s = (hipStream_t)stream;
+  if (g->stage_host_bytes < hbytes) {
+    // the previous stage's copies have landed: its begin waited on the stream
+    int rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (rc) return rc;
+    if (g->stage_host) (void)hipHostFree(g->stage_host);
+    g->stage_host = nullptr;
+    g->stage_host_bytes = 0;
+    const size_t want = hbytes * 5 / 4 + (1 << 20);
+    rc = hip_ok(hipHostMalloc(&g->stage_host, want, hipHostMallocDefault), "hipHostMalloc");
+    if (rc) return rc;
+    g->stage_host_bytes = want;
+  }
+  unsigned char *h = (unsigned char *)g->stage_host;
+  memcpy(h, codes, (size_t)cb * n);
+  if (k) {
+    memcpy(h + hb_c, zr.data(), 4 * k);
+    memcpy(h + hb_c + hb_k, outliers, 4 * k);
+  }
+  const uint32_t k32 = (uint32_t)k;
+  memcpy(h + hb_c + 2 * hb_k, &k32, 4);
+  auto alw = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t b_f = alw(4 * (size_t)n), b_i = alw(8 * (size_t)n);
+  unsigned char *w = (unsigned char *)g->stage_work;
+  const float *dp = (const float *)w;
+  float *drec = (float *)(w + 2 * b_f);
+  const int64_t *dgidx = (const int64_t *)(w + 3 * b_f);
+  unsigned char *dcodes = w + 3 * b_f + b_i;
+  uint32_t *drank = (uint32_t *)(dcodes + b_f);
+  float *dval = (float *)(dcodes + 2 * b_f);
+  uint32_t *dcnt = (uint32_t *)(dcodes + 3 * b_f);
+  int rc = hip_ok(hipMemcpyAsync(dcodes, h, (size_t)cb * n, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync");
+  if (!rc && k) rc = hip_ok(hipMemcpyAsync(drank, h + hb_c, 4 * k, hipMemcpyHostToDevice, s),
+                            "hipMemcpyAsync");
+  if (!rc && k) rc = hip_ok(hipMemcpyAsync(dval, h + hb_c + hb_k, 4 * k, hipMemcpyHostToDevice,
+                                           s), "hipMemcpyAsync");
+  if (!rc) rc = hip_ok(hipMemcpyAsync(dcnt, h + hb_c + 2 * hb_k, 4, hipMemcpyHostToDevice, s),
+                       "hipMemcpyAsync");
+  if (rc) return rc;
+  rc = latec_dequantize(dp, dcodes, n, drank, dval, dcnt, eb, o, drec, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                     (const float *)drec, dgidx, d_recon);
+  return hip_ok(hipGetLastError(), "scatter_kernel launch");
 }
 
 int latec_gnn_gate_apply(float *d_pred, float *d_scale, const float *d_ip, int64_t n,

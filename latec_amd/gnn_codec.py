@@ -35,7 +35,7 @@ import numpy as np
 
 from . import _device as D
 from . import rans as R
-from ._lib import GnnStageIO, InterpOpts, check, lib, ptr, stream_ptr
+from ._lib import GnnDecodeStageIO, GnnStageIO, InterpOpts, check, lib, ptr, stream_ptr
 from .bitstream import pack_stage, unpack_stage, zstd_compress, zstd_decompress
 from .gnn_predictor import ChunkedGNNPredictor, load_model
 from .levels import _slice_count, stage_ebs, stage_layouts, stage_strides
@@ -321,8 +321,9 @@ class GNNCompressorCodec:
         self.last_stats: dict = {}
         self.last_decoded = None  # the last uncompress's normalized reconstruction (device)
         self.count_flops = False  # count the model's GEMM flops into last_stats (syncs)
-        # each stage after the model in one library call (latec_gnn_code_stage);
-        # False runs the same steps as separate calls from Python
+        # each stage after the model in one library call (latec_gnn_code_stage;
+        # the decoder: latec_gnn_decode_stage_begin / _end around the rANS
+        # decode); False runs the same steps as separate calls from Python
         self.native_stages = True
 
     @staticmethod
@@ -492,6 +493,27 @@ class GNNCompressorCodec:
                                          ctypes.byref(io), stream_ptr()))
         k = int(n_out[0])
         return codes, outl[:k].copy(), levels, k, int(word[0])
+
+    @staticmethod
+    def _decode_stage_begin(pred_obj, plan, s, n, dkeep, p_full, sc_full, block, cand_opts,
+                            interp_kind, word, eb):
+        """latec_gnn_decode_stage_begin: the stage's prediction with the gate
+        applied; returns its scale levels (host)."""
+        io = GnnDecodeStageIO()
+        io.M, io.n = int(plan.counts[s]), int(n)
+        io.d_keep = None if dkeep is None else dkeep.data_ptr()
+        io.d_pred, io.d_scale = p_full[0].data_ptr(), sc_full[0].data_ptr()
+        io.d_gidx = pred_obj.stage_gidx(s).data_ptr()
+        io.d_block = None if block is None else block.data_ptr()
+        io.cand_opts = ctypes.addressof(cand_opts)
+        io.interp_kind, io.gate_word, io.eb = int(interp_kind), int(word), eb
+        lthr = np.ascontiguousarray(R.scale_thresholds(eb), np.float32)
+        io.level_thr = lthr.ctypes.data
+        levels = np.empty(n, np.uint8)
+        io.levels = levels.ctypes.data
+        check(lib().latec_gnn_decode_stage_begin(pred_obj.model.handle, plan.handle, s,
+                                                 ctypes.byref(io), stream_ptr()))
+        return levels
 
     def _encode(self, values, xn, norm, round_output, eb, ratio, edges, levels, radius,
                 gate, gate_fine, interp_kind, numerics, zstd_level):
@@ -680,6 +702,7 @@ class GNNCompressorCodec:
         recon.view(-1)[D.to_device(_anchor_flat_indices(axes, shape))] = D.to_device(a_recon)
         is_finest = [st == 1 for st in stage_strides(ndim, levels, stride)]
         o = opts_for(radius)
+        cand_opts = _candidate_opts(ndim)
         gi = fgi = 0
         for ci in range(pred_obj.n_chunks):
             sls = pred_obj.chunk_slices(ci)
@@ -696,22 +719,13 @@ class GNNCompressorCodec:
                     continue
                 dkeep = sched.device_keep(s)
                 p_full, sc_full = pred_obj.predict_wave_stage(s, recon, ebs[s])
-                gidx = pred_obj.stage_gidx(s)
-                if dkeep is not None:
-                    p, sc, gidx = p_full[0][dkeep], sc_full[0][dkeep], gidx[dkeep]
-                else:
-                    p, sc = p_full[0].clone(), sc_full[0].clone()
                 n = int(sched.counts[s])
                 if n == 0:
                     _, _, off = unpack_stage(payload, off, rans_levels=np.zeros(0, np.uint8),
                                              rans_tables=tables[s], radius=radius)
                     continue
-                if interp_kind:
-                    p = self._interp_candidates(recon, sls, single, plan, s, (interp_kind,),
-                                                dkeep, n)[interp_kind]
-                    sc = T.full((n,), float(ebs[s]), dtype=T.float32, device=D.device())
-                else:
-                    word = None
+                word = None
+                if not interp_kind:
                     if fine_gates is not None and is_finest[s]:
                         if fine_desc is None:
                             fine_desc = int(fine_gates[fgi])
@@ -720,6 +734,35 @@ class GNNCompressorCodec:
                     elif gates is not None:
                         word = int(gates[gi])
                         gi += 1
+                if self.native_stages:
+                    gated = bool(interp_kind) or (word is not None and (word >> 4) & 15 > 0)
+                    block = None
+                    if gated:
+                        block = recon if single else recon[(slice(None), *sls)].contiguous()
+                    h_lv = self._decode_stage_begin(pred_obj, plan, s, n, dkeep, p_full, sc_full,
+                                                    block, cand_opts, interp_kind, word or 0,
+                                                    float(ebs[s]))
+                    codes, outl, off = unpack_stage(payload, off, rans_levels=h_lv,
+                                                    rans_tables=tables[s], radius=radius)
+                    if len(codes) != n:
+                        raise ValueError("stage code count mismatch")
+                    hc = np.ascontiguousarray(codes, np.uint16 if o.code_bytes == 2
+                                              else np.uint32)
+                    ov = np.ascontiguousarray(outl, np.float32)
+                    check(lib().latec_gnn_decode_stage_end(
+                        pred_obj.model.handle, ptr(hc), ptr(ov) if len(ov) else None, len(ov),
+                        float(ebs[s]), _optp(o), ptr(recon), stream_ptr()))
+                    continue
+                gidx = pred_obj.stage_gidx(s)
+                if dkeep is not None:
+                    p, sc, gidx = p_full[0][dkeep], sc_full[0][dkeep], gidx[dkeep]
+                else:
+                    p, sc = p_full[0].clone(), sc_full[0].clone()
+                if interp_kind:
+                    p = self._interp_candidates(recon, sls, single, plan, s, (interp_kind,),
+                                                dkeep, n)[interp_kind]
+                    sc = T.full((n,), float(ebs[s]), dtype=T.float32, device=D.device())
+                else:
                     if word is not None:
                         gk, gt, gs, gd = (word >> 8) & 3, (word >> 4) & 15, word & 15, \
                             (word >> _GATE_DIR_SHIFT) & 1

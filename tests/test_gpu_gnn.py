@@ -474,3 +474,39 @@ def test_native_stage_loop_equals_python_stage_loop(tmp_path, case):
     y = codec.uncompress(got).numpy()
     assert np.abs(y - x).max() <= 1e-3 * float(x.max() - x.min()) + GC.roundtrip_slack(
         x.min(), x.max())
+    # the decoder's latec_gnn_decode_stage_begin / _end against the same
+    # steps called one by one from Python
+    rec_native = codec.last_decoded.clone()
+    codec.native_stages = False
+    y_ref = codec.uncompress(got).numpy()
+    codec.native_stages = True
+    assert np.array_equal(y.view(np.uint32), y_ref.view(np.uint32))
+    assert torch.equal(rec_native, codec.last_decoded)
+
+
+@pytest.mark.gpu
+def test_native_decode_stage_rejects_outlier_count_mismatch(tmp_path, monkeypatch):
+    """A stream whose outlier list is one value short fails with the
+    outlier count error of latec_gnn_decode_stage_end (a host check before
+    any copy or kernel); the next decode of a good stream still succeeds."""
+    from latec_amd import gnn_codec as GC
+    from latec_amd.gnn_codec import GNNCodec
+
+    ckpt = _ckpt(tmp_path, 32, 2, 4)
+    x = _field((40, 36), 6)
+    codec = GNNCodec(ckpt, numerics="fp32")
+    blob = codec.compress(x, 1e-3, levels=3, radius=8)
+    assert codec.last_stats["outliers"] > 2
+    good = codec.uncompress(blob).numpy()
+    real = GC.unpack_stage
+
+    def short(*a, **k):
+        codes, outl, off = real(*a, **k)
+        return codes, (outl[:-1] if len(outl) else outl), off
+
+    monkeypatch.setattr(GC, "unpack_stage", short)
+    with pytest.raises(ValueError, match="outlier count mismatch"):
+        codec.uncompress(blob)
+    monkeypatch.setattr(GC, "unpack_stage", real)
+    again = codec.uncompress(blob).numpy()
+    assert np.array_equal(good.view(np.uint32), again.view(np.uint32))
