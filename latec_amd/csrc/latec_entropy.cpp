@@ -64,8 +64,9 @@ struct SymInfo {
 };
 
 // Decoder lookup: one 64-bit entry per 2^12-slot bucket of the 2^24 quantile
-// range holds the symbol at the bucket's first slot as left | prob << 24 |
-// k << 48 (left, prob < 2^24, k < 2^16), so a quantile inside that symbol
+// range holds the symbol at the bucket's first slot as prob | left << 24 |
+// k << 48 (left, prob < 2^24, k < 2^16; the probability in the low bits is
+// ready for the state update with one mask), so a quantile inside that symbol
 // (every quantile of a bucket with no symbol boundary, >= 94% of buckets for
 // A <= 256) costs one 32 KiB-table load; the rest step forward through left[].
 static const int DLUT_SHIFT = 12;
@@ -110,7 +111,7 @@ static void build_model(const uint32_t *cdf, int64_t A, Model &m, bool for_decod
     for (size_t b = 0; b < m.dlut.size(); b++) {
       const uint32_t q = (uint32_t)(b << DLUT_SHIFT);
       while (k + 1 < A && m.left[k + 1] <= q) k++;
-      m.dlut[b] = (uint64_t)m.sym[k].left | ((uint64_t)m.sym[k].prob << 24) |
+      m.dlut[b] = (uint64_t)m.sym[k].prob | ((uint64_t)m.sym[k].left << 24) |
                   ((uint64_t)k << 48);
     }
   }
@@ -152,7 +153,7 @@ static inline uint32_t dec_get(uint64_t &state, const Model &m, const uint32_t *
                                int64_t &top) {
   const uint32_t qt = (uint32_t)(state & ((1u << PREC) - 1));
   const uint64_t e = m.dlut[qt >> DLUT_SHIFT];
-  uint32_t left = (uint32_t)e & 0xffffffu, prob = (uint32_t)(e >> 24) & 0xffffffu;
+  uint32_t prob = (uint32_t)e & 0xffffffu, left = (uint32_t)(e >> 24) & 0xffffffu;
   uint32_t k = (uint32_t)(e >> 48);
   if (qt - left >= prob) {  // past the bucket's first symbol
     do k++;
@@ -208,12 +209,10 @@ int64_t rans_encode(const uint32_t *syms, int64_t n, const uint32_t *cdf, int64_
   return nw;
 }
 
-// Stable counting sort of positions 0..n-1 by level: order[start[l] ..
-// start[l + 1]) holds the positions of level l in increasing order.  False on a
-// level >= 64.
+// Stable counting sort of vals by level: sorted[start[l] .. start[l + 1])
+// holds the values of level l in position order.  False on a level >= 64.
 static bool group_by_level(const uint8_t *levels, int64_t n, int64_t start[65],
-                           std::unique_ptr<uint32_t[]> *order, const uint32_t *vals,
-                           std::unique_ptr<uint32_t[]> *sorted) {
+                           const uint32_t *vals, std::unique_ptr<uint32_t[]> *sorted) {
   int64_t cnt[65] = {0};
   for (int64_t i = 0; i < n; i++) cnt[levels[i] < 64 ? levels[i] + 1 : 64]++;
   for (int64_t i = 0; i < n; i++)
@@ -222,13 +221,8 @@ static bool group_by_level(const uint8_t *levels, int64_t n, int64_t start[65],
   for (int l = 0; l < 64; l++) start[l + 1] = start[l] + cnt[l + 1];
   int64_t pos[64];
   for (int l = 0; l < 64; l++) pos[l] = start[l];
-  if (order) {
-    order->reset(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
-    for (int64_t i = 0; i < n; i++) (*order)[pos[levels[i]]++] = (uint32_t)i;
-  } else {
-    sorted->reset(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
-    for (int64_t i = 0; i < n; i++) (*sorted)[pos[levels[i]]++] = vals[i];
-  }
+  sorted->reset(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
+  for (int64_t i = 0; i < n; i++) (*sorted)[pos[levels[i]]++] = vals[i];
   return true;
 }
 
@@ -242,7 +236,7 @@ int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t 
   }
   int64_t start[65];
   std::unique_ptr<uint32_t[]> sorted;
-  if (!group_by_level(levels, n, start, nullptr, syms, &sorted)) {
+  if (!group_by_level(levels, n, start, syms, &sorted)) {
     set_error("scale level exceeds table count");
     return LATEC_E_ARG;
   }
@@ -309,22 +303,38 @@ int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
     set_error("stage too large for the rANS coder");
     return LATEC_E_ARG;
   }
-  int64_t start[65];
-  std::unique_ptr<uint32_t[]> order;
-  if (!group_by_level(levels, n, start, &order, nullptr, nullptr)) {
-    set_error("scale level exceeds table count");
-    return LATEC_E_ARG;
+  // the symbols come level by level (each level's positions in increasing
+  // order): each group is decoded into its own contiguous run of a scratch
+  // array, then one pass in position order takes every position's symbol
+  // from the front of its level's run
+  int64_t start[65] = {0};
+  for (int64_t i = 0; i < n; i++) {
+    if (levels[i] >= 64) {
+      set_error("scale level exceeds table count");
+      return LATEC_E_ARG;
+    }
+    start[levels[i] + 1]++;
   }
+  for (int l = 0; l < 64; l++) start[l + 1] += start[l];
+  // the calling thread keeps its scratch between calls up to 2^24 symbols
+  static thread_local std::vector<uint32_t> keep;
+  std::vector<uint32_t> own;
+  std::vector<uint32_t> &tmp = n <= ((int64_t)1 << 24) ? keep : own;
+  if ((int64_t)tmp.size() < n) tmp.resize((size_t)n);
+  uint32_t *t = tmp.data();
   for (int l = 0; l < 64; l++) {
     if (start[l + 1] == start[l]) continue;
     Model m;
     build_model(cdfs + (int64_t)l * (A + 1), A, m, true);
-    for (int64_t j = start[l]; j < start[l + 1]; j++) syms[order[j]] = dec_get(state, m, words, top);
+    for (int64_t j = start[l]; j < start[l + 1]; j++) t[j] = dec_get(state, m, words, top);
   }
   if (top != 0 || state != 0) {
     set_error("trailing data in rANS payload");
     return LATEC_E_FORMAT;
   }
+  int64_t pos[64];
+  for (int l = 0; l < 64; l++) pos[l] = start[l];
+  for (int64_t i = 0; i < n; i++) syms[i] = t[pos[levels[i]]++];
   return LATEC_OK;
 }
 
