@@ -209,20 +209,15 @@ int64_t rans_encode(const uint32_t *syms, int64_t n, const uint32_t *cdf, int64_
   return nw;
 }
 
-// Stable counting sort of vals by level: sorted[start[l] .. start[l + 1])
-// holds the values of level l in position order.  False on a level >= 64.
-static bool group_by_level(const uint8_t *levels, int64_t n, int64_t start[65],
-                           const uint32_t *vals, std::unique_ptr<uint32_t[]> *sorted) {
-  int64_t cnt[65] = {0};
-  for (int64_t i = 0; i < n; i++) cnt[levels[i] < 64 ? levels[i] + 1 : 64]++;
-  for (int64_t i = 0; i < n; i++)
-    if (levels[i] >= 64) return false;
+// Level grouping of positions 0..n-1 (stable): level l's positions take
+// [start[l], start[l + 1]) in increasing order.  False on a level >= 64.
+static bool group_levels(const uint8_t *levels, int64_t n, int64_t start[65]) {
+  int64_t cnt[256] = {0};
+  for (int64_t i = 0; i < n; i++) cnt[levels[i]]++;
+  for (int l = 64; l < 256; l++)
+    if (cnt[l]) return false;
   start[0] = 0;
-  for (int l = 0; l < 64; l++) start[l + 1] = start[l] + cnt[l + 1];
-  int64_t pos[64];
-  for (int l = 0; l < 64; l++) pos[l] = start[l];
-  sorted->reset(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
-  for (int64_t i = 0; i < n; i++) (*sorted)[pos[levels[i]]++] = vals[i];
+  for (int l = 0; l < 64; l++) start[l + 1] = start[l] + cnt[l];
   return true;
 }
 
@@ -235,12 +230,23 @@ int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t 
     return LATEC_E_ARG;
   }
   int64_t start[65];
-  std::unique_ptr<uint32_t[]> sorted;
-  if (!group_by_level(levels, n, start, syms, &sorted)) {
+  if (!group_levels(levels, n, start)) {
     set_error("scale level exceeds table count");
     return LATEC_E_ARG;
   }
-  std::vector<uint32_t> tmp(enc_cap(n));
+  std::unique_ptr<uint32_t[]> sorted(new uint32_t[(size_t)std::max<int64_t>(n, 1)]);
+  {
+    int64_t next[64];
+    for (int l = 0; l < 64; l++) next[l] = start[l];
+    for (int64_t i = 0; i < n; i++) sorted[next[levels[i]]++] = syms[i];
+  }
+  // words in place when the caller's array holds the worst case
+  std::unique_ptr<uint32_t[]> tmp;
+  uint32_t *out = words;
+  if (cap < enc_cap(n)) {
+    tmp.reset(new uint32_t[(size_t)enc_cap(n)]);
+    out = tmp.get();
+  }
   uint64_t state = 0;
   int64_t nw = 0;
   for (int l = 63; l >= 0; l--) {
@@ -254,15 +260,15 @@ int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t 
         set_error("code outside the narrowed rANS alphabet");
         return LATEC_E_ARG;
       }
-      enc_put(state, si[s], tmp.data(), nw);
+      enc_put(state, si[s], out, nw);
     }
   }
-  nw = enc_flush(state, tmp.data(), nw);
+  nw = enc_flush(state, out, nw);
   if (nw > cap) {
     set_error("rANS output capacity");
     return LATEC_E_CAPACITY;
   }
-  memcpy(words, tmp.data(), sizeof(uint32_t) * nw);
+  if (out != words) memcpy(words, out, sizeof(uint32_t) * nw);
   return nw;
 }
 
@@ -307,15 +313,11 @@ int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
   // order): each group is decoded into its own contiguous run of a scratch
   // array, then one pass in position order takes every position's symbol
   // from the front of its level's run
-  int64_t start[65] = {0};
-  for (int64_t i = 0; i < n; i++) {
-    if (levels[i] >= 64) {
-      set_error("scale level exceeds table count");
-      return LATEC_E_ARG;
-    }
-    start[levels[i] + 1]++;
+  int64_t start[65];
+  if (!group_levels(levels, n, start)) {
+    set_error("scale level exceeds table count");
+    return LATEC_E_ARG;
   }
-  for (int l = 0; l < 64; l++) start[l + 1] += start[l];
   // the calling thread keeps its scratch between calls up to 2^24 symbols
   static thread_local std::vector<uint32_t> keep;
   std::vector<uint32_t> own;
@@ -332,9 +334,9 @@ int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
     set_error("trailing data in rANS payload");
     return LATEC_E_FORMAT;
   }
-  int64_t pos[64];
-  for (int l = 0; l < 64; l++) pos[l] = start[l];
-  for (int64_t i = 0; i < n; i++) syms[i] = t[pos[levels[i]]++];
+  int64_t next[64];
+  for (int l = 0; l < 64; l++) next[l] = start[l];
+  for (int64_t i = 0; i < n; i++) syms[i] = t[next[levels[i]]++];
   return LATEC_OK;
 }
 
