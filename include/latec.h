@@ -258,6 +258,19 @@ int64_t latec_rans_encode(const uint32_t *syms, int64_t n,
 int64_t latec_rans_encode_levels(const uint32_t *syms, const uint8_t *levels,
                                  int64_t n, const uint32_t *cdfs, int64_t A,
                                  uint32_t *words, int64_t cap);
+/* The same from the quantizer's codes (u16/u32, code_bytes), narrowed to
+ * the alphabet of 2^kexp on each side of `radius` on the fly (rans.py:83-89;
+ * cdfs is (64, A+1) with A = 2 << kexp): rans_encode's narrowing and coding
+ * (rans.py:546-578) in one call. */
+int64_t latec_rans_encode_codes_levels(const void *codes, int code_bytes,
+                                       const uint8_t *levels, int64_t n,
+                                       uint32_t radius, int kexp,
+                                       const uint32_t *cdfs, int64_t A,
+                                       uint32_t *words, int64_t cap);
+/* choose_kexp (rans.py:70-80) over u16/u32 codes: the bit length of the
+ * largest |code - radius| of the nonzero codes, at least 1. */
+int latec_code_kexp(const void *codes, int code_bytes, int64_t n,
+                    uint32_t radius);
 /* Returns 0, or LATEC_E_FORMAT for trailing / malformed data. */
 int latec_rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf,
                       int64_t A, uint32_t *syms, int64_t n);

@@ -118,6 +118,8 @@ SIGNATURES = {
     "latec_scale_to_level": (i32, [p, i64, p, p, p]),
     "latec_rans_encode": (i64, [p, i64, p, i64, p, i64]),
     "latec_rans_encode_levels": (i64, [p, p, i64, p, i64, p, i64]),
+    "latec_rans_encode_codes_levels": (i64, [p, i32, p, i64, u32, i32, p, i64, p, i64]),
+    "latec_code_kexp": (i32, [p, i32, i64, u32]),
     "latec_rans_decode": (i32, [p, i64, p, i64, p, i64]),
     "latec_rans_decode_levels": (i32, [p, i64, p, p, i64, p, i64]),
     "latec_pack_stage": (i64, [p, i32, i64, p, i64, u32, i32, p, p, p, i64]),

@@ -180,9 +180,9 @@ def pack_stage(codes, outliers, *, rans_levels=None, rans_tables=None,
                radius: int = DEFAULT_RADIUS, zstd_level: int = 9) -> bytes:
     """bitstream.py:178-243: const-level fit (GPU) + literal/rANS back end, or
     per-point levels when ``rans_levels`` is given."""
-    blob_codes = np.asarray(codes, np.uint32).ravel()
     out = np.ascontiguousarray(outliers, np.float32).ravel()
     if rans_levels is None:
+        blob_codes = np.asarray(codes, np.uint32).ravel()
         tables = rans_tables if rans_tables is not None else R.const_tables(radius)
         R_ = tables.radius
         cb = 2 if R_ <= 32768 else 4
@@ -196,6 +196,7 @@ def pack_stage(codes, outliers, *, rans_levels=None, rans_tables=None,
         return pack_record(blob_codes, out, fit, R_, zstd_level)
     if rans_tables is None:
         raise ValueError("rans_tables are required with rans_levels")
+    blob_codes = R.code_array(codes)  # u16 codes stay u16
     kexp = R.choose_kexp(blob_codes, rans_tables.radius)
     shape = R.choose_shape(blob_codes, rans_levels, rans_tables, kexp)
     hblob = R.rans_encode(blob_codes, rans_levels, rans_tables, kexp=kexp, shape=shape)

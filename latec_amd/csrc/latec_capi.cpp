@@ -55,6 +55,9 @@ int64_t hist_words(uint32_t R);
 int64_t rans_encode(const uint32_t *, int64_t, const uint32_t *, int64_t, uint32_t *, int64_t);
 int64_t rans_encode_levels(const uint32_t *, const uint8_t *, int64_t, const uint32_t *,
                            int64_t, uint32_t *, int64_t);
+int64_t rans_encode_codes_levels(const void *, int, const uint8_t *, int64_t, uint32_t, int,
+                                 const uint32_t *, int64_t, uint32_t *, int64_t);
+int code_kexp(const void *, int, int64_t, uint32_t);
 int rans_decode(const uint32_t *, int64_t, const uint32_t *, int64_t, uint32_t *, int64_t);
 int rans_decode_levels(const uint32_t *, int64_t, const uint8_t *, const uint32_t *, int64_t,
                        uint32_t *, int64_t);
@@ -298,6 +301,20 @@ int64_t latec_rans_encode_levels(const uint32_t *syms, const uint8_t *levels, in
                                  const uint32_t *cdfs, int64_t A, uint32_t *words,
                                  int64_t cap) {
   return rans_encode_levels(syms, levels, n, cdfs, A, words, cap);
+}
+int64_t latec_rans_encode_codes_levels(const void *codes, int code_bytes,
+                                       const uint8_t *levels, int64_t n, uint32_t radius,
+                                       int kexp, const uint32_t *cdfs, int64_t A,
+                                       uint32_t *words, int64_t cap) {
+  return rans_encode_codes_levels(codes, code_bytes, levels, n, radius, kexp, cdfs, A, words,
+                                  cap);
+}
+int latec_code_kexp(const void *codes, int code_bytes, int64_t n, uint32_t radius) {
+  if (code_bytes != 2 && code_bytes != 4) {
+    set_error("latec_code_kexp: code_bytes must be 2 or 4");
+    return LATEC_E_ARG;
+  }
+  return code_kexp(codes, code_bytes, n, radius);
 }
 int latec_rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t A,
                       uint32_t *syms, int64_t n) {

@@ -221,10 +221,12 @@ static bool group_levels(const uint8_t *levels, int64_t n, int64_t start[65]) {
   return true;
 }
 
-int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t n,
-                           const uint32_t *cdfs, int64_t A, uint32_t *words, int64_t cap) {
-  // groups by level, levels pushed 63..0, each group in reverse (encode_reverse):
-  // the level-sorted symbols from last to first
+// Per-point models: the symbols sym(i) grouped by level (stable), levels
+// pushed 63..0, each group last to first (encode_reverse).
+template <class Sym>
+static int64_t encode_levels_from(Sym sym, const uint8_t *levels, int64_t n,
+                                  const uint32_t *cdfs, int64_t A, uint32_t *words,
+                                  int64_t cap) {
   if (n > (int64_t)UINT32_MAX) {
     set_error("stage too large for the rANS coder");
     return LATEC_E_ARG;
@@ -238,7 +240,7 @@ int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t 
   {
     int64_t next[64];
     for (int l = 0; l < 64; l++) next[l] = start[l];
-    for (int64_t i = 0; i < n; i++) sorted[next[levels[i]]++] = syms[i];
+    for (int64_t i = 0; i < n; i++) sorted[next[levels[i]]++] = sym(i);
   }
   // words in place when the caller's array holds the worst case
   std::unique_ptr<uint32_t[]> tmp;
@@ -270,6 +272,49 @@ int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t 
   }
   if (out != words) memcpy(words, out, sizeof(uint32_t) * nw);
   return nw;
+}
+
+int64_t rans_encode_levels(const uint32_t *syms, const uint8_t *levels, int64_t n,
+                           const uint32_t *cdfs, int64_t A, uint32_t *words, int64_t cap) {
+  return encode_levels_from([syms](int64_t i) { return syms[i]; }, levels, n, cdfs, A, words,
+                            cap);
+}
+
+// The same from the quantizer's codes (u16 or u32): a code c != 0 is the
+// symbol c - radius + 2^kexp of the narrowed alphabet (rans.py:83-89); a
+// code outside it wraps past A in uint32 arithmetic and is rejected there.
+int64_t rans_encode_codes_levels(const void *codes, int cb, const uint8_t *levels, int64_t n,
+                                 uint32_t radius, int kexp, const uint32_t *cdfs, int64_t A,
+                                 uint32_t *words, int64_t cap) {
+  if ((cb != 2 && cb != 4) || kexp < 1 || kexp > 30 || A != ((int64_t)2 << kexp)) {
+    set_error("rans_encode_codes_levels: code_bytes 2 or 4 and A = 2 << kexp");
+    return LATEC_E_ARG;
+  }
+  const uint32_t off = (1u << kexp) - radius;  // span + 1 - radius, mod 2^32
+  if (cb == 2) {
+    const uint16_t *c = (const uint16_t *)codes;
+    return encode_levels_from(
+        [c, off](int64_t i) { return c[i] ? (uint32_t)c[i] + off : 0u; }, levels, n, cdfs, A,
+        words, cap);
+  }
+  const uint32_t *c = (const uint32_t *)codes;
+  return encode_levels_from([c, off](int64_t i) { return c[i] ? c[i] + off : 0u; }, levels, n,
+                            cdfs, A, words, cap);
+}
+
+// choose_kexp (rans.py:70-80): the bit length of the largest |c - radius|
+// over the codes c != 0, at least 1.
+int code_kexp(const void *codes, int cb, int64_t n, uint32_t radius) {
+  uint32_t peak = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const uint32_t c = cb == 2 ? ((const uint16_t *)codes)[i] : ((const uint32_t *)codes)[i];
+    if (!c) continue;
+    const uint32_t d = c >= radius ? c - radius : radius - c;
+    peak = std::max(peak, d);
+  }
+  int b = 0;
+  while (b < 32 && (peak >> b)) b++;
+  return std::max(1, b);
 }
 
 // n symbols of one CDF row, each handed to sink(i, symbol) as it is decoded.

@@ -594,7 +594,7 @@ class GNNCompressorCodec:
                         (fine_gates if fine else gates).append(word)
                     stats["outliers"] += k
                     stats["stage_codes"] += n
-                    futures.append(pool.submit(pack_stage, h_codes.astype(np.uint32), h_val,
+                    futures.append(pool.submit(pack_stage, h_codes, h_val,
                                                rans_levels=h_lv, rans_tables=tables[s]))
                     continue
                 gidx = pred_obj.stage_gidx(s)
@@ -644,7 +644,7 @@ class GNNCompressorCodec:
                 h_lv = D.to_host(lv, np.uint8)
                 stats["outliers"] += k
                 stats["stage_codes"] += n
-                futures.append(pool.submit(pack_stage, h_codes.astype(np.uint32), h_val,
+                futures.append(pool.submit(pack_stage, h_codes, h_val,
                                            rans_levels=h_lv, rans_tables=tables[s]))
             pred_obj.finish_wave(recon)
         parts.extend(f.result() for f in futures)

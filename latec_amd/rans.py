@@ -60,14 +60,18 @@ def full_kexp(radius: int) -> int:
     return max(1, int(int(radius) - 1).bit_length())
 
 
+def code_array(codes) -> np.ndarray:
+    """codes as a flat contiguous u16 or u32 array (other dtypes widened to u32)."""
+    a = np.asarray(codes).ravel()
+    if a.dtype not in (np.uint16, np.uint32):
+        a = a.astype(np.uint32)
+    return np.ascontiguousarray(a)
+
+
 def choose_kexp(codes, radius: int) -> int:
-    """rans.py:70-80."""
-    codes = np.asarray(codes, np.uint32).ravel()
-    inr = codes[codes != 0]
-    if len(inr) == 0:
-        return 1
-    peak = int(np.abs(inr.astype(np.int64) - int(radius)).max())
-    return max(1, peak.bit_length())
+    """rans.py:70-80 (latec_code_kexp: one pass over the codes)."""
+    a = code_array(codes)
+    return check(lib().latec_code_kexp(ptr(a), a.itemsize, len(a), int(radius)))
 
 
 def _narrow(codes, radius: int, kexp: int) -> np.ndarray:
@@ -335,7 +339,7 @@ def rans_decode_const(blob: bytes, n_codes: int, level: int, tables: RansTables,
 def choose_shape(codes, levels64, tables: RansTables, kexp: int, *,
                  sample: float = _SELECT_SAMPLE) -> int:
     """rans.py:204-243 on the GPU-built level weights."""
-    codes = np.asarray(codes, np.uint32).ravel()
+    codes = np.asarray(codes).ravel()
     if len(codes) == 0:
         return 0
     levels = np.asarray(levels64, np.uint8).ravel()
@@ -370,7 +374,7 @@ def choose_shape(codes, levels64, tables: RansTables, kexp: int, *,
 
 def rans_encode(codes, levels64, tables: RansTables, *, kexp=None, shape: int = 0) -> bytes:
     """Per-point levels (rans.py:546-578)."""
-    codes = np.asarray(codes, np.uint32).ravel()
+    codes = code_array(codes)
     levels = np.ascontiguousarray(levels64, np.uint8).ravel()
     if codes.shape != levels.shape:
         raise ValueError("codes and levels must have the same flattened length")
@@ -378,15 +382,15 @@ def rans_encode(codes, levels64, tables: RansTables, *, kexp=None, shape: int = 
         return b""
     if int(codes.max(initial=0)) >= tables.alphabet:
         raise ValueError("code exceeds table alphabet")
-    if int(levels.max(initial=0)) >= N_LEVELS:
-        raise ValueError("scale level exceeds table count")
     kexp = full_kexp(tables.radius) if kexp is None else int(kexp)
-    syms = np.ascontiguousarray(_narrow(codes, tables.radius, kexp), np.uint32)
     cdfs = np.ascontiguousarray(build_cdfs(kexp, shape, tables.precision, 0))
-    cap = len(syms) + 2
+    cap = len(codes) + 2
     words = np.empty(cap, np.uint32)
-    nw = check(lib().latec_rans_encode_levels(ptr(syms), ptr(levels), len(syms), ptr(cdfs),
-                                              cdfs.shape[1] - 1, ptr(words), cap))
+    # narrowing (rans.py:83-89) and the level coder in one pass; levels
+    # >= 64 and codes outside the narrowed alphabet are rejected there
+    nw = check(lib().latec_rans_encode_codes_levels(
+        ptr(codes), codes.itemsize, ptr(levels), len(codes), int(tables.radius), kexp,
+        ptr(cdfs), cdfs.shape[1] - 1, ptr(words), cap))
     return words[:nw].astype("<u4", copy=False).tobytes()
 
 

@@ -78,6 +78,54 @@ def test_rans_levels_words_match_oracle(kexp, n, nlev):
     assert ok.all() and np.array_equal(words[:nw], same)
 
 
+@pytest.mark.parametrize("kexp,radius,n", [(1, 512, 9), (4, 512, 6000), (9, 512, 30000),
+                                            (7, 32768, 20000), (12, 1 << 17, 5000)])
+def test_rans_codes_levels_narrow_then_code(kexp, radius, n):
+    """latec_rans_encode_codes_levels (the quantizer's codes narrowed inside
+    the coder) writes the oracle's words for the narrowed symbols
+    (rans.py:83-89: code 0 -> 0, else code - radius + 2^kexp), from u16 and
+    u32 codes; latec_code_kexp is choose_kexp's bit length; a code outside the
+    narrowed alphabet and a level >= 64 are rejected."""
+    L = _lib.lib()
+    rng = np.random.RandomState(kexp * 7 + n)
+    cdfs = O.build_cdfs(kexp, 1, 24, 0)
+    A = cdfs.shape[1] - 1
+    span = (1 << kexp) - 1
+    levels = rng.randint(0, 64, n).astype(np.uint8)
+    syms = rng.randint(0, A, n).astype(np.uint32)
+    codes = np.where(syms == 0, 0, syms.astype(np.int64) - span - 1 + radius).astype(np.uint32)
+    assert codes.min() >= 0 and (codes[syms > 0] > 0).all()
+    ref = O.rans_encode_levels(syms, levels, cdfs)
+    kinds = [codes] + ([codes.astype(np.uint16)] if radius <= 32768 else [])
+    for c in kinds:
+        words = np.empty(n + 2, np.uint32)
+        nw = _lib.check(L.latec_rans_encode_codes_levels(
+            _lib.ptr(c), c.itemsize, _lib.ptr(levels), n, radius, kexp, _lib.ptr(cdfs), A,
+            _lib.ptr(words), len(words)))
+        assert np.array_equal(words[:nw], ref)
+        small = np.empty(4, np.uint32)  # short output array: the coder's own buffer
+        assert L.latec_rans_encode_codes_levels(
+            _lib.ptr(c), c.itemsize, _lib.ptr(levels), n, radius, kexp, _lib.ptr(cdfs), A,
+            _lib.ptr(small), len(small)) == _lib.LATEC_E_CAPACITY or len(ref) <= 4
+        nz = c[c != 0].astype(np.int64)
+        want = max(1, int(np.abs(nz - radius).max()).bit_length()) if len(nz) else 1
+        assert L.latec_code_kexp(_lib.ptr(c), c.itemsize, n, radius) == want
+    for bad in (radius + span + 1, radius - span - 2):  # one past each end
+        if bad <= 0:
+            continue
+        c = codes.copy()
+        c[n // 2] = bad
+        words = np.empty(n + 2, np.uint32)
+        rc = L.latec_rans_encode_codes_levels(_lib.ptr(c), 4, _lib.ptr(levels), n, radius, kexp,
+                                              _lib.ptr(cdfs), A, _lib.ptr(words), len(words))
+        assert rc == _lib.LATEC_E_ARG and "narrowed" in _lib.last_error()
+    lv = levels.copy()
+    lv[-1] = 64
+    rc = L.latec_rans_encode_codes_levels(_lib.ptr(codes), 4, _lib.ptr(lv), n, radius, kexp,
+                                          _lib.ptr(cdfs), A, _lib.ptr(words), len(words))
+    assert rc == _lib.LATEC_E_ARG and "table count" in _lib.last_error()
+
+
 def test_rans_trailing_data_rejected():
     cdf = O.build_cdfs(2, 0)[10]
     w = O.rans_encode_syms(np.array([1, 2, 3], np.uint32), cdf)
