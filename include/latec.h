@@ -277,6 +277,14 @@ int latec_rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf,
 int latec_rans_decode_levels(const uint32_t *words, int64_t nw,
                              const uint8_t *levels, const uint32_t *cdfs,
                              int64_t A, uint32_t *syms, int64_t n);
+/* The same, widened back to the quantizer's codes as latec_rans_encode_
+ * codes_levels narrowed them (symbol 0 -> code 0), written as u16 or u32
+ * (code_bytes); rans_decode + the widening of rans.py:581-616 in one call.
+ * LATEC_E_FORMAT also for a u16 code past 0xffff. */
+int latec_rans_decode_codes_levels(const uint32_t *words, int64_t nw,
+                                   const uint8_t *levels, const uint32_t *cdfs,
+                                   int64_t A, uint32_t radius, int kexp,
+                                   void *codes, int code_bytes, int64_t n);
 
 /* pack_stage, const-level path (bitstream.py:178-243): codes are u16/u32
  * (code_bytes), `fit` from latec_fit_*, `cdf_row` the (A+1) CDF of

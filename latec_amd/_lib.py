@@ -122,6 +122,7 @@ SIGNATURES = {
     "latec_code_kexp": (i32, [p, i32, i64, u32]),
     "latec_rans_decode": (i32, [p, i64, p, i64, p, i64]),
     "latec_rans_decode_levels": (i32, [p, i64, p, p, i64, p, i64]),
+    "latec_rans_decode_codes_levels": (i32, [p, i64, p, p, i64, u32, i32, p, i32, i64]),
     "latec_pack_stage": (i64, [p, i32, i64, p, i64, u32, i32, p, p, p, i64]),
     "latec_stage_header": (i32, [p, i64, p, p, p, p, p, p, p]),
     "latec_unpack_stage": (i64, [p, i64, u32, p, i64, p, i32, p, p, i64, p]),

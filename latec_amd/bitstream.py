@@ -266,8 +266,10 @@ def unpack_record_into(arr: np.ndarray, off: int, n_expect: int, radius: int, co
     return off + used
 
 
-def unpack_stage(buf, off, *, rans_levels=None, rans_tables=None, radius=DEFAULT_RADIUS):
-    """bitstream.py:288-344: (codes u32, outliers f32, next offset)."""
+def unpack_stage(buf, off, *, rans_levels=None, rans_tables=None, radius=DEFAULT_RADIUS,
+                 code_dtype=np.uint32):
+    """bitstream.py:288-344: (codes u32, outliers f32, next offset); with
+    ``rans_levels`` the codes come as ``code_dtype`` (u32, or u16)."""
     n_codes, hlen, kexp, shape, level, owx, literal, _ = parse_record(buf, off)
     if rans_levels is None:
         tables = rans_tables if rans_tables is not None else R.const_tables(radius)
@@ -278,8 +280,8 @@ def unpack_stage(buf, off, *, rans_levels=None, rans_tables=None, radius=DEFAULT
     if not kexp:
         raise ValueError("rANS stage record has no coder window")
     o = off + 12
-    codes = R.rans_decode(bytes(buf[o:o + hlen]), rans_levels, rans_tables, kexp=kexp,
-                          shape=shape)
+    codes = R.rans_decode(np.frombuffer(buf, np.uint8, count=hlen, offset=o), rans_levels,
+                          rans_tables, kexp=kexp, shape=shape, code_dtype=code_dtype)
     if len(codes) != n_codes:
         raise ValueError("stage code count mismatch")
     o += hlen

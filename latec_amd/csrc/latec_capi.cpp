@@ -58,6 +58,8 @@ int64_t rans_encode_levels(const uint32_t *, const uint8_t *, int64_t, const uin
 int64_t rans_encode_codes_levels(const void *, int, const uint8_t *, int64_t, uint32_t, int,
                                  const uint32_t *, int64_t, uint32_t *, int64_t);
 int code_kexp(const void *, int, int64_t, uint32_t);
+int rans_decode_codes_levels(const uint32_t *, int64_t, const uint8_t *, const uint32_t *, int64_t,
+                             uint32_t, int, void *, int, int64_t);
 int rans_decode(const uint32_t *, int64_t, const uint32_t *, int64_t, uint32_t *, int64_t);
 int rans_decode_levels(const uint32_t *, int64_t, const uint8_t *, const uint32_t *, int64_t,
                        uint32_t *, int64_t);
@@ -315,6 +317,12 @@ int latec_code_kexp(const void *codes, int code_bytes, int64_t n, uint32_t radiu
     return LATEC_E_ARG;
   }
   return code_kexp(codes, code_bytes, n, radius);
+}
+int latec_rans_decode_codes_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
+                                   const uint32_t *cdfs, int64_t A, uint32_t radius, int kexp,
+                                   void *codes, int code_bytes, int64_t n) {
+  return rans_decode_codes_levels(words, nw, levels, cdfs, A, radius, kexp, codes, code_bytes,
+                                  n);
 }
 int latec_rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t A,
                       uint32_t *syms, int64_t n) {

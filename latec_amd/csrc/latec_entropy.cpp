@@ -342,8 +342,12 @@ int rans_decode(const uint32_t *words, int64_t nw, const uint32_t *cdf, int64_t 
   return rans_decode_to(words, nw, cdf, A, n, [syms](int64_t i, uint32_t k) { syms[i] = k; });
 }
 
-int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
-                       const uint32_t *cdfs, int64_t A, uint32_t *syms, int64_t n) {
+// Per-point models: the level groups decoded into contiguous runs of a
+// scratch array, then one pass in position order hands every position's
+// symbol (from the front of its level's run) to put(i, symbol).
+template <class Put>
+static int decode_levels_to(const uint32_t *words, int64_t nw, const uint8_t *levels,
+                            const uint32_t *cdfs, int64_t A, int64_t n, Put put) {
   uint64_t state;
   int64_t top;
   if (!dec_init(state, words, nw, top)) {
@@ -354,10 +358,6 @@ int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
     set_error("stage too large for the rANS coder");
     return LATEC_E_ARG;
   }
-  // the symbols come level by level (each level's positions in increasing
-  // order): each group is decoded into its own contiguous run of a scratch
-  // array, then one pass in position order takes every position's symbol
-  // from the front of its level's run
   int64_t start[65];
   if (!group_levels(levels, n, start)) {
     set_error("scale level exceeds table count");
@@ -381,8 +381,44 @@ int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
   }
   int64_t next[64];
   for (int l = 0; l < 64; l++) next[l] = start[l];
-  for (int64_t i = 0; i < n; i++) syms[i] = t[next[levels[i]]++];
+  for (int64_t i = 0; i < n; i++) put(i, t[next[levels[i]]++]);
   return LATEC_OK;
+}
+
+int rans_decode_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
+                       const uint32_t *cdfs, int64_t A, uint32_t *syms, int64_t n) {
+  return decode_levels_to(words, nw, levels, cdfs, A, n,
+                          [syms](int64_t i, uint32_t k) { syms[i] = k; });
+}
+
+// The same, widened back to the quantizer's codes (u16 or u32): symbol
+// k != 0 is the code k + radius - 2^kexp in uint32 arithmetic, 0 stays 0
+// (rans.py:83-89 inverted); a u16 code past 0xffff is a format error.
+int rans_decode_codes_levels(const uint32_t *words, int64_t nw, const uint8_t *levels,
+                             const uint32_t *cdfs, int64_t A, uint32_t radius, int kexp,
+                             void *codes, int cb, int64_t n) {
+  if ((cb != 2 && cb != 4) || kexp < 1 || kexp > 30 || A != ((int64_t)2 << kexp)) {
+    set_error("rans_decode_codes_levels: code_bytes 2 or 4 and A = 2 << kexp");
+    return LATEC_E_ARG;
+  }
+  const uint32_t off = radius - (1u << kexp);
+  if (cb == 4) {
+    uint32_t *c = (uint32_t *)codes;
+    return decode_levels_to(words, nw, levels, cdfs, A, n,
+                            [c, off](int64_t i, uint32_t k) { c[i] = k ? k + off : 0u; });
+  }
+  uint16_t *c = (uint16_t *)codes;
+  uint32_t over = 0;
+  const int rc = decode_levels_to(words, nw, levels, cdfs, A, n, [c, off, &over](int64_t i, uint32_t k) {
+    const uint32_t v = k ? k + off : 0u;
+    over |= v >> 16;
+    c[i] = (uint16_t)v;
+  });
+  if (rc == LATEC_OK && over) {
+    set_error("decoded code exceeds 16 bits");
+    return LATEC_E_FORMAT;
+  }
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

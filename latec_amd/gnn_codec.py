@@ -742,12 +742,13 @@ class GNNCompressorCodec:
                     h_lv = self._decode_stage_begin(pred_obj, plan, s, n, dkeep, p_full, sc_full,
                                                     block, cand_opts, interp_kind, word or 0,
                                                     float(ebs[s]))
+                    cdt = np.uint16 if o.code_bytes == 2 else np.uint32
                     codes, outl, off = unpack_stage(payload, off, rans_levels=h_lv,
-                                                    rans_tables=tables[s], radius=radius)
+                                                    rans_tables=tables[s], radius=radius,
+                                                    code_dtype=cdt)
                     if len(codes) != n:
                         raise ValueError("stage code count mismatch")
-                    hc = np.ascontiguousarray(codes, np.uint16 if o.code_bytes == 2
-                                              else np.uint32)
+                    hc = np.ascontiguousarray(codes, cdt)
                     ov = np.ascontiguousarray(outl, np.float32)
                     check(lib().latec_gnn_decode_stage_end(
                         pred_obj.model.handle, ptr(hc), ptr(ov) if len(ov) else None, len(ov),

@@ -394,24 +394,29 @@ def rans_encode(codes, levels64, tables: RansTables, *, kexp=None, shape: int = 
     return words[:nw].astype("<u4", copy=False).tobytes()
 
 
-def rans_decode(blob: bytes, levels64, tables: RansTables, *, kexp=None, shape: int = 0):
-    """rans.py:581-616."""
+def rans_decode(blob, levels64, tables: RansTables, *, kexp=None, shape: int = 0,
+                code_dtype=np.uint32):
+    """rans.py:581-616: the stage's codes (``code_dtype`` u32, or u16 when
+    every code fits).  ``blob`` is bytes or a uint8 array of the words."""
     levels = np.ascontiguousarray(levels64, np.uint8).ravel()
     if len(levels) == 0:
-        if blob:
+        if len(blob):
             raise ValueError("non-empty rANS blob for an empty stage")
-        return np.empty(0, np.uint32)
+        return np.empty(0, code_dtype)
     if len(blob) % 4:
         raise ValueError("rANS payload is not aligned to 32-bit words")
-    if int(levels.max(initial=0)) >= N_LEVELS:
-        raise ValueError("scale level exceeds table count")
     kexp = full_kexp(tables.radius) if kexp is None else int(kexp)
-    words = np.frombuffer(blob, "<u4").astype(np.uint32)
+    raw = np.frombuffer(blob, np.uint8)
+    if raw.ctypes.data % 4:
+        raw = raw.copy()
+    words = raw.view("<u4")
     cdfs = np.ascontiguousarray(build_cdfs(kexp, shape, tables.precision, 0))
-    out = np.empty(len(levels), np.uint32)
-    check(lib().latec_rans_decode_levels(ptr(words), len(words), ptr(levels), ptr(cdfs),
-                                         cdfs.shape[1] - 1, ptr(out), len(levels)))
-    return _widen(out, tables.radius, kexp)
+    out = np.empty(len(levels), code_dtype)
+    # decode, level check and widening (0 stays 0) in one call
+    check(lib().latec_rans_decode_codes_levels(ptr(words), len(words), ptr(levels), ptr(cdfs),
+                                               cdfs.shape[1] - 1, int(tables.radius), kexp,
+                                               ptr(out), out.itemsize, len(levels)))
+    return out
 
 
 def model_bits(codes, levels64, tables: RansTables, *, kexp=None, shape: int = 0) -> float:

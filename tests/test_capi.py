@@ -110,6 +110,18 @@ def test_rans_codes_levels_narrow_then_code(kexp, radius, n):
         nz = c[c != 0].astype(np.int64)
         want = max(1, int(np.abs(nz - radius).max()).bit_length()) if len(nz) else 1
         assert L.latec_code_kexp(_lib.ptr(c), c.itemsize, n, radius) == want
+        # and back: the widening decoder returns the codes in the same width
+        back = np.full(n, 7, c.dtype)
+        _lib.check(L.latec_rans_decode_codes_levels(
+            _lib.ptr(ref), len(ref), _lib.ptr(levels), _lib.ptr(cdfs), A, radius, kexp,
+            _lib.ptr(back), c.itemsize, n))
+        assert np.array_equal(back, c)
+    if radius + span > 0xffff:  # a u16 code that does not fit is a format error
+        back = np.empty(n, np.uint16)
+        rc = L.latec_rans_decode_codes_levels(_lib.ptr(ref), len(ref), _lib.ptr(levels),
+                                              _lib.ptr(cdfs), A, radius, kexp, _lib.ptr(back),
+                                              2, n)
+        assert rc == _lib.LATEC_E_FORMAT and "16 bits" in _lib.last_error()
     for bad in (radius + span + 1, radius - span - 2):  # one past each end
         if bad <= 0:
             continue
