@@ -793,7 +793,7 @@ class GNNCompressorCodec:
         if gates is not None and gi != len(gates):
             raise ValueError("gate list length does not match the stream")
         self.last_decoded = recon  # normalized, as the encoder's stats["recon"]
-        values = recon[0].cpu().numpy()
+        values = D.to_host_pinned(recon[0])  # a new array: one copy at the pinned rate
         values *= vmax - vmin
         values += vmin
         out = _restore_dtype(values.reshape(original_shape), dtype)
