@@ -320,7 +320,9 @@ class GNNCompressorCodec:
         self._predictors: dict = {}
         self.last_stats: dict = {}
         self.last_decoded = None  # the last uncompress's normalized reconstruction (device)
-        self.count_flops = False  # count the model's GEMM flops into last_stats (syncs)
+        # count the model's GEMM flops and time its stage passes (HIP events)
+        # into last_stats (model_flops, model_s; syncs)
+        self.count_flops = False
         # each stage after the model in one library call (latec_gnn_code_stage;
         # the decoder: latec_gnn_decode_stage_begin / _end around the rANS
         # decode); False runs the same steps as separate calls from Python
@@ -548,7 +550,7 @@ class GNNCompressorCodec:
         pool = ThreadPoolExecutor(max_workers=8)
         futures = []
         stats = {"outliers": 0, "stage_codes": 0, "gate_words": 0}
-        events = []  # (start, end) HIP events around the model's stage passes
+        events = []  # (start, end) HIP events around the model's stage passes (count_flops)
         for ci in range(n_chunks):
             sls = pred_obj.chunk_slices(ci)
             cshape = tuple(sl.stop - sl.start for sl in sls)
@@ -563,12 +565,13 @@ class GNNCompressorCodec:
                                                rans_tables=tables[s]))
                     continue
                 dkeep = sched.device_keep(s)
-                e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
-                e0.record()
-                p_full, sc_full = pred_obj.predict_wave_stage(s, recon, ebs[s])
-                e1.record()
-                events.append((e0, e1))
                 if self.count_flops:
+                    e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+                    e0.record()
+                p_full, sc_full = pred_obj.predict_wave_stage(s, recon, ebs[s])
+                if self.count_flops:
+                    e1.record()
+                    events.append((e0, e1))
                     stats["model_flops"] = stats.get("model_flops", 0) + pred_obj.stage_flops(s)
                 n = int(sched.counts[s])
                 if n == 0:
@@ -650,7 +653,8 @@ class GNNCompressorCodec:
         parts.extend(f.result() for f in futures)
         pool.shutdown()
         T.cuda.current_stream().synchronize()
-        stats["model_s"] = sum(a.elapsed_time(b) for a, b in events) * 1e-3
+        if self.count_flops:
+            stats["model_s"] = sum(a.elapsed_time(b) for a, b in events) * 1e-3
         stats["gate_words"] = len(gates or []) + len(fine_gates or [])
         stats["gates_fired"] = sum(1 for g in (gates or []) + (fine_gates or []) if g)
         # the normalized reconstruction the decoder must reproduce bit for bit
