@@ -237,8 +237,14 @@ def ptr(a) -> C.c_void_p:
 
 
 def stream_ptr(stream=None) -> C.c_void_p:
-    """hipStream_t of a torch stream (None -> torch's current stream)."""
+    """hipStream_t of a torch stream (None -> torch's current stream on the
+    current device, read without building a Stream object: the per-stage
+    library calls make thousands of these)."""
     import torch
 
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return C.c_void_p(s.cuda_stream)
+    if stream is not None:
+        return C.c_void_p(stream.cuda_stream)
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None and torch.cuda.is_initialized():
+        return C.c_void_p(raw(torch._C._cuda_getDevice()))
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)

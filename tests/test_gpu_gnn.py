@@ -510,3 +510,17 @@ def test_native_decode_stage_rejects_outlier_count_mismatch(tmp_path, monkeypatc
     monkeypatch.setattr(GC, "unpack_stage", real)
     again = codec.uncompress(blob).numpy()
     assert np.array_equal(good.view(np.uint32), again.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_stream_ptr_is_torch_current_stream():
+    """The library calls' stream is torch's current stream, also inside a
+    `torch.cuda.stream` block."""
+    from latec_amd._lib import stream_ptr
+
+    torch.cuda.init()
+    assert (stream_ptr().value or 0) == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert (stream_ptr().value or 0) == s.cuda_stream != 0
+    assert (stream_ptr().value or 0) == torch.cuda.current_stream().cuda_stream
