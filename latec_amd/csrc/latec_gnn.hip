@@ -3276,8 +3276,9 @@ int latec_gnn_decode_stage_end(latec_gnn *g, const void *codes, const float *out
   const size_t hbytes = hb_c + 2 * hb_k + 256;
   hipStream_t s = (hipStream_t)stream;
   if (g->stage_host_bytes < hbytes) {
-    // the previous stage's copies have landed: its begin waited on the stream
-    int rc = hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+    // every copy out of the old staging has landed (whatever stream an
+    // earlier call used) before it is freed
+    int rc = hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (rc) return rc;
     if (g->stage_host) (void)hipHostFree(g->stage_host);
     g->stage_host = nullptr;
